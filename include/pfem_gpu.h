@@ -1,0 +1,182 @@
+/*
+ * pfem_gpu.h — C-ABI of the MI355X (gfx950) PFEM assembly + sparse-solve library.
+ *
+ * The library replaces, per time step, the element-matrix builders, the sparse assembly,
+ * the boundary conditions and the sparse solve of ImperatorS79/PFEM's incompressible PSPG
+ * solver and the explicit update of its weakly compressible solver.  Everything above
+ * those (Lua configuration, the time loop, dt control, CGAL remeshing) stays with the
+ * caller.  Reference citations are relative to the reference repository root.
+ *
+ * Conventions (mirroring the reference's ownership and error behaviour):
+ *  - The caller owns every host array and passes pointer + count; the library copies
+ *    what it needs and never keeps a host pointer.  The library owns all device memory
+ *    behind the opaque handles.  Handles are not copyable.
+ *  - Every function returns an int status: 0 OK, < 0 hard error (invalid argument, out of
+ *    memory, HIP error; the reference throws std::runtime_error), > 0 numerical failure
+ *    (the reference returns false: solver failure, Picard non-convergence, NaN residual).
+ *    pfem_last_error() gives the message of the calling thread's last failure.
+ *  - A case handle is not re-entrant (one HIP stream per case); distinct case handles may
+ *    be driven from distinct host threads.  Indices are int32.
+ *  - Vectors are field-blocked like the reference's q vectors (StatesFromToQ.hpp:9-34):
+ *    entry (node n, field s) is at n + s*N.  PSPG fields are u, v, [w], p.
+ */
+#ifndef PFEM_GPU_H
+#define PFEM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---------------------------------------------------------------- */
+#define PFEM_OK 0
+#define PFEM_E_INVALID (-1)
+#define PFEM_E_OOM (-2)
+#define PFEM_E_HIP (-3)
+#define PFEM_E_UNSUPPORTED (-4)
+#define PFEM_E_STATE (-5)
+#define PFEM_NOT_CONVERGED 1 /* Krylov iteration limit or Picard iterCount > maxIter */
+#define PFEM_NAN 2           /* NaN / breakdown (the reference's NaN residual branch) */
+
+/* ---- node flags (Node.hpp: m_isBound, isFree(), m_isOnFreeSurface, m_isFixed) -------- */
+#define PFEM_NODE_BOUND 1
+#define PFEM_NODE_FREE 2 /* recomputed by the library: node in no element (Node.inl isFree) */
+#define PFEM_NODE_ONFS 4
+#define PFEM_NODE_FIXED 8     /* IC.BoundaryFixed: position never updated */
+#define PFEM_NODE_DIRICHLET 16 /* boundary node whose tag has a "<Tag>V" BC function */
+
+/* ---- enums ------------------------------------------------------------------------- */
+#define PFEM_KIND_PSPG 0 /* MomContEqIncompNewton with Solver.id = "PSPG" */
+#define PFEM_KIND_WC 1   /* WCompNewtonNoT with Solver.id = "CDS_dpdt" */
+
+#define PFEM_RES_U 0    /* MomContEquation.inl:229-236 residual = "U" */
+#define PFEM_RES_U_P 1  /* "U_P" */
+#define PFEM_RES_AX_F 2 /* "Ax_f" */
+
+#define PFEM_PREC_FP64 0  /* FP64 assembly, SpMV and Krylov */
+#define PFEM_PREC_MIXED 1 /* FP32 inner BiCGSTAB on an FP32 copy of A, FP64 iterative refinement */
+
+#define PFEM_STATE_FP64 0 /* WC node state stored in FP64 */
+#define PFEM_STATE_FP32 1 /* WC node state stored in FP32 (accumulators stay FP64) */
+
+typedef struct pfem_ctx pfem_ctx;
+typedef struct pfem_case pfem_case;
+
+/* Case parameters.  Replaces the Lua tables read by the equation constructors:
+ * MomContEquation.inl:28-30,226-243 (PSPG) and ContEquation.inl:26-44,
+ * MomEquation.inl:28-29,149-153 (WC). */
+typedef struct pfem_case_desc {
+    int32_t dim;  /* 2 or 3 */
+    int32_t kind; /* PFEM_KIND_* */
+    double rho;   /* Material.rho (PSPG) */
+    double mu;    /* Material.mu */
+    double gamma; /* Material.gamma (surface tension); must be 0 (FST is not implemented) */
+    double body_force[3];
+    /* WC (ContEq / Material) */
+    double K0, K0p, rho_star;
+    int32_t meduri;     /* ContEq.stabilization: 1 "Meduri", 0 "None" */
+    int32_t wc_state;   /* PFEM_STATE_* */
+    /* PSPG Picard (MomContEq table) */
+    int32_t residual;   /* PFEM_RES_* */
+    int32_t max_iter;   /* maxIter */
+    double min_res;     /* minRes */
+    /* linear solver (replaces EigenSparseSolver, MomContEquation.hpp:6-12) */
+    int32_t precision;  /* PFEM_PREC_* */
+    int32_t krylov_max_iter;
+    double krylov_rtol; /* stop when ||b - A x|| <= rtol ||b|| (true residual, FP64) */
+    double inner_rtol;  /* mixed precision: relative tolerance of each FP32 inner solve */
+} pfem_case_desc;
+
+typedef struct pfem_solve_stats {
+    int32_t status;       /* PFEM_OK / PFEM_NOT_CONVERGED / PFEM_NAN */
+    int32_t krylov_iters; /* BiCGSTAB iterations (inner iterations in mixed mode) */
+    int32_t refinements;  /* FP64 refinement sweeps (mixed mode) */
+    int32_t restarts;
+    double relres;        /* final ||b - A x|| / ||b|| in FP64 */
+} pfem_solve_stats;
+
+typedef struct pfem_step_stats {
+    int32_t status;
+    int32_t picard_iters;
+    int32_t krylov_iters;
+    int32_t assemblies;
+    double residual;   /* final Picard residual */
+    double max_relres; /* worst linear-solve relative residual in the step */
+    double t_assemble, t_solve, t_update, t_residual; /* seconds (host wall clock) */
+} pfem_step_stats;
+
+/* ---- context ------------------------------------------------------------------------ */
+int pfem_ctx_create(int32_t device, pfem_ctx** out);
+int pfem_ctx_destroy(pfem_ctx* ctx);
+const char* pfem_last_error(void);
+int pfem_version(void);
+
+/* ---- case lifetime -------------------------------------------------------------------
+ * pfem_case_create replaces the equation constructors (MomContEquation.inl:7-256;
+ * MomEquation.inl:8-156 + ContEquation.inl:7-98). */
+int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out);
+int pfem_case_destroy(pfem_case* c);
+
+/* Upload a mesh: the state of Mesh after triangulateAlphaShape / remesh
+ * (Mesh.cpp:919-926).  conn: E x (dim+1) node indices (Element::m_nodesIndexes order);
+ * flags: N bytes of PFEM_NODE_*; pos: dim x N field-blocked coordinates.  Builds the CSR
+ * pattern on the GPU (PSPG) and resets the state to zero. */
+int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, const uint8_t* flags,
+                       const double* pos);
+
+/* PSPG: q = (dim+1) x N (u, v, [w], p).  WC: v (dim x N), p (N), rho (N), a (dim x N)
+ * concatenated, i.e. (2*dim + 2) x N — the node states of WCompNewtonNoT. */
+int pfem_case_set_state(pfem_case* c, const double* q);
+int pfem_case_get_state(pfem_case* c, double* q);
+int pfem_case_set_positions(pfem_case* c, const double* pos);
+int pfem_case_get_positions(pfem_case* c, double* pos);
+/* Dirichlet velocities, dim x N (read at PFEM_NODE_DIRICHLET nodes): the values the
+ * reference obtains from Lua "<Tag>V"(pos, t + dt) (MomContEquationPSPG.inl:208-214,
+ * MomEquation.inl:358-364).  Default zero. */
+int pfem_case_set_dirichlet(pfem_case* c, const double* vals);
+int pfem_case_sync(pfem_case* c);
+
+/* ---- PSPG building blocks -------------------------------------------------------------
+ * m_buildAbPSPG + m_applyBCPSPG (MomContEquationPSPG.inl:7-235).  q_prev: previous step
+ * state (be and free-node rhs); q_iter: node states used by tau (the current Picard
+ * iterate).  Either pointer may be NULL to use the case's device state. */
+int pfem_pspg_assemble(pfem_case* c, double dt, const double* q_prev, const double* q_iter);
+/* m_solver.analyzePattern/factorize/solve (MomContEquationPSPG.inl:281-290) on the
+ * assembled system.  x0 (nullable) is the initial guess; x_out (nullable) receives q. */
+int pfem_pspg_solve(pfem_case* c, const double* x0, double* x_out, pfem_solve_stats* stats);
+/* CSR export (parity tests).  row_ptr: n+1, col_idx/vals: nnz. */
+int pfem_get_csr_sizes(pfem_case* c, int64_t* n, int64_t* nnz);
+int pfem_get_csr(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, double* vals);
+int pfem_get_rhs(pfem_case* c, double* b);
+/* y = A x with the assembled FP64 matrix (host arrays of n). */
+int pfem_spmv(pfem_case* c, const double* x, double* y);
+
+/* One PSPG time step: PicardAlgo::solve (PicardAlgo.cpp:31-95) with the PSPG callbacks
+ * (MomContEquationPSPG.inl:262-372), positions updated as updateNodesPositionFromSave
+ * (Mesh.cpp:1238-1277).  On failure (> 0) positions and state are restored
+ * (Mesh::restoreNodesList). */
+int pfem_pspg_step(pfem_case* c, double dt, pfem_step_stats* stats);
+
+/* ---- WC ------------------------------------------------------------------------------
+ * One CDS_dpdt step (WCompNewton/Solver.cpp:236-276). */
+int pfem_wc_step(pfem_case* c, double dt, pfem_step_stats* stats);
+/* computeNextDT (WCompNewton/Solver.cpp:192-234): dt = min(sqrt(min_e sc^2 he^2 / S), max_dt). */
+int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* dt_out);
+
+/* ---- batched serving (no counterpart in the reference: many independent cases) -------
+ * Runs one time step of every listed case as one batch: the cases' assembly, BC and
+ * Krylov phases run as shared launches over a device work table (hipGraph-captured
+ * BiCGSTAB chunks).  Cases must have distinct handles on the same context.  status[i]
+ * receives each case's step status; stats may be NULL. */
+int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, const double* dts, int32_t* status,
+                    pfem_step_stats* stats);
+
+/* Device footprint of a case in bytes (admission control by HBM footprint). */
+int pfem_case_footprint(pfem_case* c, int64_t* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PFEM_GPU_H */

@@ -1,0 +1,234 @@
+"""ctypes front-end of the CPU oracle (liboracle.so built from oracle/pfem_oracle.cpp).
+
+TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, always as the checker or the timed CPU baseline — never by the product
+path (pfem_amd/).  The restatement it wraps follows the reference files cited in
+pfem_oracle.cpp; the pins that check it live in tests/test_oracle_*.py.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from dataclasses import dataclass
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_u8p = C.POINTER(C.c_uint8)
+
+
+def build(force: bool = False) -> str:
+    """Compile liboracle.so in place (make -C oracle)."""
+    if force or not os.path.exists(_LIB_PATH):
+        subprocess.run(["make", "-C", _HERE, "-B" if force else "-s"], check=True)
+    return _LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        L.orc_pspg_assemble.restype = C.c_void_p
+        L.orc_pspg_assemble.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, _dp, C.c_double,
+                                        C.c_double, _dp, C.c_double, C.c_int, C.c_int]
+        L.orc_sys_n.restype = C.c_int64
+        L.orc_sys_n.argtypes = [C.c_void_p]
+        L.orc_sys_nnz.restype = C.c_int64
+        L.orc_sys_nnz.argtypes = [C.c_void_p]
+        L.orc_sys_copy.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
+        L.orc_sys_times.argtypes = [C.c_void_p, _dp]
+        L.orc_sys_free.argtypes = [C.c_void_p]
+        L.orc_pspg_element.argtypes = [C.c_int, _dp, _dp, _dp, C.c_double, C.c_double, _dp, C.c_double, _dp, _dp, _dp]
+        L.orc_element_matrices.argtypes = [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp, _dp, _dp, _dp, _dp,
+                                           _dp]
+        L.orc_sys_solve.restype = C.c_int
+        L.orc_sys_solve.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, _ip, _dp, C.c_int]
+        L.orc_csr_bicgstab.restype = C.c_int
+        L.orc_csr_bicgstab.argtypes = [C.c_int64, _ip, _ip, _dp, _dp, _dp, C.c_int, C.c_double, C.c_int, _ip, _dp,
+                                       C.c_int]
+        L.orc_pspg_step.restype = C.c_int
+        L.orc_pspg_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    _dp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
+                                    C.c_int]
+        L.orc_wc_step.restype = C.c_int
+        L.orc_wc_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
+                                  _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _dp, C.c_int]
+        L.orc_wc_next_dt.restype = C.c_double
+        L.orc_wc_next_dt.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, C.c_double, C.c_int]
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(_ip)
+
+
+def _u8(a):
+    return a.ctypes.data_as(_u8p)
+
+
+def _g3(g):
+    out = np.zeros(3)
+    out[:len(g)] = g
+    return out
+
+
+@dataclass
+class CsrSystem:
+    n: int
+    row_ptr: np.ndarray
+    col_idx: np.ndarray
+    vals: np.ndarray
+    b: np.ndarray
+    times: np.ndarray  # Compute triplets, Push back, Assemble matrix, Assemble vector, Apply BC
+
+    def to_scipy(self):
+        import scipy.sparse as sp
+        return sp.csr_matrix((self.vals, self.col_idx, self.row_ptr), shape=(self.n, self.n))
+
+
+def pspg_assemble(mesh, q_prev, q_iter=None, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3, apply_bc=True,
+                  nthreads=0) -> CsrSystem:
+    """m_buildAbPSPG (+ m_applyBCPSPG): MomContEquationPSPG.inl:7-235."""
+    L = lib()
+    d = mesh.dim
+    N = mesh.n_nodes
+    g = _g3(g if g is not None else ([0.0, -9.81] if d == 2 else [0.0, 0.0, -9.81]))
+    q_prev = np.ascontiguousarray(q_prev, dtype=np.float64)
+    q_iter = q_prev if q_iter is None else np.ascontiguousarray(q_iter, dtype=np.float64)
+    dir_vals = np.zeros(d * N) if dir_vals is None else np.ascontiguousarray(dir_vals, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
+    pos = np.ascontiguousarray(mesh.pos, dtype=np.float64)
+    h = L.orc_pspg_assemble(d, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(q_prev), _d(q_iter), _d(dir_vals),
+                            rho, mu, _d(g), dt, 1 if apply_bc else 0, nthreads)
+    try:
+        n = L.orc_sys_n(h)
+        nnz = L.orc_sys_nnz(h)
+        rp = np.empty(n + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        v = np.empty(nnz)
+        b = np.empty(n)
+        L.orc_sys_copy(h, _i(rp), _i(ci), _d(v), _d(b))
+        t = np.empty(5)
+        L.orc_sys_times(h, _d(t))
+    finally:
+        L.orc_sys_free(h)
+    return CsrSystem(n, rp, ci, v, b, t)
+
+
+def pspg_element(dim, pos_e, q_prev_e, q_iter_e, rho=1000.0, mu=1e-3, g=None, dt=1e-3):
+    """Ae, be, tau of a standalone element (PSPG.inl:26-53) with nodes 0..dim."""
+    L = lib()
+    na = (dim + 1) * (dim + 1)
+    g = _g3(g if g is not None else ([0.0, -9.81] if dim == 2 else [0.0, 0.0, -9.81]))
+    Ae = np.empty((na, na))
+    be = np.empty(na)
+    tau = np.empty(1)
+    L.orc_pspg_element(dim, _d(np.ascontiguousarray(pos_e, dtype=np.float64)),
+                       _d(np.ascontiguousarray(q_prev_e, dtype=np.float64)),
+                       _d(np.ascontiguousarray(q_iter_e, dtype=np.float64)), rho, mu, _d(g), dt, _d(Ae), _d(be),
+                       _d(tau))
+    return Ae, be, float(tau[0])
+
+
+def element_matrices(dim, pos_e, rho=1.0, mu=1.0):
+    """M (factor rho), K (mu, ddev = diag(2..,1..)), D, C, L (factor 1), gradN, detJ, r_in."""
+    L = lib()
+    ne = dim + 1
+    nd = dim * ne
+    M = np.empty((ne, ne))
+    K = np.empty((nd, nd))
+    D = np.empty((ne, nd))
+    Cm = np.empty((ne, nd))
+    Lm = np.empty((ne, ne))
+    G = np.empty((dim, ne))
+    dj = np.empty(1)
+    rin = np.empty(1)
+    L.orc_element_matrices(dim, _d(np.ascontiguousarray(pos_e, dtype=np.float64)), rho, mu, _d(M), _d(K), _d(D),
+                           _d(Cm), _d(Lm), _d(G), _d(dj), _d(rin))
+    return dict(M=M, K=K, D=D, C=Cm, L=Lm, gradN=G, detJ=float(dj[0]), rin=float(rin[0]))
+
+
+def solve(sys: CsrSystem, kind=2, rtol=1e-10, maxit=20000, x0=None, nthreads=0):
+    """kind 0 dense LU (SciPy), 1 BiCGSTAB+Jacobi, 2 BiCGSTAB+ILU(0). Returns (status, x, iters, relres)."""
+    x = np.zeros(sys.n) if x0 is None else np.array(x0, dtype=np.float64)
+    if kind == 0:
+        import scipy.linalg as sla
+        A = sys.to_scipy().toarray()
+        x[:] = sla.solve(A, sys.b)
+        rr = float(np.linalg.norm(sys.b - A @ x) / max(np.linalg.norm(sys.b), 1e-300))
+        return 0, x, 0, rr
+    st, it, rr = csr_bicgstab(sys.n, sys.row_ptr, sys.col_idx, sys.vals, sys.b, x, 0 if kind == 1 else 1, rtol, maxit,
+                              nthreads)
+    return st, x, it, rr
+
+
+def csr_bicgstab(n, rp, ci, v, b, x, prec=0, rtol=1e-10, maxit=1000, nthreads=0):
+    L = lib()
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    st = L.orc_csr_bicgstab(n, _i(rp), _i(ci), _d(v), _d(b), _d(x), prec, rtol, maxit, C.byref(it), C.byref(rr),
+                            nthreads)
+    return st, it.value, rr.value
+
+
+RES_KIND = {"U": 0, "U_P": 1, "Ax_f": 2}
+
+
+def pspg_step(mesh, pos, q, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3, residual="Ax_f", min_res=1e-6,
+              max_iter=10, solver_kind=2, rtol=1e-10, krylov_maxit=20000, nthreads=0):
+    """One Picard time step (PicardAlgo.cpp:31-95 + PSPG.inl:262-372); pos, q updated in place."""
+    L = lib()
+    d = mesh.dim
+    N = mesh.n_nodes
+    g = _g3(g if g is not None else ([0.0, -9.81] if d == 2 else [0.0, 0.0, -9.81]))
+    dir_vals = np.zeros(d * N) if dir_vals is None else np.ascontiguousarray(dir_vals, dtype=np.float64)
+    stats = np.zeros(14)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
+    assert pos.dtype == np.float64 and pos.flags.c_contiguous and q.dtype == np.float64 and q.flags.c_contiguous
+    st = L.orc_pspg_step(d, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(q), _d(dir_vals), rho, mu, _d(g), dt,
+                         RES_KIND[residual], min_res, max_iter, solver_kind, rtol, krylov_maxit, _d(stats), nthreads)
+    keys = ["picard_iters", "krylov_iters", "assemblies", "final_res", "max_relres", "t_assemble", "t_solve",
+            "t_update", "t_residual", "t_triplets", "t_push", "t_set_from_triplets", "t_vector", "t_bc"]
+    return st, dict(zip(keys, stats.tolist()))
+
+
+def wc_step(mesh, pos, v, p, rho, a, dt, dir_vals=None, mu=1e-3, g=None, K0=2.2e9, K0p=7.0, rho_star=1000.0,
+            meduri=True, nthreads=0):
+    """One CDS_dpdt step (WCompNewton/Solver.cpp:236-276); arrays updated in place."""
+    L = lib()
+    d = mesh.dim
+    N = mesh.n_nodes
+    g = _g3(g if g is not None else ([0.0, -9.81] if d == 2 else [0.0, 0.0, -9.81]))
+    dir_vals = np.zeros(d * N) if dir_vals is None else np.ascontiguousarray(dir_vals, dtype=np.float64)
+    for arr in (pos, v, p, rho, a):
+        assert arr.dtype == np.float64 and arr.flags.c_contiguous
+    t = np.zeros(2)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
+    L.orc_wc_step(d, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(v), _d(p), _d(rho), _d(a), _d(dir_vals), mu,
+                  _d(g), K0, K0p, rho_star, 1 if meduri else 0, dt, _d(t), nthreads)
+    return t
+
+
+def wc_next_dt(mesh, pos, v, p, rho, mu=1e-3, K0=2.2e9, K0p=7.0, security_coeff=0.5, max_dt=1.0, nthreads=0):
+    """computeNextDT (WCompNewton/Solver.cpp:192-234)."""
+    L = lib()
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    return L.orc_wc_next_dt(mesh.dim, mesh.n_nodes, mesh.n_elems, _i(conn), _d(pos), _d(v), _d(p), _d(rho), mu, K0,
+                            K0p, security_coeff, max_dt, nthreads)

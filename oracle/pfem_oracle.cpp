@@ -1,0 +1,1348 @@
+// pfem_oracle.cpp — TEST INFRASTRUCTURE ONLY.
+//
+// CPU FP64 restatement of the reference PFEM per-time-step path (ImperatorS79/PFEM):
+// P1 element matrices, PSPG triplet assembly, setFromTriplets-equivalent duplicate sum,
+// Dirichlet column elimination, the Picard loop, the weakly compressible CDS_dpdt step and
+// the CFL time step.  It is the checker for the HIP library and the timed CPU baseline; it
+// is never linked into or called by the product path.
+//
+// Every function cites the reference file:line it restates (paths relative to the
+// reference repository root).  Third-party arithmetic the reference delegates to Eigen
+// (SparseMatrix::setFromTriplets, SparseLU) is restated from Eigen 3.3's published
+// algorithm (triplets are inserted per inner vector in input order, duplicates are summed
+// in that order) or replaced by a stated stand-in (dense partial-pivot LU for tiny
+// systems; BiCGSTAB + ILU(0)/Jacobi as the "Krylov stand-in for Eigen SparseLU").
+//
+// Pins (tests/test_oracle_*.py): closed-form P1 matrices, invariants (K*rigid = 0,
+// L*1 = 0, sum gradN = 0), brute-force triplet lists on 1-2 element meshes, SciPy SuperLU
+// golden solutions, lumped-mass conservation and the CFL closed form.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <limits>
+#include <numeric>
+#include <stdexcept>
+#include <vector>
+#include <omp.h>
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+static double secs(clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+enum NodeFlag : uint8_t { BOUND = 1, FREE = 2, ONFS = 4, FIXED = 8, DIRICHLET = 16 };
+
+// --------------------------------------------------------------------------------------
+// Quadrature tables: srcs/mesh/Mesh.cpp:342-530 (getGaussPoints, getGaussWeight,
+// getRefElementSize, getShapeFunctions).  nGPHD = 3 (2D) / 4 (3D):
+// srcs/simulation/physics/IncompNewton/MomContEquation.inl:12-23.
+// --------------------------------------------------------------------------------------
+template <int D> struct Quad;
+template <> struct Quad<2> {
+    static constexpr int ng = 3;
+    static constexpr double refSize = 0.5;  // Mesh.cpp:482
+    double gp[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}, {2.0 / 3.0, 1.0 / 6.0}};  // Mesh.cpp:384-386
+    double w[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};  // Mesh.cpp:451
+    double N[3][3];
+    Quad() {
+        for (int g = 0; g < 3; ++g) {  // Mesh.cpp:510-513
+            N[g][0] = 1 - gp[g][0] - gp[g][1];
+            N[g][1] = gp[g][0];
+            N[g][2] = gp[g][1];
+        }
+    }
+};
+template <> struct Quad<3> {
+    static constexpr int ng = 4;
+    static constexpr double refSize = 0.16666666666666666666666666666667;  // Mesh.cpp:485
+    double gp[4][3] = {{0.585410196624968, 0.138196601125011, 0.138196601125011},
+                       {0.138196601125011, 0.585410196624968, 0.138196601125011},
+                       {0.138196601125011, 0.138196601125011, 0.585410196624968},
+                       {0.138196601125011, 0.138196601125011, 0.138196601125011}};  // Mesh.cpp:402-405
+    double w[4] = {0.25, 0.25, 0.25, 0.25};  // Mesh.cpp:467
+    double N[4][4];
+    Quad() {
+        for (int g = 0; g < 4; ++g) {  // Mesh.cpp:516-520
+            N[g][0] = 1 - gp[g][0] - gp[g][1] - gp[g][2];
+            N[g][1] = gp[g][0];
+            N[g][2] = gp[g][1];
+            N[g][3] = gp[g][2];
+        }
+    }
+};
+
+// --------------------------------------------------------------------------------------
+// Element geometry: srcs/mesh/Element.cpp:15-135 (computeJ, computeDetJ, computeInvJ).
+// --------------------------------------------------------------------------------------
+struct Geo {
+    double J[3][3];
+    double invJ[3][3];
+    double detJ;
+};
+
+template <int D>
+static void computeGeo(const double* pos, int64_t N, const int* nodes, Geo& g) {
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) g.J[r][c] = g.invJ[r][c] = 0;
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) g.J[r][c] = pos[nodes[c + 1] + r * N] - pos[nodes[0] + r * N];
+    const auto& J = g.J;
+    if (D == 2) {
+        g.detJ = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+        g.invJ[0][0] = J[1][1] / g.detJ;
+        g.invJ[0][1] = -J[0][1] / g.detJ;
+        g.invJ[1][0] = -J[1][0] / g.detJ;
+        g.invJ[1][1] = J[0][0] / g.detJ;
+    } else {
+        g.detJ = J[0][0] * J[1][1] * J[2][2] + J[0][1] * J[1][2] * J[2][0] + J[0][2] * J[1][0] * J[2][1] -
+                 J[2][0] * J[1][1] * J[0][2] - J[2][1] * J[1][2] * J[0][0] - J[2][2] * J[1][0] * J[0][1];
+        const double dj = g.detJ;
+        g.invJ[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / dj;
+        g.invJ[0][1] = (J[2][1] * J[0][2] - J[2][2] * J[0][1]) / dj;
+        g.invJ[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / dj;
+        g.invJ[1][0] = (J[2][0] * J[1][2] - J[1][0] * J[2][2]) / dj;
+        g.invJ[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) / dj;
+        g.invJ[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) / dj;
+        g.invJ[2][0] = (J[1][0] * J[2][1] - J[2][0] * J[1][1]) / dj;
+        g.invJ[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) / dj;
+        g.invJ[2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) / dj;
+    }
+}
+
+// Element::getRin: srcs/mesh/Element.cpp:226-294.
+template <int D>
+static double getRin(const double* pos, int64_t N, const int* nodes, double detJ) {
+    auto X = [&](int k, int d) { return pos[nodes[k] + d * N]; };
+    if (D == 2) {
+        double A = detJ * Quad<2>::refSize;
+        auto dist = [&](int a, int b) {
+            double dx = X(a, 0) - X(b, 0), dy = X(a, 1) - X(b, 1);
+            return std::sqrt(dx * dx + dy * dy);
+        };
+        double a = dist(0, 1), b = dist(1, 2), c = dist(0, 2);
+        double s = (a + b + c) / 2;
+        return A / s;
+    }
+    double V = detJ * Quad<3>::refSize;
+    double x0 = X(0, 0), x1 = X(1, 0), x2 = X(2, 0), x3 = X(3, 0);
+    double y0 = X(0, 1), y1 = X(1, 1), y2 = X(2, 1), y3 = X(3, 1);
+    double z0 = X(0, 2), z1 = X(1, 2), z2 = X(2, 2), z3 = X(3, 2);
+    auto nrm = [](double a, double b, double c) { return std::sqrt(a * a + b * b + c * c); };
+    double n1 = nrm((y1 - y0) * (z2 - z0) - (z1 - z0) * (y2 - y0), (z1 - z0) * (x2 - x0) - (x1 - x0) * (z2 - z0),
+                    (x1 - x0) * (y2 - y0) - (y1 - y0) * (x2 - x0));
+    double n2 = nrm((y3 - y0) * (z2 - z0) - (z3 - z0) * (y2 - y0), (z3 - z0) * (x2 - x0) - (x3 - x0) * (z2 - z0),
+                    (x3 - x0) * (y2 - y0) - (y3 - y0) * (x2 - x0));
+    double n3 = nrm((y1 - y0) * (z3 - z0) - (z1 - z0) * (y3 - y0), (z1 - z0) * (x3 - x0) - (x1 - x0) * (z3 - z0),
+                    (x1 - x0) * (y3 - y0) - (y1 - y0) * (x3 - x0));
+    double n4 = nrm((y1 - y3) * (z2 - z3) - (z1 - z3) * (y2 - y3), (z1 - z3) * (x2 - x3) - (x1 - x3) * (z2 - z3),
+                    (x1 - x3) * (y2 - y3) - (y1 - y3) * (x2 - x3));
+    return 6 * V / (n1 + n2 + n3 + n4);
+}
+
+// --------------------------------------------------------------------------------------
+// MatrixBuilder: srcs/simulation/matricesBuilder/MatricesBuilder.inl.
+// Matrices are dense row-major arrays; ne = D+1, nv = 3 (2D) / 6 (3D) Voigt rows.
+// --------------------------------------------------------------------------------------
+template <int D> struct MB {
+    static constexpr int ne = D + 1;
+    static constexpr int nv = (D == 2) ? 3 : 6;
+    static constexpr int nd = D * ne;
+    Quad<D> q;
+
+    // getGradN: MatricesBuilder.inl:93-128 — gradN(k, 0) = -sum_r invJ(r, k); gradN(k, i+1) = invJ(i, k).
+    void gradN(const Geo& g, double G[D][ne]) const {
+        for (int k = 0; k < D; ++k) {
+            // the reference writes -a - b (- c) left to right
+            double v = -g.invJ[0][k];
+            for (int r = 1; r < D; ++r) v -= g.invJ[r][k];
+            G[k][0] = v;
+            for (int i = 0; i < D; ++i) G[k][i + 1] = g.invJ[i][k];
+        }
+    }
+    // getB: MatricesBuilder.inl:130-165.
+    void B(const double G[D][ne], double Bm[nv][nd]) const {
+        for (int r = 0; r < nv; ++r)
+            for (int c = 0; c < nd; ++c) Bm[r][c] = 0;
+        for (int j = 0; j < ne; ++j) {
+            if (D == 2) {
+                Bm[0][j] = Bm[2][j + ne] = G[0][j];
+                Bm[1][j + ne] = Bm[2][j] = G[1][j];
+            } else {
+                Bm[0][j] = Bm[3][j + ne] = Bm[4][j + 2 * ne] = G[0][j];
+                Bm[1][j + ne] = Bm[3][j] = Bm[5][j + 2 * ne] = G[1][j];
+                Bm[2][j + 2 * ne] = Bm[4][j] = Bm[5][j + ne] = G[D - 1][j];
+            }
+        }
+    }
+    // getM: MatricesBuilder.inl:217-230 (factor per Gauss point).
+    template <class F> void M(double detJ, F fac, double Mm[ne][ne]) const {
+        for (int i = 0; i < ne; ++i)
+            for (int j = 0; j < ne; ++j) Mm[i][j] = 0;
+        for (int g = 0; g < Quad<D>::ng; ++g) {
+            double f = fac(q.N[g]);
+            for (int i = 0; i < ne; ++i)
+                for (int j = 0; j < ne; ++j) Mm[i][j] += f * (q.N[g][i] * q.N[g][j]) * q.w[g];
+        }
+        const double s = detJ * Quad<D>::refSize;
+        for (int i = 0; i < ne; ++i)
+            for (int j = 0; j < ne; ++j) Mm[i][j] *= s;
+    }
+    // getK: MatricesBuilder.inl:277-289; K = detJ*refSize*fact * B^T ddev B, fact = sum_g mu w_g.
+    void K(double detJ, double mu, const double ddev[nv][nv], const double Bm[nv][nd], double Km[nd][nd]) const {
+        double fact = 0;
+        for (int g = 0; g < Quad<D>::ng; ++g) fact += mu * q.w[g];
+        const double s = detJ * Quad<D>::refSize * fact;
+        double BtD[nd][nv];
+        for (int a = 0; a < nd; ++a)
+            for (int c = 0; c < nv; ++c) {
+                double t = 0;
+                for (int r = 0; r < nv; ++r) t += Bm[r][a] * ddev[r][c];
+                BtD[a][c] = t;
+            }
+        for (int a = 0; a < nd; ++a)
+            for (int b = 0; b < nd; ++b) {
+                double t = 0;
+                for (int c = 0; c < nv; ++c) t += BtD[a][c] * Bm[c][b];
+                Km[a][b] = s * t;
+            }
+    }
+    // getD: MatricesBuilder.inl:293-305; D = detJ*refSize * sumWNT * m^T B.
+    template <class F> void Dm(double detJ, F fac, const double Bm[nv][nd], double Dout[ne][nd]) const {
+        double sumWNT[ne] = {};
+        for (int g = 0; g < Quad<D>::ng; ++g) {
+            double f = fac(q.N[g]);
+            for (int i = 0; i < ne; ++i) sumWNT[i] += f * q.N[g][i] * q.w[g];
+        }
+        const double s = detJ * Quad<D>::refSize;
+        double mB[nd];
+        for (int c = 0; c < nd; ++c) {
+            double t = 0;
+            for (int r = 0; r < D; ++r) t += Bm[r][c];  // m = (1,..,1,0,..,0)
+            mB[c] = t;
+        }
+        for (int i = 0; i < ne; ++i)
+            for (int c = 0; c < nd; ++c) Dout[i][c] = (s * sumWNT[i]) * mB[c];
+    }
+    // getC: MatricesBuilder.inl:325-337; C = detJ*refSize * gradN^T * sum_g(f w_g Ntilde_g).
+    void C(double detJ, const double G[D][ne], double Cout[ne][nd]) const {
+        double sumNW[D][nd] = {};
+        for (int g = 0; g < Quad<D>::ng; ++g)
+            for (int k = 0; k < D; ++k)
+                for (int j = 0; j < ne; ++j) sumNW[k][j + k * ne] += 1.0 * q.N[g][j] * q.w[g];
+        const double s = detJ * Quad<D>::refSize;
+        for (int i = 0; i < ne; ++i)
+            for (int c = 0; c < nd; ++c) {
+                double t = 0;
+                for (int k = 0; k < D; ++k) t += G[k][i] * sumNW[k][c];
+                Cout[i][c] = s * t;
+            }
+    }
+    // getL: MatricesBuilder.inl:309-321; L = detJ*refSize*fact * gradN^T gradN.
+    void L(double detJ, double factor, const double G[D][ne], double Lout[ne][ne]) const {
+        double fact = 0;
+        for (int g = 0; g < Quad<D>::ng; ++g) fact += factor * q.w[g];
+        const double s = detJ * Quad<D>::refSize * fact;
+        for (int i = 0; i < ne; ++i)
+            for (int j = 0; j < ne; ++j) {
+                double t = 0;
+                for (int k = 0; k < D; ++k) t += G[k][i] * G[k][j];
+                Lout[i][j] = s * t;
+            }
+    }
+    // getF: MatricesBuilder.inl:341-354; F = detJ*refSize * sum_g f_g Ntilde_g^T vec w_g.
+    template <class F> void Fv(double detJ, F fac, const double* vec, double Fout[nd]) const {
+        for (int c = 0; c < nd; ++c) Fout[c] = 0;
+        for (int g = 0; g < Quad<D>::ng; ++g) {
+            double f = fac(q.N[g]);
+            for (int k = 0; k < D; ++k)
+                for (int j = 0; j < ne; ++j) Fout[j + k * ne] += f * (q.N[g][j] * vec[k]) * q.w[g];
+        }
+        const double s = detJ * Quad<D>::refSize;
+        for (int c = 0; c < nd; ++c) Fout[c] *= s;
+    }
+    // getH: MatricesBuilder.inl:373-386; H = detJ*refSize * sum_g f_g gradN^T vec w_g.
+    void H(double detJ, const double G[D][ne], const double* vec, double Hout[ne]) const {
+        for (int i = 0; i < ne; ++i) Hout[i] = 0;
+        for (int g = 0; g < Quad<D>::ng; ++g)
+            for (int i = 0; i < ne; ++i) {
+                double t = 0;
+                for (int k = 0; k < D; ++k) t += G[k][i] * vec[k];
+                Hout[i] += 1.0 * t * q.w[g];
+            }
+        const double s = detJ * Quad<D>::refSize;
+        for (int i = 0; i < ne; ++i) Hout[i] *= s;
+    }
+};
+
+// ddev and m for the incompressible solver: MomContEquation.inl:73-95.
+template <int D> static void ddevIncomp(double dd[MB<D>::nv][MB<D>::nv]) {
+    constexpr int nv = MB<D>::nv;
+    for (int r = 0; r < nv; ++r)
+        for (int c = 0; c < nv; ++c) dd[r][c] = 0;
+    for (int r = 0; r < D; ++r) dd[r][r] = 2;
+    for (int r = D; r < nv; ++r) dd[r][r] = 1;
+}
+// ddev for the weakly compressible solver (deviatoric): WCompNewton/MomEquation.inl:63-83.
+template <int D> static void ddevWC(double dd[MB<D>::nv][MB<D>::nv]) {
+    constexpr int nv = MB<D>::nv;
+    for (int r = 0; r < nv; ++r)
+        for (int c = 0; c < nv; ++c) dd[r][c] = 0;
+    for (int r = 0; r < D; ++r)
+        for (int c = 0; c < D; ++c) dd[r][c] = (r == c) ? 4.0 / 3 : -2.0 / 3;
+    for (int r = D; r < nv; ++r) dd[r][r] = 1;
+}
+
+// --------------------------------------------------------------------------------------
+// PSPG assembly: srcs/simulation/physics/IncompNewton/MomContEquationPSPG.inl:7-146.
+// --------------------------------------------------------------------------------------
+struct Triplet {
+    int64_t row = 0, col = 0;  // default Triplet() = (0, 0, 0): PSPG.inl:17
+    double val = 0;
+};
+
+struct Params {
+    int dim;
+    int64_t N, E;
+    const int* conn;
+    const uint8_t* flags;  // FREE recomputed from connectivity (Node.inl: isFree = no element)
+    double rho, mu, g[3], dt;
+};
+
+// m_computeTauPSPG: PSPG.inl:238-259 (node states are the current Picard iterate).
+template <int D>
+static double tauPSPG(const Params& P, const Geo& geo, const int* nodes, const double* qIter) {
+    const double h = std::sqrt(Quad<D>::refSize * geo.detJ / M_PI);
+    double U = 0;
+    for (int n = 0; n < D + 1; ++n) {
+        double nodeU = 0;
+        for (int d = 0; d < D; ++d) {
+            double v = qIter[nodes[n] + d * P.N];
+            nodeU += v * v;
+        }
+        U += std::sqrt(nodeU);
+    }
+    U /= (D + 1);
+    const double dt = P.dt;
+    return 1 / std::sqrt((2 / dt) * (2 / dt) + (2 * U / h) * (2 * U / h) +
+                         9 * (4 * P.mu / (h * h * P.rho)) * (4 * P.mu / (h * h * P.rho)));
+}
+
+// Element Ae, be: PSPG.inl:26-53.
+template <int D>
+static void pspgElement(const Params& P, const MB<D>& mb, const double* pos, const int* nodes, const double* qPrev,
+                        const double* qIter, double Ae[(D + 1) * (D + 1)][(D + 1) * (D + 1)],
+                        double be[(D + 1) * (D + 1)]) {
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    Geo geo;
+    computeGeo<D>(pos, P.N, nodes, geo);
+    const double tau = tauPSPG<D>(P, geo, nodes, qIter);
+    double G[D][ne];
+    mb.gradN(geo, G);
+    double Bm[nv][nd];
+    mb.B(G, Bm);
+    double Ms[ne][ne];
+    mb.M(geo.detJ, [&](const double*) { return P.rho; }, Ms);
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j < ne; ++j) Ms[i][j] = (1 / P.dt) * Ms[i][j];
+    double dd[nv][nv];
+    ddevIncomp<D>(dd);
+    double Ke[nd][nd];
+    mb.K(geo.detJ, P.mu, dd, Bm, Ke);
+    double De[ne][nd];
+    mb.Dm(geo.detJ, [](const double*) { return 1.0; }, Bm, De);
+    double Ce[ne][nd];
+    mb.C(geo.detJ, G, Ce);
+    for (int i = 0; i < ne; ++i)
+        for (int c = 0; c < nd; ++c) Ce[i][c] = (tau / P.dt) * Ce[i][c];
+    double Le[ne][ne];
+    mb.L(geo.detJ, 1 / P.rho, G, Le);
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j < ne; ++j) Le[i][j] = tau * Le[i][j];
+    double Fe[nd];
+    mb.Fv(geo.detJ, [&](const double*) { return P.rho; }, P.g, Fe);
+    double He[ne];
+    mb.H(geo.detJ, G, P.g, He);
+    for (int i = 0; i < ne; ++i) He[i] = tau * He[i];
+
+    constexpr int na = (D + 1) * ne;
+    for (int r = 0; r < na; ++r)
+        for (int c = 0; c < na; ++c) Ae[r][c] = 0;
+    // Ae << Me_dt + Ke, -De^T, Ce_dt + De, Le   (PSPG.inl:42)
+    for (int a = 0; a < nd; ++a)
+        for (int b = 0; b < nd; ++b) {
+            double me = ((a / ne) == (b / ne)) ? Ms[a % ne][b % ne] : 0.0;
+            Ae[a][b] = me + Ke[a][b];
+        }
+    for (int a = 0; a < nd; ++a)
+        for (int j = 0; j < ne; ++j) Ae[a][nd + j] = -De[j][a];
+    for (int i = 0; i < ne; ++i)
+        for (int b = 0; b < nd; ++b) Ae[nd + i][b] = Ce[i][b] + De[i][b];
+    for (int i = 0; i < ne; ++i)
+        for (int j = 0; j < ne; ++j) Ae[nd + i][nd + j] = Le[i][j];
+    // vPrev = getElementVecState(qPrev): StatesFromToQ.hpp:137-170
+    double vPrev[nd];
+    for (int d = 0; d < D; ++d)
+        for (int i = 0; i < ne; ++i) vPrev[i + d * ne] = qPrev[nodes[i] + d * P.N];
+    // be << Fe + Me_dt*vPrev, He + Ce_dt*vPrev   (PSPG.inl:53)
+    for (int a = 0; a < nd; ++a) {
+        double t = 0;
+        for (int b = 0; b < nd; ++b) {
+            double me = ((a / ne) == (b / ne)) ? Ms[a % ne][b % ne] : 0.0;
+            t += me * vPrev[b];
+        }
+        be[a] = Fe[a] + t;
+    }
+    for (int i = 0; i < ne; ++i) {
+        double t = 0;
+        for (int b = 0; b < nd; ++b) t += Ce[i][b] * vPrev[b];
+        be[nd + i] = He[i] + t;
+    }
+}
+
+struct CsrSys {
+    int64_t n = 0;
+    std::vector<int> rowPtr, colIdx;
+    std::vector<double> vals, b;
+    double tTriplets = 0, tPush = 0, tAssemble = 0, tVector = 0, tBC = 0;
+};
+
+static std::vector<uint8_t> derivedFlags(const Params& P) {
+    std::vector<uint8_t> f(P.flags, P.flags + P.N);
+    std::vector<char> used(P.N, 0);
+    for (int64_t k = 0; k < P.E * (P.dim + 1); ++k) used[P.conn[k]] = 1;
+    for (int64_t n = 0; n < P.N; ++n) f[n] = (f[n] & ~FREE) | (used[n] ? 0 : FREE);
+    return f;
+}
+
+// m_buildAbPSPG: PSPG.inl:7-146, including the default-triplet padding (PSPG.inl:17), the
+// serial unit-diagonal push-back (PSPG.inl:108-128) and Eigen 3.3's setFromTriplets
+// (duplicates summed per inner vector in triplet order), emitted here as CSR.
+template <int D>
+static void buildAbPSPG(const Params& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
+                        const double* qIter, CsrSys& S) {
+    constexpr int ne = D + 1;
+    constexpr int na = (D + 1) * ne;
+    const int64_t N = P.N, E = P.E;
+    const int64_t tripletPerElm = (int64_t)na * na;
+    const int64_t n = (D + 1) * N;
+    MB<D> mb;
+    auto t0 = clk::now();
+    std::vector<Triplet> indexA(tripletPerElm * E);
+    std::vector<std::pair<int64_t, double>> indexb(na * E);
+#pragma omp parallel for schedule(static)
+    for (int64_t elm = 0; elm < E; ++elm) {
+        const int* nodes = P.conn + elm * ne;
+        double Ae[na][na], be[na];
+        pspgElement<D>(P, mb, pos, nodes, qPrev, qIter, Ae, be);
+        int64_t countA = 0, countb = 0;
+        for (int i = 0; i < ne; ++i) {
+            const uint8_t fi = fl[nodes[i]];
+            const bool boundOrFree = (fi & (BOUND | FREE)) != 0;
+            const bool isFree = (fi & FREE) != 0;
+            for (int j = 0; j < ne; ++j) {
+                for (int d1 = 0; d1 < D; ++d1)
+                    for (int d2 = 0; d2 <= D; ++d2) {
+                        if (!boundOrFree)
+                            indexA[tripletPerElm * elm + countA] = Triplet{nodes[i] + d1 * N, nodes[j] + d2 * N,
+                                                                           Ae[i + d1 * ne][j + d2 * ne]};
+                        countA++;
+                    }
+                for (int d2 = 0; d2 <= D; ++d2) {
+                    if (!isFree)
+                        indexA[tripletPerElm * elm + countA] =
+                            Triplet{nodes[i] + D * N, nodes[j] + d2 * N, Ae[i + D * ne][j + d2 * ne]};
+                    countA++;
+                }
+            }
+            for (int d = 0; d <= D; ++d) {
+                indexb[na * elm + countb] = {nodes[i] + d * N, be[i + d * ne]};
+                countb++;
+            }
+        }
+    }
+    auto t1 = clk::now();
+    for (int64_t nn = 0; nn < N; ++nn) {  // PSPG.inl:108-128
+        if (fl[nn] & FREE) indexA.push_back(Triplet{nn + D * N, nn + D * N, 1});
+        if (fl[nn] & (BOUND | FREE))
+            for (int d = 0; d < D; ++d) indexA.push_back(Triplet{nn + d * N, nn + d * N, 1});
+    }
+    auto t2 = clk::now();
+    // setFromTriplets (Eigen 3.3 SparseMatrix::setFromTriplets, PSPG.inl:136): count per
+    // outer vector, insert in input order, collapse duplicates by summation in insertion
+    // order, then sort.  Rows are the outer vectors here (CSR); for each (row, col) the sum
+    // order is the triplet order, as in Eigen's column-major build (whose inner vectors are
+    // also filled in triplet order before the transposed copy).
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (const auto& t : indexA) cnt[t.row + 1]++;
+    for (int64_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+    std::vector<int64_t> ecol(indexA.size());
+    std::vector<double> eval(indexA.size());
+    for (const auto& t : indexA) {
+        int64_t k = cur[t.row]++;
+        ecol[k] = t.col;
+        eval[k] = t.val;
+    }
+    std::vector<int64_t>().swap(cur);
+    S.n = n;
+    S.rowPtr.assign(n + 1, 0);
+    std::vector<int64_t> rowLen(n, 0);
+    // collapse duplicates per row (collapseDuplicates with scalar_sum_op)
+    std::vector<std::vector<std::pair<int64_t, double>>> rows(n);
+#pragma omp parallel
+    {
+        std::vector<int64_t> seen;
+#pragma omp for schedule(dynamic, 1024)
+        for (int64_t r = 0; r < n; ++r) {
+            auto& out = rows[r];
+            out.clear();
+            for (int64_t k = cnt[r]; k < cnt[r + 1]; ++k) {
+                int64_t c = ecol[k];
+                bool found = false;
+                for (auto& pr : out)
+                    if (pr.first == c) {
+                        pr.second += eval[k];
+                        found = true;
+                        break;
+                    }
+                if (!found) out.push_back({c, eval[k]});
+            }
+            std::sort(out.begin(), out.end(),
+                      [](const std::pair<int64_t, double>& a, const std::pair<int64_t, double>& b) {
+                          return a.first < b.first;
+                      });
+        }
+    }
+    for (int64_t r = 0; r < n; ++r) S.rowPtr[r + 1] = S.rowPtr[r] + (int)rows[r].size();
+    S.colIdx.resize(S.rowPtr[n]);
+    S.vals.resize(S.rowPtr[n]);
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < n; ++r) {
+        int64_t k = S.rowPtr[r];
+        for (const auto& pr : rows[r]) {
+            S.colIdx[k] = (int)pr.first;
+            S.vals[k] = pr.second;
+            ++k;
+        }
+    }
+    auto t3 = clk::now();
+    // b scatter in triplet order: PSPG.inl:139-145
+    S.b.assign(n, 0.0);
+    for (const auto& db : indexb) S.b[db.first] += db.second;
+    auto t4 = clk::now();
+    S.tTriplets += secs(t0, t1);
+    S.tPush += secs(t1, t2);
+    S.tAssemble += secs(t2, t3);
+    S.tVector += secs(t3, t4);
+}
+
+// m_applyBCPSPG: PSPG.inl:149-235.  The FST loop is skipped because gamma = 0 (checked
+// by the caller); Dirichlet values are the host-evaluated "<Tag>V"(pos, t+dt) results.
+template <int D>
+static void applyBCPSPG(const Params& P, const std::vector<uint8_t>& fl, const double* qPrev, const double* dirVals,
+                        CsrSys& S) {
+    auto t0 = clk::now();
+    const int64_t N = P.N;
+    // column index (CSC view) of the CSR: for each column the (row, k) entries in row order
+    std::vector<int64_t> ccnt(S.n + 1, 0);
+    for (size_t k = 0; k < S.colIdx.size(); ++k) ccnt[S.colIdx[k] + 1]++;
+    for (int64_t c = 0; c < S.n; ++c) ccnt[c + 1] += ccnt[c];
+    std::vector<int64_t> cpos(ccnt.begin(), ccnt.end() - 1);
+    std::vector<int64_t> cRow(S.colIdx.size()), cK(S.colIdx.size());
+    for (int64_t r = 0; r < S.n; ++r)
+        for (int64_t k = S.rowPtr[r]; k < S.rowPtr[r + 1]; ++k) {
+            int64_t c = S.colIdx[k];
+            cRow[cpos[c]] = r;
+            cK[cpos[c]] = k;
+            cpos[c]++;
+        }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        const uint8_t f = fl[nn];
+        if (f & FREE) {
+            S.b[nn + D * N] = 0;
+            if (!(f & BOUND))
+                for (int d = 0; d < D; ++d) S.b[nn + d * N] = qPrev[nn + d * N] + P.dt * P.g[d];
+        }
+        if ((f & BOUND) && (f & DIRICHLET)) {
+            for (int d = 0; d < D; ++d) {
+                const double res = dirVals[nn + d * N];
+                const int64_t col = nn + d * N;
+                S.b[col] = res;
+                for (int64_t k = ccnt[col]; k < ccnt[col + 1]; ++k) {
+                    const int64_t row = cRow[k];
+                    if (row == col) continue;
+                    const double value = S.vals[cK[k]];
+                    S.b[row] -= value * res;
+                    S.vals[cK[k]] = 0;
+                }
+            }
+        }
+    }
+    S.tBC += secs(t0, clk::now());
+}
+
+// --------------------------------------------------------------------------------------
+// Sparse linear algebra (stand-ins for Eigen SparseLU, MomContEquation.hpp:6-12).
+// --------------------------------------------------------------------------------------
+static void spmv(int64_t n, const int* rp, const int* ci, const double* v, const double* x, double* y) {
+#pragma omp parallel for schedule(static)
+    for (int64_t r = 0; r < n; ++r) {
+        double s = 0;
+        for (int64_t k = rp[r]; k < rp[r + 1]; ++k) s += v[k] * x[ci[k]];
+        y[r] = s;
+    }
+}
+static double dot(int64_t n, const double* a, const double* b) {
+    double s = 0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (int64_t i = 0; i < n; ++i) s += a[i] * b[i];
+    return s;
+}
+
+struct Precond {
+    int kind = 0;  // 0 Jacobi, 1 ILU(0)
+    int64_t n = 0;
+    const int* rp = nullptr;
+    const int* ci = nullptr;
+    std::vector<double> dinv, lu;
+    std::vector<int64_t> diag;
+    void setup(int k, int64_t nn, const int* rp_, const int* ci_, const double* v) {
+        kind = k;
+        n = nn;
+        rp = rp_;
+        ci = ci_;
+        diag.assign(n, -1);
+        for (int64_t r = 0; r < n; ++r)
+            for (int64_t q = rp[r]; q < rp[r + 1]; ++q)
+                if (ci[q] == r) diag[r] = q;
+        if (kind == 0) {
+            dinv.resize(n);
+            for (int64_t r = 0; r < n; ++r) dinv[r] = 1.0 / v[diag[r]];
+            return;
+        }
+        // ILU(0) (IKJ variant) on the CSR pattern; columns are sorted within rows.
+        lu.assign(v, v + rp[n]);
+        std::vector<int64_t> pos(n, -1);
+        for (int64_t i = 0; i < n; ++i) {
+            for (int64_t q = rp[i]; q < rp[i + 1]; ++q) pos[ci[q]] = q;
+            for (int64_t q = rp[i]; q < rp[i + 1] && ci[q] < i; ++q) {
+                const int64_t k = ci[q];
+                const double lik = lu[q] / lu[diag[k]];
+                lu[q] = lik;
+                for (int64_t qq = diag[k] + 1; qq < rp[k + 1]; ++qq) {
+                    const int64_t p = pos[ci[qq]];
+                    if (p >= 0) lu[p] -= lik * lu[qq];
+                }
+            }
+            for (int64_t q = rp[i]; q < rp[i + 1]; ++q) pos[ci[q]] = -1;
+        }
+    }
+    void apply(const double* x, double* y) const {
+        if (kind == 0) {
+#pragma omp parallel for schedule(static)
+            for (int64_t r = 0; r < n; ++r) y[r] = dinv[r] * x[r];
+            return;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            double s = x[i];
+            for (int64_t q = rp[i]; q < diag[i]; ++q) s -= lu[q] * y[ci[q]];
+            y[i] = s;
+        }
+        for (int64_t i = n - 1; i >= 0; --i) {
+            double s = y[i];
+            for (int64_t q = diag[i] + 1; q < rp[i + 1]; ++q) s -= lu[q] * y[ci[q]];
+            y[i] = s / lu[diag[i]];
+        }
+    }
+};
+
+// Right-preconditioned BiCGSTAB (van der Vorst 1992).  Stops on ||b - A x|| <= rtol ||b||,
+// verified on the true residual.
+static int bicgstab(int64_t n, const int* rp, const int* ci, const double* v, const double* b, double* x,
+                    int precKind, double rtol, int maxit, int* itersOut, double* relresOut) {
+    Precond M;
+    M.setup(precKind, n, rp, ci, v);
+    std::vector<double> r(n), rh(n), p(n, 0), vv(n, 0), ph(n), s(n), sh(n), t(n);
+    const double bn = std::sqrt(dot(n, b, b));
+    if (bn == 0) {
+        std::fill(x, x + n, 0.0);
+        *itersOut = 0;
+        *relresOut = 0;
+        return 0;
+    }
+    int it = 0;
+    for (int restart = 0; restart < 20; ++restart) {
+        spmv(n, rp, ci, v, x, r.data());
+        for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+        double rn = std::sqrt(dot(n, r.data(), r.data()));
+        if (rn <= rtol * bn) {
+            *itersOut = it;
+            *relresOut = rn / bn;
+            return 0;
+        }
+        if (it >= maxit) break;
+        rh = r;
+        double rho = 1, alpha = 1, omega = 1;
+        std::fill(p.begin(), p.end(), 0.0);
+        std::fill(vv.begin(), vv.end(), 0.0);
+        bool breakdown = false;
+        while (it < maxit) {
+            double rhoNew = dot(n, rh.data(), r.data());
+            if (rhoNew == 0 || !std::isfinite(rhoNew)) {
+                breakdown = true;
+                break;
+            }
+            const double beta = (rhoNew / rho) * (alpha / omega);
+            rho = rhoNew;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) p[i] = r[i] + beta * (p[i] - omega * vv[i]);
+            M.apply(p.data(), ph.data());
+            spmv(n, rp, ci, v, ph.data(), vv.data());
+            const double rv = dot(n, rh.data(), vv.data());
+            alpha = rho / rv;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) s[i] = r[i] - alpha * vv[i];
+            ++it;
+            const double sn = std::sqrt(dot(n, s.data(), s.data()));
+            if (sn <= rtol * bn) {
+#pragma omp parallel for schedule(static)
+                for (int64_t i = 0; i < n; ++i) x[i] += alpha * ph[i];
+                break;
+            }
+            M.apply(s.data(), sh.data());
+            spmv(n, rp, ci, v, sh.data(), t.data());
+            const double tt = dot(n, t.data(), t.data());
+            omega = (tt == 0) ? 0 : dot(n, t.data(), s.data()) / tt;
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < n; ++i) {
+                x[i] += alpha * ph[i] + omega * sh[i];
+                r[i] = s[i] - omega * t[i];
+            }
+            const double rn2 = std::sqrt(dot(n, r.data(), r.data()));
+            if (!std::isfinite(rn2)) return 2;
+            if (rn2 <= rtol * bn || omega == 0) break;
+        }
+        (void)breakdown;
+    }
+    spmv(n, rp, ci, v, x, r.data());
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - r[i];
+    const double rn = std::sqrt(dot(n, r.data(), r.data()));
+    *itersOut = it;
+    *relresOut = rn / bn;
+    if (!std::isfinite(rn)) return 2;
+    return rn <= rtol * bn ? 0 : 1;
+}
+
+// Dense LU with partial pivoting (tiny systems only).
+static int denseSolve(int64_t n, const int* rp, const int* ci, const double* v, const double* b, double* x) {
+    std::vector<double> A(n * n, 0.0);
+    for (int64_t r = 0; r < n; ++r)
+        for (int64_t k = rp[r]; k < rp[r + 1]; ++k) A[r * n + ci[k]] += v[k];
+    std::vector<double> y(b, b + n);
+    for (int64_t c = 0; c < n; ++c) {
+        int64_t piv = c;
+        for (int64_t r = c + 1; r < n; ++r)
+            if (std::fabs(A[r * n + c]) > std::fabs(A[piv * n + c])) piv = r;
+        if (A[piv * n + c] == 0) return 1;
+        if (piv != c) {
+            for (int64_t k = 0; k < n; ++k) std::swap(A[c * n + k], A[piv * n + k]);
+            std::swap(y[c], y[piv]);
+        }
+        for (int64_t r = c + 1; r < n; ++r) {
+            const double f = A[r * n + c] / A[c * n + c];
+            if (f == 0) continue;
+            for (int64_t k = c; k < n; ++k) A[r * n + k] -= f * A[c * n + k];
+            y[r] -= f * y[c];
+        }
+    }
+    for (int64_t r = n - 1; r >= 0; --r) {
+        double s = y[r];
+        for (int64_t k = r + 1; k < n; ++k) s -= A[r * n + k] * x[k];
+        x[r] = s / A[r * n + r];
+    }
+    return 0;
+}
+
+struct SolveCfg {
+    int kind;  // 0 dense LU, 1 BiCGSTAB+Jacobi, 2 BiCGSTAB+ILU(0)
+    double rtol;
+    int maxit;
+};
+
+static int linearSolve(const CsrSys& S, const SolveCfg& cfg, double* x, int* iters, double* relres) {
+    if (cfg.kind == 0) {
+        *iters = 0;
+        int st = denseSolve(S.n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), S.b.data(), x);
+        std::vector<double> r(S.n);
+        spmv(S.n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), x, r.data());
+        double rn = 0, bn = 0;
+        for (int64_t i = 0; i < S.n; ++i) {
+            rn += (S.b[i] - r[i]) * (S.b[i] - r[i]);
+            bn += S.b[i] * S.b[i];
+        }
+        *relres = bn > 0 ? std::sqrt(rn / bn) : 0;
+        return st;
+    }
+    return bicgstab(S.n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), S.b.data(), x, cfg.kind == 2 ? 1 : 0,
+                    cfg.rtol, cfg.maxit, iters, relres);
+}
+
+template <int D>
+static void assembleAndBC(const Params& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
+                          const double* qIter, const double* dirVals, CsrSys& S) {
+    buildAbPSPG<D>(P, fl, pos, qPrev, qIter, S);
+    applyBCPSPG<D>(P, fl, qPrev, dirVals, S);
+}
+
+// One PSPG time step: PicardAlgo::solve (srcs/simulation/nonLinearAlgo/PicardAlgo.cpp:31-95)
+// with the PSPG callbacks (PSPG.inl:262-372).  On failure pos and q are left unchanged
+// (restoreNodesList restores positions and states, Mesh.cpp:1019-1043).
+struct StepStats {
+    int picardIters = 0;
+    int krylovIters = 0;
+    int assemblies = 0;
+    double finalRes = 0;
+    double maxRelres = 0;
+    double tAssemble = 0, tSolve = 0, tUpdate = 0, tResidual = 0;
+    double tTriplets = 0, tPush = 0, tSetFromTriplets = 0, tVector = 0, tBC = 0;
+};
+
+template <int D>
+static int pspgStep(const Params& P, double* pos, double* q, const double* dirVals, int resKind, double minRes,
+                    int maxIter, const SolveCfg& cfg, StepStats& st) {
+    const int64_t N = P.N;
+    const int64_t n = (D + 1) * N;
+    auto fl = derivedFlags(P);
+    std::vector<double> posSave(pos, pos + D * N);
+    std::vector<double> qPrev(q, q + n);
+    std::vector<double> qIter(n, 0.0), qIterPrev(qPrev);
+    std::vector<double> states(qPrev);  // node states (u, v, [w], p) used by tau
+    CsrSys S;
+    auto restore = [&]() { std::copy(posSave.begin(), posSave.end(), pos); };
+    auto t0 = clk::now();
+    assembleAndBC<D>(P, fl, pos, qPrev.data(), states.data(), dirVals, S);  // prepare
+    st.assemblies++;
+    st.tAssemble += secs(t0, clk::now());
+    unsigned iterCount = 0;
+    double res = std::numeric_limits<double>::max();
+    std::vector<double> guess(qPrev);
+    while (res > minRes) {
+        if (iterCount > (unsigned)maxIter) {
+            restore();
+            st.picardIters = iterCount;
+            st.finalRes = res;
+            return 1;
+        }
+        // solve callback (PSPG.inl:278-313)
+        auto ts = clk::now();
+        std::vector<double> x(guess);
+        int kit = 0;
+        double rr = 0;
+        int s = linearSolve(S, cfg, x.data(), &kit, &rr);
+        st.krylovIters += kit;
+        st.maxRelres = std::max(st.maxRelres, rr);
+        st.tSolve += secs(ts, clk::now());
+        if (s == 2 || (cfg.kind == 0 && s != 0)) {
+            restore();
+            st.picardIters = iterCount;
+            return s == 2 ? 2 : 1;
+        }
+        qIter = x;
+        guess = x;
+        auto tu = clk::now();
+        states = qIter;
+        for (int64_t nn = 0; nn < N; ++nn)  // updateNodesPositionFromSave (Mesh.cpp:1238-1277)
+            if (!(P.flags[nn] & FIXED))
+                for (int d = 0; d < D; ++d) pos[nn + d * N] = posSave[nn + d * N] + qIter[nn + d * N] * P.dt;
+        st.tUpdate += secs(tu, clk::now());
+        auto ta = clk::now();
+        S = CsrSys();
+        assembleAndBC<D>(P, fl, pos, qPrev.data(), states.data(), dirVals, S);
+        st.assemblies++;
+        st.tAssemble += secs(ta, clk::now());
+        st.tTriplets += S.tTriplets;
+        st.tPush += S.tPush;
+        st.tSetFromTriplets += S.tAssemble;
+        st.tVector += S.tVector;
+        st.tBC += S.tBC;
+        // residual callback (PSPG.inl:314-372)
+        auto tr = clk::now();
+        if (resKind == 2) {
+            std::vector<double> Ax(n);
+            spmv(n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), qIter.data(), Ax.data());
+            double s2 = 0;
+            for (int64_t i = 0; i < n; ++i) s2 += (Ax[i] - S.b[i]) * (Ax[i] - S.b[i]);
+            res = std::sqrt(s2);
+        } else {
+            double num = 0, den = 0, resV, resP = 0;
+            for (int64_t nn = 0; nn < N; ++nn)
+                if (!(fl[nn] & FREE))
+                    for (int d = 0; d < D; ++d) {
+                        const double a = qIter[nn + d * N], b = qIterPrev[nn + d * N];
+                        num += (a - b) * (a - b);
+                        den += b * b;
+                    }
+            resV = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+            if (resKind == 1) {
+                num = den = 0;
+                for (int64_t nn = 0; nn < N; ++nn)
+                    if (!(fl[nn] & FREE)) {
+                        const double a = qIter[nn + D * N], b = qIterPrev[nn + D * N];
+                        num += (a - b) * (a - b);
+                        den += b * b;
+                    }
+                resP = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+            }
+            res = std::max(resV, resP);
+        }
+        st.tResidual += secs(tr, clk::now());
+        qIterPrev = qIter;
+        if (std::isnan(res)) {
+            restore();
+            st.picardIters = iterCount + 1;
+            st.finalRes = res;
+            return 2;
+        }
+        iterCount++;
+    }
+    std::copy(qIter.begin(), qIter.end(), q);
+    st.picardIters = iterCount;
+    st.finalRes = res;
+    return 0;
+}
+
+// --------------------------------------------------------------------------------------
+// Weakly compressible CDS_dpdt step: srcs/simulation/physics/WCompNewton/Solver.cpp:236-276,
+// ContEquation.inl:123-166,334-413 (Meduri / None stabilisation, Tait-Murnaghan EOS
+// :304-332), MomEquation.inl:201-374.
+// --------------------------------------------------------------------------------------
+struct WCParams {
+    int dim;
+    int64_t N, E;
+    const int* conn;
+    const uint8_t* flags;
+    double mu, g[3], K0, K0p, rhoStar;
+    int meduri;
+};
+
+template <int D>
+static void wcStep(const WCParams& W, double dt, double* pos, double* v, double* p, double* rho, double* a,
+                   const double* dirVals, double* tCont, double* tMom) {
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    const int64_t N = W.N, E = W.E;
+    Params P{};
+    P.dim = D;
+    P.N = N;
+    P.E = E;
+    P.conn = W.conn;
+    P.flags = W.flags;
+    auto fl = derivedFlags(P);
+    MB<D> mb;
+    // v1/2 = v + dt/2 a ; x += dt v1/2 (non-fixed)   Solver.cpp:249-254, Mesh.cpp:1101-1137
+    for (int64_t i = 0; i < D * N; ++i) v[i] = v[i] + 0.5 * dt * a[i];
+    for (int64_t nn = 0; nn < N; ++nn)
+        if (!(W.flags[nn] & FIXED))
+            for (int d = 0; d < D; ++d) pos[nn + d * N] += v[nn + d * N] * dt;
+    auto t0 = clk::now();
+    // ---- continuity (m_buildSystemdpdt, ContEquation.inl:353-413)
+    std::vector<double> invM(N, 0.0), F0(N, 0.0);
+    std::vector<double> MLe(E * ne), F0e(E * ne);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < E; ++e) {
+        const int* nodes = W.conn + e * ne;
+        Geo geo;
+        computeGeo<D>(pos, N, nodes, geo);
+        double Me[ne][ne];
+        mb.M(geo.detJ, [](const double*) { return 1.0; }, Me);
+        double ML[ne];
+        for (int i = 0; i < ne; ++i) {  // lump2: MatricesBuilder.hpp:87-102
+            ML[i] = 0;
+            for (int j = 0; j < ne; ++j) ML[i] += Me[i][j];
+        }
+        double Pe[ne], Ve[nd];
+        for (int i = 0; i < ne; ++i) Pe[i] = p[nodes[i]];
+        for (int d = 0; d < D; ++d)
+            for (int i = 0; i < ne; ++i) Ve[i + d * ne] = v[nodes[i] + d * N];
+        double G[D][ne], Bm[nv][nd], De[ne][nd];
+        mb.gradN(geo, G);
+        mb.B(G, Bm);
+        mb.Dm(geo.detJ,
+              [&](const double* Ng) {  // factor K0 + K0p * (N . P): ContEquation.inl:80-86
+                  double pg = 0;
+                  for (int i = 0; i < ne; ++i) pg += Ng[i] * Pe[i];
+                  return W.K0 + W.K0p * pg;
+              },
+              Bm, De);
+        for (int i = 0; i < ne; ++i) {
+            double dv = 0;
+            for (int c = 0; c < nd; ++c) dv += De[i][c] * Ve[c];
+            double mp = 0;
+            if (W.meduri)
+                for (int j = 0; j < ne; ++j) mp += Me[i][j] * Pe[j];
+            else
+                mp = ML[i] * Pe[i];
+            F0e[e * ne + i] = -dt * dv + mp;
+            MLe[e * ne + i] = ML[i];
+        }
+    }
+    for (int64_t e = 0; e < E; ++e)
+        for (int i = 0; i < ne; ++i) {
+            invM[W.conn[e * ne + i]] += MLe[e * ne + i];
+            F0[W.conn[e * ne + i]] += F0e[e * ne + i];
+        }
+    for (int64_t nn = 0; nn < N; ++nn) invM[nn] = 1 / invM[nn];
+    for (int64_t nn = 0; nn < N; ++nn)  // m_applyBCdpdt: ContEquation.inl:334-350
+        if (fl[nn] & FREE) {
+            F0[nn] = 0;
+            invM[nn] = 1;
+        }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        p[nn] = invM[nn] * F0[nn];
+        rho[nn] = std::pow((W.K0p / W.K0) * p[nn] + 1, 1 / W.K0p) * W.rhoStar;  // :319-332
+    }
+    auto t1 = clk::now();
+    // ---- momentum (m_buildSystem, MomEquation.inl:229-302)
+    std::vector<double> invMv(D * N, 0.0), F(D * N, 0.0);
+    std::vector<double> Mle(E * nd), Fte(E * nd);
+    double dd[nv][nv];
+    ddevWC<D>(dd);
+#pragma omp parallel for schedule(static)
+    for (int64_t e = 0; e < E; ++e) {
+        const int* nodes = W.conn + e * ne;
+        Geo geo;
+        computeGeo<D>(pos, N, nodes, geo);
+        double Ve[nd], Pe[ne], Re[ne];
+        for (int d = 0; d < D; ++d)
+            for (int i = 0; i < ne; ++i) Ve[i + d * ne] = v[nodes[i] + d * N];
+        for (int i = 0; i < ne; ++i) {
+            Pe[i] = p[nodes[i]];
+            Re[i] = rho[nodes[i]];
+        }
+        auto rhoG = [&](const double* Ng) {
+            double r = 0;
+            for (int i = 0; i < ne; ++i) r += Ng[i] * Re[i];
+            return r;
+        };
+        double G[D][ne], Bm[nv][nd];
+        mb.gradN(geo, G);
+        mb.B(G, Bm);
+        double Ms[ne][ne];
+        mb.M(geo.detJ, rhoG, Ms);
+        // diagBlock then lump (row sums onto the diagonal): MatricesBuilder.hpp:104-129
+        double lumped[ne];
+        for (int i = 0; i < ne; ++i) {
+            double s = Ms[i][i];
+            for (int j = 0; j < ne; ++j)
+                if (j != i) s += Ms[i][j];
+            lumped[i] = s;
+        }
+        double Ke[nd][nd], De[ne][nd], Fe[nd];
+        mb.K(geo.detJ, W.mu, dd, Bm, Ke);
+        mb.Dm(geo.detJ, [](const double*) { return 1.0; }, Bm, De);
+        mb.Fv(geo.detJ, rhoG, W.g, Fe);
+        for (int r = 0; r < nd; ++r) {
+            double kv = 0;
+            for (int c = 0; c < nd; ++c) kv += Ke[r][c] * Ve[c];
+            double dp = 0;
+            for (int j = 0; j < ne; ++j) dp += De[j][r] * Pe[j];
+            Fte[e * nd + r] = -kv + dp + Fe[r];
+            Mle[e * nd + r] = lumped[r % ne];
+        }
+    }
+    for (int64_t e = 0; e < E; ++e)
+        for (int i = 0; i < ne; ++i)
+            for (int d = 0; d < D; ++d) {
+                const int64_t gi = W.conn[e * ne + i] + d * N;
+                invMv[gi] += Mle[e * nd + i + d * ne];
+                F[gi] += Fte[e * nd + i + d * ne];
+            }
+    for (int64_t i = 0; i < D * N; ++i) invMv[i] = 1 / invMv[i];
+    for (int64_t nn = 0; nn < N; ++nn) {  // m_applyBC: MomEquation.inl:305-374 (gamma = 0)
+        const uint8_t f = fl[nn];
+        if ((f & FREE) && !(f & BOUND)) {
+            for (int d = 0; d < D; ++d) {
+                F[nn + d * N] = W.g[d];
+                invMv[nn + d * N] = 1;
+            }
+        } else if ((f & BOUND) && (f & DIRICHLET)) {
+            for (int d = 0; d < D; ++d) {
+                F[nn + d * N] = dirVals[nn + d * N];
+                invMv[nn + d * N] = 1;
+            }
+        }
+    }
+    for (int64_t i = 0; i < D * N; ++i) {  // MomEquation.inl:216-222
+        a[i] = invMv[i] * F[i];
+        v[i] = v[i] + 0.5 * dt * a[i];
+    }
+    auto t2 = clk::now();
+    if (tCont) *tCont += secs(t0, t1);
+    if (tMom) *tMom += secs(t1, t2);
+}
+
+// computeNextDT: WCompNewton/Solver.cpp:192-234 (without the heat equation).
+template <int D>
+static double wcNextDT(const WCParams& W, const double* pos, const double* v, const double* p, const double* rho,
+                       double securityCoeff, double maxDT) {
+    constexpr int ne = D + 1;
+    const int64_t N = W.N;
+    double timeStep = std::numeric_limits<double>::max();
+#pragma omp parallel for reduction(min : timeStep)
+    for (int64_t e = 0; e < W.E; ++e) {
+        const int* nodes = W.conn + e * ne;
+        Geo geo;
+        computeGeo<D>(pos, N, nodes, geo);
+        const double he = 2 * getRin<D>(pos, N, nodes, geo.detJ);
+        double maxS = 0;
+        for (int k = 0; k < ne; ++k) {
+            const int nn = nodes[k];
+            const double c2 = (W.K0 + W.K0p * p[nn]) / rho[nn];  // ContEquation.inl:117-121
+            double u2 = 0;
+            for (int d = 0; d < D; ++d) u2 += v[nn + d * N] * v[nn + d * N];  // MomEquation.inl:184-192
+            const double alpha = W.mu / rho[nn];                              // MomEquation.inl:195-198
+            maxS = std::max(std::max(u2, c2), maxS);
+            maxS = std::max(maxS, 4 * alpha * alpha / (he * he));
+        }
+        timeStep = std::min(timeStep, securityCoeff * securityCoeff * he * he / maxS);
+    }
+    return std::min(std::sqrt(timeStep), maxDT);
+}
+
+}  // namespace
+
+// ======================================================================================
+// C interface (ctypes) — used by tests/, __graft_entry__.smoke() and bench.py's CPU leg.
+// ======================================================================================
+extern "C" {
+
+struct orc_sys {
+    CsrSys S;
+};
+
+static void setThreads(int nthreads) {
+    if (nthreads > 0) omp_set_num_threads(nthreads);
+}
+
+// m_buildAbPSPG (+ m_applyBCPSPG when apply_bc != 0).
+orc_sys* orc_pspg_assemble(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, const double* pos,
+                           const double* q_prev, const double* q_iter, const double* dir_vals, double rho, double mu,
+                           const double* g, double dt, int apply_bc, int nthreads) {
+    setThreads(nthreads);
+    Params P{};
+    P.dim = dim;
+    P.N = N;
+    P.E = E;
+    P.conn = conn;
+    P.flags = flags;
+    P.rho = rho;
+    P.mu = mu;
+    P.dt = dt;
+    for (int d = 0; d < 3; ++d) P.g[d] = d < dim ? g[d] : 0.0;
+    auto fl = derivedFlags(P);
+    auto* h = new orc_sys();
+    if (dim == 2) {
+        buildAbPSPG<2>(P, fl, pos, q_prev, q_iter, h->S);
+        if (apply_bc) applyBCPSPG<2>(P, fl, q_prev, dir_vals, h->S);
+    } else {
+        buildAbPSPG<3>(P, fl, pos, q_prev, q_iter, h->S);
+        if (apply_bc) applyBCPSPG<3>(P, fl, q_prev, dir_vals, h->S);
+    }
+    return h;
+}
+int64_t orc_sys_n(const orc_sys* h) { return h->S.n; }
+int64_t orc_sys_nnz(const orc_sys* h) { return h->S.rowPtr.empty() ? 0 : h->S.rowPtr.back(); }
+void orc_sys_copy(const orc_sys* h, int* row_ptr, int* col_idx, double* vals, double* b) {
+    const auto& S = h->S;
+    if (row_ptr) std::copy(S.rowPtr.begin(), S.rowPtr.end(), row_ptr);
+    if (col_idx) std::copy(S.colIdx.begin(), S.colIdx.end(), col_idx);
+    if (vals) std::copy(S.vals.begin(), S.vals.end(), vals);
+    if (b) std::copy(S.b.begin(), S.b.end(), b);
+}
+void orc_sys_times(const orc_sys* h, double* out5) {
+    out5[0] = h->S.tTriplets;
+    out5[1] = h->S.tPush;
+    out5[2] = h->S.tAssemble;
+    out5[3] = h->S.tVector;
+    out5[4] = h->S.tBC;
+}
+void orc_sys_free(orc_sys* h) { delete h; }
+
+// Element-level access for closed-form pins: Ae, be of one element (row-major (d+1)^2 x (d+1)^2).
+void orc_pspg_element(int dim, const double* pos_e, const double* qprev_e, const double* qiter_e, double rho, double mu,
+                      const double* g, double dt, double* Ae_out, double* be_out, double* tau_out) {
+    // pos_e: dim x (dim+1) field-blocked for a standalone element with nodes 0..dim
+    Params P{};
+    P.dim = dim;
+    P.N = dim + 1;
+    P.E = 1;
+    P.rho = rho;
+    P.mu = mu;
+    P.dt = dt;
+    for (int d = 0; d < 3; ++d) P.g[d] = d < dim ? g[d] : 0.0;
+    int nodes[4] = {0, 1, 2, 3};
+    if (dim == 2) {
+        double Ae[9][9], be[9];
+        MB<2> mb;
+        pspgElement<2>(P, mb, pos_e, nodes, qprev_e, qiter_e, Ae, be);
+        std::memcpy(Ae_out, Ae, sizeof(Ae));
+        std::memcpy(be_out, be, sizeof(be));
+        Geo geo;
+        computeGeo<2>(pos_e, P.N, nodes, geo);
+        *tau_out = tauPSPG<2>(P, geo, nodes, qiter_e);
+    } else {
+        double Ae[16][16], be[16];
+        MB<3> mb;
+        pspgElement<3>(P, mb, pos_e, nodes, qprev_e, qiter_e, Ae, be);
+        std::memcpy(Ae_out, Ae, sizeof(Ae));
+        std::memcpy(be_out, be, sizeof(be));
+        Geo geo;
+        computeGeo<3>(pos_e, P.N, nodes, geo);
+        *tau_out = tauPSPG<3>(P, geo, nodes, qiter_e);
+    }
+}
+
+// Element matrices (factor 1 / given) for invariant pins: M (rho), K (mu, incompressible ddev),
+// D, C, L (factor 1), gradN, detJ.  Sizes: ne*ne, nd*nd, ne*nd, ne*nd, ne*ne, dim*ne.
+void orc_element_matrices(int dim, const double* pos_e, double rho, double mu, double* M, double* K, double* Dm,
+                          double* C, double* L, double* G, double* detJ, double* rin) {
+    int nodes[4] = {0, 1, 2, 3};
+    if (dim == 2) {
+        MB<2> mb;
+        Geo geo;
+        computeGeo<2>(pos_e, 3, nodes, geo);
+        double Gm[2][3], Bm[3][6], Mm[3][3], Km[6][6], Dd[3][6], Cc[3][6], Ll[3][3], dd[3][3];
+        mb.gradN(geo, Gm);
+        mb.B(Gm, Bm);
+        mb.M(geo.detJ, [&](const double*) { return rho; }, Mm);
+        ddevIncomp<2>(dd);
+        mb.K(geo.detJ, mu, dd, Bm, Km);
+        mb.Dm(geo.detJ, [](const double*) { return 1.0; }, Bm, Dd);
+        mb.C(geo.detJ, Gm, Cc);
+        mb.L(geo.detJ, 1.0, Gm, Ll);
+        std::memcpy(M, Mm, sizeof(Mm));
+        std::memcpy(K, Km, sizeof(Km));
+        std::memcpy(Dm, Dd, sizeof(Dd));
+        std::memcpy(C, Cc, sizeof(Cc));
+        std::memcpy(L, Ll, sizeof(Ll));
+        std::memcpy(G, Gm, sizeof(Gm));
+        *detJ = geo.detJ;
+        *rin = getRin<2>(pos_e, 3, nodes, geo.detJ);
+    } else {
+        MB<3> mb;
+        Geo geo;
+        computeGeo<3>(pos_e, 4, nodes, geo);
+        double Gm[3][4], Bm[6][12], Mm[4][4], Km[12][12], Dd[4][12], Cc[4][12], Ll[4][4], dd[6][6];
+        mb.gradN(geo, Gm);
+        mb.B(Gm, Bm);
+        mb.M(geo.detJ, [&](const double*) { return rho; }, Mm);
+        ddevIncomp<3>(dd);
+        mb.K(geo.detJ, mu, dd, Bm, Km);
+        mb.Dm(geo.detJ, [](const double*) { return 1.0; }, Bm, Dd);
+        mb.C(geo.detJ, Gm, Cc);
+        mb.L(geo.detJ, 1.0, Gm, Ll);
+        std::memcpy(M, Mm, sizeof(Mm));
+        std::memcpy(K, Km, sizeof(Km));
+        std::memcpy(Dm, Dd, sizeof(Dd));
+        std::memcpy(C, Cc, sizeof(Cc));
+        std::memcpy(L, Ll, sizeof(Ll));
+        std::memcpy(G, Gm, sizeof(Gm));
+        *detJ = geo.detJ;
+        *rin = getRin<3>(pos_e, 4, nodes, geo.detJ);
+    }
+}
+
+// Linear solve of an oracle system: kind 0 dense LU, 1 BiCGSTAB+Jacobi, 2 BiCGSTAB+ILU(0).
+int orc_sys_solve(const orc_sys* h, int kind, double rtol, int maxit, double* x, int* iters, double* relres,
+                  int nthreads) {
+    setThreads(nthreads);
+    SolveCfg cfg{kind, rtol, maxit};
+    return linearSolve(h->S, cfg, x, iters, relres);
+}
+
+// CSR solve (for timing the Krylov stand-in on arbitrary systems).
+int orc_csr_bicgstab(int64_t n, const int* rp, const int* ci, const double* v, const double* b, double* x, int prec,
+                     double rtol, int maxit, int* iters, double* relres, int nthreads) {
+    setThreads(nthreads);
+    return bicgstab(n, rp, ci, v, b, x, prec, rtol, maxit, iters, relres);
+}
+
+// Whole PSPG time step (Picard).  stats_out (16 doubles): picardIters, krylovIters,
+// assemblies, finalRes, maxRelres, tAssemble, tSolve, tUpdate, tResidual, tTriplets, tPush,
+// tSetFromTriplets, tVector, tBC.
+int orc_pspg_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, double* pos, double* q,
+                  const double* dir_vals, double rho, double mu, const double* g, double dt, int res_kind,
+                  double min_res, int max_iter, int solver_kind, double rtol, int krylov_maxit, double* stats_out,
+                  int nthreads) {
+    setThreads(nthreads);
+    Params P{};
+    P.dim = dim;
+    P.N = N;
+    P.E = E;
+    P.conn = conn;
+    P.flags = flags;
+    P.rho = rho;
+    P.mu = mu;
+    P.dt = dt;
+    for (int d = 0; d < 3; ++d) P.g[d] = d < dim ? g[d] : 0.0;
+    SolveCfg cfg{solver_kind, rtol, krylov_maxit};
+    StepStats st;
+    int s = (dim == 2) ? pspgStep<2>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st)
+                       : pspgStep<3>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st);
+    if (stats_out) {
+        double o[14] = {(double)st.picardIters, (double)st.krylovIters, (double)st.assemblies, st.finalRes,
+                        st.maxRelres,           st.tAssemble,           st.tSolve,          st.tUpdate,
+                        st.tResidual,           st.tTriplets,           st.tPush,           st.tSetFromTriplets,
+                        st.tVector,             st.tBC};
+        std::memcpy(stats_out, o, sizeof(o));
+    }
+    return s;
+}
+
+// One WC CDS_dpdt step; state arrays are updated in place.  times_out: [tCont, tMom].
+int orc_wc_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, double* pos, double* v,
+                double* p, double* rho, double* a, const double* dir_vals, double mu, const double* g, double K0,
+                double K0p, double rho_star, int meduri, double dt, double* times_out, int nthreads) {
+    setThreads(nthreads);
+    WCParams W{};
+    W.dim = dim;
+    W.N = N;
+    W.E = E;
+    W.conn = conn;
+    W.flags = flags;
+    W.mu = mu;
+    for (int d = 0; d < 3; ++d) W.g[d] = d < dim ? g[d] : 0.0;
+    W.K0 = K0;
+    W.K0p = K0p;
+    W.rhoStar = rho_star;
+    W.meduri = meduri;
+    double tc = 0, tm = 0;
+    if (dim == 2)
+        wcStep<2>(W, dt, pos, v, p, rho, a, dir_vals, &tc, &tm);
+    else
+        wcStep<3>(W, dt, pos, v, p, rho, a, dir_vals, &tc, &tm);
+    if (times_out) {
+        times_out[0] = tc;
+        times_out[1] = tm;
+    }
+    return 0;
+}
+
+double orc_wc_next_dt(int dim, int64_t N, int64_t E, const int* conn, const double* pos, const double* v,
+                      const double* p, const double* rho, double mu, double K0, double K0p, double security_coeff,
+                      double max_dt, int nthreads) {
+    setThreads(nthreads);
+    WCParams W{};
+    W.dim = dim;
+    W.N = N;
+    W.E = E;
+    W.conn = conn;
+    W.mu = mu;
+    W.K0 = K0;
+    W.K0p = K0p;
+    return dim == 2 ? wcNextDT<2>(W, pos, v, p, rho, security_coeff, max_dt)
+                    : wcNextDT<3>(W, pos, v, p, rho, security_coeff, max_dt);
+}
+
+}  // extern "C"

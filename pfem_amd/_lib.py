@@ -1,0 +1,130 @@
+"""ctypes binding of libpfem_gpu.so (include/pfem_gpu.h).
+
+The HIP library is the only implementation: if it is not built, loading fails with an
+error instead of falling back to anything else.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpfem_gpu.so")
+
+PFEM_OK = 0
+PFEM_E_INVALID = -1
+PFEM_E_OOM = -2
+PFEM_E_HIP = -3
+PFEM_E_UNSUPPORTED = -4
+PFEM_E_STATE = -5
+PFEM_NOT_CONVERGED = 1
+PFEM_NAN = 2
+
+KIND_PSPG = 0
+KIND_WC = 1
+RES = {"U": 0, "U_P": 1, "Ax_f": 2}
+PREC_FP64 = 0
+PREC_MIXED = 1
+STATE_FP64 = 0
+STATE_FP32 = 1
+
+EXPORTS = [
+    "pfem_ctx_create", "pfem_ctx_destroy", "pfem_last_error", "pfem_version", "pfem_case_create",
+    "pfem_case_destroy", "pfem_case_set_mesh", "pfem_case_set_state", "pfem_case_get_state",
+    "pfem_case_set_positions", "pfem_case_get_positions", "pfem_case_set_dirichlet", "pfem_case_sync",
+    "pfem_pspg_assemble", "pfem_pspg_solve", "pfem_get_csr_sizes", "pfem_get_csr", "pfem_get_rhs", "pfem_spmv",
+    "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
+]
+
+
+class CaseDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("kind", C.c_int32), ("rho", C.c_double), ("mu", C.c_double), ("gamma", C.c_double),
+        ("body_force", C.c_double * 3), ("K0", C.c_double), ("K0p", C.c_double), ("rho_star", C.c_double),
+        ("meduri", C.c_int32), ("wc_state", C.c_int32), ("residual", C.c_int32), ("max_iter", C.c_int32),
+        ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
+        ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("krylov_iters", C.c_int32), ("refinements", C.c_int32),
+                ("restarts", C.c_int32), ("relres", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class StepStats(C.Structure):
+    _fields_ = [("status", C.c_int32), ("picard_iters", C.c_int32), ("krylov_iters", C.c_int32),
+                ("assemblies", C.c_int32), ("residual", C.c_double), ("max_relres", C.c_double),
+                ("t_assemble", C.c_double), ("t_solve", C.c_double), ("t_update", C.c_double),
+                ("t_residual", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_lib = None
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_u8p = C.POINTER(C.c_uint8)
+_vp = C.c_void_p
+
+
+def lib():
+    """Load libpfem_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m pfem_amd.build` "
+                           "(the HIP library is required; there is no fallback)")
+    L = C.CDLL(LIB_PATH)
+    sig = {
+        "pfem_ctx_create": [C.c_int32, C.POINTER(_vp)],
+        "pfem_ctx_destroy": [_vp],
+        "pfem_last_error": [],
+        "pfem_version": [],
+        "pfem_case_create": [_vp, C.POINTER(CaseDesc), C.POINTER(_vp)],
+        "pfem_case_destroy": [_vp],
+        "pfem_case_set_mesh": [_vp, C.c_int32, C.c_int32, _ip, _u8p, _dp],
+        "pfem_case_set_state": [_vp, _dp],
+        "pfem_case_get_state": [_vp, _dp],
+        "pfem_case_set_positions": [_vp, _dp],
+        "pfem_case_get_positions": [_vp, _dp],
+        "pfem_case_set_dirichlet": [_vp, _dp],
+        "pfem_case_sync": [_vp],
+        "pfem_pspg_assemble": [_vp, C.c_double, _dp, _dp],
+        "pfem_pspg_solve": [_vp, _dp, _dp, C.POINTER(SolveStats)],
+        "pfem_get_csr_sizes": [_vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)],
+        "pfem_get_csr": [_vp, _ip, _ip, _dp],
+        "pfem_get_rhs": [_vp, _dp],
+        "pfem_spmv": [_vp, _dp, _dp],
+        "pfem_pspg_step": [_vp, C.c_double, C.POINTER(StepStats)],
+        "pfem_wc_step": [_vp, C.c_double, C.POINTER(StepStats)],
+        "pfem_wc_next_dt": [_vp, C.c_double, C.c_double, _dp],
+        "pfem_batch_step": [_vp, C.c_int32, C.POINTER(_vp), _dp, _ip, C.POINTER(StepStats)],
+        "pfem_case_footprint": [_vp, C.POINTER(C.c_int64)],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.pfem_last_error.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+class PfemError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+def check(rc: int, allow_positive: bool = False) -> int:
+    """< 0 raises (the reference throws); > 0 is a numerical failure (the reference returns false)."""
+    if rc < 0 or (rc > 0 and not allow_positive):
+        msg = lib().pfem_last_error()
+        raise PfemError(rc, msg.decode() if msg else "")
+    return rc

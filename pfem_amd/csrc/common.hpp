@@ -1,0 +1,124 @@
+// Shared device helpers for the PFEM gfx950 kernels.
+//
+// Every kernel in this library is launched over a *work table*: a device array of
+// BlockTask entries, one per workgroup.  Entry b tells workgroup b which case it works
+// on and which slice of that case it owns (block index and block count within the
+// case); work inside a case is grid-strided over that slice.  A single case is a table
+// with one case; a serving batch is a table with many.  Because the table lives in
+// device memory, the same launch (or captured hipGraph) serves any batch composition.
+//
+// Reductions are deterministic: each workgroup reduces in a fixed tree, writes one
+// partial per quantity, and the last workgroup of the case (detected with a counter)
+// sums the partials in block order and finalises the case's scalars.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace pfem {
+
+constexpr int kBlock = 256;  // threads per workgroup (4 wavefronts of 64)
+constexpr int kWaves = kBlock / 64;
+
+struct BlockTask {
+    int case_id;  // index into the launch's case array; < 0 marks an idle workgroup
+    int block;    // workgroup index within the case
+    int nblocks;  // workgroups given to the case in this launch
+    int pad;
+};
+
+// Node flag bits (PFEM_NODE_* in include/pfem_gpu.h).
+enum : uint8_t { NODE_BOUND = 1, NODE_FREE = 2, NODE_ONFS = 4, NODE_FIXED = 8, NODE_DIRICHLET = 16 };
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+__device__ __forceinline__ double wave_min(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Sum NV values over the workgroup; the result is valid in thread 0.
+template <int NV>
+__device__ __forceinline__ void block_sum(double (&v)[NV]) {
+    __shared__ double red[NV][kWaves];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+        double s = wave_sum(v[j]);
+        if (lane == 0) red[j][wave] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) {
+            double s = red[j][0];
+#pragma unroll
+            for (int w = 1; w < kWaves; ++w) s += red[j][w];
+            v[j] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Two-level deterministic reduction.  Every workgroup of a case calls this with its
+// thread-local contributions; nblocks must not exceed kMaxCaseBlocks (views.hpp), the
+// size of each case's partials buffer in NV-tuples.  Returns true in exactly one workgroup (the last to
+// arrive), in which `tot` (valid in thread 0) holds the case-wide sums taken in block
+// order.  `partials` holds nblocks*NV doubles for this case; `counter` must be zero on
+// entry and is reset to zero by the last workgroup.
+template <int NV>
+__device__ bool reduce_last_block(double (&v)[NV], double* partials, unsigned* counter, int block, int nblocks,
+                                  double (&tot)[NV]) {
+    __shared__ bool is_last;
+    block_sum<NV>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            __hip_atomic_store(&partials[block * NV + j], v[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __threadfence();
+        unsigned prev = atomicAdd(counter, 1u);
+        is_last = (prev == (unsigned)(nblocks - 1));
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    __threadfence();
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    // thread t sums blocks t, t + kBlock, ... in increasing order; then a fixed tree
+    for (int bidx = threadIdx.x; bidx < nblocks; bidx += kBlock) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j)
+            acc[j] += __hip_atomic_load(&partials[bidx * NV + j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int j = 0; j < NV; ++j) tot[j] = acc[j];
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    return true;
+}
+
+// Positive doubles order like their bit patterns: min over positive values via integer atomics.
+__device__ __forceinline__ void atomic_min_pos_double(unsigned long long* addr, double v) {
+    atomicMin(addr, (unsigned long long)__double_as_longlong(v));
+}
+
+template <typename T>
+__device__ __forceinline__ T ld(const T* p) {
+    return __builtin_nontemporal_load(p);
+}
+
+}  // namespace pfem

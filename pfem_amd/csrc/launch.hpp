@@ -1,0 +1,35 @@
+// Host-side declarations of the kernel launchers (one translation unit per kernel family).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include "common.hpp"
+#include "views.hpp"
+
+namespace pfem {
+
+void launch_pspg_assemble(int dim, const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
+void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
+void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
+
+template <typename T>
+void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_spmv(int lanes, int n, const int* rp, const int* ci, const T* va, const T* x, T* y, int nblocks,
+                 hipStream_t s);
+void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
+
+void launch_update_pos(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_res_u(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
+
+template <typename S>
+void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_tasks, const BlockTask* elem2,
+                    int n_elem2, const BlockTask* elem3, int n_elem3, hipStream_t s);
+template <typename S>
+void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, const BlockTask* elem3, int n_elem3,
+                   hipStream_t s);
+
+}  // namespace pfem

@@ -1,0 +1,51 @@
+// K0: CSR pattern of the PSPG system, built on the GPU from element connectivity.
+#pragma once
+
+#include <cstdint>
+#include "hostutil.hpp"
+
+namespace pfem {
+
+// DOF layout follows the reference's field-blocked q vectors (StatesFromToQ.hpp:21-34):
+// row/column (node i, field s) is i + s*N, s = 0..dim-1 velocity, s = dim pressure.
+//
+// Row (i, s) is "full" when the reference emits element triplets into it
+// (MomContEquationPSPG.inl:68,81): momentum rows of nodes that are neither boundary
+// nor free, continuity rows of non-free nodes.  A full row holds, for t = 0..dim and
+// every node j of i's sorted node-graph neighbourhood (i itself included), column
+// j + t*N; so its length is (dim+1)*deg(i) and column (j, t) sits at offset
+// t*deg(i) + rank_i(j).  Every other row holds only its diagonal (the unit triplets
+// pushed at MomContEquationPSPG.inl:108-128).  Row starts are
+//   s <  dim: s*mom_nnz + mom_ptr[i],   s == dim: dim*mom_nnz + cont_ptr[i],
+// where mom_ptr / cont_ptr are prefix sums of the momentum / continuity row lengths.
+struct Pattern {
+    int dim = 0, N = 0, E = 0, n = 0;
+    long long nnz = 0;
+    int mom_nnz = 0, cont_nnz = 0;
+    int graph_nnz = 0;  // node-graph entries (node pairs sharing an element, incl. self)
+    int n_dir = 0;      // Dirichlet column entries off the diagonal
+    int max_deg = 0;
+    DevBuf conn;       // int  [ne][E]  element connectivity, structure of arrays
+    DevBuf flags;      // u8   [N]      node flags with NODE_FREE derived from connectivity
+    DevBuf nbr_ptr;    // int  [N+1]
+    DevBuf nbr_idx;    // int  [graph_nnz]
+    DevBuf node_info;  // int4 [N]      {mom_ptr, cont_ptr, deg, flags}
+    DevBuf ranks;      // u8   [E][16]  rank of node b in node a's neighbourhood, a*ne+b
+    DevBuf row_ptr;    // int  [n+1]
+    DevBuf col_idx;    // int  [nnz]
+    DevBuf diag_pos;   // int  [n]      position of the diagonal entry of each row
+    DevBuf dir;        // int4 [n_dir]  {position, row, column, 0} of Dirichlet column entries
+};
+
+// Builds every array of `pat` on `stream` (synchronises the stream before returning,
+// since array sizes are data dependent).  conn_aos: device, E x (dim+1) row-major;
+// flags_in: device, N bytes (NODE_FREE is recomputed).
+void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,
+                   Pattern& pat);
+
+// Only the structure-of-arrays connectivity and the derived node flags (what the
+// weakly compressible step needs; no matrix).
+void build_mesh_basic(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,
+                      Pattern& pat);
+
+}  // namespace pfem

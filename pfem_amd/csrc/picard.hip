@@ -1,0 +1,87 @@
+// Device side of the Picard loop (PicardAlgo.cpp:31-95 with the PSPG callbacks of
+// MomContEquationPSPG.inl:262-372): position update from the saved nodes, the Ax_f
+// residual ||A q - b||_2 and the U / U_P relative-change residuals, batched over cases.
+#include "common.hpp"
+#include "views.hpp"
+
+namespace pfem {
+
+// updateNodesPositionFromSave (Mesh.cpp:1238-1277): x = x_saved + dt q_v for non-fixed nodes.
+__global__ __launch_bounds__(kBlock) void k_update_pos(const PicardView* __restrict__ views,
+                                                       const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const PicardView& V = views[task.case_id];
+    for (int i = task.block * kBlock + threadIdx.x; i < V.N; i += task.nblocks * kBlock) {
+        if (V.flags[i] & NODE_FIXED) continue;
+        for (int k = 0; k < V.dim; ++k) {
+            const long long q = (long long)k * V.N + i;
+            V.pos[q] = V.pos_save[q] + V.x[q] * V.dt;
+        }
+    }
+}
+
+template <int L>
+__global__ __launch_bounds__(kBlock) void k_res_axf(const PicardView* __restrict__ views,
+                                                    const BlockTask* __restrict__ tasks) {
+    constexpr int RPB = kBlock / L;
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const PicardView& V = views[task.case_id];
+    const int lane = threadIdx.x % L;
+    double part[1] = {0.0};
+    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
+        const int beg = V.row_ptr[row], end = V.row_ptr[row + 1];
+        double acc = 0;
+        for (int k = beg + lane; k < end; k += L) acc += V.vals[k] * V.x[V.col_idx[k]];
+#pragma unroll
+        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
+        if (lane == 0) {
+            const double r = acc - V.b[row];
+            part[0] += r * r;
+        }
+    }
+    double tot[1];
+    if (reduce_last_block<1>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0)
+        V.out[0] = tot[0];
+}
+
+// sum over non-free nodes of |du|^2, |u_prev|^2 (velocity) and dp^2, p_prev^2.
+__global__ __launch_bounds__(kBlock) void k_res_u(const PicardView* __restrict__ views,
+                                                  const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const PicardView& V = views[task.case_id];
+    double part[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = task.block * kBlock + threadIdx.x; i < V.N; i += task.nblocks * kBlock) {
+        if (V.flags[i] & NODE_FREE) continue;
+        for (int k = 0; k <= V.dim; ++k) {
+            const long long q = (long long)k * V.N + i;
+            const double a = V.x[q], b = V.x_prev[q];
+            const int slot = (k < V.dim) ? 0 : 2;
+            part[slot] += (a - b) * (a - b);
+            part[slot + 1] += b * b;
+        }
+    }
+    double tot[4];
+    if (reduce_last_block<4>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0)
+        for (int j = 0; j < 4; ++j) V.out[j] = tot[j];
+}
+
+void launch_update_pos(const PicardView* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_update_pos<<<nt, kBlock, 0, s>>>(v, t);
+}
+
+void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, hipStream_t s) {
+    switch (lanes) {
+        case 4: k_res_axf<4><<<nt, kBlock, 0, s>>>(v, t); break;
+        case 8: k_res_axf<8><<<nt, kBlock, 0, s>>>(v, t); break;
+        case 16: k_res_axf<16><<<nt, kBlock, 0, s>>>(v, t); break;
+        case 32: k_res_axf<32><<<nt, kBlock, 0, s>>>(v, t); break;
+        default: k_res_axf<64><<<nt, kBlock, 0, s>>>(v, t); break;
+    }
+}
+
+void launch_res_u(const PicardView* v, const BlockTask* t, int nt, hipStream_t s) { k_res_u<<<nt, kBlock, 0, s>>>(v, t); }
+
+}  // namespace pfem

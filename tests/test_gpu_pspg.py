@@ -1,0 +1,192 @@
+"""GPU parity of the PSPG path (K0 pattern, K1 assembly, K2 BC, K3/K5 solve, Picard step)
+against the CPU oracle, through the C-ABI.
+
+Bars: CSR pattern (row_ptr, col_idx) bit-exact; assembled values within 1e-12 of the
+oracle relative to each row's largest magnitude (the GPU sums element contributions with
+FP64 atomics, so the order of the duplicate sums differs from Eigen's triplet order);
+b within 1e-12 relative to its field block's largest magnitude; solutions to the stated
+relative residual on the oracle's own system.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import pfem_amd
+from oracle import oracle as O
+from pfem_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def small_meshes():
+    return {
+        "2d_dambreak": mg.dambreak_2d(),  # has free wall nodes
+        "2d_column": mg.column_2d_nodes(400, h=0.02, seed=3),
+        "3d_column": mg.dambreak_3d(h=0.05, col=(0.5, 0.5, 1.0), seed=1),
+    }
+
+
+def rand_state(mesh, seed):
+    rng = np.random.default_rng(seed)
+    q = mg.pspg_hydrostatic(mesh)
+    d = mesh.dim
+    N = mesh.n_nodes
+    q[:d * N] = rng.normal(scale=0.2, size=d * N)
+    return q
+
+
+def row_rel_err(rp, ref, got):
+    worst = 0.0
+    for r in range(len(rp) - 1):
+        a, b = rp[r], rp[r + 1]
+        scale = np.abs(ref[a:b]).max()
+        if scale == 0:
+            assert np.all(got[a:b] == 0)
+            continue
+        worst = max(worst, np.abs(got[a:b] - ref[a:b]).max() / scale)
+    return worst
+
+
+def block_rel_err(d, N, ref, got):
+    worst = 0.0
+    for s in range(d + 1):
+        r = ref[s * N:(s + 1) * N]
+        g = got[s * N:(s + 1) * N]
+        scale = np.abs(r).max()
+        worst = max(worst, np.abs(g - r).max() / scale if scale > 0 else np.abs(g).max())
+    return worst
+
+
+def make_case(ctx, mesh, **kw):
+    c = pfem_amd.Case(ctx, mesh.dim, kind="PSPG", **kw)
+    c.set_mesh(mesh)
+    return c
+
+
+@pytest.mark.parametrize("name", ["2d_dambreak", "2d_column", "3d_column"])
+@pytest.mark.parametrize("with_bc", [False, True])
+def test_assembly_parity(ctx, name, with_bc):
+    mesh = small_meshes()[name]
+    say(f"[assembly] {name} bc={with_bc} N={mesh.n_nodes} E={mesh.n_elems}")
+    d, N = mesh.dim, mesh.n_nodes
+    qprev = rand_state(mesh, 1)
+    qiter = rand_state(mesh, 2)
+    rng = np.random.default_rng(5)
+    dv = rng.normal(scale=0.5, size=d * N) if with_bc else np.zeros(d * N)
+    dt = 1e-3 if d == 2 else 5e-4
+    ref = O.pspg_assemble(mesh, qprev, qiter, dir_vals=dv, dt=dt, apply_bc=True)
+    c = make_case(ctx, mesh)
+    c.set_dirichlet(dv)
+    c.assemble(dt, q_prev=qprev, q_iter=qiter)
+    rp, ci, va = c.csr()
+    np.testing.assert_array_equal(rp, ref.row_ptr)
+    np.testing.assert_array_equal(ci, ref.col_idx)
+    assert row_rel_err(rp, ref.vals, va) <= 1e-12
+    assert block_rel_err(d, N, ref.b, c.rhs()) <= 1e-12
+    c.close()
+
+
+def test_pattern_is_reproducible(ctx):
+    mesh = small_meshes()["3d_column"]
+    c1 = make_case(ctx, mesh)
+    c2 = make_case(ctx, mesh)
+    for c in (c1, c2):
+        c.assemble(5e-4, q_prev=rand_state(mesh, 1))
+    a, b = c1.csr(), c2.csr()
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[1], b[1])
+
+
+@pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
+def test_spmv_matches_scipy(ctx, name):
+    mesh = small_meshes()[name]
+    say(f"[spmv] {name}")
+    c = make_case(ctx, mesh)
+    q = rand_state(mesh, 3)
+    c.assemble(1e-3, q_prev=q)
+    rp, ci, va = c.csr()
+    import scipy.sparse as sp
+
+    A = sp.csr_matrix((va, ci, rp))
+    x = np.random.default_rng(9).normal(size=A.shape[0])
+    y = c.spmv(x)
+    ref = A @ x
+    assert np.abs(y - ref).max() <= 1e-12 * np.abs(A).sum(axis=1).max() * np.abs(x).max()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
+def test_solve_relres_on_oracle_system(ctx, name, precision):
+    mesh = small_meshes()[name]
+    say(f"[solve] {name} {precision}")
+    d, N = mesh.dim, mesh.n_nodes
+    q = rand_state(mesh, 4)
+    dt = 1e-3 if d == 2 else 5e-4
+    ref = O.pspg_assemble(mesh, q, q, dt=dt)
+    c = make_case(ctx, mesh, precision=precision, krylov_rtol=1e-10)
+    c.assemble(dt, q_prev=q, q_iter=q)
+    rc, x, st = c.solve(x0=np.zeros_like(q))
+    assert rc == 0, st
+    A = ref.to_scipy()
+    relres = np.linalg.norm(ref.b - A @ x) / np.linalg.norm(ref.b)
+    assert relres <= 1e-9, (relres, st)
+    import scipy.sparse.linalg as spla
+
+    xs = spla.spsolve(A.tocsc(), ref.b)
+    assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
+def test_picard_step_parity(ctx, name):
+    mesh = small_meshes()[name]
+    say(f"[picard] {name}")
+    d, N = mesh.dim, mesh.n_nodes
+    dt = 1e-3 if d == 2 else 5e-4
+    q0 = mg.pspg_hydrostatic(mesh)
+    # oracle: Picard with a tight Krylov stand-in
+    pos_o = mesh.pos.reshape(-1).copy()
+    q_o = q0.copy()
+    c = make_case(ctx, mesh, krylov_rtol=1e-11)
+    c.set_state(q0)
+    for step in range(2):
+        st_o, info = O.pspg_step(mesh, pos_o, q_o, dt=dt, solver_kind=2, rtol=1e-13)
+        rc, st = c.pspg_step(dt)
+        say(f"  step {step}: oracle iters={info['picard_iters']} gpu iters={st['picard_iters']} "
+            f"krylov={st['krylov_iters']} res={st['residual']:.3e}")
+        assert st_o == 0 and rc == 0
+        assert abs(st["picard_iters"] - info["picard_iters"]) <= 1
+        q_g = c.get_state()
+        pos_g = c.get_positions().reshape(-1)
+        assert block_rel_err(d, N, q_o, q_g) <= 1e-6
+        np.testing.assert_allclose(pos_g, pos_o, rtol=0, atol=1e-9 * np.abs(pos_o).max())
+
+
+def test_picard_failure_restores(ctx):
+    mesh = small_meshes()["2d_dambreak"]
+    q0 = mg.pspg_hydrostatic(mesh)
+    c = make_case(ctx, mesh, min_res=0.0, max_iter=0)
+    c.set_state(q0)
+    pos0 = c.get_positions().copy()
+    rc, st = c.pspg_step(1e-3)
+    assert rc == 1  # PFEM_NOT_CONVERGED: iterCount > maxIter
+    np.testing.assert_array_equal(c.get_positions(), pos0)
+    np.testing.assert_array_equal(c.get_state(), q0)
+
+
+def test_invalid_arguments_raise(ctx):
+    with pytest.raises(pfem_amd.PfemError):
+        pfem_amd.Case(ctx, 4)
+    with pytest.raises(pfem_amd.PfemError):
+        pfem_amd.Case(ctx, 2, gamma=0.07)  # surface tension is not implemented
+    mesh = mg.dambreak_2d()
+    bad = mg.PfemMesh(2, mesh.pos, mesh.conn.copy(), mesh.flags, mesh.h)
+    bad.conn[0, 0] = mesh.n_nodes + 5
+    c = pfem_amd.Case(ctx, 2)
+    with pytest.raises(pfem_amd.PfemError):
+        c.set_mesh(bad)
