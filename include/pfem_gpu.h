@@ -86,6 +86,8 @@ typedef struct pfem_case_desc {
     int32_t krylov_max_iter;
     double krylov_rtol; /* stop when ||b - A x|| <= rtol ||b|| (true residual, FP64) */
     double inner_rtol;  /* mixed precision: relative tolerance of each FP32 inner solve */
+    int32_t inner_max_iter; /* mixed precision: iteration cap of each FP32 inner solve */
+    int32_t pad0;
 } pfem_case_desc;
 
 typedef struct pfem_solve_stats {
@@ -171,6 +173,13 @@ int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* 
  * receives each case's step status; stats may be NULL. */
 int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, const double* dts, int32_t* status,
                     pfem_step_stats* stats);
+
+/* Instrumentation (no counterpart in the reference): times, with HIP events on the case's
+ * stream, the first SpMV launch of every BiCGSTAB iteration of single-case solves.
+ * out4 (nullable) receives the statistics gathered since the last call:
+ * {avg ms FP64, launches FP64, avg ms FP32, launches FP32}; enable = 1 then resets and
+ * (re)starts timing, enable = 0 stops it. */
+int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4);
 
 /* Device footprint of a case in bytes (admission control by HBM footprint). */
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);

@@ -63,7 +63,7 @@ class Case:
     def __init__(self, ctx: Context, dim: int, kind: str = "PSPG", rho=1000.0, mu=1e-3, gamma=0.0, body_force=None,
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
-                 inner_rtol=1e-4):
+                 inner_rtol=1e-3, inner_max_iter=300):
         d = _lib.CaseDesc()
         d.dim = dim
         d.kind = _lib.KIND_PSPG if kind == "PSPG" else _lib.KIND_WC
@@ -82,6 +82,7 @@ class Case:
         d.max_iter, d.min_res = max_iter, min_res
         d.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
         d.krylov_max_iter, d.krylov_rtol, d.inner_rtol = krylov_max_iter, krylov_rtol, inner_rtol
+        d.inner_max_iter = inner_max_iter
         self.ctx = ctx
         self.dim = dim
         self.kind = kind
@@ -194,6 +195,13 @@ class Case:
         st = _lib.StepStats()
         check(lib().pfem_wc_step(self._h, dt, C.byref(st)))
         return st.as_dict()
+
+    def profile_spmv(self, enable: bool):
+        """Start (True) or stop (False) live event timing of the BiCGSTAB SpMV kernel; returns
+        {avg_ms_fp64, launches_fp64, avg_ms_fp32, launches_fp32} gathered since the last call."""
+        out = np.zeros(4)
+        check(lib().pfem_profile_spmv(self._h, 1 if enable else 0, _dp(out)))
+        return dict(avg_ms_fp64=out[0], launches_fp64=int(out[1]), avg_ms_fp32=out[2], launches_fp32=int(out[3]))
 
     def wc_next_dt(self, security_coeff, max_dt):
         out = C.c_double(0)

@@ -34,6 +34,7 @@ EXPORTS = [
     "pfem_case_set_positions", "pfem_case_get_positions", "pfem_case_set_dirichlet", "pfem_case_sync",
     "pfem_pspg_assemble", "pfem_pspg_solve", "pfem_get_csr_sizes", "pfem_get_csr", "pfem_get_rhs", "pfem_spmv",
     "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
+    "pfem_profile_spmv",
 ]
 
 
@@ -43,7 +44,7 @@ class CaseDesc(C.Structure):
         ("body_force", C.c_double * 3), ("K0", C.c_double), ("K0p", C.c_double), ("rho_star", C.c_double),
         ("meduri", C.c_int32), ("wc_state", C.c_int32), ("residual", C.c_int32), ("max_iter", C.c_int32),
         ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
-        ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double),
+        ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("pad0", C.c_int32),
     ]
 
 
@@ -106,6 +107,7 @@ def lib():
         "pfem_wc_next_dt": [_vp, C.c_double, C.c_double, _dp],
         "pfem_batch_step": [_vp, C.c_int32, C.POINTER(_vp), _dp, _ip, C.POINTER(StepStats)],
         "pfem_case_footprint": [_vp, C.POINTER(C.c_int64)],
+        "pfem_profile_spmv": [_vp, C.c_int32, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -109,12 +109,25 @@ struct Runner {
     int nt_elem[2] = {0, 0}, nt_node = 0, nt_row = 0;
     int lanes = 8;
     hipEvent_t ev[6] = {};
+    // live timing of the first SpMV kernel of each BiCGSTAB iteration (single-case runs)
+    bool prof = false;
+    hipEvent_t pev[128] = {};
+    double prof_ms[2] = {0, 0};
+    long long prof_count[2] = {0, 0};
     explicit Runner(hipStream_t st) : s(st) {
         for (auto& e : ev) PFEM_HIP(hipEventCreate(&e));
     }
     ~Runner() {
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& e : pev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    void enable_profiling(bool on) {
+        if (on && !pev[0])
+            for (auto& e : pev) PFEM_HIP(hipEventCreate(&e));
+        prof = on;
+        if (on) prof_ms[0] = prof_ms[1] = 0, prof_count[0] = prof_count[1] = 0;
     }
 
     template <class V>
@@ -191,6 +204,12 @@ static void assemble_active(Runner& R) {
     PFEM_HIP(hipGetLastError());
 }
 
+static float ev_ms(hipEvent_t a, hipEvent_t b) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0;
+    return ms;
+}
+
 struct KryResult {
     int status = PFEM_OK;
     int iters = 0;
@@ -212,7 +231,7 @@ static void krylov_run(Runner& R, const std::vector<pfem_case*>& cs, bool verify
         KScal& k = c->h_scal[which];
         std::memset(&k, 0, sizeof(KScal));
         k.rtol = which == 0 ? c->d.krylov_rtol : c->d.inner_rtol;
-        k.maxit = c->d.krylov_max_iter;
+        k.maxit = which == 0 ? c->d.krylov_max_iter : std::max(1, c->d.inner_max_iter);
         k.req_init = 1;
         PFEM_HIP(hipMemcpyAsync(c->scal.as<KScal>() + which, &k, sizeof(KScal), hipMemcpyHostToDevice, R.s));
     }
@@ -221,11 +240,22 @@ static void krylov_run(Runner& R, const std::vector<pfem_case*>& cs, bool verify
     launch_kry_init<T>(R.lanes, v, t, R.nt_row, R.s);
     std::vector<char> fin(nc, 0), verifying(nc, 0);
     int chunk = 4;
+    const bool prof = R.prof && nc == 1;
+    int prof_chunk = 0, prof_before = 0;
     for (;;) {
         for (pfem_case* c : cs)
             PFEM_HIP(hipMemcpyAsync(&c->h_scal[which], c->scal.as<KScal>() + which, sizeof(KScal),
                                     hipMemcpyDeviceToHost, R.s));
         PFEM_HIP(hipStreamSynchronize(R.s));
+        if (prof_chunk > 0) {
+            // only iterations that did work (the case may converge inside the chunk)
+            const int worked = std::min(prof_chunk, cs[0]->h_scal[which].iters - prof_before);
+            for (int it = 0; it < worked; ++it) {
+                R.prof_ms[which] += ev_ms(R.pev[2 * it], R.pev[2 * it + 1]);
+                R.prof_count[which]++;
+            }
+            prof_chunk = 0;
+        }
         bool need_init = false, all_final = true;
         for (size_t i = 0; i < nc; ++i) {
             if (fin[i]) continue;
@@ -272,7 +302,11 @@ static void krylov_run(Runner& R, const std::vector<pfem_case*>& cs, bool verify
             launch_kry_init<T>(R.lanes, v, t, R.nt_row, R.s);
             continue;
         }
-        launch_kry_iters<T>(R.lanes, chunk, v, t, R.nt_row, R.s);
+        if (prof) {
+            prof_chunk = chunk;
+            prof_before = cs[0]->h_scal[which].iters;
+        }
+        launch_kry_iters<T>(R.lanes, chunk, v, t, R.nt_row, R.s, prof ? R.pev : nullptr);
         PFEM_HIP(hipGetLastError());
         chunk = std::min(chunk * 2, 64);
     }
@@ -306,10 +340,17 @@ static void solve_active(Runner& R, const std::vector<pfem_case*>& cs, std::vect
         }
     }
     if (!gmx.empty()) {
+        // FP64 iterative refinement around capped FP32 inner BiCGSTAB solves.  A case whose
+        // true FP64 residual stops decreasing (the FP32 recurrence can lose accuracy on
+        // badly conditioned systems) finishes with FP64 BiCGSTAB from its current iterate.
         std::vector<pfem_case*> act = gmx;
         std::vector<size_t> idx(gmx.size());
         for (size_t j = 0; j < gmx.size(); ++j) idx[j] = imx[j];
-        const int max_ref = 30;
+        std::vector<double> best(nc, std::numeric_limits<double>::infinity());
+        std::vector<int> stall(nc, 0);
+        std::vector<pfem_case*> fallback;
+        std::vector<size_t> fidx;
+        const int max_ref = 40;
         for (int ref = 0; !act.empty(); ++ref) {
             if (R.act != act) R.set_active(act);
             launch_ir_residual(R.lanes, R.v_ref.as<RefineView>(), R.t_row.as<BlockTask>(), R.nt_row, R.s);
@@ -320,17 +361,26 @@ static void solve_active(Runner& R, const std::vector<pfem_case*>& cs, std::vect
             std::vector<size_t> nidx;
             for (size_t j = 0; j < act.size(); ++j) {
                 pfem_case* c = act[j];
-                auto& s = st[idx[j]];
+                const size_t i = idx[j];
+                auto& s = st[i];
                 const double rr = c->h_out[0], bb = c->h_out[1];
                 const double relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
                 s.relres = relres;
                 s.refinements = ref;
-                if (!std::isfinite(relres)) s.status = PFEM_NAN;
-                else if (relres <= c->d.krylov_rtol) s.status = PFEM_OK;
-                else if (ref >= max_ref || s.krylov_iters >= c->d.krylov_max_iter) s.status = PFEM_NOT_CONVERGED;
-                else {
+                if (relres <= c->d.krylov_rtol) {
+                    s.status = PFEM_OK;
+                    continue;
+                }
+                if (std::isfinite(relres) && relres < 0.5 * best[i]) stall[i] = 0;
+                else stall[i]++;
+                if (std::isfinite(relres)) best[i] = std::min(best[i], relres);
+                if (stall[i] >= 3 || !std::isfinite(relres) || ref >= max_ref ||
+                    s.krylov_iters >= c->d.krylov_max_iter) {
+                    fallback.push_back(c);
+                    fidx.push_back(i);
+                } else {
                     next.push_back(c);
-                    nidx.push_back(idx[j]);
+                    nidx.push_back(i);
                 }
             }
             if (next.empty()) break;
@@ -346,13 +396,22 @@ static void solve_active(Runner& R, const std::vector<pfem_case*>& cs, std::vect
             act = next;
             idx = nidx;
         }
+        if (!fallback.empty()) {
+            // a non-finite iterate restarts from zero
+            for (size_t j = 0; j < fallback.size(); ++j)
+                if (!std::isfinite(st[fidx[j]].relres))
+                    PFEM_HIP(hipMemsetAsync(fallback[j]->x.p, 0, sizeof(double) * (size_t)fallback[j]->n, R.s));
+            std::vector<KryResult> kr;
+            krylov_run<double>(R, fallback, true, kr);
+            for (size_t j = 0; j < fallback.size(); ++j) {
+                auto& s = st[fidx[j]];
+                s.status = kr[j].status;
+                s.krylov_iters += kr[j].iters;
+                s.restarts += kr[j].restarts + 1;
+                s.relres = kr[j].bb > 0 ? std::sqrt(kr[j].rr / kr[j].bb) : 0.0;
+            }
+        }
     }
-}
-
-static float ev_ms(hipEvent_t a, hipEvent_t b) {
-    float ms = 0;
-    if (hipEventElapsedTime(&ms, a, b) != hipSuccess) ms = 0;
-    return ms;
 }
 
 // One Picard time step for each case of `cs` (PicardAlgo.cpp:31-95).
@@ -734,7 +793,8 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
             PFEM_REQUIRE(d.precision == PFEM_PREC_FP64 || d.precision == PFEM_PREC_MIXED, PFEM_E_INVALID,
                          "unknown precision");
             PFEM_REQUIRE(d.krylov_rtol > 0 && d.krylov_max_iter > 0, PFEM_E_INVALID, "invalid Krylov settings");
-            PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED || d.inner_rtol > 0, PFEM_E_INVALID, "inner_rtol must be > 0");
+            PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED || (d.inner_rtol > 0 && d.inner_max_iter > 0), PFEM_E_INVALID,
+                         "inner_rtol and inner_max_iter must be > 0");
         } else {
             PFEM_REQUIRE(d.K0 > 0 && d.K0p > 0 && d.rho_star > 0, PFEM_E_INVALID, "invalid Tait parameters");
             PFEM_REQUIRE(d.wc_state == PFEM_STATE_FP64 || d.wc_state == PFEM_STATE_FP32, PFEM_E_INVALID,
@@ -1077,6 +1137,21 @@ int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* 
         const double dt = std::min(std::sqrt(dt2), max_dt);
         PFEM_REQUIRE(!std::isnan(dt), PFEM_NAN, "NaN time step");
         *dt_out = dt;
+        return PFEM_OK;
+    });
+}
+
+int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4) {
+    return guarded([&] {
+        require_case(c);
+        pfem::Runner& R = case_runner(c);
+        if (out4) {
+            for (int k = 0; k < 2; ++k) {
+                out4[2 * k] = R.prof_count[k] ? R.prof_ms[k] / (double)R.prof_count[k] : 0.0;
+                out4[2 * k + 1] = (double)R.prof_count[k];
+            }
+        }
+        R.enable_profiling(enable != 0);
         return PFEM_OK;
     });
 }

@@ -262,10 +262,14 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const RefineView* __restri
 }
 
 // Launchers ---------------------------------------------------------------------------
+// ev (optional): a pair of events recorded around the first SpMV of the iteration, for
+// live timing of that kernel.
 template <typename T, int L>
-static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s, hipEvent_t* ev) {
     k_kry_p<T><<<nt, kBlock, 0, s>>>(v, t);
+    if (ev) (void)hipEventRecord(ev[0], s);
     k_kry_spmv_v<T, L><<<nt, kBlock, 0, s>>>(v, t);
+    if (ev) (void)hipEventRecord(ev[1], s);
     k_kry_s<T><<<nt, kBlock, 0, s>>>(v, t);
     k_kry_spmv_t<T, L><<<nt, kBlock, 0, s>>>(v, t);
     k_kry_x<T><<<nt, kBlock, 0, s>>>(v, t);
@@ -286,8 +290,10 @@ void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int 
 }
 
 template <typename T>
-void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
-    for (int it = 0; it < iters; ++it) PFEM_DISPATCH_L(lanes, (iter_launch<T, LL>(v, t, nt, s)));
+void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s,
+                      hipEvent_t* ev) {
+    for (int it = 0; it < iters; ++it)
+        PFEM_DISPATCH_L(lanes, (iter_launch<T, LL>(v, t, nt, s, ev ? ev + 2 * it : nullptr)));
 }
 
 template <typename T>
@@ -306,8 +312,10 @@ void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream
 
 template void launch_kry_init<double>(int, const KrylovView<double>*, const BlockTask*, int, hipStream_t);
 template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockTask*, int, hipStream_t);
-template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, hipStream_t);
-template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, hipStream_t);
+template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, hipStream_t,
+                                       hipEvent_t*);
+template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, hipStream_t,
+                                      hipEvent_t*);
 template void launch_spmv<double>(int, int, const int*, const int*, const double*, const double*, double*, int,
                                   hipStream_t);
 template void launch_spmv<float>(int, int, const int*, const int*, const float*, const float*, float*, int,

@@ -14,7 +14,8 @@ void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntas
 template <typename T>
 void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s);
 template <typename T>
-void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s,
+                      hipEvent_t* ev = nullptr);
 template <typename T>
 void launch_spmv(int lanes, int n, const int* rp, const int* ci, const T* va, const T* x, T* y, int nblocks,
                  hipStream_t s);

@@ -57,4 +57,6 @@ def test_wc_step_parity(ctx, name, state):
         assert rel(po, pg) <= tol
         assert rel(ro, rg) <= tol
         assert np.abs(ag - ao).max() <= tol * max(np.abs(ao).max(), 9.81)
-        np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0, atol=1e-12)
+        # positions are FP64 in both states; with FP32 velocities they drift by dt * |dv|
+        np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0,
+                                   atol=1e-12 if state == "fp64" else 1e-9)
