@@ -181,6 +181,14 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
  * (re)starts timing, enable = 0 stops it. */
 int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4);
 
+/* Instrumentation (no counterpart in the reference): back-to-back timings, in
+ * microseconds per launch, of kernels on this case's assembled system.  out8: {node-block
+ * SpMV FP64, node-block SpMV FP32 (0 unless mixed precision), streaming read of the nnz
+ * FP64 values, copy kernel, bytes moved by the copy, nnz, node-graph nnz, lanes per node}. */
+int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8);
+/* Instrumentation: XCD (XCC) id on which each of nblocks single-wavefront workgroups ran. */
+int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
+
 /* Device footprint of a case in bytes (admission control by HBM footprint). */
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);
 

@@ -34,7 +34,7 @@ EXPORTS = [
     "pfem_case_set_positions", "pfem_case_get_positions", "pfem_case_set_dirichlet", "pfem_case_sync",
     "pfem_pspg_assemble", "pfem_pspg_solve", "pfem_get_csr_sizes", "pfem_get_csr", "pfem_get_rhs", "pfem_spmv",
     "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
-    "pfem_profile_spmv",
+    "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd",
 ]
 
 
@@ -108,6 +108,8 @@ def lib():
         "pfem_batch_step": [_vp, C.c_int32, C.POINTER(_vp), _dp, _ip, C.POINTER(StepStats)],
         "pfem_case_footprint": [_vp, C.POINTER(C.c_int64)],
         "pfem_profile_spmv": [_vp, C.c_int32, _dp],
+        "pfem_bench_kernels": [_vp, C.c_int32, _dp],
+        "pfem_probe_xcd": [_vp, C.c_int32, _ip],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

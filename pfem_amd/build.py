@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfem_gpu.so")
-SOURCES = ["pattern.hip", "pspg.hip", "krylov.hip", "picard.hip", "wc.hip", "capi.cpp"]
+SOURCES = ["pattern.hip", "pspg.hip", "krylov.hip", "picard.hip", "wc.hip", "microbench.hip", "capi.cpp"]
 ARCH = os.environ.get("PFEM_OFFLOAD_ARCH", "gfx950")
 
 CXXFLAGS = [

@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -49,14 +50,44 @@ int guarded(F&& f) {
 }
 
 constexpr int kElemCap = 8192;  // workgroups per case for element / node passes
-constexpr int kRowCap = pfem::kMaxCaseBlocks;
+// workgroups per case in row launches (<= kMaxCaseBlocks, the partials capacity);
+// PFEM_ROW_BLOCKS overrides it (tuning)
+const int kRowCap = [] {
+    const char* e = std::getenv("PFEM_ROW_BLOCKS");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 64 && v <= pfem::kMaxCaseBlocks) ? v : 1024;
+}();
 
 int grid_for_rows(int n) { return std::max(1, std::min((n + pfem::kBlock - 1) / pfem::kBlock, kRowCap)); }
+// workgroups per case in the row (Krylov) launches; KrylovView::nblk must equal it
+int row_blocks(long long n) { return (int)std::max<long long>(1, std::min<long long>((n + pfem::kBlock - 1) / pfem::kBlock, kRowCap)); }
 
-int pick_lanes(double avg_row) {
-    if (avg_row >= 40) return 16;
-    if (avg_row >= 12) return 8;
+// lanes per node of the node-block SpMV: the node-graph degree (incl. the node itself) is
+// ~7 for 2D Kuhn triangulations and ~15 for 3D Kuhn tetrahedralisations
+int pick_lanes(double avg_deg) {
+    // PFEM_SPMV_LANES (4, 8 or 16) overrides the choice (tuning)
+    static const int forced = [] {
+        const char* e = std::getenv("PFEM_SPMV_LANES");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    }();
+    if (forced) return forced;
+    if (avg_deg > 8) return 16;
+    if (avg_deg > 4) return 8;
     return 4;
+}
+
+template <class T>
+pfem::NodeMat<T> node_mat(const pfem::Pattern& P, int dim, const T* vals) {
+    pfem::NodeMat<T> A{};
+    A.N = P.N;
+    A.nf = dim + 1;
+    A.mom_nnz = P.mom_nnz;
+    A.info = P.node_info.as<int4>();
+    A.nbr_ptr = P.nbr_ptr.as<int>();
+    A.nbr_idx = P.nbr_idx.as<int>();
+    A.vals = vals;
+    return A;
 }
 }  // namespace
 
@@ -80,7 +111,7 @@ struct pfem_case {
     pfem::Pattern pat;
     DevBuf pos, pos_save, dir_vals;
     // PSPG system, Picard state and Krylov workspace
-    DevBuf q_prev, x, q_itprev, vals, b, vals32, dinv, dinv32, kv64, kv32;
+    DevBuf q_prev, x, q_itprev, vals, b, vals32, binv, binv32, kv64, kv32;
     DevBuf partials, counters, scal, out;
     // WC state and accumulators
     DevBuf wstate, wacc, wdt2;
@@ -105,9 +136,10 @@ struct Runner {
     hipStream_t s = nullptr;
     std::vector<pfem_case*> act;
     DevBuf v_pspg, v_pic, v_k64, v_k32, v_ref, v_wc64, v_wc32;
-    DevBuf t_elem[2], t_node, t_row;
-    int nt_elem[2] = {0, 0}, nt_node = 0, nt_row = 0;
+    DevBuf t_elem[2], t_asm[2], t_node, t_row;
+    int nt_elem[2] = {0, 0}, nt_asm[2] = {0, 0}, nt_node = 0, nt_row = 0;
     int lanes = 8;
+    int par = 0;  // slot parity of the next BiCGSTAB iteration launch
     hipEvent_t ev[6] = {};
     // live timing of the first SpMV kernel of each BiCGSTAB iteration (single-case runs)
     bool prof = false;
@@ -124,8 +156,8 @@ struct Runner {
             if (e) (void)hipEventDestroy(e);
     }
     void enable_profiling(bool on) {
-        if (on && !pev[0])
-            for (auto& e : pev) PFEM_HIP(hipEventCreate(&e));
+        if (on && !pev[0])  // timing-only events: no system-scope fence (no L2 write-back) on record
+            for (auto& e : pev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         prof = on;
         if (on) prof_ms[0] = prof_ms[1] = 0, prof_count[0] = prof_count[1] = 0;
     }
@@ -162,23 +194,27 @@ struct Runner {
         upload_views(v_ref, &pfem_case::rv);
         upload_views(v_wc64, &pfem_case::wc64);
         upload_views(v_wc32, &pfem_case::wc32);
-        std::vector<BlockTask> e2, e3, nodes, rows;
-        double nnz = 0, nrows = 0;
+        std::vector<BlockTask> e2, e3, a2, a3, nodes, rows;
+        double gnnz = 0, gnodes = 0;
         for (size_t i = 0; i < cs.size(); ++i) {
             pfem_case* c = cs[i];
             append(c->dim == 2 ? e2 : e3, (int)i, c->E, kElemCap);
+            // node-major assembly: kAsmLanes (16) lanes per node
+            append(c->dim == 2 ? a2 : a3, (int)i, (long long)c->N * 16, kElemCap);
             append(nodes, (int)i, c->N, kElemCap);
-            append(rows, (int)i, c->n, kRowCap);
+            for (int b = 0, nb = row_blocks(c->n); b < nb; ++b) rows.push_back(BlockTask{(int)i, b, nb, 0});
             if (c->d.kind == PFEM_KIND_PSPG) {
-                nnz += (double)c->pat.nnz;
-                nrows += c->n;
+                gnnz += (double)c->pat.graph_nnz;
+                gnodes += c->N;
             }
         }
         nt_elem[0] = upload_tasks(t_elem[0], e2, s);
         nt_elem[1] = upload_tasks(t_elem[1], e3, s);
+        nt_asm[0] = upload_tasks(t_asm[0], a2, s);
+        nt_asm[1] = upload_tasks(t_asm[1], a3, s);
         nt_node = upload_tasks(t_node, nodes, s);
         nt_row = upload_tasks(t_row, rows, s);
-        lanes = pick_lanes(nrows > 0 ? nnz / nrows : 16.0);
+        lanes = pick_lanes(gnodes > 0 ? gnnz / gnodes : 15.0);
     }
 
     // Refresh the views only (same set, changed per-step scalars such as dt).
@@ -197,8 +233,8 @@ struct Runner {
 // ---------------------------------------------------------------------------------------
 static void assemble_active(Runner& R) {
     const PspgView* v = R.v_pspg.as<PspgView>();
-    if (R.nt_elem[0]) launch_pspg_assemble(2, v, R.te(0), R.nt_elem[0], R.s);
-    if (R.nt_elem[1]) launch_pspg_assemble(3, v, R.te(1), R.nt_elem[1], R.s);
+    if (R.nt_asm[0]) launch_pspg_assemble(2, v, R.t_asm[0].as<BlockTask>(), R.nt_asm[0], R.s);
+    if (R.nt_asm[1]) launch_pspg_assemble(3, v, R.t_asm[1].as<BlockTask>(), R.nt_asm[1], R.s);
     launch_pspg_bc(v, R.t_node.as<BlockTask>(), R.nt_node, R.s);
     launch_pspg_precond(v, R.t_row.as<BlockTask>(), R.nt_row, R.s);
     PFEM_HIP(hipGetLastError());
@@ -237,7 +273,8 @@ static void krylov_run(Runner& R, const std::vector<pfem_case*>& cs, bool verify
     }
     const KrylovView<T>* v = which == 0 ? (const KrylovView<T>*)R.v_k64.p : (const KrylovView<T>*)R.v_k32.p;
     const BlockTask* t = R.t_row.as<BlockTask>();
-    launch_kry_init<T>(R.lanes, v, t, R.nt_row, R.s);
+    // init_fin writes the slot the next iteration reads as its previous one
+    launch_kry_init<T>(R.lanes, v, t, R.nt_row, (int)nc, R.par ^ 1, R.s);
     std::vector<char> fin(nc, 0), verifying(nc, 0);
     int chunk = 4;
     const bool prof = R.prof && nc == 1;
@@ -299,14 +336,15 @@ static void krylov_run(Runner& R, const std::vector<pfem_case*>& cs, bool verify
         }
         if (all_final) break;
         if (need_init) {
-            launch_kry_init<T>(R.lanes, v, t, R.nt_row, R.s);
+            launch_kry_init<T>(R.lanes, v, t, R.nt_row, (int)nc, R.par ^ 1, R.s);
             continue;
         }
         if (prof) {
             prof_chunk = chunk;
             prof_before = cs[0]->h_scal[which].iters;
         }
-        launch_kry_iters<T>(R.lanes, chunk, v, t, R.nt_row, R.s, prof ? R.pev : nullptr);
+        launch_kry_iters<T>(R.lanes, chunk, v, t, R.nt_row, R.par, R.s, prof ? R.pev : nullptr);
+        R.par = (R.par + chunk) & 1;
         PFEM_HIP(hipGetLastError());
         chunk = std::min(chunk * 2, 64);
     }
@@ -585,22 +623,23 @@ void fill_views(pfem_case* c) {
     pv.N = N;
     pv.E = c->E;
     pv.mom_nnz = P.mom_nnz;
-    pv.n_dir = P.n_dir;
     pv.conn = P.conn.as<int>();
     pv.pos = c->pos.as<double>();
     pv.q_prev = c->q_prev.as<double>();
     pv.q_iter = c->x.as<double>();
     pv.node_info = P.node_info.as<int4>();
-    pv.ranks = P.ranks.as<uint4>();
-    pv.dir = P.dir.as<int4>();
+    pv.nbr_ptr = P.nbr_ptr.as<int>();
+    pv.nbr_idx = P.nbr_idx.as<int>();
+    pv.pei_ptr = P.pei_ptr.as<int>();
+    pv.pei = P.pei.as<uint32_t>();
     pv.dir_vals = c->dir_vals.as<double>();
     pv.vals = c->vals.as<double>();
     pv.b = c->b.as<double>();
     const bool mixed = c->d.precision == PFEM_PREC_MIXED;
     pv.vals32 = mixed ? c->vals32.as<float>() : nullptr;
     pv.diag_pos = P.diag_pos.as<int>();
-    pv.dinv = c->dinv.as<double>();
-    pv.dinv32 = mixed ? c->dinv32.as<float>() : nullptr;
+    pv.binv = c->binv.as<double>();
+    pv.binv32 = mixed ? c->binv32.as<float>() : nullptr;
     pv.nnz = P.nnz;
     pv.rho = c->d.rho;
     pv.mu = c->d.mu;
@@ -618,9 +657,7 @@ void fill_views(pfem_case* c) {
     pc.pos_save = c->pos_save.as<double>();
     pc.x = c->x.as<double>();
     pc.x_prev = c->q_itprev.as<double>();
-    pc.row_ptr = P.row_ptr.as<int>();
-    pc.col_idx = P.col_idx.as<int>();
-    pc.vals = c->vals.as<double>();
+    pc.A = node_mat<double>(P, dim, c->vals.as<double>());
     pc.b = c->b.as<double>();
     pc.partials = c->partials.as<double>();
     pc.counters = c->counters.as<unsigned>();
@@ -630,10 +667,11 @@ void fill_views(pfem_case* c) {
     k = KrylovView<double>{};
     k.n = (int)n;
     k.x_zero = 0;
-    k.row_ptr = P.row_ptr.as<int>();
-    k.col_idx = P.col_idx.as<int>();
-    k.vals = c->vals.as<double>();
-    k.dinv = c->dinv.as<double>();
+    k.nodes = N;
+    k.nf = dim + 1;
+    k.nblk = row_blocks(n);
+    k.A = node_mat<double>(P, dim, c->vals.as<double>());
+    k.binv = c->binv.as<double>();
     k.b = c->b.as<double>();
     k.x = c->x.as<double>();
     double* w = c->kv64.as<double>();
@@ -655,10 +693,11 @@ void fill_views(pfem_case* c) {
         float* u = c->kv32.as<float>();
         f.n = (int)n;
         f.x_zero = 1;
-        f.row_ptr = P.row_ptr.as<int>();
-        f.col_idx = P.col_idx.as<int>();
-        f.vals = c->vals32.as<float>();
-        f.dinv = c->dinv32.as<float>();
+        f.nodes = N;
+        f.nf = dim + 1;
+        f.nblk = row_blocks(n);
+        f.A = node_mat<float>(P, dim, c->vals32.as<float>());
+        f.binv = c->binv32.as<float>();
         f.b = u;
         f.x = u + n;
         f.r = u + 2 * n;
@@ -675,9 +714,7 @@ void fill_views(pfem_case* c) {
         auto& r = c->rv;
         r = RefineView{};
         r.n = (int)n;
-        r.row_ptr = P.row_ptr.as<int>();
-        r.col_idx = P.col_idx.as<int>();
-        r.vals = c->vals.as<double>();
+        r.A = node_mat<double>(P, dim, c->vals.as<double>());
         r.b = c->b.as<double>();
         r.x = c->x.as<double>();
         r.r = c->kv64.as<double>();
@@ -844,7 +881,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         c->dir_vals.reserve(sizeof(double) * (size_t)dim * N);
         PFEM_HIP(hipMemcpyAsync(c->pos.p, pos, sizeof(double) * (size_t)dim * N, hipMemcpyHostToDevice, c->stream));
         PFEM_HIP(hipMemsetAsync(c->dir_vals.p, 0, sizeof(double) * (size_t)dim * N, c->stream));
-        c->partials.reserve(sizeof(double) * pfem::kMaxCaseBlocks * 4);
+        c->partials.reserve(sizeof(double) * pfem::kMaxCaseBlocks * pfem::kPartStride);
         c->counters.reserve(sizeof(unsigned) * 8);
         c->scal.reserve(sizeof(KScal) * 2);
         c->out.reserve(sizeof(double) * 4);
@@ -858,16 +895,16 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             c->q_itprev.reserve(sizeof(double) * n);
             c->vals.reserve(sizeof(double) * nnz);
             c->b.reserve(sizeof(double) * n);
-            c->dinv.reserve(sizeof(double) * n);
+            c->binv.reserve(sizeof(double) * (size_t)(dim + 1) * (dim + 1) * N);
             c->kv64.reserve(sizeof(double) * n * 8);
             PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * n, c->stream));
             PFEM_HIP(hipMemsetAsync(c->x.p, 0, sizeof(double) * n, c->stream));
             if (c->d.precision == PFEM_PREC_MIXED) {
                 c->vals32.reserve(sizeof(float) * nnz);
-                c->dinv32.reserve(sizeof(float) * n);
+                c->binv32.reserve(sizeof(float) * (size_t)(dim + 1) * (dim + 1) * N);
                 c->kv32.reserve(sizeof(float) * n * 10);
             }
-            c->lanes = pick_lanes((double)nnz / (double)n);
+            c->lanes = pick_lanes((double)c->pat.graph_nnz / (double)N);
             fill_views(c);
         } else {
             pfem::build_mesh_basic(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
@@ -977,10 +1014,10 @@ int pfem_case_footprint(pfem_case* c, int64_t* bytes) {
         PFEM_REQUIRE(bytes != nullptr, PFEM_E_INVALID, "null output");
         size_t b = 0;
         for (DevBuf* p : {&c->pos, &c->pos_save, &c->dir_vals, &c->q_prev, &c->x, &c->q_itprev, &c->vals, &c->b,
-                          &c->vals32, &c->dinv, &c->dinv32, &c->kv64, &c->kv32, &c->partials, &c->counters, &c->scal,
+                          &c->vals32, &c->binv, &c->binv32, &c->kv64, &c->kv32, &c->partials, &c->counters, &c->scal,
                           &c->out, &c->wstate, &c->wacc, &c->wdt2, &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr,
                           &c->pat.nbr_idx, &c->pat.node_info, &c->pat.ranks, &c->pat.row_ptr, &c->pat.col_idx,
-                          &c->pat.diag_pos, &c->pat.dir})
+                          &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei})
             b += p->bytes;
         *bytes = (int64_t)b;
         return PFEM_OK;
@@ -1070,8 +1107,8 @@ int pfem_spmv(pfem_case* c, const double* x, double* y) {
         double* dx = c->kv64.as<double>();
         double* dy = dx + c->n;
         PFEM_HIP(hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, c->stream));
-        pfem::launch_spmv<double>(c->lanes, c->n, c->pat.row_ptr.as<int>(), c->pat.col_idx.as<int>(),
-                                  c->vals.as<double>(), dx, dy, grid_for_rows(c->n), c->stream);
+        pfem::launch_spmv<double>(c->lanes, node_mat<double>(c->pat, c->dim, c->vals.as<double>()), dx, dy,
+                                  grid_for_rows(c->N), c->stream);
         PFEM_HIP(hipGetLastError());
         PFEM_HIP(hipMemcpyAsync(y, dy, nb, hipMemcpyDeviceToHost, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
@@ -1152,6 +1189,65 @@ int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4) {
             }
         }
         R.enable_profiling(enable != 0);
+        return PFEM_OK;
+    });
+}
+
+int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG && out8 && reps > 0, PFEM_E_INVALID, "invalid arguments");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        hipEvent_t e0, e1;
+        PFEM_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
+        PFEM_HIP(hipEventCreateWithFlags(&e1, hipEventDisableSystemFence));
+        auto timed = [&](auto&& launch) {
+            launch();  // warm-up
+            PFEM_HIP(hipEventRecord(e0, c->stream));
+            for (int r = 0; r < reps; ++r) launch();
+            PFEM_HIP(hipEventRecord(e1, c->stream));
+            PFEM_HIP(hipEventSynchronize(e1));
+            PFEM_HIP(hipGetLastError());
+            return 1e3 * (double)pfem::ev_ms(e0, e1) / reps;  // microseconds per launch
+        };
+        const int n = c->n;
+        double* w = c->kv64.as<double>();
+        for (int k = 0; k < 8; ++k) out8[k] = 0;
+        out8[0] = timed([&] {
+            pfem::launch_spmv<double>(c->lanes, node_mat<double>(c->pat, c->dim, c->vals.as<double>()),
+                                      c->q_prev.as<double>(), w + n, grid_for_rows(c->N), c->stream);
+        });
+        if (c->d.precision == PFEM_PREC_MIXED) {
+            float* u = c->kv32.as<float>();
+            out8[1] = timed([&] {
+                pfem::launch_spmv<float>(c->lanes, node_mat<float>(c->pat, c->dim, c->vals32.as<float>()), u, u + n,
+                                         grid_for_rows(c->N), c->stream);
+            });
+        }
+        const long long nnz = c->pat.nnz;
+        out8[2] = timed([&] { pfem::launch_stream_read(c->vals.as<double>(), nnz, w, 2048, c->stream); });
+        const long long ncopy = std::min<long long>(nnz, 4LL * n);
+        out8[3] = timed([&] { pfem::launch_copy(c->vals.as<double>(), w + 4LL * n, ncopy, 2048, c->stream); });
+        out8[4] = (double)ncopy * 16.0;
+        out8[5] = (double)nnz;
+        out8[6] = (double)c->pat.graph_nnz;
+        out8[7] = (double)c->lanes;
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return PFEM_OK;
+    });
+}
+
+int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out) {
+    return guarded([&] {
+        PFEM_REQUIRE(ctx && out && nblocks > 0 && nblocks <= (1 << 20), PFEM_E_INVALID, "invalid arguments");
+        PFEM_HIP(hipSetDevice(ctx->device));
+        DevBuf d;
+        d.reserve(sizeof(int) * (size_t)nblocks);
+        pfem::launch_xcd_probe(d.as<int>(), nblocks, ctx->stream);
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(out, d.p, sizeof(int) * (size_t)nblocks, hipMemcpyDeviceToHost, ctx->stream));
+        PFEM_HIP(hipStreamSynchronize(ctx->stream));
         return PFEM_OK;
     });
 }

@@ -1,226 +1,377 @@
-// K3 (CSR SpMV) and K5 (Jacobi-preconditioned BiCGSTAB) for a batch of cases, plus the
-// FP64 iterative-refinement kernels of the mixed-precision solve.
+// K3 (CSR SpMV) and K5 (block-Jacobi-preconditioned BiCGSTAB) for a batch of cases, plus
+// the FP64 iterative-refinement kernels of the mixed-precision solve.
 //
 // Replaces SparseLU<SparseMatrix<double>, COLAMDOrdering<int>> (analyzePattern / factorize /
 // solve at MomContEquationPSPG.inl:281-290) with right-preconditioned BiCGSTAB
-// (van der Vorst 1992).  One iteration is five launches:
-//   p  = r + beta (p - omega v);  ph = D^-1 p
-//   v  = A ph                      -> (rh, v)        -> alpha = rho / (rh, v)
-//   s  = r - alpha v;  sh = D^-1 s -> (s, s)         -> early exit when ||s|| <= tol
-//   t  = A sh                      -> (t, s), (t, t) -> omega
-//   x += alpha ph + omega sh;  r = s - omega t -> (rh, r), (r, r) -> beta, rho, convergence
-// Every dot product is accumulated in FP64 and reduced deterministically; the scalars
-// are finalised on the device by the last workgroup of each reduction, so the host
-// only reads the convergence flag every few iterations.
+// (van der Vorst 1992).  The preconditioner is block Jacobi over the nf x nf diagonal
+// block of each node (its velocity and pressure DOFs).  One iteration is five launches:
+//   K_p : p  = r + beta (p - omega v);  ph = B^-1 p        (beta from (rh, r) and slot q)
+//   K_v : v  = A ph                          -> (rh, v)
+//   K_s : s  = r - alpha v;  sh = B^-1 s     -> (s, s)      (alpha = rho / (rh, v))
+//   K_t : t  = A sh                          -> (t, s), (t, t)   (skipped when ||s|| <= tol)
+//   K_x : x += alpha ph + omega sh;  r = s - omega t -> (rh, r), (r, r)
+// Reductions: every workgroup writes its FP64 partial sums to its own slot and the next
+// kernel's workgroups each re-reduce them in block order — deterministic, and ordered by
+// the kernel boundary, so no device-scope fence (an L2 write-back on gfx950) is needed.
+// Scalars: two slots used alternately by iteration parity (KScal in views.hpp).
 //
-// SpMV: L lanes per row (L | 64), rows grid-strided over the case's workgroups; the
-// L lanes of a row read consecutive values and column indices (coalesced) and reduce
-// with cross-lane shuffles.
+// Work split: workgroups are dispatched round-robin over the 8 XCDs; the workgroups of a
+// case that land on one XCD own one contiguous range of nodes (every field of those
+// nodes), so the SpMV's x gathers (neighbour nodes) and the vectors handed from one
+// kernel to the next stay in that XCD's L2.  Vector kernels are node-major (a thread owns
+// the nf DOFs of a node); SpMV kernels use L lanes per row (L | 64) reading consecutive
+// values and column indices and reducing with cross-lane shuffles.
 #include "common.hpp"
+#include "nodespmv.hpp"
 #include "views.hpp"
 
 namespace pfem {
 
-template <typename T, int L>
-__device__ __forceinline__ T row_dot(const int* __restrict__ rp, const int* __restrict__ ci, const T* __restrict__ va,
-                                     const T* __restrict__ x, long long row, int lane) {
-    const int beg = rp[row], end = rp[row + 1];
-    T acc = 0;
-    for (int k = beg + lane; k < end; k += L) acc += va[k] * x[ci[k]];
-#pragma unroll
-    for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
-    return acc;
+constexpr int kXcd = 8;
+
+// The node range and the stride over it owned by this workgroup.
+struct Seg {
+    int n0;   // first node of the range
+    int ns;   // nodes in the range
+    int lb;   // index of this workgroup among the case's workgroups on the range
+    int cnt;  // workgroups of the case on the range
+};
+
+__device__ __forceinline__ Seg xcd_segment(const BlockTask& t, int nodes) {
+    int seg, nseg, lb, cnt;
+    if (t.nblocks < kXcd) {
+        seg = t.block;
+        nseg = t.nblocks;
+        lb = 0;
+        cnt = 1;
+    } else {
+        const int g0 = (int)blockIdx.x - t.block;  // global id of the case's first workgroup
+        seg = ((int)blockIdx.x % kXcd - g0 % kXcd + kXcd) % kXcd;  // == t.block % kXcd
+        nseg = kXcd;
+        lb = (t.block - seg) / kXcd;
+        cnt = (t.nblocks - seg + kXcd - 1) / kXcd;
+    }
+    Seg s;
+    s.n0 = (int)((long long)nodes * seg / nseg);
+    s.ns = (int)((long long)nodes * (seg + 1) / nseg) - s.n0;
+    s.lb = lb;
+    s.cnt = cnt;
+    return s;
 }
 
-// y = A x (plain SpMV; used for the roofline measurement and the tests).
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_spmv(int n, const int* __restrict__ rp, const int* __restrict__ ci,
-                                                 const T* __restrict__ va, const T* __restrict__ x, T* __restrict__ y,
-                                                 int nblocks) {
-    constexpr int RPB = kBlock / L;
-    const int lane = threadIdx.x % L;
-    for (long long row = (long long)blockIdx.x * RPB + threadIdx.x / L; row < n; row += (long long)nblocks * RPB) {
-        const T acc = row_dot<T, L>(rp, ci, va, x, row, lane);
-        if (lane == 0) y[row] = acc;
+// Local row index l in [0, nf*ns) -> global row (field-blocked layout).
+__device__ __forceinline__ long long seg_row(const Seg& s, int nodes, long long l) {
+    const long long f = l / s.ns;
+    return f * nodes + s.n0 + (l - f * s.ns);
+}
+
+// out = B^-1 in for one node (nf <= 4 fields, SoA blocks).
+template <typename T>
+__device__ __forceinline__ void block_apply(const T* __restrict__ binv, int nodes, int nf, long long i, const T (&in)[kMaxNf],
+                                            T (&out)[kMaxNf]) {
+#pragma unroll
+    for (int s = 0; s < kMaxNf; ++s) {
+        T acc = 0;
+        if (s < nf) {
+#pragma unroll
+            for (int t = 0; t < kMaxNf; ++t)
+                if (t < nf) acc += binv[(long long)(s * nf + t) * nodes + i] * in[t];
+        }
+        out[s] = acc;
     }
 }
 
+// Sum of partial slots over the producer's workgroups, in block order; valid in every thread.
+template <int NV>
+__device__ __forceinline__ void reduce_partials(const double* __restrict__ P, int nblk, const int (&slot)[NV],
+                                                double (&out)[NV]) {
+    __shared__ double bc[NV];
+    double acc[NV];
+#pragma unroll
+    for (int j = 0; j < NV; ++j) acc[j] = 0.0;
+    for (int b = threadIdx.x; b < nblk; b += kBlock)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) acc[j] += P[b * kPartStride + slot[j]];
+    block_sum<NV>(acc);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) bc[j] = acc[j];
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < NV; ++j) out[j] = bc[j];
+}
+
+// Writes this workgroup's partial sums (thread-local contributions) to its slots.
+template <int NV>
+__device__ __forceinline__ void write_partials(double (&v)[NV], double* __restrict__ P, int block,
+                                               const int (&slot)[NV]) {
+    block_sum<NV>(v);
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int j = 0; j < NV; ++j) P[block * kPartStride + slot[j]] = v[j];
+}
+
+// y = A x (plain node-block SpMV; pfem_spmv and the Picard paths).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_spmv(NodeMat<T> A, const T* __restrict__ x, T* __restrict__ y) {
+    constexpr int NPB = kBlock / K;
+    const int lane = threadIdx.x % K;
+    for (long long i = (long long)blockIdx.x * NPB + threadIdx.x / K; i < A.N; i += (long long)gridDim.x * NPB) {
+        T yv[kMaxNf];
+        node_spmv<T, K>(A, x, i, lane, yv);
+        if (lane < A.nf) y[i + (long long)lane * A.N] = pick(yv, lane);
+    }
+}
+
+// (Re)start: r = b - A x (or b when x_zero), rh = r, p = v = 0.
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __restrict__ views,
                                                      const BlockTask* __restrict__ tasks) {
-    constexpr int RPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
-    if (!V.sc->req_init) return;  // only cases the host asked to (re)start
+    if (!V.sc->req_init) return;
+    constexpr int NPB = kBlock / L;
+    const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
-    double part[2] = {0.0, 0.0};
-    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
-        const T ax = V.x_zero ? T(0) : row_dot<T, L>(V.row_ptr, V.col_idx, V.vals, V.x, row, lane);
-        if (lane == 0) {
+    double part[3] = {0.0, 0.0, 0.0};
+    for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
+        const long long i = sg.n0 + l;
+        T ax[kMaxNf] = {T(0), T(0), T(0), T(0)};
+        if (!V.x_zero) node_spmv<T, L>(V.A, V.x, i, lane, ax);
+        if (lane < V.nf) {
+            const long long row = i + (long long)lane * V.nodes;
             const T bi = V.b[row];
-            const T r = bi - ax;
+            const T r = bi - pick(ax, lane);
             V.r[row] = r;
             V.rh[row] = r;
             V.p[row] = T(0);
             V.v[row] = T(0);
             if (V.x_zero) V.x[row] = T(0);
             part[0] += (double)r * (double)r;
-            part[1] += (double)bi * (double)bi;
+            part[1] += (double)r * (double)r;
+            part[2] += (double)bi * (double)bi;
         }
     }
-    double tot[2];
-    if (reduce_last_block<2>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0) {
-        KScal& S = *V.sc;
-        S.rr = tot[0];
-        S.bb = tot[1];
-        S.tol2 = S.rtol * S.rtol * tot[1];
-        S.rho = tot[0];
-        S.alpha = 1.0;
-        S.omega = 1.0;
-        S.beta = 0.0;
-        S.s_conv = 0;
-        S.req_init = 0;
-        if (tot[1] == 0.0 || tot[0] <= S.tol2) S.done = 1;
-        else if (!isfinite(tot[0])) S.done = 2;
-        else if (S.iters >= S.maxit) S.done = 3;
-        else S.done = 0;
+    constexpr int slots[3] = {P_RHR, P_RR, P_BB};
+    write_partials<3>(part, V.partials, task.block, slots);
+}
+
+// One workgroup per case: finalises a (re)start and writes slot `par` as the "previous"
+// slot of the next iteration (which runs with parity par ^ 1).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __restrict__ views, int ncases, int par) {
+    for (int c = blockIdx.x; c < ncases; c += gridDim.x) {
+        const KrylovView<T>& V = views[c];
+        if (!V.sc->req_init) continue;
+        constexpr int slots[3] = {P_RHR, P_RR, P_BB};
+        double tot[3];
+        reduce_partials<3>(V.partials, V.nblk, slots, tot);
+        if (threadIdx.x == 0) {
+            KScal& S = *V.sc;
+            S.bb = tot[2];
+            S.tol2 = S.rtol * S.rtol * tot[2];
+            S.rr = tot[1];
+            S.slot[par].rho = tot[0];
+            S.slot[par].alpha = 1.0;
+            S.slot[par].omega = 1.0;
+            S.req_init = 0;
+            if (tot[2] == 0.0 || tot[1] <= S.tol2) S.done = 1;
+            else if (!isfinite(tot[1])) S.done = 2;
+            else if (S.iters >= S.maxit) S.done = 3;
+            else S.done = 0;
+        }
+        __syncthreads();
     }
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restrict__ views,
-                                                  const BlockTask* __restrict__ tasks) {
+                                                  const BlockTask* __restrict__ tasks, int par) {
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
-    const KScal& S = *V.sc;
+    KScal& S = *V.sc;
     if (S.done) return;
-    const T beta = (T)S.beta, omega = (T)S.omega;
-    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < V.n; i += (long long)task.nblocks * kBlock) {
-        const T p = V.r[i] + beta * (V.p[i] - omega * V.v[i]);
-        V.p[i] = p;
-        V.ph[i] = V.dinv[i] * p;
+    constexpr int slots[2] = {P_RHR, P_RR};
+    double tot[2];
+    reduce_partials<2>(V.partials, V.nblk, slots, tot);
+    const KSlot prev = S.slot[par ^ 1];
+    const double beta = (tot[0] / prev.rho) * (prev.alpha / prev.omega);
+    int stop = 0;
+    if (tot[1] <= S.tol2) stop = 1;
+    else if (!isfinite(tot[1]) || tot[0] == 0.0 || !isfinite(beta)) stop = 2;
+    else if (S.iters >= S.maxit) stop = 3;
+    if (stop) {
+        if (task.block == 0 && threadIdx.x == 0) {
+            S.rr = tot[1];
+            S.done = stop;
+        }
+        return;
+    }
+    const T b = (T)beta, om = (T)prev.omega;
+    const Seg sg = xcd_segment(task, V.nodes);
+    const int nf = V.nf;
+    for (long long l = (long long)sg.lb * kBlock + threadIdx.x; l < sg.ns; l += (long long)sg.cnt * kBlock) {
+        const long long i = sg.n0 + l;
+        T pv[kMaxNf], ph[kMaxNf];
+#pragma unroll
+        for (int f = 0; f < kMaxNf; ++f) {
+            pv[f] = T(0);
+            if (f < nf) {
+                const long long q = (long long)f * V.nodes + i;
+                pv[f] = V.r[q] + b * (V.p[q] - om * V.v[q]);
+                V.p[q] = pv[f];
+            }
+        }
+        block_apply<T>(V.binv, V.nodes, nf, i, pv, ph);
+#pragma unroll
+        for (int f = 0; f < kMaxNf; ++f)
+            if (f < nf) V.ph[(long long)f * V.nodes + i] = ph[f];
+    }
+    if (task.block == 0 && threadIdx.x == 0) {
+        S.slot[par].rho = tot[0];
+        S.rr = tot[1];
     }
 }
 
 template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_kry_spmv_v(const KrylovView<T>* __restrict__ views,
-                                                       const BlockTask* __restrict__ tasks) {
-    constexpr int RPB = kBlock / L;
+__global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restrict__ views,
+                                                  const BlockTask* __restrict__ tasks) {
+    constexpr int NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
     if (V.sc->done) return;
+    const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
     double part[1] = {0.0};
-    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
-        const T y = row_dot<T, L>(V.row_ptr, V.col_idx, V.vals, V.ph, row, lane);
-        if (lane == 0) {
-            V.v[row] = y;
-            part[0] += (double)V.rh[row] * (double)y;
+    for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
+        const long long i = sg.n0 + l;
+        T y[kMaxNf];
+        node_spmv<T, L>(V.A, V.ph, i, lane, y);
+        if (lane < V.nf) {
+            const long long row = i + (long long)lane * V.nodes;
+            const T yv = pick(y, lane);
+            V.v[row] = yv;
+            part[0] += (double)V.rh[row] * (double)yv;
         }
     }
-    double tot[1];
-    if (reduce_last_block<1>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0) {
-        KScal& S = *V.sc;
-        S.rv = tot[0];
-        const double alpha = S.rho / tot[0];
-        if (tot[0] == 0.0 || !isfinite(alpha)) S.done = 2;
-        else S.alpha = alpha;
-    }
+    constexpr int slots[1] = {P_RV};
+    write_partials<1>(part, V.partials, task.block, slots);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kry_s(const KrylovView<T>* __restrict__ views,
+                                                  const BlockTask* __restrict__ tasks, int par) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const KrylovView<T>& V = views[task.case_id];
+    KScal& S = *V.sc;
+    if (S.done) return;
+    constexpr int rs[1] = {P_RV};
+    double rv[1];
+    reduce_partials<1>(V.partials, V.nblk, rs, rv);
+    const double alpha = S.slot[par].rho / rv[0];
+    if (rv[0] == 0.0 || !isfinite(alpha)) {
+        if (task.block == 0 && threadIdx.x == 0) S.done = 2;
+        return;
+    }
+    const T a = (T)alpha;
+    const Seg sg = xcd_segment(task, V.nodes);
+    const int nf = V.nf;
+    double part[1] = {0.0};
+    for (long long l = (long long)sg.lb * kBlock + threadIdx.x; l < sg.ns; l += (long long)sg.cnt * kBlock) {
+        const long long i = sg.n0 + l;
+        T sv[kMaxNf], sh[kMaxNf];
+#pragma unroll
+        for (int f = 0; f < kMaxNf; ++f) {
+            sv[f] = T(0);
+            if (f < nf) {
+                const long long q = (long long)f * V.nodes + i;
+                sv[f] = V.r[q] - a * V.v[q];
+                V.s[q] = sv[f];
+                part[0] += (double)sv[f] * (double)sv[f];
+            }
+        }
+        block_apply<T>(V.binv, V.nodes, nf, i, sv, sh);
+#pragma unroll
+        for (int f = 0; f < kMaxNf; ++f)
+            if (f < nf) V.sh[(long long)f * V.nodes + i] = sh[f];
+    }
+    constexpr int slots[1] = {P_SS};
+    write_partials<1>(part, V.partials, task.block, slots);
+    if (task.block == 0 && threadIdx.x == 0) S.slot[par].alpha = alpha;
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restrict__ views,
                                                   const BlockTask* __restrict__ tasks) {
+    constexpr int NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
     if (V.sc->done) return;
-    const T alpha = (T)V.sc->alpha;
-    double part[1] = {0.0};
-    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < V.n; i += (long long)task.nblocks * kBlock) {
-        const T s = V.r[i] - alpha * V.v[i];
-        V.s[i] = s;
-        V.sh[i] = V.dinv[i] * s;
-        part[0] += (double)s * (double)s;
-    }
-    double tot[1];
-    if (reduce_last_block<1>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0) {
-        KScal& S = *V.sc;
-        S.ss = tot[0];
-        S.s_conv = (tot[0] <= S.tol2) ? 1 : 0;
-    }
-}
-
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_kry_spmv_t(const KrylovView<T>* __restrict__ views,
-                                                       const BlockTask* __restrict__ tasks) {
-    constexpr int RPB = kBlock / L;
-    const BlockTask task = tasks[blockIdx.x];
-    if (task.case_id < 0) return;
-    const KrylovView<T>& V = views[task.case_id];
-    if (V.sc->done || V.sc->s_conv) return;
+    constexpr int ss_slot[1] = {P_SS};
+    double ss[1];
+    reduce_partials<1>(V.partials, V.nblk, ss_slot, ss);
+    if (ss[0] <= V.sc->tol2) return;  // ||s|| converged: K_x finishes the iteration
+    const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
     double part[2] = {0.0, 0.0};
-    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
-        const T y = row_dot<T, L>(V.row_ptr, V.col_idx, V.vals, V.sh, row, lane);
-        if (lane == 0) {
-            V.t[row] = y;
-            part[0] += (double)y * (double)V.s[row];
-            part[1] += (double)y * (double)y;
+    for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
+        const long long i = sg.n0 + l;
+        T y[kMaxNf];
+        node_spmv<T, L>(V.A, V.sh, i, lane, y);
+        if (lane < V.nf) {
+            const long long row = i + (long long)lane * V.nodes;
+            const T yv = pick(y, lane);
+            V.t[row] = yv;
+            part[0] += (double)yv * (double)V.s[row];
+            part[1] += (double)yv * (double)yv;
         }
     }
-    double tot[2];
-    if (reduce_last_block<2>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0) {
-        KScal& S = *V.sc;
-        S.omega = (tot[1] == 0.0) ? 0.0 : tot[0] / tot[1];
-    }
+    constexpr int slots[2] = {P_TS, P_TT};
+    write_partials<2>(part, V.partials, task.block, slots);
 }
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kry_x(const KrylovView<T>* __restrict__ views,
-                                                  const BlockTask* __restrict__ tasks) {
+                                                  const BlockTask* __restrict__ tasks, int par) {
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
-    if (V.sc->done) return;
-    const bool sconv = V.sc->s_conv != 0;
-    const T alpha = (T)V.sc->alpha, omega = (T)V.sc->omega;
+    KScal& S = *V.sc;
+    if (S.done) return;
+    constexpr int sl[3] = {P_SS, P_TS, P_TT};
+    double tot[3];
+    reduce_partials<3>(V.partials, V.nblk, sl, tot);
+    const bool sconv = tot[0] <= S.tol2;
+    const double alpha = S.slot[par].alpha;
+    const double omega = sconv ? 1.0 : (tot[2] == 0.0 ? 0.0 : tot[1] / tot[2]);
+    const T a = (T)alpha, om = (T)omega;
+    const Seg sg = xcd_segment(task, V.nodes);
+    const long long nloc = (long long)V.nf * sg.ns;
     double part[2] = {0.0, 0.0};
-    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < V.n; i += (long long)task.nblocks * kBlock) {
+    for (long long l = (long long)sg.lb * kBlock + threadIdx.x; l < nloc; l += (long long)sg.cnt * kBlock) {
+        const long long i = seg_row(sg, V.nodes, l);
+        T r;
         if (sconv) {
-            V.x[i] += alpha * V.ph[i];
+            V.x[i] += a * V.ph[i];
+            r = V.s[i];
         } else {
-            V.x[i] += alpha * V.ph[i] + omega * V.sh[i];
-            const T r = V.s[i] - omega * V.t[i];
-            V.r[i] = r;
-            part[0] += (double)V.rh[i] * (double)r;
-            part[1] += (double)r * (double)r;
+            V.x[i] += a * V.ph[i] + om * V.sh[i];
+            r = V.s[i] - om * V.t[i];
         }
+        V.r[i] = r;
+        part[0] += (double)V.rh[i] * (double)r;
+        part[1] += (double)r * (double)r;
     }
-    double tot[2];
-    if (reduce_last_block<2>(part, V.partials, V.counters, task.block, task.nblocks, tot) && threadIdx.x == 0) {
-        KScal& S = *V.sc;
+    constexpr int slots[2] = {P_RHR, P_RR};
+    write_partials<2>(part, V.partials, task.block, slots);
+    if (task.block == 0 && threadIdx.x == 0) {
+        S.slot[par].omega = omega;
         S.iters += 1;
-        if (sconv) {
-            S.rr = S.ss;
-            S.done = 1;
-        } else {
-            S.rr = tot[1];
-            if (tot[1] <= S.tol2) {
-                S.done = 1;
-            } else if (!isfinite(tot[1]) || !isfinite(tot[0])) {
-                S.done = 2;
-            } else if (S.omega == 0.0 || tot[0] == 0.0) {
-                S.done = 2;  // breakdown: the host restarts from the current x
-            } else {
-                S.beta = (tot[0] / S.rho) * (S.alpha / S.omega);
-                S.rho = tot[0];
-                if (S.iters >= S.maxit) S.done = 3;
-            }
-        }
     }
 }
 
@@ -228,17 +379,19 @@ __global__ __launch_bounds__(kBlock) void k_kry_x(const KrylovView<T>* __restric
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_ir_residual(const RefineView* __restrict__ views,
                                                         const BlockTask* __restrict__ tasks) {
-    constexpr int RPB = kBlock / L;
+    constexpr int NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const RefineView& V = views[task.case_id];
     const int lane = threadIdx.x % L;
     double part[2] = {0.0, 0.0};
-    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
-        const double ax = row_dot<double, L>(V.row_ptr, V.col_idx, V.vals, V.x, row, lane);
-        if (lane == 0) {
+    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < V.A.N; i += (long long)task.nblocks * NPB) {
+        double ax[kMaxNf];
+        node_spmv<double, L>(V.A, V.x, i, lane, ax);
+        if (lane < V.A.nf) {
+            const long long row = i + (long long)lane * V.A.N;
             const double bi = V.b[row];
-            const double r = bi - ax;
+            const double r = bi - pick(ax, lane);
             V.r[row] = r;
             if (V.r32) V.r32[row] = (float)r;
             part[0] += r * r;
@@ -265,41 +418,42 @@ __global__ __launch_bounds__(kBlock) void k_ir_update(const RefineView* __restri
 // ev (optional): a pair of events recorded around the first SpMV of the iteration, for
 // live timing of that kernel.
 template <typename T, int L>
-static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s, hipEvent_t* ev) {
-    k_kry_p<T><<<nt, kBlock, 0, s>>>(v, t);
+static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s, int par, hipEvent_t* ev) {
+    k_kry_p<T><<<nt, kBlock, 0, s>>>(v, t, par);
     if (ev) (void)hipEventRecord(ev[0], s);
-    k_kry_spmv_v<T, L><<<nt, kBlock, 0, s>>>(v, t);
+    k_kry_v<T, L><<<nt, kBlock, 0, s>>>(v, t);
     if (ev) (void)hipEventRecord(ev[1], s);
-    k_kry_s<T><<<nt, kBlock, 0, s>>>(v, t);
-    k_kry_spmv_t<T, L><<<nt, kBlock, 0, s>>>(v, t);
-    k_kry_x<T><<<nt, kBlock, 0, s>>>(v, t);
+    k_kry_s<T><<<nt, kBlock, 0, s>>>(v, t, par);
+    k_kry_t<T, L><<<nt, kBlock, 0, s>>>(v, t);
+    k_kry_x<T><<<nt, kBlock, 0, s>>>(v, t, par);
 }
 
+// lanes = K, the lanes per field of the node-block SpMV (a node takes kMaxNf * K lanes)
 #define PFEM_DISPATCH_L(lanes, CALL)                      \
     switch (lanes) {                                       \
         case 4: { constexpr int LL = 4; CALL; } break;     \
         case 8: { constexpr int LL = 8; CALL; } break;     \
-        case 16: { constexpr int LL = 16; CALL; } break;   \
-        case 32: { constexpr int LL = 32; CALL; } break;   \
-        default: { constexpr int LL = 64; CALL; } break;   \
+        default: { constexpr int LL = 16; CALL; } break;   \
     }
 
 template <typename T>
-void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int ncases, int par,
+                     hipStream_t s) {
     PFEM_DISPATCH_L(lanes, (k_kry_init<T, LL><<<nt, kBlock, 0, s>>>(v, t)));
+    k_kry_init_fin<T><<<ncases < 1024 ? ncases : 1024, kBlock, 0, s>>>(v, ncases, par);
 }
 
+// Launches `iters` iterations; iteration j uses slot parity (par0 + j) & 1.
 template <typename T>
-void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s,
-                      hipEvent_t* ev) {
+void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, int par0,
+                      hipStream_t s, hipEvent_t* ev) {
     for (int it = 0; it < iters; ++it)
-        PFEM_DISPATCH_L(lanes, (iter_launch<T, LL>(v, t, nt, s, ev ? ev + 2 * it : nullptr)));
+        PFEM_DISPATCH_L(lanes, (iter_launch<T, LL>(v, t, nt, s, (par0 + it) & 1, ev ? ev + 2 * it : nullptr)));
 }
 
 template <typename T>
-void launch_spmv(int lanes, int n, const int* rp, const int* ci, const T* va, const T* x, T* y, int nblocks,
-                 hipStream_t s) {
-    PFEM_DISPATCH_L(lanes, (k_spmv<T, LL><<<nblocks, kBlock, 0, s>>>(n, rp, ci, va, x, y, nblocks)));
+void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s) {
+    PFEM_DISPATCH_L(lanes, (k_spmv<T, LL><<<nblocks, kBlock, 0, s>>>(A, x, y)));
 }
 
 void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s) {
@@ -310,15 +464,13 @@ void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream
     k_ir_update<<<nt, kBlock, 0, s>>>(v, t);
 }
 
-template void launch_kry_init<double>(int, const KrylovView<double>*, const BlockTask*, int, hipStream_t);
-template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockTask*, int, hipStream_t);
-template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, hipStream_t,
+template void launch_kry_init<double>(int, const KrylovView<double>*, const BlockTask*, int, int, int, hipStream_t);
+template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockTask*, int, int, int, hipStream_t);
+template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, int, hipStream_t,
                                        hipEvent_t*);
-template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, hipStream_t,
+template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, int, hipStream_t,
                                       hipEvent_t*);
-template void launch_spmv<double>(int, int, const int*, const int*, const double*, const double*, double*, int,
-                                  hipStream_t);
-template void launch_spmv<float>(int, int, const int*, const int*, const float*, const float*, float*, int,
-                                 hipStream_t);
+template void launch_spmv<double>(int, const NodeMat<double>&, const double*, double*, int, hipStream_t);
+template void launch_spmv<float>(int, const NodeMat<float>&, const float*, float*, int, hipStream_t);
 
 }  // namespace pfem

@@ -12,13 +12,13 @@ void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, h
 void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 
 template <typename T>
-void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int ncases, int par,
+                     hipStream_t s);
 template <typename T>
-void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, hipStream_t s,
-                      hipEvent_t* ev = nullptr);
+void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, int par0,
+                      hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
-void launch_spmv(int lanes, int n, const int* rp, const int* ci, const T* va, const T* x, T* y, int nblocks,
-                 hipStream_t s);
+void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s);
 void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 
@@ -32,5 +32,9 @@ void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_
 template <typename S>
 void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, const BlockTask* elem3, int n_elem3,
                    hipStream_t s);
+
+void launch_stream_read(const double* a, long long n, double* out, int nblocks, hipStream_t s);
+void launch_copy(const double* a, double* b, long long n, int nblocks, hipStream_t s);
+void launch_xcd_probe(int* out, int nblocks, hipStream_t s);
 
 }  // namespace pfem

@@ -190,40 +190,57 @@ __global__ void k_ranks(int E, const int* __restrict__ soa, const int* __restric
     }
 }
 
-__device__ __forceinline__ bool is_dirichlet(uint8_t f) {
-    return (f & NODE_BOUND) && (f & NODE_DIRICHLET) && !(f & NODE_FREE);
+__device__ __forceinline__ uint32_t rank_at(const uint4 r, int q) {
+    const uint32_t w = (q < 4) ? r.x : (q < 8) ? r.y : (q < 12) ? r.z : r.w;
+    return (w >> (8 * (q & 3))) & 0xffu;
 }
 
-// Entries (row, col) with col a velocity DOF of a Dirichlet node and row != col.
-template <bool FILL>
-__global__ void k_dir(int N, int dim, int mom_nnz, const int4* __restrict__ info, const int* __restrict__ nbr_ptr,
-                      const int* __restrict__ nbr_idx, int* __restrict__ count, const int* __restrict__ offset,
-                      int4* __restrict__ out) {
-    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < N; nd += gridDim.x * blockDim.x) {
-        const int4 inf_n = info[nd];
-        if (!is_dirichlet((uint8_t)inf_n.w)) {
-            if (!FILL) count[nd] = 0;
-            continue;
+// Pair-element incidence: count, fill (cursor order) and per-list sort by element index.
+template <int NE>
+__global__ void k_pei_count(int E, const int* __restrict__ soa, const uint4* __restrict__ ranks,
+                            const int* __restrict__ nbr_ptr, int* __restrict__ cnt) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        const uint4 rk = ranks[e];
+#pragma unroll
+        for (int a = 0; a < NE; ++a) {
+            const int base = nbr_ptr[soa[(long long)a * E + e]];
+#pragma unroll
+            for (int b = 0; b < NE; ++b) atomicAdd(&cnt[base + rank_at(rk, a * NE + b)], 1);
         }
-        int c = 0;
-        long long o = FILL ? offset[nd] : 0;
-        for (int q = nbr_ptr[nd]; q < nbr_ptr[nd + 1]; ++q) {
-            const int j = nbr_idx[q];
-            const int4 inf = info[j];
-            const int rk = find_rank(nbr_idx, nbr_ptr[j], inf.z, nd);
-            for (int s = 0; s <= dim; ++s) {
-                if (!row_full(inf, s, dim)) continue;
-                const long long rs = row_start(inf, s, dim, mom_nnz);
-                const int row = j + s * N;
-                for (int d = 0; d < dim; ++d) {
-                    const int col = nd + d * N;
-                    if (row == col) continue;
-                    if (FILL) out[o + c] = make_int4((int)(rs + (long long)d * inf.z + rk), row, col, 0);
-                    ++c;
-                }
+    }
+}
+
+template <int NE>
+__global__ void k_pei_fill(int E, const int* __restrict__ soa, const uint4* __restrict__ ranks,
+                           const int* __restrict__ nbr_ptr, const int* __restrict__ pei_ptr, int* __restrict__ cursor,
+                           uint32_t* __restrict__ pei) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x) {
+        const uint4 rk = ranks[e];
+#pragma unroll
+        for (int a = 0; a < NE; ++a) {
+            const int base = nbr_ptr[soa[(long long)a * E + e]];
+#pragma unroll
+            for (int b = 0; b < NE; ++b) {
+                const int m = base + (int)rank_at(rk, a * NE + b);
+                const int pos = pei_ptr[m] + atomicAdd(&cursor[m], 1);
+                pei[pos] = ((uint32_t)e << 4) | (uint32_t)(a << 2) | (uint32_t)b;
             }
         }
-        if (!FILL) count[nd] = c;
+    }
+}
+
+__global__ void k_pei_sort(int M, const int* __restrict__ pei_ptr, uint32_t* __restrict__ pei) {
+    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
+        const int beg = pei_ptr[m], end = pei_ptr[m + 1];
+        for (int q = beg + 1; q < end; ++q) {  // insertion sort (lists hold at most a few dozen entries)
+            const uint32_t v = pei[q];
+            int r = q - 1;
+            while (r >= beg && pei[r] > v) {
+                pei[r + 1] = pei[r];
+                --r;
+            }
+            pei[r + 1] = v;
+        }
     }
 }
 
@@ -372,23 +389,34 @@ void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* fl
         k_ranks<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.nbr_ptr.as<int>(),
                                                             pat.nbr_idx.as<int>(), pat.ranks.as<uint8_t>(),
                                                             err.as<int>());
-    // Dirichlet column entries: count, scan, fill
-    k_dir<false><<<grid_for(N, B, 65536), B, 0, stream>>>(N, dim, pat.mom_nnz, pat.node_info.as<int4>(),
-                                                          pat.nbr_ptr.as<int>(), pat.nbr_idx.as<int>(),
-                                                          len_m.as<int>(), nullptr, nullptr);
-    PFEM_HIP(hipMemsetAsync(len_m.as<int>() + N, 0, sizeof(int), stream));
-    exclusive_sum(len_m.as<int>(), ptr_m.as<int>(), N + 1, stream, tmp);
-    int ndir = 0;
-    PFEM_HIP(hipMemcpyAsync(&ndir, ptr_m.as<int>() + N, sizeof(int), hipMemcpyDeviceToHost, stream));
     PFEM_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     PFEM_HIP(hipStreamSynchronize(stream));
     PFEM_REQUIRE(herr == 0, E_INVALID, "element slot ranks inconsistent with the node graph");
-    pat.n_dir = ndir;
-    pat.dir.reserve(sizeof(int4) * (size_t)(ndir > 0 ? ndir : 1));
-    if (ndir > 0)
-        k_dir<true><<<grid_for(N, B, 65536), B, 0, stream>>>(N, dim, pat.mom_nnz, pat.node_info.as<int4>(),
-                                                             pat.nbr_ptr.as<int>(), pat.nbr_idx.as<int>(), nullptr,
-                                                             ptr_m.as<int>(), pat.dir.as<int4>());
+    // pair-element incidence (gather assembly)
+    PFEM_REQUIRE((long long)E < (1LL << 28), E_UNSUPPORTED, "more than 2^28 elements");
+    DevBuf cnt, cursor;
+    cnt.reserve(sizeof(int) * (size_t)(M + 1));
+    cursor.reserve(sizeof(int) * (size_t)M);
+    PFEM_HIP(hipMemsetAsync(cnt.p, 0, sizeof(int) * (size_t)(M + 1), stream));
+    PFEM_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int) * (size_t)M, stream));
+    pat.pei_ptr.reserve(sizeof(int) * (size_t)(M + 1));
+    pat.pei.reserve(sizeof(uint32_t) * (size_t)npairs);
+    if (dim == 2)
+        k_pei_count<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+                                                                pat.nbr_ptr.as<int>(), cnt.as<int>());
+    else
+        k_pei_count<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+                                                                pat.nbr_ptr.as<int>(), cnt.as<int>());
+    exclusive_sum(cnt.as<int>(), pat.pei_ptr.as<int>(), M + 1, stream, tmp);
+    if (dim == 2)
+        k_pei_fill<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+                                                               pat.nbr_ptr.as<int>(), pat.pei_ptr.as<int>(),
+                                                               cursor.as<int>(), pat.pei.as<uint32_t>());
+    else
+        k_pei_fill<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+                                                               pat.nbr_ptr.as<int>(), pat.pei_ptr.as<int>(),
+                                                               cursor.as<int>(), pat.pei.as<uint32_t>());
+    k_pei_sort<<<grid_for(M, B, 65536), B, 0, stream>>>(M, pat.pei_ptr.as<int>(), pat.pei.as<uint32_t>());
     PFEM_HIP(hipGetLastError());
     PFEM_HIP(hipStreamSynchronize(stream));
 }

@@ -23,7 +23,6 @@ struct Pattern {
     long long nnz = 0;
     int mom_nnz = 0, cont_nnz = 0;
     int graph_nnz = 0;  // node-graph entries (node pairs sharing an element, incl. self)
-    int n_dir = 0;      // Dirichlet column entries off the diagonal
     int max_deg = 0;
     DevBuf conn;       // int  [ne][E]  element connectivity, structure of arrays
     DevBuf flags;      // u8   [N]      node flags with NODE_FREE derived from connectivity
@@ -34,7 +33,11 @@ struct Pattern {
     DevBuf row_ptr;    // int  [n+1]
     DevBuf col_idx;    // int  [nnz]
     DevBuf diag_pos;   // int  [n]      position of the diagonal entry of each row
-    DevBuf dir;        // int4 [n_dir]  {position, row, column, 0} of Dirichlet column entries
+    // pair-element incidence: for node-graph entry m = (i, j), the elements containing
+    // both nodes, sorted by element index, encoded e << 4 | a << 2 | b with conn[e][a] = i
+    // and conn[e][b] = j
+    DevBuf pei_ptr;    // int  [graph_nnz + 1]
+    DevBuf pei;        // u32  [E * ne * ne]
 };
 
 // Builds every array of `pat` on `stream` (synchronises the stream before returning,

@@ -2,6 +2,7 @@
 // MomContEquationPSPG.inl:262-372): position update from the saved nodes, the Ax_f
 // residual ||A q - b||_2 and the U / U_P relative-change residuals, batched over cases.
 #include "common.hpp"
+#include "nodespmv.hpp"
 #include "views.hpp"
 
 namespace pfem {
@@ -24,20 +25,17 @@ __global__ __launch_bounds__(kBlock) void k_update_pos(const PicardView* __restr
 template <int L>
 __global__ __launch_bounds__(kBlock) void k_res_axf(const PicardView* __restrict__ views,
                                                     const BlockTask* __restrict__ tasks) {
-    constexpr int RPB = kBlock / L;
+    constexpr int NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const PicardView& V = views[task.case_id];
     const int lane = threadIdx.x % L;
     double part[1] = {0.0};
-    for (long long row = (long long)task.block * RPB + threadIdx.x / L; row < V.n; row += (long long)task.nblocks * RPB) {
-        const int beg = V.row_ptr[row], end = V.row_ptr[row + 1];
-        double acc = 0;
-        for (int k = beg + lane; k < end; k += L) acc += V.vals[k] * V.x[V.col_idx[k]];
-#pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
-        if (lane == 0) {
-            const double r = acc - V.b[row];
+    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < V.N; i += (long long)task.nblocks * NPB) {
+        double ax[kMaxNf];
+        node_spmv<double, L>(V.A, V.x, i, lane, ax);
+        if (lane < V.A.nf) {
+            const double r = pick(ax, lane) - V.b[i + (long long)lane * V.N];
             part[0] += r * r;
         }
     }
@@ -76,9 +74,7 @@ void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, 
     switch (lanes) {
         case 4: k_res_axf<4><<<nt, kBlock, 0, s>>>(v, t); break;
         case 8: k_res_axf<8><<<nt, kBlock, 0, s>>>(v, t); break;
-        case 16: k_res_axf<16><<<nt, kBlock, 0, s>>>(v, t); break;
-        case 32: k_res_axf<32><<<nt, kBlock, 0, s>>>(v, t); break;
-        default: k_res_axf<64><<<nt, kBlock, 0, s>>>(v, t); break;
+        default: k_res_axf<16><<<nt, kBlock, 0, s>>>(v, t); break;
     }
 }
 

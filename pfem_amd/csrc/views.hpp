@@ -10,48 +10,72 @@ namespace pfem {
 // PSPG assembly + boundary conditions (MomContEquationPSPG.inl:7-235).
 struct PspgView {
     int dim, N, E, mom_nnz;
-    int n_dir, pad0;
     const int* conn;          // [ne][E]
     const double* pos;        // [dim][N]
     const double* q_prev;     // [(dim+1)N]  states at the start of the time step (be, free-node rhs)
     const double* q_iter;     // [(dim+1)N]  current Picard iterate (node states used by tau)
     const int4* node_info;    // [N] {mom_ptr, cont_ptr, deg, flags}
-    const uint4* ranks;       // [E] packed 8-bit slot ranks
-    const int4* dir;          // [n_dir] {position, row, column, 0}
+    const int* nbr_ptr;       // [N+1] node graph
+    const int* nbr_idx;       // [graph_nnz]
+    const int* pei_ptr;       // [graph_nnz+1] pair-element incidence
+    const uint32_t* pei;      // [E*ne*ne]  e << 4 | a << 2 | b, sorted by e within each pair
     const double* dir_vals;   // [dim][N] Dirichlet velocities (read at Dirichlet nodes)
     double* vals;             // [nnz]
     double* b;                // [(dim+1)N]
     float* vals32;            // [nnz] FP32 copy for the mixed-precision inner solve (may be null)
     const int* diag_pos;      // [n]
-    double* dinv;             // [n] Jacobi inverse diagonal (FP64)
-    float* dinv32;            // [n] (may be null)
+    double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
+    float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
     double rho, mu, dt;
     double g[3];
 };
 
-// Scalars of one BiCGSTAB solve, finalised by the last workgroup of each reduction.
+// The assembled PSPG matrix seen through its node structure.  Values are the CSR values
+// (pattern.hpp layout); the column structure is read from the node graph instead of the
+// CSR column indices: the full rows (i, s) of node i all hold, for t = 0..nf-1, the
+// columns nbr_k + t*N of i's deg neighbours at offset t*deg + k, and every other row
+// holds only its diagonal.
+template <typename T>
+struct NodeMat {
+    int N, nf, mom_nnz, pad;
+    const int4* info;    // [N] {mom_ptr, cont_ptr, deg, flags}
+    const int* nbr_ptr;  // [N+1]
+    const int* nbr_idx;  // [graph_nnz]
+    const T* vals;       // [nnz]
+};
+
+// Scalars of one BiCGSTAB solve.  The per-iteration scalars live in two slots used
+// alternately (the host passes the slot parity of each iteration): a kernel only writes
+// the current slot and only reads the previous one or fields written by earlier kernels,
+// so no workgroup ever reads a value another workgroup of the same launch is writing.
+struct KSlot {
+    double rho, alpha, omega, pad;
+};
 struct KScal {
-    double rho, alpha, omega, beta;
-    double rr, bb, ss, rv;
+    KSlot slot[2];
+    double rr;    // last ||r||^2
+    double bb;    // ||b||^2
     double tol2;  // absolute squared tolerance on ||r||^2
     double rtol;
     int done;     // 0 running, 1 converged, 2 breakdown or non-finite, 3 iteration limit
     int iters;
     int maxit;
-    int s_conv;
     int req_init;  // set by the host: the next init launch (re)starts this case
-    int pad;
 };
 
+// Slots of the per-workgroup partial sums (kPartStride doubles per workgroup).
+enum : int { P_RHR = 0, P_RR = 1, P_RV = 2, P_SS = 3, P_TS = 4, P_TT = 5, P_BB = 6, kPartStride = 8 };
+
 // BiCGSTAB workspace of one case; T is the inner arithmetic type (double or float).
+// Rows are n = nf * nodes (field-blocked); workgroups are assigned XCD-local node ranges.
 template <typename T>
 struct KrylovView {
     int n, x_zero;
-    const int* row_ptr;
-    const int* col_idx;
-    const T* vals;
-    const T* dinv;
+    int nodes, nf;
+    int nblk, pad;  // workgroups of this case in the Krylov launches (partials producers)
+    NodeMat<T> A;
+    const T* binv;  // [nf*nf][nodes] inverse of each node's diagonal (nf x nf) block, SoA
     const T* b;
     T* x;
     T* r;
@@ -62,7 +86,7 @@ struct KrylovView {
     T* t;
     T* ph;
     T* sh;
-    double* partials;    // [kMaxCaseBlocks * 4]
+    double* partials;    // [kMaxCaseBlocks * kPartStride]
     unsigned* counters;  // [8]
     KScal* sc;
 };
@@ -70,9 +94,7 @@ struct KrylovView {
 // Mixed-precision iterative refinement (FP64 outer residual, FP32 inner solve).
 struct RefineView {
     int n, pad;
-    const int* row_ptr;
-    const int* col_idx;
-    const double* vals;  // FP64 system
+    NodeMat<double> A;   // FP64 system
     const double* b;
     double* x;           // FP64 solution
     double* r;           // FP64 residual
@@ -116,9 +138,7 @@ struct PicardView {
     const double* pos_save;
     const double* x;       // new iterate q
     const double* x_prev;  // previous iterate (U / U_P residuals)
-    const int* row_ptr;
-    const int* col_idx;
-    const double* vals;
+    NodeMat<double> A;
     const double* b;
     double* partials;
     unsigned* counters;

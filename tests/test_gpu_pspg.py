@@ -190,3 +190,22 @@ def test_invalid_arguments_raise(ctx):
     c = pfem_amd.Case(ctx, 2)
     with pytest.raises(pfem_amd.PfemError):
         c.set_mesh(bad)
+
+
+def test_assembly_is_deterministic(ctx):
+    """The gather assembly has no atomics: two cases with the same inputs give bitwise-identical
+    A and b, and re-assembling gives the same bits again."""
+    mesh = small_meshes()["3d_column"]
+    q = rand_state(mesh, 11)
+    rng = np.random.default_rng(12)
+    dv = rng.normal(size=3 * mesh.n_nodes)
+    out = []
+    for _ in range(2):
+        c = make_case(ctx, mesh)
+        c.set_dirichlet(dv)
+        c.assemble(5e-4, q_prev=q, q_iter=q)
+        c.assemble(5e-4, q_prev=q, q_iter=q)
+        out.append((c.csr()[2].copy(), c.rhs().copy()))
+        c.close()
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
