@@ -4,7 +4,7 @@
 Default workload: the 3D incompressible PSPG dam break run as a plain-FP64 drop-in
 (0.5 x 0.5 x 1.0 m column, h = 0.01 m, 262,701 nodes / 1,500,000 tets, 4 dofs per node,
 rho = 1000, mu = 1e-3, g = (0, 0, -9.81), dt = 5e-4, Picard minRes 1e-6 on the Ax_f
-residual, BiCGSTAB + Jacobi to a true relative residual of 1e-9).  One step is one
+residual, BiCGSTAB + block Jacobi to a true relative residual of 1e-9).  One step is one
 Picard time step: assembly + BC, Krylov solves, position updates and re-assemblies,
 all resident in HBM.
 

@@ -2,8 +2,9 @@
 against the CPU oracle, through the C-ABI.
 
 Bars: CSR pattern (row_ptr, col_idx) bit-exact; assembled values within 1e-12 of the
-oracle relative to each row's largest magnitude (the GPU sums element contributions with
-FP64 atomics, so the order of the duplicate sums differs from Eigen's triplet order);
+oracle relative to each row's largest magnitude (the GPU gathers each entry's element
+contributions in element order, Eigen's triplet order, but evaluates the element terms
+with its own operation order);
 b within 1e-12 relative to its field block's largest magnitude; solutions to the stated
 relative residual on the oracle's own system.
 """
