@@ -84,7 +84,9 @@ typedef struct pfem_case_desc {
     /* linear solver (replaces EigenSparseSolver, MomContEquation.hpp:6-12) */
     int32_t precision;  /* PFEM_PREC_* */
     int32_t krylov_max_iter;
-    double krylov_rtol; /* stop when ||b - A x|| <= rtol ||b|| (true residual, FP64) */
+    double krylov_rtol; /* stop when ||b - A x|| <= rtol ||b|| (true residual, FP64); inside a
+                           Picard step with the Ax_f residual also ||b - A x|| <= 0.1 min_res,
+                           so that the inexact solve never decides the absolute Picard test */
     double inner_rtol;  /* mixed precision: relative tolerance of each FP32 inner solve */
     int32_t inner_max_iter; /* mixed precision: iteration cap of each FP32 inner solve */
     int32_t pad0;
@@ -108,11 +110,14 @@ typedef struct pfem_step_stats {
     double t_assemble, t_solve, t_update, t_residual; /* seconds (host wall clock) */
 } pfem_step_stats;
 
-/* ---- context ------------------------------------------------------------------------ */
+/* ---- context ------------------------------------------------------------------------
+ * One context per device; it owns the stream of pfem_batch_step. */
 int pfem_ctx_create(int32_t device, pfem_ctx** out);
 int pfem_ctx_destroy(pfem_ctx* ctx);
 const char* pfem_last_error(void);
 int pfem_version(void);
+/* Monotonic clock (nanoseconds) of the serving timestamps. */
+int64_t pfem_now_ns(void);
 
 /* ---- case lifetime -------------------------------------------------------------------
  * pfem_case_create replaces the equation constructors (MomContEquation.inl:7-256;
@@ -167,12 +172,55 @@ int pfem_wc_step(pfem_case* c, double dt, pfem_step_stats* stats);
 int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* dt_out);
 
 /* ---- batched serving (no counterpart in the reference: many independent cases) -------
- * Runs one time step of every listed case as one batch: the cases' assembly, BC and
- * Krylov phases run as shared launches over a device work table (hipGraph-captured
- * BiCGSTAB chunks).  Cases must have distinct handles on the same context.  status[i]
- * receives each case's step status; stats may be NULL. */
+ * Runs one time step of every listed case as one batch: each operation of the cases'
+ * steps (assembly, BC, BiCGSTAB iterations, residuals, WC passes) is one launch over a
+ * device work table of all cases that need it.  Cases must be distinct handles on the
+ * same context.  status[i] receives each case's step status; stats may be NULL.  Each
+ * case's result is bitwise the same as stepping it alone (PSPG). */
 int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, const double* dts, int32_t* status,
                     pfem_step_stats* stats);
+
+/* Continuous-batching server: a request is one time step of one case (a PSPG Picard step,
+ * or a WC step followed by computeNextDT).  Requests are admitted in arrival order when an
+ * engine slot is free and the case's step workspace (matrix values, rhs, preconditioner,
+ * Picard iterates, Krylov vectors: pfem_case_workspace_bytes) fits in the server's HBM
+ * arena; the mesh, pattern and state stay resident with the case.  A request that
+ * arrives while others run joins the running batch at the next engine tick.  The server
+ * runs on its own host thread and HIP stream; a case with a pending request must not be
+ * used by any other call until its result has been polled. */
+typedef struct pfem_server pfem_server;
+
+typedef struct pfem_server_desc {
+    int64_t hbm_budget;       /* bytes of the workspace arena; 0: half of the free device memory */
+    int32_t max_inflight;     /* requests in flight at once (engine slots), 1..4096 */
+    int32_t max_chunk;        /* BiCGSTAB iterations per engine tick at most (0: 64) */
+    double wc_security_coeff; /* CFL securityCoeff of the WC requests' next time step */
+    double wc_max_dt;         /* maxDT of the WC requests' next time step */
+} pfem_server_desc;
+
+typedef struct pfem_server_result {
+    int64_t tag;
+    int32_t status; /* the step's status (PFEM_OK, PFEM_NOT_CONVERGED, PFEM_NAN, < 0 errors) */
+    int32_t kind;   /* PFEM_KIND_* */
+    pfem_step_stats stats;
+    double next_dt; /* WC: computeNextDT after the step */
+    int64_t t_arrival_ns, t_admit_ns, t_done_ns; /* pfem_now_ns clock */
+} pfem_server_result;
+
+int pfem_server_create(pfem_ctx* ctx, const pfem_server_desc* desc, pfem_server** out);
+/* Waits for every submitted request, then frees the server (unpolled results are dropped). */
+int pfem_server_destroy(pfem_server* s);
+/* Queues one step of `c` with time step dt.  arrival_ns (pfem_now_ns clock; <= 0: now) is
+ * the request's arrival: it is not admitted before then and its latency counts from it.
+ * PFEM_E_OOM when the case's workspace exceeds the arena, PFEM_E_STATE when the case
+ * already has a pending request. */
+int pfem_server_submit(pfem_server* s, pfem_case* c, double dt, int64_t tag, int64_t arrival_ns);
+/* Moves up to max completed results into out (*n_out of them), waiting up to timeout_ms
+ * (< 0: until one is available) when none is ready. */
+int pfem_server_poll(pfem_server* s, pfem_server_result* out, int32_t max, int32_t timeout_ms, int32_t* n_out);
+/* {engine ticks, peak requests in flight, arena bytes, arena peak bytes, submitted,
+ *  completed, arena bytes in use, requests not yet arrived} */
+int pfem_server_stats(pfem_server* s, int64_t* out8);
 
 /* Instrumentation (no counterpart in the reference): times, with HIP events on the case's
  * stream, the first SpMV launch of every BiCGSTAB iteration of single-case solves.
@@ -189,8 +237,11 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8);
 /* Instrumentation: XCD (XCC) id on which each of nblocks single-wavefront workgroups ran. */
 int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
 
-/* Device footprint of a case in bytes (admission control by HBM footprint). */
+/* Device footprint of a case in bytes: the resident mesh, pattern and state plus the
+ * step workspace. */
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);
+/* Bytes of the step workspace alone (what a serving request borrows from the arena). */
+int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
 
 #ifdef __cplusplus
 }

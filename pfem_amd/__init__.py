@@ -9,7 +9,8 @@ reference's equation/solver interface for this path (ImperatorS79/PFEM):
 * ``WCompNewtonStep`` — the weakly compressible CDS_dpdt step
   (``WCompNewton/Solver.cpp:236-276``) and ``computeNextDT`` (``:192-234``).
 * ``Case`` exposes the building blocks (assemble, solve, CSR export) used by the parity
-  tests, and ``batch_step`` runs many independent cases as one batch.
+  tests, ``batch_step`` runs many independent cases as one batch, and ``Server`` is the
+  continuous-batching request loop (one request = one time step of one case).
 """
 from __future__ import annotations
 
@@ -21,8 +22,8 @@ from . import _lib
 from ._lib import PfemError, check, lib
 from .meshgen import BOUND, DIRICHLET, FIXED, FREE, ONFS, PfemMesh
 
-__all__ = ["Context", "Case", "MomContEqIncompNewton", "WCompNewtonStep", "batch_step", "PfemError", "PfemMesh",
-           "BOUND", "FREE", "ONFS", "FIXED", "DIRICHLET"]
+__all__ = ["Context", "Case", "MomContEqIncompNewton", "WCompNewtonStep", "batch_step", "Server", "now_ns",
+           "PfemError", "PfemMesh", "BOUND", "FREE", "ONFS", "FIXED", "DIRICHLET"]
 
 
 def _dp(a):
@@ -148,6 +149,11 @@ class Case:
         check(lib().pfem_case_footprint(self._h, C.byref(b)))
         return b.value
 
+    def workspace_bytes(self) -> int:
+        b = C.c_int64(0)
+        check(lib().pfem_case_workspace_bytes(self._h, C.byref(b)))
+        return b.value
+
     # ---- PSPG building blocks -------------------------------------------------------
     def assemble(self, dt, q_prev=None, q_iter=None):
         qp = None if q_prev is None else np.ascontiguousarray(q_prev, dtype=np.float64)
@@ -249,3 +255,50 @@ def batch_step(ctx: Context, cases, dts):
     stats = (_lib.StepStats * n)()
     check(lib().pfem_batch_step(ctx._h, n, hs, _dp(d), _ip(status), stats))
     return status, [s.as_dict() for s in stats]
+
+
+def now_ns() -> int:
+    """The serving clock (pfem_now_ns, monotonic nanoseconds)."""
+    return int(lib().pfem_now_ns())
+
+
+class Server:
+    """Continuous-batching request loop (pfem_server_*): one request is one time step of one
+    case; results come back through ``poll``."""
+
+    def __init__(self, ctx: Context, hbm_budget: int = 0, max_inflight: int = 64, max_chunk: int = 0,
+                 wc_security_coeff: float = 0.5, wc_max_dt: float = 1.0):
+        d = _lib.ServerDesc()
+        d.hbm_budget, d.max_inflight, d.max_chunk = int(hbm_budget), int(max_inflight), int(max_chunk)
+        d.wc_security_coeff, d.wc_max_dt = wc_security_coeff, wc_max_dt
+        self._h = C.c_void_p()
+        check(lib().pfem_server_create(ctx._h, C.byref(d), C.byref(self._h)))
+        self._buf = (_lib.ServerResult * 256)()
+
+    def submit(self, case: Case, dt: float, tag: int, arrival_ns: int = 0):
+        check(lib().pfem_server_submit(self._h, case._h, dt, int(tag), int(arrival_ns)))
+
+    def poll(self, max_results: int = 256, timeout_ms: int = 0):
+        """Completed results (dicts with tag, status, stats, next_dt and the timestamps)."""
+        n = C.c_int32(0)
+        m = min(max_results, len(self._buf))
+        check(lib().pfem_server_poll(self._h, self._buf, m, int(timeout_ms), C.byref(n)))
+        return [self._buf[i].as_dict() for i in range(n.value)]
+
+    def stats(self):
+        out = (C.c_int64 * 8)()
+        check(lib().pfem_server_stats(self._h, out))
+        keys = ["ticks", "peak_inflight", "arena_bytes", "arena_peak", "submitted", "completed", "arena_used",
+                "not_arrived"]
+        return dict(zip(keys, [int(v) for v in out]))
+
+    def close(self):
+        if self._h:
+            lib().pfem_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

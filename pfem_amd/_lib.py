@@ -34,7 +34,8 @@ EXPORTS = [
     "pfem_case_set_positions", "pfem_case_get_positions", "pfem_case_set_dirichlet", "pfem_case_sync",
     "pfem_pspg_assemble", "pfem_pspg_solve", "pfem_get_csr_sizes", "pfem_get_csr", "pfem_get_rhs", "pfem_spmv",
     "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
-    "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd",
+    "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd", "pfem_now_ns", "pfem_case_workspace_bytes",
+    "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
 ]
 
 
@@ -64,6 +65,22 @@ class StepStats(C.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class ServerDesc(C.Structure):
+    _fields_ = [("hbm_budget", C.c_int64), ("max_inflight", C.c_int32), ("max_chunk", C.c_int32),
+                ("wc_security_coeff", C.c_double), ("wc_max_dt", C.c_double)]
+
+
+class ServerResult(C.Structure):
+    _fields_ = [("tag", C.c_int64), ("status", C.c_int32), ("kind", C.c_int32), ("stats", StepStats),
+                ("next_dt", C.c_double), ("t_arrival_ns", C.c_int64), ("t_admit_ns", C.c_int64),
+                ("t_done_ns", C.c_int64)]
+
+    def as_dict(self):
+        d = {k: getattr(self, k) for k, _ in self._fields_ if k != "stats"}
+        d["stats"] = self.stats.as_dict()
+        return d
 
 
 _lib = None
@@ -110,12 +127,20 @@ def lib():
         "pfem_profile_spmv": [_vp, C.c_int32, _dp],
         "pfem_bench_kernels": [_vp, C.c_int32, _dp],
         "pfem_probe_xcd": [_vp, C.c_int32, _ip],
+        "pfem_now_ns": [],
+        "pfem_case_workspace_bytes": [_vp, C.POINTER(C.c_int64)],
+        "pfem_server_create": [_vp, C.POINTER(ServerDesc), C.POINTER(_vp)],
+        "pfem_server_destroy": [_vp],
+        "pfem_server_submit": [_vp, _vp, C.c_double, C.c_int64, C.c_int64],
+        "pfem_server_poll": [_vp, C.POINTER(ServerResult), C.c_int32, C.c_int32, _ip],
+        "pfem_server_stats": [_vp, C.POINTER(C.c_int64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)
         fn.argtypes = args
         fn.restype = C.c_int
     L.pfem_last_error.restype = C.c_char_p
+    L.pfem_now_ns.restype = C.c_int64
     _lib = L
     return L
 

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfem_gpu.so")
-SOURCES = ["pattern.hip", "pspg.hip", "krylov.hip", "picard.hip", "wc.hip", "microbench.hip", "capi.cpp"]
+SOURCES = ["pattern.hip", "pspg.hip", "krylov.hip", "picard.hip", "wc.hip", "microbench.hip", "engine.cpp", "capi.cpp"]
 ARCH = os.environ.get("PFEM_OFFLOAD_ARCH", "gfx950")
 
 CXXFLAGS = [
@@ -41,7 +41,7 @@ def build(verbose: bool = False) -> str:
     with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = LIB + ".tmp"
-    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp, *objs]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stdout}\n{r.stderr}")

@@ -1,0 +1,997 @@
+// Tick engine: the batched driver of every PSPG and weakly compressible request.
+//
+// The PSPG step restates PicardAlgo::solve (nonLinearAlgo/PicardAlgo.cpp:31-95) with the
+// PSPG callbacks of MomContEquationPSPG.inl:262-372 as a per-request state machine:
+//   prepare (save nodes, q = qPrev) -> assemble -> [ solve -> update positions ->
+//   re-assemble -> residual -> test ]*  -> commit, or restore on failure.
+// The linear solve replaces SparseLU (MomContEquationPSPG.inl:281-290) with BiCGSTAB +
+// block Jacobi in FP64, or FP64 iterative refinement around FP32 BiCGSTAB.  Every
+// decision is taken on the host from the slot scalars read back at the end of a tick;
+// every operation is one launch over all requests that need it in that tick.
+#include "runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+
+namespace pfem {
+
+int64_t now_ns() {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+
+namespace {
+
+constexpr int kElemCap = 8192;  // workgroups per case for element / node passes
+
+// workgroups per case in row launches (<= kMaxCaseBlocks, the partials capacity);
+// PFEM_ROW_BLOCKS overrides it (tuning)
+const int kRowCap = [] {
+    const char* e = std::getenv("PFEM_ROW_BLOCKS");
+    const int v = e ? std::atoi(e) : 0;
+    return (v >= 64 && v <= kMaxCaseBlocks) ? v : 1024;
+}();
+
+size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+int lane_index(int lanes) { return lanes == 4 ? 0 : (lanes == 8 ? 1 : 2); }
+constexpr int kLanes[3] = {4, 8, 16};
+
+enum : unsigned {
+    OP_COPY = 1u << 0,
+    OP_UPD = 1u << 1,
+    OP_ASM = 1u << 2,
+    OP_RES = 1u << 3,
+    OP_IRU = 1u << 4,
+    OP_IRR = 1u << 5,
+    OP_K64_INIT = 1u << 6,
+    OP_K64_RUN = 1u << 7,
+    OP_K32_INIT = 1u << 8,
+    OP_K32_RUN = 1u << 9,
+    OP_WC = 1u << 10,
+    OP_CFL = 1u << 11,
+};
+
+enum : int { W_NONE = 0, W_K64 = 1, W_K32 = 2, W_IR = 3, W_POST = 4 };
+
+enum Tab : int {
+    T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
+    T_RES, T_IRR = T_RES + 3, T_K64 = T_IRR + 3, T_K32 = T_K64 + 3, T_F64 = T_K32 + 3, T_F32 = T_F64 + 3,
+    T_WCN = T_F32 + 3, T_WCE2 = T_WCN + 2, T_WCE3 = T_WCE2 + 2, T_CFE2 = T_WCE3 + 2, T_CFE3 = T_CFE2 + 2,
+    T_NTAB = T_CFE3 + 2
+};
+
+void append(std::vector<BlockTask>& t, int slot, long long work, int cap, int pad = 0) {
+    long long nb = (work + kBlock - 1) / kBlock;
+    nb = std::max<long long>(1, std::min<long long>(nb, cap));
+    for (int b = 0; b < nb; ++b) t.push_back(BlockTask{slot, b, (int)nb, pad});
+}
+
+void append_rows(std::vector<BlockTask>& t, int slot, long long n) {
+    const int nb = row_blocks(n);
+    for (int b = 0; b < nb; ++b) t.push_back(BlockTask{slot, b, nb, 0});
+}
+
+template <class T>
+NodeMat<T> node_mat(const Pattern& P, int dim, const T* vals) {
+    NodeMat<T> A{};
+    A.N = P.N;
+    A.nf = dim + 1;
+    A.mom_nnz = P.mom_nnz;
+    A.info = P.node_info.as<int4>();
+    A.nbr_ptr = P.nbr_ptr.as<int>();
+    A.nbr_idx = P.nbr_idx.as<int>();
+    A.vals = vals;
+    return A;
+}
+
+template <class T>
+void pinned_alloc(T*& p, size_t count) {
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    PFEM_HIP(hipHostMalloc((void**)&p, sizeof(T) * std::max<size_t>(count, 1), hipHostMallocDefault));
+    std::memset((void*)p, 0, sizeof(T) * std::max<size_t>(count, 1));
+}
+
+int pow2ceil(int v) {
+    int p = 1;
+    while (p < v && p < (1 << 20)) p <<= 1;
+    return p;
+}
+
+}  // namespace
+
+int row_blocks(long long n) {
+    return (int)std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, kRowCap));
+}
+
+// lanes per field of the node-block SpMV: the node-graph degree (incl. the node itself)
+// is ~7 for 2D Kuhn triangulations and ~15 for 3D Kuhn tetrahedralisations
+int pick_lanes(double avg_deg) {
+    // PFEM_SPMV_LANES (4, 8 or 16) overrides the choice (tuning)
+    static const int forced = [] {
+        const char* e = std::getenv("PFEM_SPMV_LANES");
+        const int v = e ? std::atoi(e) : 0;
+        return (v == 4 || v == 8 || v == 16) ? v : 0;
+    }();
+    if (forced) return forced;
+    if (avg_deg > 8) return 16;
+    if (avg_deg > 4) return 8;
+    return 4;
+}
+
+// ---------------------------------------------------------------------------------------
+// Workspace and arena
+// ---------------------------------------------------------------------------------------
+namespace {
+struct WsLayout {
+    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals32, binv32, kv32, total;
+};
+WsLayout ws_layout(const pfem_case* c) {
+    WsLayout L{};
+    const size_t n = (size_t)c->n, nnz = (size_t)c->pat.nnz, nf = (size_t)c->dim + 1, N = (size_t)c->N;
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        const size_t at = o;
+        o += align256(bytes);
+        return at;
+    };
+    L.vals = take(8 * nnz);
+    L.b = take(8 * n);
+    L.binv = take(8 * nf * nf * N);
+    L.x = take(8 * n);
+    L.x_prev = take(8 * n);
+    L.pos_save = take(8 * (size_t)c->dim * N);
+    L.kv64 = take(8 * 8 * n);
+    if (c->d.precision == PFEM_PREC_MIXED) {
+        L.vals32 = take(4 * nnz);
+        L.binv32 = take(4 * nf * nf * N);
+        L.kv32 = take(4 * 10 * n);
+    }
+    L.total = o;
+    return L;
+}
+}  // namespace
+
+size_t workspace_bytes(const pfem_case* c) {
+    if (c->d.kind != PFEM_KIND_PSPG || !c->has_mesh) return 0;
+    return ws_layout(c).total;
+}
+
+void carve_workspace(pfem_case* c, char* base) {
+    const WsLayout L = ws_layout(c);
+    Workspace& w = c->ws;
+    w.base = base;
+    w.bytes = L.total;
+    w.vals = reinterpret_cast<double*>(base + L.vals);
+    w.b = reinterpret_cast<double*>(base + L.b);
+    w.binv = reinterpret_cast<double*>(base + L.binv);
+    w.x = reinterpret_cast<double*>(base + L.x);
+    w.x_prev = reinterpret_cast<double*>(base + L.x_prev);
+    w.pos_save = reinterpret_cast<double*>(base + L.pos_save);
+    w.kv64 = reinterpret_cast<double*>(base + L.kv64);
+    const bool mixed = c->d.precision == PFEM_PREC_MIXED;
+    w.vals32 = mixed ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
+    w.binv32 = mixed ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
+    w.kv32 = mixed ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
+}
+
+void own_workspace(pfem_case* c, hipStream_t s) {
+    if (c->ws.base || c->d.kind != PFEM_KIND_PSPG || !c->has_mesh) return;
+    const size_t wb = workspace_bytes(c);
+    c->ws.own.reserve(wb);
+    carve_workspace(c, c->ws.own.as<char>());
+    PFEM_HIP(hipMemcpyAsync(c->ws.x, c->q_prev.p, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToDevice, s));
+}
+
+void Arena::init(size_t bytes) {
+    bytes = align256(std::max<size_t>(bytes, 256));
+    mem_.reserve(bytes);
+    cap_ = bytes;
+    used_ = peak_ = 0;
+    free_.clear();
+    free_[0] = bytes;
+}
+
+bool Arena::alloc(size_t bytes, size_t& off) {
+    bytes = align256(std::max<size_t>(bytes, 256));
+    for (auto it = free_.begin(); it != free_.end(); ++it) {
+        if (it->second < bytes) continue;
+        off = it->first;
+        const size_t rem = it->second - bytes;
+        free_.erase(it);
+        if (rem) free_[off + bytes] = rem;
+        used_ += bytes;
+        peak_ = std::max(peak_, used_);
+        return true;
+    }
+    return false;
+}
+
+void Arena::release(size_t off, size_t bytes) {
+    bytes = align256(std::max<size_t>(bytes, 256));
+    used_ -= bytes;
+    auto it = free_.emplace(off, bytes).first;
+    auto nx = std::next(it);
+    if (nx != free_.end() && it->first + it->second == nx->first) {
+        it->second += nx->second;
+        free_.erase(nx);
+    }
+    if (it != free_.begin()) {
+        auto pv = std::prev(it);
+        if (pv->first + pv->second == it->first) {
+            pv->second += it->second;
+            free_.erase(it);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------
+// Engine
+// ---------------------------------------------------------------------------------------
+struct Engine::Impl {
+    int device = 0;
+    hipStream_t s = nullptr;
+    int S = 0;
+    Arena* arena = nullptr;
+    DevBuf d_pv, d_pic, d_k64, d_k32, d_ref, d_wc64, d_wc32, d_scal, d_tab;
+    PspgView* h_pv = nullptr;
+    PicardView* h_pic = nullptr;
+    KrylovView<double>* h_k64 = nullptr;
+    KrylovView<float>* h_k32 = nullptr;
+    RefineView* h_ref = nullptr;
+    WcView<double>* h_wc64 = nullptr;
+    WcView<float>* h_wc32 = nullptr;
+    SlotScal* h_scal = nullptr;   // readback
+    SlotScal* h_reset = nullptr;  // staging of host-written slot scalars
+    BlockTask* h_tab = nullptr;
+    size_t tab_cap = 0;
+    std::vector<int> free_slots;
+    int par = 0;
+    bool prof = false;
+    hipEvent_t ev[8] = {};
+    hipEvent_t pev[128] = {};
+
+    ~Impl() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : pev)
+            if (e) (void)hipEventDestroy(e);
+        for (void* p : {(void*)h_pv, (void*)h_pic, (void*)h_k64, (void*)h_k32, (void*)h_ref, (void*)h_wc64,
+                        (void*)h_wc32, (void*)h_scal, (void*)h_reset, (void*)h_tab})
+            if (p) (void)hipHostFree(p);
+    }
+
+    void alloc_slots(int n) {
+        S = n;
+        d_pv.reserve(sizeof(PspgView) * n);
+        d_pic.reserve(sizeof(PicardView) * n);
+        d_k64.reserve(sizeof(KrylovView<double>) * n);
+        d_k32.reserve(sizeof(KrylovView<float>) * n);
+        d_ref.reserve(sizeof(RefineView) * n);
+        d_wc64.reserve(sizeof(WcView<double>) * n);
+        d_wc32.reserve(sizeof(WcView<float>) * n);
+        d_scal.reserve(sizeof(SlotScal) * n);
+        PFEM_HIP(hipMemsetAsync(d_scal.p, 0, sizeof(SlotScal) * n, s));
+        pinned_alloc(h_pv, n);
+        pinned_alloc(h_pic, n);
+        pinned_alloc(h_k64, n);
+        pinned_alloc(h_k32, n);
+        pinned_alloc(h_ref, n);
+        pinned_alloc(h_wc64, n);
+        pinned_alloc(h_wc32, n);
+        pinned_alloc(h_scal, n);
+        pinned_alloc(h_reset, n);
+        PFEM_HIP(hipStreamSynchronize(s));
+        free_slots.clear();
+        for (int i = n - 1; i >= 0; --i) free_slots.push_back(i);
+    }
+
+    SlotScal* dslot(int slot) const { return d_scal.as<SlotScal>() + slot; }
+
+    template <class V>
+    void upload(DevBuf& d, V* h, int slot) {
+        PFEM_HIP(hipMemcpyAsync(d.as<V>() + slot, h + slot, sizeof(V), hipMemcpyHostToDevice, s));
+    }
+
+    void load_views(Req* r);
+    void reset_kscal(int slot, int which, double rtol, int maxit, double atol = 0.0);
+    void restart_kscal(int slot, int which);
+};
+
+Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl) {
+    p_->device = device;
+    p_->s = s;
+    p_->arena = arena;
+    for (auto& e : p_->ev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    p_->alloc_slots(std::max(1, slots));
+}
+
+Engine::~Engine() = default;
+
+void Engine::ensure_slots(int s) {
+    PFEM_REQUIRE(idle(), E_STATE, "engine resized while busy");
+    if (s > p_->S) p_->alloc_slots(s);
+}
+
+void Engine::enable_profiling(bool on) {
+    if (on && !p_->pev[0])  // timing-only events: no system-scope fence on record
+        for (auto& e : p_->pev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    p_->prof = on;
+    if (on) prof_ms[0] = prof_ms[1] = 0, prof_count[0] = prof_count[1] = 0;
+}
+
+void Engine::enqueue(Req* r) { waitq_.push_back(r); }
+
+// Views of the request's case, pointing the scalars at the slot, uploaded to the slot.
+void Engine::Impl::load_views(Req* r) {
+    pfem_case* c = r->c;
+    const int slot = r->slot;
+    const int N = c->N, dim = c->dim;
+    const long long n = c->n;
+    SlotScal* ds = dslot(slot);
+    if (c->d.kind == PFEM_KIND_PSPG) {
+        const Pattern& P = c->pat;
+        const Workspace& w = c->ws;
+        const bool mixed = c->d.precision == PFEM_PREC_MIXED;
+        PspgView& pv = h_pv[slot];
+        pv = PspgView{};
+        pv.dim = dim;
+        pv.N = N;
+        pv.E = c->E;
+        pv.mom_nnz = P.mom_nnz;
+        pv.conn = P.conn.as<int>();
+        pv.pos = c->pos.as<double>();
+        pv.q_prev = c->q_prev.as<double>();
+        pv.q_iter = w.x;
+        pv.node_info = P.node_info.as<int4>();
+        pv.nbr_ptr = P.nbr_ptr.as<int>();
+        pv.nbr_idx = P.nbr_idx.as<int>();
+        pv.pei_ptr = P.pei_ptr.as<int>();
+        pv.pei = P.pei.as<uint32_t>();
+        pv.dir_vals = c->dir_vals.as<double>();
+        pv.vals = w.vals;
+        pv.b = w.b;
+        pv.vals32 = mixed ? w.vals32 : nullptr;
+        pv.diag_pos = P.diag_pos.as<int>();
+        pv.binv = w.binv;
+        pv.binv32 = mixed ? w.binv32 : nullptr;
+        pv.nnz = P.nnz;
+        pv.rho = c->d.rho;
+        pv.mu = c->d.mu;
+        pv.dt = r->dt;
+        for (int k = 0; k < 3; ++k) pv.g[k] = c->d.body_force[k];
+
+        PicardView& pc = h_pic[slot];
+        pc = PicardView{};
+        pc.dim = dim;
+        pc.N = N;
+        pc.n = (int)n;
+        pc.res_kind = c->d.residual;
+        pc.dt = r->dt;
+        pc.flags = P.flags.as<uint8_t>();
+        pc.pos = c->pos.as<double>();
+        pc.pos_save = w.pos_save;
+        pc.q_prev = c->q_prev.as<double>();
+        pc.x = w.x;
+        pc.x_prev = w.x_prev;
+        pc.A = node_mat<double>(P, dim, w.vals);
+        pc.b = w.b;
+        pc.partials = c->partials.as<double>();
+        pc.counters = c->counters.as<unsigned>();
+        pc.out = ds->out;
+
+        KrylovView<double>& k = h_k64[slot];
+        k = KrylovView<double>{};
+        k.n = (int)n;
+        k.x_zero = 0;
+        k.nodes = N;
+        k.nf = dim + 1;
+        k.nblk = row_blocks(n);
+        k.A = node_mat<double>(P, dim, w.vals);
+        k.binv = w.binv;
+        k.b = w.b;
+        k.x = w.x;
+        double* v = w.kv64;
+        k.r = v;
+        k.rh = v + n;
+        k.p = v + 2 * n;
+        k.v = v + 3 * n;
+        k.s = v + 4 * n;
+        k.t = v + 5 * n;
+        k.ph = v + 6 * n;
+        k.sh = v + 7 * n;
+        k.partials = c->partials.as<double>();
+        k.counters = c->counters.as<unsigned>();
+        k.sc = &ds->k[0];
+
+        KrylovView<float>& f = h_k32[slot];
+        f = KrylovView<float>{};
+        RefineView& rv = h_ref[slot];
+        rv = RefineView{};
+        if (mixed) {
+            float* u = w.kv32;
+            f.n = (int)n;
+            f.x_zero = 1;
+            f.nodes = N;
+            f.nf = dim + 1;
+            f.nblk = row_blocks(n);
+            f.A = node_mat<float>(P, dim, w.vals32);
+            f.binv = w.binv32;
+            f.b = u;
+            f.x = u + n;
+            f.r = u + 2 * n;
+            f.rh = u + 3 * n;
+            f.p = u + 4 * n;
+            f.v = u + 5 * n;
+            f.s = u + 6 * n;
+            f.t = u + 7 * n;
+            f.ph = u + 8 * n;
+            f.sh = u + 9 * n;
+            f.partials = c->partials.as<double>();
+            f.counters = c->counters.as<unsigned>();
+            f.sc = &ds->k[1];
+            rv.n = (int)n;
+            rv.A = node_mat<double>(P, dim, w.vals);
+            rv.b = w.b;
+            rv.x = w.x;
+            rv.r = w.kv64;
+            rv.r32 = u;
+            rv.d32 = u + n;
+            rv.partials = c->partials.as<double>();
+            rv.counters = c->counters.as<unsigned>();
+            rv.out = ds->out;
+        }
+        upload(d_pv, h_pv, slot);
+        upload(d_pic, h_pic, slot);
+        upload(d_k64, h_k64, slot);
+        upload(d_k32, h_k32, slot);
+        upload(d_ref, h_ref, slot);
+    } else {
+        auto set = [&](auto& V, auto* base) {
+            using S = std::remove_pointer_t<decltype(base)>;
+            V = WcView<S>{};
+            V.dim = dim;
+            V.N = N;
+            V.E = c->E;
+            V.conn = c->pat.conn.as<int>();
+            V.flags = c->pat.flags.as<uint8_t>();
+            V.pos = c->pos.as<double>();
+            // layout of pfem_case_set_state: v (dim x N), p (N), rho (N), a (dim x N)
+            V.v = base;
+            V.p = base + (size_t)dim * N;
+            V.rho = base + (size_t)(dim + 1) * N;
+            V.a = base + (size_t)(dim + 2) * N;
+            V.dir_vals = c->dir_vals.as<double>();
+            double* acc = c->wacc.as<double>();
+            V.m_cont = acc;
+            V.f_cont = acc + N;
+            V.m_mom = acc + 2 * (size_t)N;
+            V.f_mom = acc + 3 * (size_t)N;
+            V.dt2 = &ds->dt2;
+            V.mu = c->d.mu;
+            V.K0 = c->d.K0;
+            V.K0p = c->d.K0p;
+            V.rho_star = c->d.rho_star;
+            for (int k = 0; k < 3; ++k) V.g[k] = c->d.body_force[k];
+            V.dt = r->dt;
+            V.sc2 = r->sc * r->sc;
+            V.meduri = c->d.meduri;
+        };
+        if (c->d.wc_state == PFEM_STATE_FP32) {
+            set(h_wc32[slot], c->wstate.as<float>());
+            upload(d_wc32, h_wc32, slot);
+        } else {
+            set(h_wc64[slot], c->wstate.as<double>());
+            upload(d_wc64, h_wc64, slot);
+        }
+    }
+}
+
+void Engine::Impl::reset_kscal(int slot, int which, double rtol, int maxit, double atol) {
+    KScal& k = h_reset[slot].k[which];
+    std::memset(&k, 0, sizeof(KScal));
+    k.rtol = rtol;
+    k.atol2 = atol > 0 ? atol * atol : 0.0;
+    k.maxit = maxit;
+    k.req_init = 1;
+    PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
+}
+
+// Restart of a finished solve from its current iterate (the device copy equals the last
+// readback: no kernel writes a finished solve's scalars).
+void Engine::Impl::restart_kscal(int slot, int which) {
+    KScal& k = h_reset[slot].k[which];
+    k = h_scal[slot].k[which];
+    k.done = 0;
+    k.req_init = 1;
+    PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
+}
+
+namespace {
+
+// Absolute cap on the linear residual of a Picard step with the absolute Ax_f residual:
+// ||b - A q|| <= 0.1 minRes, so that the linear-solve error (which a direct solver keeps
+// near machine precision) never decides whether ||A q - b|| <= minRes.
+double step_atol(const Req* r) {
+    const pfem_case_desc& d = r->c->d;
+    return (r->kind == RK_STEP && d.residual == PFEM_RES_AX_F && d.min_res > 0) ? 0.1 * d.min_res : 0.0;
+}
+
+// Operations of a solve start (FP64 BiCGSTAB or the first refinement residual).
+void solve_start(Engine::Impl& I, Req* r) {
+    pfem_case* c = r->c;
+    r->ss = pfem_solve_stats{};
+    r->verifying = false;
+    r->fallback = false;
+    r->restarts = 0;
+    r->kit_prev = -1;
+    if (c->d.precision == PFEM_PREC_MIXED) {
+        r->ref = 0;
+        r->stall = 0;
+        r->best = std::numeric_limits<double>::infinity();
+        r->ops |= OP_IRR;
+        r->wait = W_IR;
+    } else {
+        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
+        r->ops |= OP_K64_INIT | OP_K64_RUN;
+        r->wait = W_K64;
+    }
+}
+
+void finish(Req* r, int status, unsigned copy_ops) {
+    r->status = status;
+    r->final_ = true;
+    if (copy_ops) {
+        r->ops |= OP_COPY;
+        r->copy |= copy_ops;
+    }
+    r->wait = W_NONE;
+}
+
+// Top of the Picard loop (PicardAlgo.cpp:47-65): `while (res > minRes)`; iterCount > maxIter
+// restores the nodes and fails.
+void picard_top(Engine::Impl& I, Req* r) {
+    pfem_case* c = r->c;
+    if (!(r->res > c->d.min_res)) {
+        finish(r, PFEM_OK, COPY_COMMIT);
+    } else if (r->iter > c->d.max_iter) {
+        finish(r, PFEM_NOT_CONVERGED, COPY_RESTORE);
+    } else {
+        r->ops |= OP_COPY;
+        r->copy |= COPY_SAVEIT;
+        solve_start(I, r);
+    }
+}
+
+void solve_done(Engine::Impl& I, Req* r, int status) {
+    (void)I;
+    r->ss.status = status;
+    r->st.krylov_iters += r->ss.krylov_iters;
+    r->st.max_relres = std::max(r->st.max_relres, r->ss.relres);
+    if (r->kind == RK_SOLVE) {
+        finish(r, status, 0);
+        return;
+    }
+    // PSPG solve callback (MomContEquationPSPG.inl:278-313): failure restores the nodes
+    if (status != PFEM_OK) {
+        finish(r, status, COPY_RESTORE);
+        return;
+    }
+    r->ops |= OP_UPD | OP_ASM | OP_RES;
+    r->st.assemblies++;
+    r->wait = W_POST;
+}
+
+// Chunk-size hint from the residual decay between two readbacks.
+int estimate_remaining(Req* r, const KScal& k) {
+    int est = 8;
+    if (r->kit_prev >= 0 && k.iters > r->kit_prev && k.rr > 0 && r->krr_prev > k.rr && k.tol2 > 0) {
+        const double rate = std::log(k.rr / r->krr_prev) / (double)(k.iters - r->kit_prev);
+        const double rem = std::log(k.tol2 / k.rr) / rate;
+        est = (int)std::min(1e6, std::max(1.0, std::ceil(rem)));
+    }
+    r->kit_prev = k.iters;
+    r->krr_prev = k.rr;
+    return est;
+}
+
+}  // namespace
+
+void Engine::abort_all(int code, std::vector<Req*>& done) {
+    for (Req* r : inflight_) waitq_.push_back(r);
+    inflight_.clear();
+    for (Req* r : waitq_) {
+        r->status = code;
+        r->completed = true;
+        r->t_done = now_ns();
+        if (r->arena_ws && p_->arena) {
+            p_->arena->release(r->c->ws.off, r->c->ws.bytes);
+            r->c->ws.clear();
+            r->arena_ws = false;
+        }
+        if (r->slot >= 0) p_->free_slots.push_back(r->slot);
+        r->slot = -1;
+        done.push_back(r);
+    }
+    waitq_.clear();
+}
+
+void Engine::drain(std::vector<Req*>& done) {
+    while (!idle()) tick(done);
+}
+
+void Engine::tick(std::vector<Req*>& done) {
+    Impl& I = *p_;
+    const hipStream_t s = I.s;
+    // ---- admission (FIFO): a free slot, and the step workspace of a PSPG request -------
+    while (!waitq_.empty() && !I.free_slots.empty()) {
+        Req* r = waitq_.front();
+        pfem_case* c = r->c;
+        if (c->d.kind == PFEM_KIND_PSPG && !c->ws.base) {
+            const size_t wb = workspace_bytes(c);
+            if (I.arena) {
+                size_t off = 0;
+                if (!I.arena->alloc(wb, off)) break;  // wait until memory frees up
+                carve_workspace(c, I.arena->base() + off);
+                c->ws.arena = I.arena;
+                c->ws.off = off;
+                r->arena_ws = true;
+            } else {
+                own_workspace(c, s);
+            }
+        }
+        waitq_.pop_front();
+        r->slot = I.free_slots.back();
+        I.free_slots.pop_back();
+        r->t_admit = now_ns();
+        r->ops = 0;
+        r->copy = 0;
+        r->final_ = false;
+        r->iter = 0;
+        r->res = std::numeric_limits<double>::max();
+        r->first_asm = true;
+        r->st = pfem_step_stats{};
+        I.load_views(r);
+        switch (r->kind) {
+            case RK_ASSEMBLE:
+                r->ops = OP_ASM;
+                r->final_ = true;
+                break;
+            case RK_SOLVE:
+                solve_start(I, r);
+                break;
+            case RK_STEP:
+                r->ops = OP_COPY | OP_ASM;
+                r->copy = COPY_PREP;
+                r->st.assemblies = 1;
+                solve_start(I, r);
+                break;
+            case RK_WC:
+                r->ops = (r->wc_step ? OP_WC : 0u) | (r->wc_cfl ? OP_CFL : 0u);
+                r->final_ = true;
+                break;
+        }
+        inflight_.push_back(r);
+    }
+    peak_inflight = std::max(peak_inflight, (int)inflight_.size());
+    if (inflight_.empty()) return;
+    ++ticks;
+
+    // ---- chunk size of this tick's BiCGSTAB iterations --------------------------------
+    int min_hint = 1 << 20, nrun = 0;
+    for (Req* r : inflight_) {
+        if (!(r->ops & (OP_K64_RUN | OP_K32_RUN))) continue;
+        ++nrun;
+        int hint = 8;
+        if (!(r->ops & (OP_K64_INIT | OP_K32_INIT))) {
+            const int w = (r->ops & OP_K64_RUN) ? 0 : 1;
+            hint = estimate_remaining(r, I.h_scal[r->slot].k[w]);
+        } else {
+            r->kit_prev = -1;
+            if (r->verifying) hint = 4;
+        }
+        min_hint = std::min(min_hint, hint);
+    }
+    const int chunk = nrun ? std::max(4, std::min(max_chunk, pow2ceil(min_hint))) : 0;
+
+    // ---- work tables ------------------------------------------------------------------
+    std::vector<BlockTask> tab[T_NTAB];
+    for (Req* r : inflight_) {
+        pfem_case* c = r->c;
+        const int sl = r->slot;
+        const unsigned op = r->ops;
+        const int li = lane_index(c->lanes);
+        if (op & OP_COPY) append(tab[T_COPY], sl, c->N, kElemCap, (int)r->copy);
+        if (op & OP_UPD) append(tab[T_UPD], sl, c->N, kElemCap);
+        if (op & OP_ASM) {
+            append(tab[c->dim == 2 ? T_ASM2 : T_ASM3], sl, (long long)c->N * 16, kElemCap);
+            append(tab[T_BC], sl, c->N, kElemCap);
+            append_rows(tab[T_PRE], sl, c->n);
+        }
+        if (op & OP_RES) {
+            if (c->d.residual == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n);
+            else append_rows(tab[T_RESU], sl, c->n);
+        }
+        if (op & OP_IRU) append_rows(tab[T_IRU], sl, c->n);
+        if (op & OP_IRR) append_rows(tab[T_IRR + li], sl, c->n);
+        if (op & OP_K64_RUN) append_rows(tab[T_K64 + li], sl, c->n);
+        if (op & OP_K32_RUN) append_rows(tab[T_K32 + li], sl, c->n);
+        if (op & OP_K64_INIT) tab[T_F64 + li].push_back(BlockTask{sl, 0, 1, 0});
+        if (op & OP_K32_INIT) tab[T_F32 + li].push_back(BlockTask{sl, 0, 1, 0});
+        const int wi = c->d.wc_state == PFEM_STATE_FP32 ? 1 : 0;
+        if (op & OP_WC) {
+            append(tab[T_WCN + wi], sl, c->N, kElemCap);
+            append(tab[(c->dim == 2 ? T_WCE2 : T_WCE3) + wi], sl, c->E, kElemCap);
+        }
+        if (op & OP_CFL) {
+            append(tab[(c->dim == 2 ? T_CFE2 : T_CFE3) + wi], sl, c->E, kElemCap);
+            SlotScal& h = I.h_reset[sl];
+            h.dt2 = 0x7fefffffffffffffULL;  // DBL_MAX
+            PFEM_HIP(hipMemcpyAsync(&I.dslot(sl)->dt2, &h.dt2, sizeof(h.dt2), hipMemcpyHostToDevice, s));
+        }
+    }
+    size_t off[T_NTAB + 1];
+    off[0] = 0;
+    for (int t = 0; t < T_NTAB; ++t) off[t + 1] = off[t] + tab[t].size();
+    const size_t total = off[T_NTAB];
+    if (total > I.tab_cap) {
+        I.tab_cap = std::max<size_t>(total * 2, 4096);
+        pinned_alloc(I.h_tab, I.tab_cap);
+        I.d_tab.reserve(sizeof(BlockTask) * I.tab_cap);
+    }
+    for (int t = 0; t < T_NTAB; ++t)
+        if (!tab[t].empty()) std::memcpy(I.h_tab + off[t], tab[t].data(), sizeof(BlockTask) * tab[t].size());
+    if (total) PFEM_HIP(hipMemcpyAsync(I.d_tab.p, I.h_tab, sizeof(BlockTask) * total, hipMemcpyHostToDevice, s));
+    auto T = [&](int t) { return I.d_tab.as<BlockTask>() + off[t]; };
+    auto NT = [&](int t) { return (int)tab[t].size(); };
+
+    // ---- launches ---------------------------------------------------------------------
+    const bool timed = inflight_.size() == 1;
+    Req* solo = timed ? inflight_[0] : nullptr;
+    auto mark = [&](int k) {
+        if (timed) PFEM_HIP(hipEventRecord(I.ev[k], s));
+    };
+    const PspgView* vpv = I.d_pv.as<PspgView>();
+    const PicardView* vpic = I.d_pic.as<PicardView>();
+    mark(0);
+    if (NT(T_COPY)) launch_pic_copy(vpic, T(T_COPY), NT(T_COPY), s);
+    if (NT(T_UPD)) launch_update_pos(vpic, T(T_UPD), NT(T_UPD), s);
+    mark(1);
+    if (NT(T_ASM2)) launch_pspg_assemble(2, vpv, T(T_ASM2), NT(T_ASM2), s);
+    if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
+    if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
+    if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
+    mark(2);
+    for (int li = 0; li < 3; ++li)
+        if (NT(T_RES + li)) launch_res_axf(kLanes[li], vpic, T(T_RES + li), NT(T_RES + li), s);
+    if (NT(T_RESU)) launch_res_u(vpic, T(T_RESU), NT(T_RESU), s);
+    mark(3);
+    const RefineView* vref = I.d_ref.as<RefineView>();
+    if (NT(T_IRU)) launch_ir_update(vref, T(T_IRU), NT(T_IRU), s);
+    for (int li = 0; li < 3; ++li)
+        if (NT(T_IRR + li)) launch_ir_residual(kLanes[li], vref, T(T_IRR + li), NT(T_IRR + li), s);
+    const KrylovView<double>* vk64 = I.d_k64.as<KrylovView<double>>();
+    const KrylovView<float>* vk32 = I.d_k32.as<KrylovView<float>>();
+    for (int li = 0; li < 3; ++li) {
+        if (NT(T_F64 + li))
+            launch_kry_init<double>(kLanes[li], vk64, T(T_K64 + li), NT(T_K64 + li), T(T_F64 + li), NT(T_F64 + li),
+                                    I.par ^ 1, s);
+        if (NT(T_F32 + li))
+            launch_kry_init<float>(kLanes[li], vk32, T(T_K32 + li), NT(T_K32 + li), T(T_F32 + li), NT(T_F32 + li),
+                                   I.par ^ 1, s);
+    }
+    // live SpMV timing: one request iterating alone
+    int prof_w = -1, prof_before = 0;
+    if (I.prof && nrun == 1) {
+        for (Req* r : inflight_)
+            if (r->ops & (OP_K64_RUN | OP_K32_RUN)) {
+                prof_w = (r->ops & OP_K64_RUN) ? 0 : 1;
+                const bool init = r->ops & (prof_w == 0 ? OP_K64_INIT : OP_K32_INIT);
+                prof_before = init ? I.h_reset[r->slot].k[prof_w].iters : I.h_scal[r->slot].k[prof_w].iters;
+            }
+    }
+    for (int li = 0; li < 3; ++li) {
+        if (NT(T_K64 + li))
+            launch_kry_iters<double>(kLanes[li], chunk, vk64, T(T_K64 + li), NT(T_K64 + li), I.par, s,
+                                     prof_w == 0 ? I.pev : nullptr);
+        if (NT(T_K32 + li))
+            launch_kry_iters<float>(kLanes[li], chunk, vk32, T(T_K32 + li), NT(T_K32 + li), I.par, s,
+                                    prof_w == 1 ? I.pev : nullptr);
+    }
+    if (nrun) I.par = (I.par + chunk) & 1;
+    const WcView<double>* vw64 = I.d_wc64.as<WcView<double>>();
+    const WcView<float>* vw32 = I.d_wc32.as<WcView<float>>();
+    if (NT(T_WCN + 0))
+        launch_wc_step<double>(vw64, T(T_WCN), NT(T_WCN), T(T_WCE2), NT(T_WCE2), T(T_WCE3), NT(T_WCE3), s);
+    if (NT(T_WCN + 1))
+        launch_wc_step<float>(vw32, T(T_WCN + 1), NT(T_WCN + 1), T(T_WCE2 + 1), NT(T_WCE2 + 1), T(T_WCE3 + 1),
+                              NT(T_WCE3 + 1), s);
+    if (NT(T_CFE2) + NT(T_CFE3)) launch_wc_cfl<double>(vw64, T(T_CFE2), NT(T_CFE2), T(T_CFE3), NT(T_CFE3), s);
+    if (NT(T_CFE2 + 1) + NT(T_CFE3 + 1))
+        launch_wc_cfl<float>(vw32, T(T_CFE2 + 1), NT(T_CFE2 + 1), T(T_CFE3 + 1), NT(T_CFE3 + 1), s);
+    mark(4);
+    PFEM_HIP(hipGetLastError());
+    PFEM_HIP(hipMemcpyAsync(I.h_scal, I.d_scal.p, sizeof(SlotScal) * (size_t)I.S, hipMemcpyDeviceToHost, s));
+    PFEM_HIP(hipStreamSynchronize(s));
+
+    // ---- timings ----------------------------------------------------------------------
+    auto ems = [&](int a, int b) {
+        float ms = 0;
+        if (hipEventElapsedTime(&ms, I.ev[a], I.ev[b]) != hipSuccess) ms = 0;
+        return 1e-3 * (double)ms;
+    };
+    if (timed && solo->kind == RK_STEP) {
+        pfem_step_stats& st = solo->st;
+        if (solo->ops & OP_UPD) st.t_update += ems(0, 1);
+        if (solo->ops & OP_ASM) {
+            if (solo->first_asm) st.t_assemble += ems(1, 2);
+            else st.t_residual += ems(1, 2);
+        }
+        if (solo->ops & OP_RES) st.t_residual += ems(2, 3);
+        st.t_solve += ems(3, 4);
+    }
+    if (prof_w >= 0) {
+        for (Req* r : inflight_)
+            if (r->ops & (prof_w == 0 ? OP_K64_RUN : OP_K32_RUN)) {
+                const int worked = std::min(chunk, I.h_scal[r->slot].k[prof_w].iters - prof_before);
+                for (int it = 0; it < worked; ++it) {
+                    float ms = 0;
+                    if (hipEventElapsedTime(&ms, I.pev[2 * it], I.pev[2 * it + 1]) == hipSuccess) {
+                        prof_ms[prof_w] += ms;
+                        prof_count[prof_w]++;
+                    }
+                }
+            }
+    }
+
+    // ---- advance every request --------------------------------------------------------
+    std::vector<Req*> still;
+    still.reserve(inflight_.size());
+    for (Req* r : inflight_) {
+        pfem_case* c = r->c;
+        const unsigned ran = r->ops;
+        if (ran & OP_ASM) r->first_asm = false;
+        r->ops = 0;
+        r->copy = 0;
+        if (!r->final_) {
+            const SlotScal& h = I.h_scal[r->slot];
+            switch (r->wait) {
+                case W_K64: {  // FP64 BiCGSTAB, convergence confirmed on the true residual
+                    const KScal& k = h.k[0];
+                    if (k.done == 0) {
+                        r->ops = OP_K64_RUN;
+                    } else if (k.done == 1 && !r->verifying) {
+                        r->verifying = true;
+                        I.restart_kscal(r->slot, 0);
+                        r->ops = OP_K64_INIT | OP_K64_RUN;
+                    } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
+                        r->restarts++;
+                        r->verifying = false;
+                        I.restart_kscal(r->slot, 0);
+                        r->ops = OP_K64_INIT | OP_K64_RUN;
+                    } else {
+                        int stt = PFEM_OK;
+                        if (k.done == 2) stt = PFEM_NAN;
+                        else if (k.done == 3) stt = (k.bb > 0 && std::isfinite(k.rr)) ? PFEM_NOT_CONVERGED : PFEM_NAN;
+                        r->ss.krylov_iters += k.iters;
+                        r->ss.restarts += r->restarts + (r->fallback ? 1 : 0);
+                        r->ss.relres = k.bb > 0 ? std::sqrt(k.rr / k.bb) : 0.0;
+                        solve_done(I, r, stt);
+                    }
+                    break;
+                }
+                case W_K32: {  // FP32 inner solve of one refinement sweep
+                    const KScal& k = h.k[1];
+                    if (k.done == 0) {
+                        r->ops = OP_K32_RUN;
+                    } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
+                        r->restarts++;
+                        I.restart_kscal(r->slot, 1);
+                        r->ops = OP_K32_INIT | OP_K32_RUN;
+                    } else {
+                        r->ss.krylov_iters += k.iters;
+                        r->ss.restarts += r->restarts;
+                        r->ref++;
+                        r->ops = OP_IRU | OP_IRR;
+                        r->wait = W_IR;
+                    }
+                    break;
+                }
+                case W_IR: {  // FP64 refinement residual
+                    const double rr = h.out[0], bb = h.out[1];
+                    const double relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+                    r->ss.relres = relres;
+                    r->ss.refinements = r->ref;
+                    const double atol = step_atol(r);
+                    if (relres <= c->d.krylov_rtol && (atol <= 0 || std::sqrt(rr) <= atol)) {
+                        solve_done(I, r, PFEM_OK);
+                        break;
+                    }
+                    if (std::isfinite(relres) && relres < 0.5 * r->best) r->stall = 0;
+                    else r->stall++;
+                    if (std::isfinite(relres)) r->best = std::min(r->best, relres);
+                    if (r->stall >= 3 || !std::isfinite(relres) || r->ref >= 40 ||
+                        r->ss.krylov_iters >= c->d.krylov_max_iter) {
+                        // the FP32 recurrence stopped helping: FP64 BiCGSTAB from the iterate
+                        // (from zero when the iterate is not finite)
+                        if (!std::isfinite(relres)) {
+                            r->ops |= OP_COPY;
+                            r->copy |= COPY_ZERO_X;
+                        }
+                        r->fallback = true;
+                        r->verifying = false;
+                        r->restarts = 0;
+                        r->kit_prev = -1;
+                        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
+                        r->ops |= OP_K64_INIT | OP_K64_RUN;
+                        r->wait = W_K64;
+                    } else {
+                        r->restarts = 0;
+                        r->kit_prev = -1;
+                        I.reset_kscal(r->slot, 1, c->d.inner_rtol, std::max(1, c->d.inner_max_iter));
+                        r->ops = OP_K32_INIT | OP_K32_RUN;
+                        r->wait = W_K32;
+                    }
+                    break;
+                }
+                case W_POST: {  // residual callback (MomContEquationPSPG.inl:314-372)
+                    const double* o = h.out;
+                    double res;
+                    if (c->d.residual == PFEM_RES_AX_F) {
+                        res = std::sqrt(o[0]);
+                    } else {
+                        const double big = std::numeric_limits<double>::max();
+                        const double rv = (o[1] == 0) ? big : std::sqrt(o[0] / o[1]);
+                        const double rp =
+                            (c->d.residual == PFEM_RES_U_P) ? ((o[3] == 0) ? big : std::sqrt(o[2] / o[3])) : 0.0;
+                        res = std::max(rv, rp);
+                    }
+                    r->res = res;
+                    r->st.residual = res;
+                    if (std::isnan(res)) {
+                        r->st.picard_iters = r->iter + 1;
+                        finish(r, PFEM_NAN, COPY_RESTORE);
+                    } else {
+                        r->iter++;
+                        r->st.picard_iters = r->iter;
+                        picard_top(I, r);
+                    }
+                    break;
+                }
+                default:
+                    break;
+            }
+            if (r->final_ && r->ops) {  // completes after its last operations run
+                still.push_back(r);
+                continue;
+            }
+        }
+        if (r->final_) {
+            if (r->kind == RK_WC && r->wc_cfl) {
+                double dt2;
+                std::memcpy(&dt2, &I.h_scal[r->slot].dt2, sizeof(dt2));
+                r->next_dt = std::min(std::sqrt(dt2), r->max_dt);
+            }
+            r->t_done = now_ns();
+            r->completed = true;
+            if (r->arena_ws) {
+                I.arena->release(c->ws.off, c->ws.bytes);
+                c->ws.clear();
+                r->arena_ws = false;
+            }
+            I.free_slots.push_back(r->slot);
+            r->slot = -1;
+            done.push_back(r);
+            continue;
+        }
+        still.push_back(r);
+    }
+    inflight_.swap(still);
+}
+
+}  // namespace pfem

@@ -157,12 +157,13 @@ __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __rest
     write_partials<3>(part, V.partials, task.block, slots);
 }
 
-// One workgroup per case: finalises a (re)start and writes slot `par` as the "previous"
-// slot of the next iteration (which runs with parity par ^ 1).
+// One workgroup per listed case: finalises a (re)start and writes slot `par` as the
+// "previous" slot of the next iteration (which runs with parity par ^ 1).
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __restrict__ views, int ncases, int par) {
-    for (int c = blockIdx.x; c < ncases; c += gridDim.x) {
-        const KrylovView<T>& V = views[c];
+__global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __restrict__ views,
+                                                         const BlockTask* __restrict__ ids, int count, int par) {
+    for (int c = blockIdx.x; c < count; c += gridDim.x) {
+        const KrylovView<T>& V = views[ids[c].case_id];
         if (!V.sc->req_init) continue;
         constexpr int slots[3] = {P_RHR, P_RR, P_BB};
         double tot[3];
@@ -171,6 +172,7 @@ __global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __
             KScal& S = *V.sc;
             S.bb = tot[2];
             S.tol2 = S.rtol * S.rtol * tot[2];
+            if (S.atol2 > 0.0 && S.atol2 < S.tol2) S.tol2 = S.atol2;
             S.rr = tot[1];
             S.slot[par].rho = tot[0];
             S.slot[par].alpha = 1.0;
@@ -437,10 +439,10 @@ static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipS
     }
 
 template <typename T>
-void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int ncases, int par,
-                     hipStream_t s) {
+void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, const BlockTask* ids, int nids,
+                     int par, hipStream_t s) {
     PFEM_DISPATCH_L(lanes, (k_kry_init<T, LL><<<nt, kBlock, 0, s>>>(v, t)));
-    k_kry_init_fin<T><<<ncases < 1024 ? ncases : 1024, kBlock, 0, s>>>(v, ncases, par);
+    k_kry_init_fin<T><<<nids < 1024 ? nids : 1024, kBlock, 0, s>>>(v, ids, nids, par);
 }
 
 // Launches `iters` iterations; iteration j uses slot parity (par0 + j) & 1.
@@ -464,8 +466,10 @@ void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream
     k_ir_update<<<nt, kBlock, 0, s>>>(v, t);
 }
 
-template void launch_kry_init<double>(int, const KrylovView<double>*, const BlockTask*, int, int, int, hipStream_t);
-template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockTask*, int, int, int, hipStream_t);
+template void launch_kry_init<double>(int, const KrylovView<double>*, const BlockTask*, int, const BlockTask*, int,
+                                      int, hipStream_t);
+template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockTask*, int, const BlockTask*, int, int,
+                                     hipStream_t);
 template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, int, hipStream_t,
                                        hipEvent_t*);
 template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, int, hipStream_t,

@@ -11,9 +11,10 @@ void launch_pspg_assemble(int dim, const PspgView* views, const BlockTask* tasks
 void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 
+// ids: the cases (case_id of each entry) whose KScal::req_init is set, nids of them
 template <typename T>
-void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int ncases, int par,
-                     hipStream_t s);
+void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, const BlockTask* ids, int nids,
+                     int par, hipStream_t s);
 template <typename T>
 void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, int par0,
                       hipStream_t s, hipEvent_t* ev = nullptr);
@@ -22,6 +23,7 @@ void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, 
 void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 
+void launch_pic_copy(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_update_pos(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_res_u(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);

@@ -1,10 +1,11 @@
-// K0: node graph, field-blocked DOF CSR, element slot ranks and Dirichlet column list,
-// all built on the device from the element connectivity.
+// K0: node graph, field-blocked DOF CSR row structure, diagonal positions and the
+// pair-element incidence of the gather assembly, all built on the device from the
+// element connectivity.
 //
 // Replaces the pattern side of Eigen's SparseMatrix::setFromTriplets at
 // MomContEquationPSPG.inl:136 (the (row, col) set the triplet list of
-// MomContEquationPSPG.inl:55-128 produces) and the column walk of the Dirichlet
-// elimination at MomContEquationPSPG.inl:216-228.
+// MomContEquationPSPG.inl:55-128 produces).  Column indices are not stored: the kernels
+// address values through the node graph, and build_col_idx writes them for export.
 #include <hipcub/hipcub.hpp>
 
 #include "common.hpp"
@@ -118,18 +119,23 @@ __global__ void k_row_ptr(int N, int dim, int mom_nnz, long long nnz, const int4
     }
 }
 
-// One thread per node-graph entry (i, j): writes column j + t*N of every full row (i, s).
-__global__ void k_col_full(int M, int N, int dim, int mom_nnz, const unsigned long long* __restrict__ ukeys,
-                           const int* __restrict__ nbr_ptr, const int4* __restrict__ info, int* __restrict__ col_idx) {
-    for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
-        const unsigned long long key = ukeys[m];
-        const int i = (int)(key / (unsigned long long)N), j = (int)(key % (unsigned long long)N);
-        const int k = m - nbr_ptr[i];
+// One thread per node: the columns of every row of the node (CSR export only; the hot
+// kernels read values through the node graph).
+__global__ void k_col_node(int N, int dim, int mom_nnz, const int4* __restrict__ info, const int* __restrict__ nbr_ptr,
+                           const int* __restrict__ nbr_idx, int* __restrict__ col_idx) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < N; i += gridDim.x * blockDim.x) {
         const int4 inf = info[i];
+        const int nb = nbr_ptr[i];
         for (int s = 0; s <= dim; ++s) {
-            if (!row_full(inf, s, dim)) continue;
-            const long long base = row_start(inf, s, dim, mom_nnz) + k;
-            for (int t = 0; t <= dim; ++t) col_idx[base + (long long)t * inf.z] = j + t * N;
+            const long long rs = row_start(inf, s, dim, mom_nnz);
+            if (!row_full(inf, s, dim)) {
+                col_idx[rs] = i + s * N;
+                continue;
+            }
+            for (int k = 0; k < inf.z; ++k) {
+                const int j = nbr_idx[nb + k];
+                for (int t = 0; t <= dim; ++t) col_idx[rs + (long long)t * inf.z + k] = j + t * N;
+            }
         }
     }
 }
@@ -154,7 +160,7 @@ __global__ void k_col_diag(int N, int dim, int mom_nnz, const int4* __restrict__
             if (row_full(inf, s, dim)) {
                 diag_pos[i + (long long)s * N] = (int)(rs + (long long)s * inf.z + self_rank);
             } else {
-                col_idx[rs] = i + s * N;
+                if (col_idx) col_idx[rs] = i + s * N;
                 diag_pos[i + (long long)s * N] = (int)rs;
             }
         }
@@ -370,24 +376,23 @@ void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* fl
 
     const long long n = pat.n;
     pat.row_ptr.reserve(sizeof(int) * (size_t)(n + 1));
-    pat.col_idx.reserve(sizeof(int) * (size_t)pat.nnz);
     pat.diag_pos.reserve(sizeof(int) * (size_t)n);
     k_row_ptr<<<grid_for(n + 1, B, 65536), B, 0, stream>>>(N, dim, pat.mom_nnz, pat.nnz, pat.node_info.as<int4>(),
                                                            pat.row_ptr.as<int>());
-    k_col_full<<<grid_for(M, B, 65536), B, 0, stream>>>(M, N, dim, pat.mom_nnz, ukeys, pat.nbr_ptr.as<int>(),
-                                                        pat.node_info.as<int4>(), pat.col_idx.as<int>());
     k_col_diag<<<grid_for(N, B, 65536), B, 0, stream>>>(N, dim, pat.mom_nnz, pat.node_info.as<int4>(),
-                                                        pat.nbr_ptr.as<int>(), pat.nbr_idx.as<int>(),
-                                                        pat.col_idx.as<int>(), pat.diag_pos.as<int>());
-    pat.ranks.reserve(16 * (size_t)E);
+                                                        pat.nbr_ptr.as<int>(), pat.nbr_idx.as<int>(), nullptr,
+                                                        pat.diag_pos.as<int>());
+    // element slot ranks: only needed to build the pair-element incidence
+    DevBuf ranks;
+    ranks.reserve(16 * (size_t)E);
     PFEM_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
     if (dim == 2)
         k_ranks<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.nbr_ptr.as<int>(),
-                                                            pat.nbr_idx.as<int>(), pat.ranks.as<uint8_t>(),
+                                                            pat.nbr_idx.as<int>(), ranks.as<uint8_t>(),
                                                             err.as<int>());
     else
         k_ranks<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.nbr_ptr.as<int>(),
-                                                            pat.nbr_idx.as<int>(), pat.ranks.as<uint8_t>(),
+                                                            pat.nbr_idx.as<int>(), ranks.as<uint8_t>(),
                                                             err.as<int>());
     PFEM_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     PFEM_HIP(hipStreamSynchronize(stream));
@@ -402,23 +407,30 @@ void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* fl
     pat.pei_ptr.reserve(sizeof(int) * (size_t)(M + 1));
     pat.pei.reserve(sizeof(uint32_t) * (size_t)npairs);
     if (dim == 2)
-        k_pei_count<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+        k_pei_count<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), ranks.as<uint4>(),
                                                                 pat.nbr_ptr.as<int>(), cnt.as<int>());
     else
-        k_pei_count<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+        k_pei_count<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), ranks.as<uint4>(),
                                                                 pat.nbr_ptr.as<int>(), cnt.as<int>());
     exclusive_sum(cnt.as<int>(), pat.pei_ptr.as<int>(), M + 1, stream, tmp);
     if (dim == 2)
-        k_pei_fill<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+        k_pei_fill<3><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), ranks.as<uint4>(),
                                                                pat.nbr_ptr.as<int>(), pat.pei_ptr.as<int>(),
                                                                cursor.as<int>(), pat.pei.as<uint32_t>());
     else
-        k_pei_fill<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), pat.ranks.as<uint4>(),
+        k_pei_fill<4><<<grid_for(E, B, 65536), B, 0, stream>>>(E, pat.conn.as<int>(), ranks.as<uint4>(),
                                                                pat.nbr_ptr.as<int>(), pat.pei_ptr.as<int>(),
                                                                cursor.as<int>(), pat.pei.as<uint32_t>());
     k_pei_sort<<<grid_for(M, B, 65536), B, 0, stream>>>(M, pat.pei_ptr.as<int>(), pat.pei.as<uint32_t>());
     PFEM_HIP(hipGetLastError());
     PFEM_HIP(hipStreamSynchronize(stream));
+}
+
+void build_col_idx(const Pattern& pat, int* col_idx, hipStream_t stream) {
+    const int B = kBlock;
+    k_col_node<<<grid_for(pat.N, B, 65536), B, 0, stream>>>(pat.N, pat.dim, pat.mom_nnz, pat.node_info.as<int4>(),
+                                                            pat.nbr_ptr.as<int>(), pat.nbr_idx.as<int>(), col_idx);
+    PFEM_HIP(hipGetLastError());
 }
 
 }  // namespace pfem

@@ -29,9 +29,7 @@ struct Pattern {
     DevBuf nbr_ptr;    // int  [N+1]
     DevBuf nbr_idx;    // int  [graph_nnz]
     DevBuf node_info;  // int4 [N]      {mom_ptr, cont_ptr, deg, flags}
-    DevBuf ranks;      // u8   [E][16]  rank of node b in node a's neighbourhood, a*ne+b
     DevBuf row_ptr;    // int  [n+1]
-    DevBuf col_idx;    // int  [nnz]
     DevBuf diag_pos;   // int  [n]      position of the diagonal entry of each row
     // pair-element incidence: for node-graph entry m = (i, j), the elements containing
     // both nodes, sorted by element index, encoded e << 4 | a << 2 | b with conn[e][a] = i
@@ -45,6 +43,10 @@ struct Pattern {
 // flags_in: device, N bytes (NODE_FREE is recomputed).
 void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,
                    Pattern& pat);
+
+// Writes the nnz CSR column indices of `pat` into col_idx (device) on `stream` (export
+// only: the kernels address values through the node graph).
+void build_col_idx(const Pattern& pat, int* col_idx, hipStream_t stream);
 
 // Only the structure-of-arrays connectivity and the derived node flags (what the
 // weakly compressible step needs; no matrix).

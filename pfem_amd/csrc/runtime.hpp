@@ -1,0 +1,189 @@
+// Host runtime of the PFEM gfx950 library: contexts, cases, step workspaces, the serving
+// arena and the tick engine that drives every time step.
+//
+// Every computation goes through an Engine.  A request (one PSPG Picard step, one linear
+// solve, one assembly, or one weakly compressible step) occupies an engine *slot* while it
+// runs.  Each tick the engine builds device work tables over all in-flight requests,
+// grouped by the operation each request needs next, launches each operation once for
+// its whole group, reads every slot's scalars back with one copy, and advances each
+// request's state machine.  New requests are admitted at the start of every tick, so a
+// request that arrives while others are on the GPU joins the running batch (continuous
+// batching).  The direct C-ABI calls run the same engine with one slot.
+#pragma once
+
+#include <atomic>
+#include <condition_variable>
+#include <cstdint>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "../../include/pfem_gpu.h"
+#include "hostutil.hpp"
+#include "launch.hpp"
+#include "pattern.hpp"
+
+struct pfem_case;
+
+namespace pfem {
+
+class Arena;
+
+// The per-step workspace of a PSPG case: everything rebuilt or overwritten inside one
+// time step (matrix values, rhs, block-Jacobi inverse, Picard iterates, saved positions,
+// Krylov vectors).  Either owned by the case (direct calls) or borrowed from a serving
+// arena for the duration of one request.
+struct Workspace {
+    char* base = nullptr;
+    size_t bytes = 0;
+    Arena* arena = nullptr;  // non-null: borrowed at `off`
+    size_t off = 0;
+    DevBuf own;
+    double* vals = nullptr;
+    double* b = nullptr;
+    double* binv = nullptr;
+    double* x = nullptr;
+    double* x_prev = nullptr;
+    double* pos_save = nullptr;
+    double* kv64 = nullptr;  // 8 n: r, rh, p, v, s, t, ph, sh
+    float* vals32 = nullptr;
+    float* binv32 = nullptr;
+    float* kv32 = nullptr;  // 10 n: b, x, r, rh, p, v, s, t, ph, sh
+    // Forgets the arrays (and frees an owned allocation).
+    void clear() {
+        own.release();
+        base = nullptr;
+        bytes = 0;
+        arena = nullptr;
+        off = 0;
+        vals = b = binv = x = x_prev = pos_save = kv64 = nullptr;
+        vals32 = binv32 = kv32 = nullptr;
+    }
+};
+
+// Bytes of the step workspace of a PSPG case (0 for weakly compressible cases).
+size_t workspace_bytes(const pfem_case* c);
+// Points the workspace arrays into [base, base + workspace_bytes).
+void carve_workspace(pfem_case* c, char* base);
+// Gives the case its own workspace (if it has none) with the Picard iterate x = q_prev.
+void own_workspace(pfem_case* c, hipStream_t s);
+
+// First-fit allocator over one device allocation.
+class Arena {
+public:
+    void init(size_t bytes);
+    bool alloc(size_t bytes, size_t& off);
+    void release(size_t off, size_t bytes);
+    char* base() const { return static_cast<char*>(mem_.p); }
+    size_t capacity() const { return cap_; }
+    size_t used() const { return used_; }
+    size_t peak() const { return peak_; }
+
+private:
+    DevBuf mem_;
+    size_t cap_ = 0, used_ = 0, peak_ = 0;
+    std::map<size_t, size_t> free_;  // offset -> bytes
+};
+
+enum ReqKind : int { RK_ASSEMBLE = 0, RK_SOLVE = 1, RK_STEP = 2, RK_WC = 3 };
+
+struct Req {
+    pfem_case* c = nullptr;
+    ReqKind kind = RK_STEP;
+    double dt = 0;
+    int64_t tag = 0;
+    bool wc_step = true, wc_cfl = false;
+    double sc = 0.5, max_dt = 0;
+    // results
+    int status = PFEM_OK;
+    pfem_step_stats st{};
+    pfem_solve_stats ss{};  // the last linear solve
+    double next_dt = 0;
+    int64_t t_arrival = 0, t_admit = 0, t_done = 0;
+    bool completed = false;
+    // engine state
+    int slot = -1;
+    unsigned ops = 0;  // operations to enqueue at the next tick (OP_* in engine.cpp)
+    unsigned copy = 0; // COPY_* bits of the copy operation
+    int wait = 0;      // what the readback after those operations decides
+    bool final_ = false;
+    bool arena_ws = false;
+    int iter = 0;
+    double res = 0;
+    bool verifying = false, fallback = false;
+    int restarts = 0;
+    int kit_prev = -1;
+    double krr_prev = 0;
+    int ref = 0, stall = 0;
+    double best = 0;
+    bool first_asm = true;
+};
+
+class Engine {
+public:
+    struct Impl;
+    Engine(int device, hipStream_t s, int slots, Arena* arena);
+    ~Engine();
+    Engine(const Engine&) = delete;
+    Engine& operator=(const Engine&) = delete;
+
+    // Queues a request; it is admitted (FIFO) when a slot and its workspace are free.
+    void enqueue(Req* r);
+    // One tick; completed requests are appended to `done`.
+    void tick(std::vector<Req*>& done);
+    // Ticks until every queued and in-flight request has completed.
+    void drain(std::vector<Req*>& done);
+    bool idle() const { return waitq_.empty() && inflight_.empty(); }
+    int inflight() const { return (int)inflight_.size(); }
+    // Grows the slot count (only while idle).
+    void ensure_slots(int s);
+    // Fails every queued and in-flight request with `code` (after a device error).
+    void abort_all(int code, std::vector<Req*>& done);
+
+    void enable_profiling(bool on);
+    double prof_ms[2] = {0, 0};
+    long long prof_count[2] = {0, 0};
+    long long ticks = 0;
+    int peak_inflight = 0;
+    int max_chunk = 64;
+
+private:
+    std::unique_ptr<Impl> p_;
+    std::deque<Req*> waitq_;
+    std::vector<Req*> inflight_;
+};
+
+int64_t now_ns();
+int row_blocks(long long n);
+int pick_lanes(double avg_deg);
+
+}  // namespace pfem
+
+struct pfem_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<pfem::Engine> engine;  // pfem_batch_step
+    std::mutex mu;
+};
+
+struct pfem_case {
+    pfem_ctx* ctx = nullptr;
+    pfem_case_desc d{};
+    hipStream_t stream = nullptr;
+    int dim = 0, N = 0, E = 0, n = 0;
+    bool has_mesh = false;
+    pfem::Pattern pat;
+    // persistent device data
+    pfem::DevBuf pos, dir_vals, q_prev;
+    pfem::DevBuf partials, counters;
+    pfem::DevBuf wstate, wacc;
+    pfem::Workspace ws;
+    int lanes = 8;
+    std::atomic<int> busy{0};  // a serving request holds the case
+    std::unique_ptr<pfem::Engine> engine;  // direct calls
+    ~pfem_case();
+};

@@ -58,6 +58,7 @@ struct KScal {
     double bb;    // ||b||^2
     double tol2;  // absolute squared tolerance on ||r||^2
     double rtol;
+    double atol2; // > 0: ||r||^2 must also fall below it (absolute cap), 0: relative only
     int done;     // 0 running, 1 converged, 2 breakdown or non-finite, 3 iteration limit
     int iters;
     int maxit;
@@ -135,14 +136,32 @@ struct PicardView {
     double dt;
     const uint8_t* flags;
     double* pos;
-    const double* pos_save;
-    const double* x;       // new iterate q
-    const double* x_prev;  // previous iterate (U / U_P residuals)
+    double* pos_save;
+    double* q_prev;  // state at the start of the step (committed when the loop converges)
+    double* x;       // Picard iterate q
+    double* x_prev;  // previous iterate (U / U_P residuals)
     NodeMat<double> A;
     const double* b;
     double* partials;
     unsigned* counters;
     double* out;  // [4]
+};
+
+// Bookkeeping copies of the Picard step, one bit per copy (BlockTask::pad of the copy table).
+enum : int {
+    COPY_PREP = 1,     // pos_save = pos, x = q_prev, x_prev = q_prev   (prepare callback)
+    COPY_SAVEIT = 2,   // x_prev = x                                   (next Picard iteration)
+    COPY_COMMIT = 4,   // q_prev = x                                   (converged step)
+    COPY_RESTORE = 8,  // pos = pos_save                               (Mesh::restoreNodesList)
+    COPY_ZERO_X = 16,  // x = 0                                        (restart from zero)
+};
+
+// Scalars of one case in an engine slot, read back by the host once per tick.
+struct SlotScal {
+    KScal k[2];              // FP64 solve, FP32 inner solve
+    double out[4];           // residual sums (Picard residual, refinement residual)
+    unsigned long long dt2;  // CFL minimum (bits of a positive double)
+    double pad[3];
 };
 
 constexpr int kMaxCaseBlocks = 2048;

@@ -168,6 +168,29 @@ def test_picard_step_parity(ctx, name):
         np.testing.assert_allclose(pos_g, pos_o, rtol=0, atol=1e-9 * np.abs(pos_o).max())
 
 
+@pytest.mark.parametrize("residual", ["Ax_f", "U", "U_P"])
+def test_picard_residual_matches_oracle(ctx, residual):
+    """minRes = 0 and maxIter = 1 fix the loop at exactly two Picard iterations (then
+    iterCount > maxIter fails and restores): the residual after the second iteration is the
+    same quantity in both, whichever residual kind is selected."""
+    mesh = small_meshes()["2d_dambreak"]
+    say(f"[picard residual] {residual}")
+    dt = 1e-3
+    q0 = mg.pspg_hydrostatic(mesh)
+    pos_o = mesh.pos.reshape(-1).copy()
+    q_o = q0.copy()
+    st_o, info = O.pspg_step(mesh, pos_o, q_o, dt=dt, residual=residual, min_res=0.0, max_iter=1, solver_kind=2,
+                             rtol=1e-13)
+    c = make_case(ctx, mesh, residual=residual, min_res=0.0, max_iter=1, krylov_rtol=1e-12)
+    c.set_state(q0)
+    rc, st = c.pspg_step(dt)
+    say(f"  oracle res={info['final_res']:.6e} iters={info['picard_iters']}  gpu res={st['residual']:.6e} "
+        f"iters={st['picard_iters']}")
+    assert st_o == 1 and rc == 1
+    assert st["picard_iters"] == info["picard_iters"] == 2
+    assert st["residual"] == pytest.approx(info["final_res"], rel=1e-5)
+
+
 def test_picard_failure_restores(ctx):
     mesh = small_meshes()["2d_dambreak"]
     q0 = mg.pspg_hydrostatic(mesh)
