@@ -1,21 +1,34 @@
 #!/usr/bin/env python3
 """Benchmark of the PFEM per-time-step hot path on one MI355X.
 
-Default workload: the 3D incompressible PSPG dam break run as a plain-FP64 drop-in
-(0.5 x 0.5 x 1.0 m column, h = 0.01 m, 262,701 nodes / 1,500,000 tets, 4 dofs per node,
-rho = 1000, mu = 1e-3, g = (0, 0, -9.81), dt = 5e-4, Picard minRes 1e-6 on the Ax_f
-residual, BiCGSTAB + block Jacobi to a true relative residual of 1e-9).  One step is one
-Picard time step: assembly + BC, Krylov solves, position updates and re-assemblies,
-all resident in HBM.
+Headline workload: the serving stream — 512 independent cases (seeded), 60 % 2D / 40 % 3D,
+node counts log-uniform in [2k, 300k], half incompressible PSPG dam-break columns, half
+weakly compressible sloshing tanks; each request is one time step of one case (a Picard
+step with Ax_f minRes 1e-6, maxIter 10 and linear solves to a true relative residual of
+1e-8; or a CDS_dpdt step followed by computeNextDT).  Two paths serve the same requests:
 
-Prints ONE JSON line: steps/s with p50/p99 step latency, the roofline of the dominant
-kernel (the BiCGSTAB SpMV, timed live with HIP events on the case's stream) and the CPU
-baseline (the oracle restatement, bounded sample on the host cores).
+* plain FP64: one request at a time through the direct C-ABI calls (FP64 BiCGSTAB, FP64
+  WC state) — the drop-in replacement of the reference's per-case step;
+* batched mixed: the continuous-batching server (requests join the running batch at every
+  engine tick, admitted by step-workspace footprint), FP32 inner BiCGSTAB with FP64
+  iterative refinement, FP32 WC state with FP64 accumulators.
+
+Each path runs the stream twice: saturated (every request queued at t0: throughput, the
+headline steps/s) and with seeded Poisson arrivals at 200 requests/s (p50/p99 latency from
+arrival to result).  States are reset to the initial condition before every run, so both
+paths advance the same states by one step.
+
+Also measured: the 3D PSPG dam break (262,701 nodes, 1.5M tets) as a plain FP64 drop-in,
+with the BiCGSTAB SpMV timed live with HIP events (the roofline of the dominant kernel),
+and the CPU oracle on a bounded sample of the stream (the CPU baseline).
+
+Prints ONE JSON line.
 """
 from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -27,12 +40,207 @@ sys.path.insert(0, ROOT)
 
 METRIC = "steps/s + p50/p99 step latency, batched mixed vs plain FP64 GPU, relres≤1e-8"
 HBM_PEAK_GBPS = 8000.0
+K0, K0P, RHO_STAR = 2.2e9, 7.0, 1000.0
+RATE = 200.0
+WC_SC, WC_MAXDT = 0.5, 1e-3
 
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
+# ------------------------------------------------------------------------------------------
+# the stream
+# ------------------------------------------------------------------------------------------
+def build_stream(n, seed=2024):
+    """Seeded request list: kind, dim, target node count and Poisson arrival offset (s)."""
+    rng = np.random.default_rng(seed)
+    dims = np.where(rng.random(n) < 0.6, 2, 3)
+    nodes = np.exp(rng.uniform(math.log(2000), math.log(300000), n)).astype(int)
+    kinds = np.where(rng.random(n) < 0.5, "PSPG", "WC")
+    gaps = rng.exponential(1.0 / RATE, n)
+    arrivals = np.cumsum(gaps) - gaps[0]
+    return [dict(i=i, kind=str(kinds[i]), dim=int(dims[i]), target=int(nodes[i]), arrival=float(arrivals[i]))
+            for i in range(n)]
+
+
+def make_mesh(req):
+    from pfem_amd import meshgen as mg
+
+    seed = 1000 + req["i"]
+    if req["kind"] == "PSPG":
+        if req["dim"] == 2:
+            return mg.column_2d_nodes(req["target"], h=0.01, seed=seed)
+        return mg.column_3d_nodes(req["target"], h=0.01, seed=seed)
+    return mg.tank_nodes(req["dim"], req["target"], h=0.01, seed=seed)
+
+
+def initial_state(req, mesh):
+    from pfem_amd import meshgen as mg
+
+    if req["kind"] == "PSPG":
+        return mg.pspg_hydrostatic(mesh)
+    v, p, rho, a = mg.wc_tait_hydrostatic(mesh, K0=K0, K0p=K0P, rho_star=RHO_STAR)
+    return np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)])
+
+
+def make_case(ctx, req, mesh, variant):
+    import pfem_amd
+
+    if req["kind"] == "PSPG":
+        c = pfem_amd.Case(ctx, req["dim"], kind="PSPG", rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6,
+                          max_iter=10, precision="mixed" if variant == "batched" else "fp64", krylov_rtol=1e-8,
+                          krylov_max_iter=20000, inner_rtol=1e-3, inner_max_iter=300)
+    else:
+        c = pfem_amd.Case(ctx, req["dim"], kind="WC", mu=1e-3, K0=K0, K0p=K0P, rho_star=RHO_STAR,
+                          wc_state="fp32" if variant == "batched" else "fp64")
+    c.set_mesh(mesh)
+    return c
+
+
+def pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else None
+
+
+# ------------------------------------------------------------------------------------------
+# the two paths
+# ------------------------------------------------------------------------------------------
+def run_plain(stream, cases, poisson):
+    """One request at a time through the direct calls, in arrival order."""
+    import pfem_amd
+
+    lat, fails, relres, kit = [], 0, 0.0, 0
+    t0 = pfem_amd.now_ns()
+    for req in stream:
+        c = cases[req["i"]]
+        arr = t0 + int(req["arrival"] * 1e9) if poisson else t0
+        while poisson and pfem_amd.now_ns() < arr:
+            time.sleep(max(0.0, (arr - pfem_amd.now_ns()) * 1e-9 - 2e-4))
+        if req["kind"] == "PSPG":
+            rc, st = c.pspg_step(req["dt"])
+            fails += rc != 0
+            relres = max(relres, st["max_relres"])
+            kit += st["krylov_iters"]
+        else:
+            c.wc_step(req["dt"])
+            c.wc_next_dt(WC_SC, WC_MAXDT)
+        lat.append((pfem_amd.now_ns() - arr) * 1e-6)
+    total = (pfem_amd.now_ns() - t0) * 1e-9
+    return dict(steps_per_s=len(stream) / total, seconds=total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99),
+                failed=int(fails), max_relres=relres, krylov_iters=int(kit))
+
+
+def run_batched(ctx, stream, cases, poisson, budget, slots):
+    import pfem_amd
+
+    srv = pfem_amd.Server(ctx, hbm_budget=budget, max_inflight=slots, wc_security_coeff=WC_SC, wc_max_dt=WC_MAXDT)
+    t0 = pfem_amd.now_ns() + 2_000_000
+    for req in stream:
+        arr = t0 + int(req["arrival"] * 1e9) if poisson else t0
+        srv.submit(cases[req["i"]], req["dt"], tag=req["i"], arrival_ns=arr)
+    res = []
+    deadline = time.time() + 600
+    while len(res) < len(stream):
+        if time.time() > deadline:
+            raise SystemExit("batched run did not finish")
+        res += srv.poll(timeout_ms=100)
+    st = srv.stats()
+    srv.close()
+    t_end = max(r["t_done_ns"] for r in res)
+    total = (t_end - t0) * 1e-9
+    lat = [(r["t_done_ns"] - r["t_arrival_ns"]) * 1e-6 for r in res]
+    pspg = [r for r in res if r["kind"] == 0]
+    return dict(steps_per_s=len(stream) / total, seconds=total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99),
+                failed=int(sum(r["status"] != 0 for r in res)),
+                max_relres=max([r["stats"]["max_relres"] for r in pspg], default=0.0),
+                krylov_iters=int(sum(r["stats"]["krylov_iters"] for r in pspg)), ticks=st["ticks"],
+                peak_inflight=st["peak_inflight"], arena_bytes=st["arena_bytes"], arena_peak=st["arena_peak"])
+
+
+def reset(stream, cases, meshes, states):
+    for req in stream:
+        c = cases[req["i"]]
+        c.set_state(states[req["i"]])
+        c.set_positions(meshes[req["i"]].pos)
+
+
+def accuracy(stream, plain, batched):
+    """Largest per-field difference of the batched-mixed results from the plain-FP64 ones,
+    relative to the field's largest magnitude (velocity, pressure, positions)."""
+    worst = {"pspg_velocity": 0.0, "pspg_pressure": 0.0, "pspg_position": 0.0, "wc_velocity": 0.0,
+             "wc_pressure": 0.0}
+    for req in stream:
+        i, d = req["i"], req["dim"]
+        a, b = plain[i].get_state(), batched[i].get_state()
+        N = plain[i].N
+        rel = lambda x, y: float(np.abs(x - y).max() / max(np.abs(x).max(), 1e-300))  # noqa: E731
+        k = "pspg" if req["kind"] == "PSPG" else "wc"
+        worst[k + "_velocity"] = max(worst[k + "_velocity"], rel(a[:d * N], b[:d * N]))
+        worst[k + "_pressure"] = max(worst[k + "_pressure"], rel(a[d * N:(d + 1) * N], b[d * N:(d + 1) * N]))
+        if k == "pspg":
+            worst["pspg_position"] = max(worst["pspg_position"],
+                                         rel(plain[i].get_positions(), batched[i].get_positions()))
+    return worst
+
+
+# ------------------------------------------------------------------------------------------
+# CPU baseline: the oracle on a bounded sample of the stream
+# ------------------------------------------------------------------------------------------
+def cpu_baseline(stream, meshes, states, threads, budget_s=20.0):
+    """Oracle (C++ FP64 restatement, BiCGSTAB + ILU(0) as the stand-in for SparseLU) on the
+    smallest requests of each kind and dimension until about budget_s of CPU time; the
+    stream's CPU time is extrapolated per (kind, dim) by node count."""
+    from oracle import oracle as O
+
+    groups = {}
+    for req in stream:
+        groups.setdefault((req["kind"], req["dim"]), []).append(req)
+    per_node = {}
+    sample = []
+    t_all = time.perf_counter()
+    for key, reqs in sorted(groups.items()):
+        reqs = sorted(reqs, key=lambda r: meshes[r["i"]].n_nodes)
+        spent, nodes = 0.0, 0
+        for req in reqs:
+            if spent > budget_s / len(groups) and nodes > 0:
+                break
+            m = meshes[req["i"]]
+            q = states[req["i"]].copy()
+            pos = m.pos.reshape(-1).copy()
+            t0 = time.perf_counter()
+            if req["kind"] == "PSPG":
+                O.pspg_step(m, pos, q, dt=req["dt"], residual="Ax_f", min_res=1e-6, max_iter=10, solver_kind=2,
+                            rtol=1e-8, nthreads=threads)
+            else:
+                d, N = m.dim, m.n_nodes
+                v, p, rho, a = (q[:d * N].copy(), q[d * N:(d + 1) * N].copy(), q[(d + 1) * N:(d + 2) * N].copy(),
+                                q[(d + 2) * N:].copy())
+                O.wc_step(m, pos, v, p, rho, a, req["dt"], mu=1e-3, K0=K0, K0p=K0P, rho_star=RHO_STAR,
+                          nthreads=threads)
+                O.wc_next_dt(m, pos, v, p, rho, mu=1e-3, K0=K0, K0p=K0P, security_coeff=WC_SC, max_dt=WC_MAXDT,
+                             nthreads=threads)
+            spent += time.perf_counter() - t0
+            nodes += m.n_nodes
+            sample.append(req["i"])
+        per_node[key] = spent / nodes
+    est = sum(per_node[(r["kind"], r["dim"])] * meshes[r["i"]].n_nodes for r in stream)
+    return {
+        "value": len(stream) / est,
+        "unit": "steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": (f"oracle C++ FP64 restatement (triplets, setFromTriplets-equivalent sum, Dirichlet elimination, "
+                   f"Picard loop with BiCGSTAB+ILU(0) to relres 1e-8 as the SparseLU stand-in; WC CDS_dpdt + CFL) "
+                   f"on the {len(sample)} smallest requests of each kind/dimension "
+                   f"({time.perf_counter() - t_all:.1f} s), extrapolated to the {len(stream)}-request stream "
+                   f"per kind/dimension by node count"),
+        "seconds_per_node": {f"{k}{d}D": v for (k, d), v in per_node.items()},
+    }
+
+
+# ------------------------------------------------------------------------------------------
+# the 3D dam break drop-in and the SpMV roofline
+# ------------------------------------------------------------------------------------------
 def spmv_fused_bytes(nodes, n, nnz, graph_nnz, vbytes=8):
     """Algorithmic bytes of the first BiCGSTAB SpMV (v = A ph; dot (rh, v)) in the node-block
     format: the nnz values once, the node graph once (neighbour indices, pointers and the
@@ -46,145 +254,175 @@ def graph_nnz_from_csr(rp, dim, nodes):
     return int(np.sum(cont // (dim + 1)))
 
 
-def cpu_baseline(mesh, q, dt, asm_per_step, krylov_per_step, threads):
-    """Oracle (C++ FP64 restatement) on the host cores: one full assembly + BC of the same
-    system and 20 BiCGSTAB+Jacobi iterations on it, scaled to one step with the GPU run's
-    per-step counts of assemblies and Krylov iterations."""
-    from oracle import oracle as O
-
-    t0 = time.perf_counter()
-    S = O.pspg_assemble(mesh, q, q, dt=dt, nthreads=threads)
-    t_asm = time.perf_counter() - t0
-    x = np.zeros(S.n)
-    its = 20
-    t0 = time.perf_counter()
-    O.csr_bicgstab(S.n, S.row_ptr, S.col_idx, S.vals, S.b, x, prec=0, rtol=1e-30, maxit=its, nthreads=threads)
-    t_it = (time.perf_counter() - t0) / its
-    t_step = asm_per_step * t_asm + krylov_per_step * t_it
-    return {
-        "value": 1.0 / t_step,
-        "unit": "steps/s",
-        "cores": threads,
-        "kind": "port",
-        "sample": (f"oracle C++ restatement on the full workload: 1 assembly+BC ({t_asm:.2f} s: triplets, push-back, "
-                   f"setFromTriplets-equivalent, Dirichlet elimination) + {its} BiCGSTAB+Jacobi iterations "
-                   f"({t_it * 1e3:.1f} ms each), scaled by the GPU run's {asm_per_step:.1f} assemblies and "
-                   f"{krylov_per_step:.0f} Krylov iterations per step"),
-        "t_assemble_s": t_asm,
-        "t_krylov_iter_s": t_it,
-    }
-
-
-def run_c2(args):
+def run_dambreak3d(ctx, steps, warmup):
     import pfem_amd
     from pfem_amd import meshgen as mg
 
     dt = 5e-4
     mesh = mg.dambreak_3d(h=0.01, col=(0.5, 0.5, 1.0), seed=0)
     q0 = mg.pspg_hydrostatic(mesh)
-    log(f"[bench] mesh N={mesh.n_nodes} E={mesh.n_elems}")
-    ctx = pfem_amd.Context(0)
     case = pfem_amd.MomContEqIncompNewton(ctx, 3, rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6, max_iter=10,
                                           precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000)
-    t0 = time.perf_counter()
     case.set_mesh(mesh)
-    log(f"[bench] set_mesh (GPU pattern build) {time.perf_counter() - t0:.3f} s")
     case.set_state(q0)
-    for w in range(args.warmup):
+    for w in range(max(1, warmup)):
         ok = case.solve(dt)
-        log(f"[bench] warmup {w}: ok={ok} {case.last_stats}")
-    if args.warmup < 1:
-        case.assemble(dt)
+        log(f"[bench] dam break 3D warmup {w}: ok={ok} {case.last_stats}")
     rp, _, _ = case.csr()
-    n = len(rp) - 1
-    nnz = int(rp[-1])
+    n, nnz = len(rp) - 1, int(rp[-1])
     case.profile_spmv(True)
-    lib = pfem_amd.lib()
-    lib.pfem_case_sync(case._h)
-    lat = []
-    stats = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
+    lat, stats = [], []
+    t0 = time.perf_counter()
+    for k in range(steps):
         t1 = time.perf_counter()
         ok = case.solve(dt)
-        lat.append(time.perf_counter() - t1)
+        lat.append((time.perf_counter() - t1) * 1e3)
         stats.append(case.last_stats)
-        log(f"[bench] step {k}: {lat[-1] * 1e3:.1f} ms picard={case.last_stats['picard_iters']} "
-            f"krylov={case.last_stats['krylov_iters']} relres={case.last_stats['max_relres']:.2e} ok={ok}")
         if not ok:
-            raise SystemExit(f"step {k} failed: {case.last_stats}")
-    lib.pfem_case_sync(case._h)
-    total = time.perf_counter() - t_start
+            raise SystemExit(f"dam break step {k} failed: {case.last_stats}")
+    total = time.perf_counter() - t0
     prof = case.profile_spmv(False)
-    lat_ms = np.array(lat) * 1e3
-    krylov = float(np.mean([s["krylov_iters"] for s in stats]))
-    asm = float(np.mean([s["assemblies"] for s in stats]))
-    t_asm = float(np.mean([s["t_assemble"] for s in stats]))
-    spmv_ms = prof["avg_ms_fp64"]
     gnnz = graph_nnz_from_csr(rp, 3, mesh.n_nodes)
     bytes_spmv = spmv_fused_bytes(mesh.n_nodes, n, nnz, gnnz)
+    spmv_ms = prof["avg_ms_fp64"]
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
-    out = {
-        "metric": METRIC,
-        "value": args.steps / total,
-        "unit": "steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": total / args.steps * 1e3,
-        "p50_ms": float(np.percentile(lat_ms, 50)),
-        "p99_ms": float(np.percentile(lat_ms, 99)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f64",
-        "data": "synthetic",
-        "config": {
-            "workload": "3D PSPG dam break, plain FP64 drop-in (column 0.5x0.5x1.0 m, h=0.01 m, seeded jitter)",
-            "nodes": mesh.n_nodes, "tets": mesh.n_elems, "unknowns": n, "nnz": nnz, "graph_nnz": gnnz, "dt": dt,
-            "picard": "Ax_f, minRes 1e-6, maxIter 10", "solver": "BiCGSTAB + block Jacobi, true relres <= 1e-9",
-        },
-        "picard_iters_per_step": float(np.mean([s["picard_iters"] for s in stats])),
-        "krylov_iters_per_step": krylov,
-        "assemblies_per_step": asm,
-        "max_relres": float(max(s["max_relres"] for s in stats)),
-        "elements_per_s": mesh.n_elems / t_asm if t_asm > 0 else None,
-        "spmv_GBps": achieved,
-        "roofline": {
-            "kernel": "k_kry_v<double,16> (node-block SpMV v = A ph with fused (rh, v))",
-            "bound": "hbm",
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBPS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
-            "avg_launch_ms": spmv_ms,
-            "launches_timed": prof["launches_fp64"],
-            "algorithmic_bytes_per_launch": bytes_spmv,
-        },
+    t_asm = float(np.mean([s["t_assemble"] for s in stats]))
+    out = dict(
+        workload="3D PSPG dam break, plain FP64 drop-in (column 0.5x0.5x1.0 m, h=0.01 m, seeded jitter)",
+        nodes=mesh.n_nodes, tets=mesh.n_elems, unknowns=n, nnz=nnz, graph_nnz=gnnz, dt=dt,
+        steps_per_s=steps / total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99),
+        picard_iters_per_step=float(np.mean([s["picard_iters"] for s in stats])),
+        krylov_iters_per_step=float(np.mean([s["krylov_iters"] for s in stats])),
+        assemblies_per_step=float(np.mean([s["assemblies"] for s in stats])),
+        max_relres=float(max(s["max_relres"] for s in stats)),
+        elements_per_s=mesh.n_elems / t_asm if t_asm > 0 else None,
+    )
+    roof = {
+        "kernel": "k_kry_v<double,16> (node-block SpMV v = A ph with fused (rh, v)), 3D dam break 262,701 nodes",
+        "bound": "hbm",
+        "achieved": achieved,
+        "peak": HBM_PEAK_GBPS,
+        "unit": "GB/s",
+        "frac": achieved / HBM_PEAK_GBPS,
+        "traffic": None,
+        "avg_launch_ms": spmv_ms,
+        "launches_timed": prof["launches_fp64"],
+        "algorithmic_bytes_per_launch": bytes_spmv,
     }
-    if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
-        log(f"[bench] CPU baseline with {threads} threads")
-        out["cpu_baseline"] = cpu_baseline(mesh, q0, dt, asm, krylov, threads)
     case.close()
-    ctx.close()
-    return out
+    return out, roof
 
 
+# ------------------------------------------------------------------------------------------
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=512, help="requests in the serving stream")
+    ap.add_argument("--warmup", type=int, default=2, help="untimed warm-up requests per path")
+    ap.add_argument("--slots", type=int, default=64, help="server requests in flight at once")
+    ap.add_argument("--budget-gb", type=float, default=64.0, help="server workspace arena (GiB)")
+    ap.add_argument("--dambreak-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (each case runs whole on one GPU; "
-              "multi-GPU sharding is a design only)", file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (cases run whole on one GPU; "
+              "spreading cases or partitions over several GPUs is a design only)", file=sys.stderr)
         sys.exit(2)
-    out = run_c2(args)
+    import pfem_amd
+
+    ctx = pfem_amd.Context(0)
+    n_req = max(1, args.steps)
+    stream = build_stream(n_req)
+    t0 = time.perf_counter()
+    meshes = {r["i"]: make_mesh(r) for r in stream}
+    states = {r["i"]: initial_state(r, meshes[r["i"]]) for r in stream}
+    log(f"[bench] stream: {n_req} requests, {sum(m.n_nodes for m in meshes.values())} nodes, "
+        f"{sum(m.n_elems for m in meshes.values())} elements; meshes in {time.perf_counter() - t0:.1f} s")
+    t0 = time.perf_counter()
+    plain = {r["i"]: make_case(ctx, r, meshes[r["i"]], "plain") for r in stream}
+    batched = {r["i"]: make_case(ctx, r, meshes[r["i"]], "batched") for r in stream}
+    for r in stream:
+        plain[r["i"]].reserve_workspace()
+        if r["kind"] == "PSPG":
+            r["dt"] = 1e-3 if r["dim"] == 2 else 5e-4
+        else:
+            c = plain[r["i"]]
+            c.set_state(states[r["i"]])
+            r["dt"] = c.wc_next_dt(WC_SC, WC_MAXDT)
+    log(f"[bench] {2 * n_req} cases resident in {time.perf_counter() - t0:.1f} s")
+    budget = int(args.budget_gb * (1 << 30))
+
+    # warm-up: a few requests through both paths (kernel loading, allocator)
+    warm = stream[:max(0, min(args.warmup, n_req))]
+    if warm:
+        reset(warm, plain, meshes, states)
+        reset(warm, batched, meshes, states)
+        run_plain(warm, plain, poisson=False)
+        run_batched(ctx, warm, batched, poisson=False, budget=budget, slots=args.slots)
+
+    results = {}
+    for name, poisson in (("saturated", False), ("poisson", True)):
+        reset(stream, plain, meshes, states)
+        results[f"plain_{name}"] = run_plain(stream, plain, poisson)
+        log(f"[bench] plain FP64 {name}: {results[f'plain_{name}']}")
+        reset(stream, batched, meshes, states)
+        results[f"batched_{name}"] = run_batched(ctx, stream, batched, poisson, budget, args.slots)
+        log(f"[bench] batched mixed {name}: {results[f'batched_{name}']}")
+    acc = accuracy(stream, plain, batched)
+    log(f"[bench] batched mixed vs plain FP64 (same initial states, one step): {acc}")
+    for c in list(plain.values()) + list(batched.values()):
+        c.close()
+
+    db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
+    log(f"[bench] dam break 3D: {db}")
+    bs, ps = results["batched_saturated"], results["plain_saturated"]
+    bp, pp = results["batched_poisson"], results["plain_poisson"]
+    n2 = sum(r["dim"] == 2 for r in stream)
+    npspg = sum(r["kind"] == "PSPG" for r in stream)
+    out = {
+        "metric": METRIC,
+        "value": bs["steps_per_s"],
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": n_req,
+        "warmup": len(warm),
+        "ms_per_step": 1e3 / bs["steps_per_s"],
+        "p50_ms": bp["p50_ms"],
+        "p99_ms": bp["p99_ms"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": (f"serving stream: {n_req} independent cases (seed 2024), {n2} 2D / {n_req - n2} 3D, "
+                         f"{npspg} PSPG dam-break columns / {n_req - npspg} WC sloshing tanks, nodes log-uniform "
+                         f"2k-300k, one time step per request; Poisson arrivals {RATE:.0f}/s for latency, "
+                         f"saturated for throughput"),
+            "picard": "Ax_f, minRes 1e-6, maxIter 10", "linear_solve": "BiCGSTAB + block Jacobi, relres <= 1e-8",
+            "batched": "continuous batching, FP32 inner BiCGSTAB + FP64 refinement, FP32 WC state",
+            "plain": "one request at a time, FP64 BiCGSTAB, FP64 WC state",
+            "slots": args.slots, "arena_GiB": args.budget_gb,
+        },
+        "speedup_vs_plain_fp64": bs["steps_per_s"] / ps["steps_per_s"],
+        "plain_fp64": {"steps_per_s": ps["steps_per_s"], "poisson_steps_per_s": pp["steps_per_s"],
+                       "poisson_p50_ms": pp["p50_ms"], "poisson_p99_ms": pp["p99_ms"], "failed": ps["failed"],
+                       "max_relres": max(ps["max_relres"], pp["max_relres"])},
+        "batched_mixed": {"steps_per_s": bs["steps_per_s"], "poisson_steps_per_s": bp["steps_per_s"],
+                          "poisson_p50_ms": bp["p50_ms"], "poisson_p99_ms": bp["p99_ms"], "failed": bs["failed"],
+                          "max_relres": max(bs["max_relres"], bp["max_relres"]), "ticks": bs["ticks"],
+                          "peak_inflight": bs["peak_inflight"], "arena_peak_bytes": bs["arena_peak"]},
+        "max_relres": max(bs["max_relres"], bp["max_relres"], ps["max_relres"], pp["max_relres"]),
+        "accuracy_vs_fp64": acc,
+        "dambreak3d_fp64": db,
+        "roofline": roof,
+    }
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or min(os.cpu_count() or 1, 16)
+        log(f"[bench] CPU baseline with {threads} threads")
+        out["cpu_baseline"] = cpu_baseline(stream, meshes, states, threads)
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

@@ -55,7 +55,12 @@ extern "C" {
 #define PFEM_RES_AX_F 2 /* "Ax_f" */
 
 #define PFEM_PREC_FP64 0  /* FP64 assembly, SpMV and Krylov */
-#define PFEM_PREC_MIXED 1 /* FP32 inner BiCGSTAB on an FP32 copy of A, FP64 iterative refinement */
+#define PFEM_PREC_MIXED 1    /* BiCGSTAB whose SpMVs read an FP32 copy of A's values (vectors,
+                                products and sums in FP64), with residual replacement by the
+                                FP64 residual b - A x; converges on the FP64 system */
+#define PFEM_PREC_MIXED_IR 2 /* FP32 inner BiCGSTAB (FP32 vectors) on the FP32 copy of A inside
+                                FP64 iterative refinement; falls back to FP64 BiCGSTAB when the
+                                FP32 corrections stop reducing the FP64 residual */
 
 #define PFEM_STATE_FP64 0 /* WC node state stored in FP64 */
 #define PFEM_STATE_FP32 1 /* WC node state stored in FP32 (accumulators stay FP64) */
@@ -87,10 +92,15 @@ typedef struct pfem_case_desc {
     double krylov_rtol; /* stop when ||b - A x|| <= rtol ||b|| (true residual, FP64); inside a
                            Picard step with the Ax_f residual also ||b - A x|| <= 0.1 min_res,
                            so that the inexact solve never decides the absolute Picard test */
-    double inner_rtol;  /* mixed precision: relative tolerance of each FP32 inner solve */
-    int32_t inner_max_iter; /* mixed precision: iteration cap of each FP32 inner solve */
-    int32_t pad0;
+    double inner_rtol;  /* PFEM_PREC_MIXED_IR: relative tolerance of each FP32 inner solve */
+    int32_t inner_max_iter; /* PFEM_PREC_MIXED_IR: iteration cap of each FP32 inner solve */
+    int32_t precond;        /* PFEM_PRECOND_* */
 } pfem_case_desc;
+
+/* BiCGSTAB preconditioner of a PSPG case (chosen at pfem_case_set_mesh). */
+#define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 5,000 nodes and 3D cases of >= 150,000 */
+#define PFEM_PRECOND_JACOBI 1 /* block Jacobi over each node's (dim+1) x (dim+1) diagonal block */
+#define PFEM_PRECOND_AMG 2    /* aggregation multigrid V-cycle, block-Jacobi smoothing */
 
 typedef struct pfem_solve_stats {
     int32_t status;       /* PFEM_OK / PFEM_NOT_CONVERGED / PFEM_NAN */
@@ -242,6 +252,12 @@ int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);
 /* Bytes of the step workspace alone (what a serving request borrows from the arena). */
 int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
+/* Allocates the case's own step workspace and direct-call resources now (the direct calls
+ * otherwise allocate them at their first use), or frees the workspace (an idle case keeps only its mesh, pattern and state; a
+ * served case then borrows the server's arena again).  The assembled system and the Picard
+ * iterate are lost on release. */
+int pfem_case_reserve_workspace(pfem_case* c);
+int pfem_case_release_workspace(pfem_case* c);
 
 #ifdef __cplusplus
 }

@@ -64,7 +64,7 @@ class Case:
     def __init__(self, ctx: Context, dim: int, kind: str = "PSPG", rho=1000.0, mu=1e-3, gamma=0.0, body_force=None,
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
-                 inner_rtol=1e-3, inner_max_iter=300):
+                 inner_rtol=1e-3, inner_max_iter=300, precond="auto"):
         d = _lib.CaseDesc()
         d.dim = dim
         d.kind = _lib.KIND_PSPG if kind == "PSPG" else _lib.KIND_WC
@@ -81,9 +81,14 @@ class Case:
             raise PfemError(_lib.PFEM_E_INVALID, "unknown residual type: " + str(residual))
         d.residual = _lib.RES[residual]
         d.max_iter, d.min_res = max_iter, min_res
-        d.precision = _lib.PREC_MIXED if precision == "mixed" else _lib.PREC_FP64
+        if precision not in _lib.PRECISION:
+            raise PfemError(_lib.PFEM_E_INVALID, "unknown precision: " + str(precision))
+        d.precision = _lib.PRECISION[precision]
         d.krylov_max_iter, d.krylov_rtol, d.inner_rtol = krylov_max_iter, krylov_rtol, inner_rtol
         d.inner_max_iter = inner_max_iter
+        if precond not in _lib.PRECOND:
+            raise PfemError(_lib.PFEM_E_INVALID, "unknown preconditioner: " + str(precond))
+        d.precond = _lib.PRECOND[precond]
         self.ctx = ctx
         self.dim = dim
         self.kind = kind
@@ -153,6 +158,12 @@ class Case:
         b = C.c_int64(0)
         check(lib().pfem_case_workspace_bytes(self._h, C.byref(b)))
         return b.value
+
+    def reserve_workspace(self):
+        check(lib().pfem_case_reserve_workspace(self._h))
+
+    def release_workspace(self):
+        check(lib().pfem_case_release_workspace(self._h))
 
     # ---- PSPG building blocks -------------------------------------------------------
     def assemble(self, dt, q_prev=None, q_iter=None):

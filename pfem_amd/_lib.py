@@ -25,6 +25,9 @@ KIND_WC = 1
 RES = {"U": 0, "U_P": 1, "Ax_f": 2}
 PREC_FP64 = 0
 PREC_MIXED = 1
+PREC_MIXED_IR = 2
+PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "mixed_ir": PREC_MIXED_IR}
+PRECOND = {"auto": 0, "jacobi": 1, "amg": 2}
 STATE_FP64 = 0
 STATE_FP32 = 1
 
@@ -36,6 +39,7 @@ EXPORTS = [
     "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
     "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd", "pfem_now_ns", "pfem_case_workspace_bytes",
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
+    "pfem_case_reserve_workspace", "pfem_case_release_workspace",
 ]
 
 
@@ -45,7 +49,7 @@ class CaseDesc(C.Structure):
         ("body_force", C.c_double * 3), ("K0", C.c_double), ("K0p", C.c_double), ("rho_star", C.c_double),
         ("meduri", C.c_int32), ("wc_state", C.c_int32), ("residual", C.c_int32), ("max_iter", C.c_int32),
         ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
-        ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("pad0", C.c_int32),
+        ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("precond", C.c_int32),
     ]
 
 
@@ -134,6 +138,8 @@ def lib():
         "pfem_server_submit": [_vp, _vp, C.c_double, C.c_int64, C.c_int64],
         "pfem_server_poll": [_vp, C.POINTER(ServerResult), C.c_int32, C.c_int32, _ip],
         "pfem_server_stats": [_vp, C.POINTER(C.c_int64)],
+        "pfem_case_reserve_workspace": [_vp],
+        "pfem_case_release_workspace": [_vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -1,0 +1,388 @@
+// Aggregation multigrid for the PSPG systems: per-assembly setup (Galerkin coarse
+// operators, block-Jacobi inverses, dense inverse of the coarsest level) and the V-cycle
+// used as BiCGSTAB's right preconditioner, each batched over cases and levels through work
+// tables.  The hierarchy's structure comes from amg.cpp; see amg.hpp for the transfer.
+//
+// V-cycle on level l (input r_l, output z_l; level 0 reads p or s and writes ph or sh):
+//   w = omega B^-1 r                      (k_amg_pre, block-Jacobi pre-smoothing)
+//   t = r - A w                           (k_amg_res)
+//   r_{l+1} = P^T t                       (k_amg_restrict, gather over aggregate members)
+//   z_{l+1} = V-cycle(l+1)  /  A_c^-1 r_c (k_amg_coarsest, dense inverse on the coarsest)
+//   w += P z_{l+1}                        (k_amg_prolong)
+//   z = w + omega B^-1 (r - A w)          (k_amg_post, post-smoothing)
+// Every reduction is a fixed-order gather, so a case's V-cycle does not depend on the batch.
+#include "common.hpp"
+#include "nodespmv.hpp"
+#include "views.hpp"
+
+namespace pfem {
+
+namespace {
+
+__device__ __forceinline__ bool row_full_flags(uint8_t f, int s, int D) {
+    return (s < D) ? (f & (NODE_BOUND | NODE_FREE)) == 0 : (f & NODE_FREE) == 0;
+}
+
+__device__ __forceinline__ long long row_start_of(const int4& inf, int s, int D, int mom_nnz) {
+    return (s < D) ? (long long)s * mom_nnz + inf.x : (long long)D * mom_nnz + inf.y;
+}
+
+template <typename T>
+__device__ __forceinline__ const T* level_in(const AmgView<T>& V, int which) {
+    return V.level == 0 ? V.r0[which] : V.r;
+}
+
+// Invert the nf x nf block m (nf <= 4; rows/cols >= nf padded with identity) by
+// Gauss-Jordan with partial pivoting; the inverse is left in m[.][4..7].
+__device__ __forceinline__ void invert4(double (&m)[4][8]) {
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        for (int r = c + 1; r < 4; ++r)
+            if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
+        if (piv != c)
+            for (int k = 0; k < 8; ++k) {
+                const double tmp = m[c][k];
+                m[c][k] = m[piv][k];
+                m[piv][k] = tmp;
+            }
+        const double inv = m[c][c] != 0.0 ? 1.0 / m[c][c] : 0.0;
+        for (int k = 0; k < 8; ++k) m[c][k] *= inv;
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double fct = m[r][c];
+            if (fct != 0.0)
+                for (int k = 0; k < 8; ++k) m[r][k] -= fct * m[c][k];
+        }
+    }
+}
+
+}  // namespace
+
+// ---- setup --------------------------------------------------------------------------------
+
+// Galerkin coarse operator of level l >= 1: each entry block (I, J) sums the finer blocks of
+// its gather list.  L lanes per coarse node, lane k owns entry k of the node's list.
+template <typename T, int L>
+__device__ __forceinline__ void galerkin_body(const AmgView<T>& V, const BlockTask& task) {
+    constexpr int NPB = kBlock / L;
+    const int lane = threadIdx.x % L;
+    const int nf = V.A.nf, D = nf - 1;
+    const NodeMat<T>& F = V.F;
+    for (long long I = (long long)task.block * NPB + threadIdx.x / L; I < V.A.N; I += (long long)task.nblocks * NPB) {
+        const int4 inf = V.A.info[I];
+        const int deg = inf.z;
+        const int self = V.self_rank[I];
+        for (int k = lane; k < deg; k += L) {
+            const int mc = V.A.nbr_ptr[I] + k;
+            double blk[kMaxNf][kMaxNf];
+#pragma unroll
+            for (int s = 0; s < kMaxNf; ++s)
+#pragma unroll
+                for (int t = 0; t < kMaxNf; ++t) blk[s][t] = 0.0;
+            for (int q = V.gal_ptr[mc]; q < V.gal_ptr[mc + 1]; ++q) {
+                const int2 src = V.gal_src[q];
+                const int i = src.x, kf = src.y;
+                const int4 fi = F.info[i];
+                const int j = F.nbr_idx[F.nbr_ptr[i] + kf];
+                const uint8_t ffi = (uint8_t)fi.w;
+                const uint8_t ffj = V.fmask ? (uint8_t)F.info[j].w : (uint8_t)0;
+#pragma unroll
+                for (int s = 0; s < kMaxNf; ++s) {
+                    if (s >= nf) continue;
+                    if (V.fmask && !row_full_flags(ffi, s, D)) continue;
+                    const long long rs = row_start_of(fi, s, D, F.mom_nnz) + kf;
+#pragma unroll
+                    for (int t = 0; t < kMaxNf; ++t) {
+                        if (t >= nf) continue;
+                        if (V.fmask && !row_full_flags(ffj, t, D)) continue;
+                        blk[s][t] += (double)F.vals[rs + (long long)t * fi.z];
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kMaxNf; ++s) {
+                if (s >= nf) continue;
+                const long long rs = row_start_of(inf, s, D, V.A.mom_nnz) + k;
+#pragma unroll
+                for (int t = 0; t < kMaxNf; ++t) {
+                    if (t >= nf) continue;
+                    double v = blk[s][t];
+                    if (k == self && s == t && v == 0.0) v = 1.0;  // DOF with no member DOF: decoupled
+                    V.vals_out[rs + (long long)t * deg] = (T)v;
+                }
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_galerkin(const AmgView<T>* __restrict__ views,
+                                                         const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (V.lanes == 4) galerkin_body<T, 4>(V, task);
+    else if (V.lanes == 8) galerkin_body<T, 8>(V, task);
+    else galerkin_body<T, 16>(V, task);
+}
+
+// Block-Jacobi inverse of a coarse level (the diagonal nf x nf block of each node).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_bjac(const AmgView<T>* __restrict__ views,
+                                                     const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    const int nf = V.A.nf, D = nf - 1, N = V.A.N;
+    for (long long I = (long long)task.block * kBlock + threadIdx.x; I < N; I += (long long)task.nblocks * kBlock) {
+        const int4 inf = V.A.info[I];
+        const int self = V.self_rank[I];
+        double m[4][8];
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 8; ++t) m[s][t] = (t - 4 == s) ? 1.0 : 0.0;
+        for (int s = 0; s < nf; ++s) {
+            const long long rs = row_start_of(inf, s, D, V.A.mom_nnz) + self;
+            for (int t = 0; t < nf; ++t) m[s][t] = (double)V.A.vals[rs + (long long)t * inf.z];
+        }
+        for (int s = nf; s < 4; ++s) m[s][s] = 1.0;
+        invert4(m);
+        for (int s = 0; s < nf; ++s)
+            for (int t = 0; t < nf; ++t) V.binv_out[(long long)(s * nf + t) * N + I] = (T)m[s][4 + t];
+    }
+}
+
+// Dense inverse of the coarsest level (one workgroup per case; Gauss-Jordan in LDS without
+// pivoting: the coarse Galerkin operators keep positive diagonals; a vanishing pivot is
+// replaced by 1, which decouples that unknown).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_dense(const AmgView<T>* __restrict__ views,
+                                                      const BlockTask* __restrict__ tasks) {
+    __shared__ double a[kAmgMaxDense * kAmgMaxDense];
+    __shared__ double pivrow[kAmgMaxDense];
+    __shared__ double colk[kAmgMaxDense];
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    const int n = V.nc, N = V.A.N, nf = V.A.nf, D = nf - 1;
+    for (int e = threadIdx.x; e < n * n; e += kBlock) a[e] = 0.0;
+    __syncthreads();
+    // scatter the node-block values: unknown (I, s) is I + s N
+    for (int I = threadIdx.x; I < N; I += kBlock) {
+        const int4 inf = V.A.info[I];
+        const int nb = V.A.nbr_ptr[I];
+        for (int k = 0; k < inf.z; ++k) {
+            const int J = V.A.nbr_idx[nb + k];
+            for (int s = 0; s < nf; ++s) {
+                const long long rs = row_start_of(inf, s, D, V.A.mom_nnz) + k;
+                for (int t = 0; t < nf; ++t) a[(I + s * N) * n + (J + t * N)] = (double)V.A.vals[rs + (long long)t * inf.z];
+            }
+        }
+    }
+    __syncthreads();
+    // in-place Gauss-Jordan inversion: step k scales row k by 1/a_kk and eliminates column k
+    // from every other row, storing the inverse's column k in place of the eliminated one
+    for (int k = 0; k < n; ++k) {
+        double p = a[k * n + k];
+        p = (p != 0.0) ? 1.0 / p : 1.0;
+        for (int j = threadIdx.x; j < n; j += kBlock) {
+            pivrow[j] = (j == k) ? p : a[k * n + j] * p;
+            colk[j] = a[j * n + k];
+        }
+        __syncthreads();
+        for (int e = threadIdx.x; e < n * n; e += kBlock) {
+            const int i = e / n, j = e - i * n;
+            if (i == k) a[e] = pivrow[j];
+            else if (j == k) a[e] = -colk[i] * p;
+            else a[e] -= colk[i] * pivrow[j];
+        }
+        __syncthreads();
+    }
+    for (int e = threadIdx.x; e < n * n; e += kBlock) V.dense[e] = (T)a[e];
+}
+
+// ---- V-cycle -----------------------------------------------------------------------------
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_pre(const AmgView<T>* __restrict__ views,
+                                                    const BlockTask* __restrict__ tasks, int which) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    const T* in = level_in(V, which);
+    const int nf = V.A.nf, N = V.A.N;
+    const T om = (T)V.omega;
+    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < N; i += (long long)task.nblocks * kBlock) {
+        T x[kMaxNf];
+#pragma unroll
+        for (int t = 0; t < kMaxNf; ++t) x[t] = (t < nf) ? in[i + (long long)t * N] : T(0);
+#pragma unroll
+        for (int s = 0; s < kMaxNf; ++s) {
+            if (s >= nf) continue;
+            T acc = 0;
+#pragma unroll
+            for (int t = 0; t < kMaxNf; ++t)
+                if (t < nf) acc += V.binv[(long long)(s * nf + t) * N + i] * x[t];
+            V.w[i + (long long)s * N] = om * acc;
+        }
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void res_body(const AmgView<T>& V, const BlockTask& task, int which) {
+    constexpr int NPB = kBlock / L;
+    const int lane = threadIdx.x % L;
+    const T* in = level_in(V, which);
+    const int N = V.A.N;
+    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
+        T y[kMaxNf];
+        node_spmv<L>(V.A, (const T*)V.w, i, lane, y);
+        if (lane < V.A.nf) {
+            const long long q = i + (long long)lane * N;
+            V.t[q] = in[q] - pick(y, lane);
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict__ views,
+                                                    const BlockTask* __restrict__ tasks, int which) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    if (V.lanes == 4) res_body<T, 4>(V, task, which);
+    else if (V.lanes == 8) res_body<T, 8>(V, task, which);
+    else res_body<T, 16>(V, task, which);
+}
+
+// r_{l+1}(I, s) = sum over the members i of aggregate I of t(i, s) (level 0: full rows only).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __restrict__ views,
+                                                         const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    const int nf = V.A.nf, D = nf - 1, N = V.A.N, Nc = V.next_N;
+    for (long long I = (long long)task.block * kBlock + threadIdx.x; I < Nc; I += (long long)task.nblocks * kBlock) {
+        double acc[kMaxNf] = {0.0, 0.0, 0.0, 0.0};
+        for (int m = V.agg_ptr[I]; m < V.agg_ptr[I + 1]; ++m) {
+            const int i = V.agg_nodes[m];
+            const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
+#pragma unroll
+            for (int s = 0; s < kMaxNf; ++s)
+                if (s < nf && (!V.mask || row_full_flags(f, s, D))) acc[s] += (double)V.t[i + (long long)s * N];
+        }
+#pragma unroll
+        for (int s = 0; s < kMaxNf; ++s)
+            if (s < nf) V.next_r[I + (long long)s * Nc] = (T)acc[s];
+    }
+}
+
+// z = A_c^-1 r on the coarsest level (one workgroup per case).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __restrict__ views,
+                                                         const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    const int n = V.nc;
+    for (int row = threadIdx.x; row < n; row += kBlock) {
+        double acc = 0.0;
+        for (int j = 0; j < n; ++j) acc += (double)V.dense[row * n + j] * (double)V.r[j];
+        V.z[row] = (T)acc;
+    }
+}
+
+// w(i, s) += z_{l+1}(agg(i), s) (level 0: full rows only).
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_prolong(const AmgView<T>* __restrict__ views,
+                                                        const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    const int nf = V.A.nf, D = nf - 1, N = V.A.N, Nc = V.next_N;
+    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < N; i += (long long)task.nblocks * kBlock) {
+        const int I = V.fine_agg[i];
+        if (I < 0) continue;
+        const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
+#pragma unroll
+        for (int s = 0; s < kMaxNf; ++s)
+            if (s < nf && (!V.mask || row_full_flags(f, s, D)))
+                V.w[i + (long long)s * N] += V.next_z[I + (long long)s * Nc];
+    }
+}
+
+template <typename T, int L>
+__device__ __forceinline__ void post_body(const AmgView<T>& V, const BlockTask& task, int which) {
+    constexpr int NPB = kBlock / L;
+    const int lane = threadIdx.x % L;
+    const T* in = level_in(V, which);
+    const int N = V.A.N, nf = V.A.nf;
+    const T om = (T)V.omega;
+    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
+        T y[kMaxNf];
+        node_spmv<L>(V.A, (const T*)V.w, i, lane, y);
+        if (lane < nf) {
+            T d[kMaxNf];
+#pragma unroll
+            for (int t = 0; t < kMaxNf; ++t) d[t] = (t < nf) ? in[i + (long long)t * N] - pick(y, t) : T(0);
+            T acc = 0;
+#pragma unroll
+            for (int t = 0; t < kMaxNf; ++t)
+                if (t < nf) acc += V.binv[(long long)(lane * nf + t) * N + i] * d[t];
+            const long long q = i + (long long)lane * N;
+            const T out = V.w[q] + om * acc;
+            if (V.level == 0) V.out0[which][q] = out;
+            else V.z[q] = out;
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_post(const AmgView<T>* __restrict__ views,
+                                                     const BlockTask* __restrict__ tasks, int which) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (*V.done_flag) return;
+    if (V.lanes == 4) post_body<T, 4>(V, task, which);
+    else if (V.lanes == 8) post_body<T, 8>(V, task, which);
+    else post_body<T, 16>(V, task, which);
+}
+
+// ---- launchers ---------------------------------------------------------------------------
+void launch_amg_galerkin(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_galerkin<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_bjac(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_bjac<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_dense(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_dense<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_pre(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_pre<double><<<nt, kBlock, 0, s>>>(v, t, which);
+}
+void launch_amg_res(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_res<double><<<nt, kBlock, 0, s>>>(v, t, which);
+}
+void launch_amg_restrict(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_restrict<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_coarsest(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_coarsest<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_prolong(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_prolong<double><<<nt, kBlock, 0, s>>>(v, t);
+}
+void launch_amg_post(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_post<double><<<nt, kBlock, 0, s>>>(v, t, which);
+}
+
+}  // namespace pfem

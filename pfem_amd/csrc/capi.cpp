@@ -96,7 +96,26 @@ size_t persistent_bytes(pfem_case* c) {
                       &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr, &c->pat.nbr_idx, &c->pat.node_info,
                       &c->pat.row_ptr, &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei})
         b += p->bytes;
+    for (auto& L : c->amg.lev)
+        for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,
+                          &L.gal_ptr, &L.gal_src})
+            b += p->bytes;
     return b;
+}
+
+// Multigrid for the systems whose block-Jacobi BiCGSTAB iteration count grows with the
+// number of nodes across the domain: 2D cases from 5,000 nodes, 3D cases from 150,000
+// (measured: below these the extra V-cycle work outweighs the saved iterations;
+// PFEM_AMG_MIN_NODES_2D / _3D override the thresholds).
+bool wants_amg(const pfem_case* c) {
+    if (c->d.precond == PFEM_PRECOND_JACOBI) return false;
+    if (c->d.precond == PFEM_PRECOND_AMG) return true;
+    auto env = [](const char* k, int dflt) {
+        const char* e = std::getenv(k);
+        return e ? std::atoi(e) : dflt;
+    };
+    static const int min2 = env("PFEM_AMG_MIN_NODES_2D", 5000), min3 = env("PFEM_AMG_MIN_NODES_3D", 150000);
+    return c->N >= (c->dim == 2 ? min2 : min3);
 }
 
 }  // namespace
@@ -246,10 +265,15 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
             PFEM_REQUIRE(d.rho > 0, PFEM_E_INVALID, "rho must be > 0");
             PFEM_REQUIRE(d.residual >= 0 && d.residual <= 2, PFEM_E_INVALID, "unknown residual kind");
             PFEM_REQUIRE(d.max_iter >= 0, PFEM_E_INVALID, "maxIter must be >= 0");
-            PFEM_REQUIRE(d.precision == PFEM_PREC_FP64 || d.precision == PFEM_PREC_MIXED, PFEM_E_INVALID,
+            PFEM_REQUIRE(d.precision == PFEM_PREC_FP64 || d.precision == PFEM_PREC_MIXED ||
+                             d.precision == PFEM_PREC_MIXED_IR,
+                         PFEM_E_INVALID,
                          "unknown precision");
             PFEM_REQUIRE(d.krylov_rtol > 0 && d.krylov_max_iter > 0, PFEM_E_INVALID, "invalid Krylov settings");
-            PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED || (d.inner_rtol > 0 && d.inner_max_iter > 0), PFEM_E_INVALID,
+            PFEM_REQUIRE(d.precond >= PFEM_PRECOND_AUTO && d.precond <= PFEM_PRECOND_AMG, PFEM_E_INVALID,
+                         "unknown preconditioner");
+            PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED_IR || (d.inner_rtol > 0 && d.inner_max_iter > 0),
+                         PFEM_E_INVALID,
                          "inner_rtol and inner_max_iter must be > 0");
         } else {
             PFEM_REQUIRE(d.K0 > 0 && d.K0p > 0 && d.rho_star > 0, PFEM_E_INVALID, "invalid Tait parameters");
@@ -308,6 +332,8 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             c->q_prev.reserve(sizeof(double) * n);
             PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * n, c->stream));
             c->lanes = pfem::pick_lanes((double)c->pat.graph_nnz / (double)N);
+            c->amg = pfem::AmgHierarchy{};
+            if (wants_amg(c)) pfem::build_amg(c->pat, dim, pfem::kAmgMaxDense / (dim + 1), c->stream, c->amg);
         } else {
             pfem::build_mesh_basic(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             const size_t ss = wc_state_size(c);
@@ -421,6 +447,26 @@ int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes) {
     return guarded([&] {
         PFEM_REQUIRE(c != nullptr && bytes != nullptr, PFEM_E_INVALID, "null argument");
         *bytes = (int64_t)pfem::workspace_bytes(c);
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_reserve_workspace(pfem_case* c) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        pfem::own_workspace(c, c->stream);
+        case_engine(c);
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_release_workspace(pfem_case* c) {
+    return guarded([&] {
+        require_case(c);
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        c->ws.clear();
         return PFEM_OK;
     });
 }
@@ -617,7 +663,7 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
             pfem::launch_spmv<double>(c->lanes, node_mat<double>(c->pat, c->dim, c->ws.vals), c->q_prev.as<double>(),
                                       w + n, grid_for_rows(c->N), c->stream);
         });
-        if (c->d.precision == PFEM_PREC_MIXED) {
+        if (c->d.precision == PFEM_PREC_MIXED_IR) {
             float* u = c->ws.kv32;
             out8[1] = timed([&] {
                 pfem::launch_spmv<float>(c->lanes, node_mat<float>(c->pat, c->dim, c->ws.vals32), u, u + n,

@@ -62,7 +62,11 @@ enum Tab : int {
     T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
     T_RES, T_IRR = T_RES + 3, T_K64 = T_IRR + 3, T_K32 = T_K64 + 3, T_F64 = T_K32 + 3, T_F32 = T_F64 + 3,
     T_WCN = T_F32 + 3, T_WCE2 = T_WCN + 2, T_WCE3 = T_WCE2 + 2, T_CFE2 = T_WCE3 + 2, T_CFE3 = T_CFE2 + 2,
-    T_NTAB = T_CFE3 + 2
+    // multigrid setup per level (Galerkin, block Jacobi), coarsest dense inverse
+    T_AG = T_CFE3 + 2, T_AB = T_AG + kAmgMaxLevels, T_AD = T_AB + kAmgMaxLevels,
+    // V-cycle per level: node ops, SpMV ops, restriction; coarsest solve
+    T_VN = T_AD + 1, T_VS = T_VN + kAmgMaxLevels, T_VR = T_VS + kAmgMaxLevels, T_VC = T_VR + kAmgMaxLevels,
+    T_NTAB = T_VC + 1
 };
 
 void append(std::vector<BlockTask>& t, int slot, long long work, int cap, int pad = 0) {
@@ -130,6 +134,8 @@ int pick_lanes(double avg_deg) {
 namespace {
 struct WsLayout {
     size_t vals, b, binv, x, x_prev, pos_save, kv64, vals32, binv32, kv32, total;
+    size_t amg_w0, amg_t0, amg_dense;
+    size_t amg_lv[kAmgMaxLevels][6];
 };
 WsLayout ws_layout(const pfem_case* c) {
     WsLayout L{};
@@ -147,10 +153,23 @@ WsLayout ws_layout(const pfem_case* c) {
     L.x_prev = take(8 * n);
     L.pos_save = take(8 * (size_t)c->dim * N);
     L.kv64 = take(8 * 8 * n);
-    if (c->d.precision == PFEM_PREC_MIXED) {
-        L.vals32 = take(4 * nnz);
+    if (c->d.precision == PFEM_PREC_MIXED || c->d.precision == PFEM_PREC_MIXED_IR) L.vals32 = take(4 * nnz);
+    if (c->d.precision == PFEM_PREC_MIXED_IR) {
         L.binv32 = take(4 * nf * nf * N);
         L.kv32 = take(4 * 10 * n);
+    }
+    const AmgHierarchy& H = c->amg;
+    if (H.levels > 1) {
+        L.amg_w0 = take(8 * n);
+        L.amg_t0 = take(8 * n);
+        for (int l = 1; l < H.levels; ++l) {
+            const AmgLevel& lv = H.lev[l];
+            const size_t nl = (size_t)nf * lv.N;
+            L.amg_lv[l][0] = take(8 * (size_t)lv.nnz);
+            L.amg_lv[l][1] = take(8 * nf * nf * (size_t)lv.N);
+            for (int k = 2; k < 6; ++k) L.amg_lv[l][k] = take(8 * nl);
+        }
+        L.amg_dense = take(8 * (size_t)H.coarse_n * H.coarse_n);
     }
     L.total = o;
     return L;
@@ -174,10 +193,22 @@ void carve_workspace(pfem_case* c, char* base) {
     w.x_prev = reinterpret_cast<double*>(base + L.x_prev);
     w.pos_save = reinterpret_cast<double*>(base + L.pos_save);
     w.kv64 = reinterpret_cast<double*>(base + L.kv64);
-    const bool mixed = c->d.precision == PFEM_PREC_MIXED;
-    w.vals32 = mixed ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
-    w.binv32 = mixed ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
-    w.kv32 = mixed ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
+    const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
+    const bool v32 = ir || c->d.precision == PFEM_PREC_MIXED;
+    w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
+    w.binv32 = ir ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
+    w.kv32 = ir ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
+    const AmgHierarchy& H = c->amg;
+    w.amg_w0 = w.amg_t0 = w.amg_dense = nullptr;
+    for (auto& lv : w.amg_lv)
+        for (auto& p : lv) p = nullptr;
+    if (H.levels > 1) {
+        w.amg_w0 = reinterpret_cast<double*>(base + L.amg_w0);
+        w.amg_t0 = reinterpret_cast<double*>(base + L.amg_t0);
+        w.amg_dense = reinterpret_cast<double*>(base + L.amg_dense);
+        for (int l = 1; l < H.levels; ++l)
+            for (int k = 0; k < 6; ++k) w.amg_lv[l][k] = reinterpret_cast<double*>(base + L.amg_lv[l][k]);
+    }
 }
 
 void own_workspace(pfem_case* c, hipStream_t s) {
@@ -238,7 +269,9 @@ struct Engine::Impl {
     hipStream_t s = nullptr;
     int S = 0;
     Arena* arena = nullptr;
-    DevBuf d_pv, d_pic, d_k64, d_k32, d_ref, d_wc64, d_wc32, d_scal, d_tab;
+    DevBuf d_pv, d_pic, d_k64, d_k32, d_ref, d_wc64, d_wc32, d_scal, d_tab, d_amg;
+    AmgView<double>* h_amg = nullptr;  // [S * kAmgMaxLevels]
+    double amg_omega = 0.9;  // fewest BiCGSTAB iterations among 0.5 / 0.7 / 0.9 on 2D and 3D PSPG systems
     PspgView* h_pv = nullptr;
     PicardView* h_pic = nullptr;
     KrylovView<double>* h_k64 = nullptr;
@@ -262,7 +295,7 @@ struct Engine::Impl {
         for (auto& e : pev)
             if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)h_pv, (void*)h_pic, (void*)h_k64, (void*)h_k32, (void*)h_ref, (void*)h_wc64,
-                        (void*)h_wc32, (void*)h_scal, (void*)h_reset, (void*)h_tab})
+                        (void*)h_wc32, (void*)h_scal, (void*)h_reset, (void*)h_tab, (void*)h_amg})
             if (p) (void)hipHostFree(p);
     }
 
@@ -276,7 +309,9 @@ struct Engine::Impl {
         d_wc64.reserve(sizeof(WcView<double>) * n);
         d_wc32.reserve(sizeof(WcView<float>) * n);
         d_scal.reserve(sizeof(SlotScal) * n);
+        d_amg.reserve(sizeof(AmgView<double>) * n * kAmgMaxLevels);
         PFEM_HIP(hipMemsetAsync(d_scal.p, 0, sizeof(SlotScal) * n, s));
+        pinned_alloc(h_amg, (size_t)n * kAmgMaxLevels);
         pinned_alloc(h_pv, n);
         pinned_alloc(h_pic, n);
         pinned_alloc(h_k64, n);
@@ -299,8 +334,8 @@ struct Engine::Impl {
     }
 
     void load_views(Req* r);
-    void reset_kscal(int slot, int which, double rtol, int maxit, double atol = 0.0);
-    void restart_kscal(int slot, int which);
+    void reset_kscal(int slot, int which, double rtol, int maxit, double atol = 0.0, double repl2 = 0.0);
+    void restart_kscal(int slot, int which, int mode);
 };
 
 Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl) {
@@ -308,6 +343,11 @@ Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl
     p_->s = s;
     p_->arena = arena;
     for (auto& e : p_->ev) PFEM_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+    // PFEM_AMG_OMEGA overrides the multigrid smoother damping (tuning)
+    if (const char* e = std::getenv("PFEM_AMG_OMEGA")) {
+        const double w = std::atof(e);
+        if (w > 0.0 && w <= 1.5) p_->amg_omega = w;
+    }
     p_->alloc_slots(std::max(1, slots));
 }
 
@@ -337,7 +377,8 @@ void Engine::Impl::load_views(Req* r) {
     if (c->d.kind == PFEM_KIND_PSPG) {
         const Pattern& P = c->pat;
         const Workspace& w = c->ws;
-        const bool mixed = c->d.precision == PFEM_PREC_MIXED;
+        const bool mixed = c->d.precision == PFEM_PREC_MIXED_IR;  // FP32 inner solves
+        const bool v32 = mixed || c->d.precision == PFEM_PREC_MIXED;  // FP32 copy of the values
         PspgView& pv = h_pv[slot];
         pv = PspgView{};
         pv.dim = dim;
@@ -356,7 +397,7 @@ void Engine::Impl::load_views(Req* r) {
         pv.dir_vals = c->dir_vals.as<double>();
         pv.vals = w.vals;
         pv.b = w.b;
-        pv.vals32 = mixed ? w.vals32 : nullptr;
+        pv.vals32 = v32 ? w.vals32 : nullptr;
         pv.diag_pos = P.diag_pos.as<int>();
         pv.binv = w.binv;
         pv.binv32 = mixed ? w.binv32 : nullptr;
@@ -393,6 +434,8 @@ void Engine::Impl::load_views(Req* r) {
         k.nf = dim + 1;
         k.nblk = row_blocks(n);
         k.A = node_mat<double>(P, dim, w.vals);
+        k.use32 = c->d.precision == PFEM_PREC_MIXED ? 1 : 0;
+        k.A32 = node_mat<float>(P, dim, k.use32 ? w.vals32 : nullptr);
         k.binv = w.binv;
         k.b = w.b;
         k.x = w.x;
@@ -408,6 +451,68 @@ void Engine::Impl::load_views(Req* r) {
         k.partials = c->partials.as<double>();
         k.counters = c->counters.as<unsigned>();
         k.sc = &ds->k[0];
+        const AmgHierarchy& H = c->amg;
+        k.amg = H.levels > 1 ? 1 : 0;
+        if (k.amg) {
+            AmgView<double>* av = h_amg + (size_t)slot * kAmgMaxLevels;
+            for (int l = 0; l < H.levels; ++l) {
+                AmgView<double>& V = av[l];
+                V = AmgView<double>{};
+                V.level = l;
+                V.omega = amg_omega;
+                V.done_flag = &ds->k[0].done;
+                if (l == 0) {
+                    V.A = node_mat<double>(P, dim, w.vals);
+                    V.binv = w.binv;
+                    V.lanes = c->lanes;
+                    V.mask = 1;
+                    V.r0[0] = k.p;
+                    V.r0[1] = k.s;
+                    V.out0[0] = k.ph;
+                    V.out0[1] = k.sh;
+                    V.w = w.amg_w0;
+                    V.t = w.amg_t0;
+                } else {
+                    const AmgLevel& L = H.lev[l];
+                    NodeMat<double> A{};
+                    A.N = L.N;
+                    A.nf = dim + 1;
+                    A.mom_nnz = L.mom_nnz;
+                    A.info = L.node_info.as<int4>();
+                    A.nbr_ptr = L.nbr_ptr.as<int>();
+                    A.nbr_idx = L.nbr_idx.as<int>();
+                    A.vals = w.amg_lv[l][0];
+                    V.A = A;
+                    V.binv = w.amg_lv[l][1];
+                    V.lanes = L.lanes;
+                    V.r = w.amg_lv[l][2];
+                    V.w = w.amg_lv[l][3];
+                    V.t = w.amg_lv[l][4];
+                    V.z = w.amg_lv[l][5];
+                    V.F = av[l - 1].A;
+                    V.fmask = l == 1 ? 1 : 0;
+                    V.gal_ptr = L.gal_ptr.as<int>();
+                    V.gal_src = L.gal_src.as<int2>();
+                    V.self_rank = L.self_rank.as<int>();
+                    V.vals_out = w.amg_lv[l][0];
+                    V.binv_out = w.amg_lv[l][1];
+                }
+                if (l + 1 < H.levels) {
+                    const AmgLevel& Lc = H.lev[l + 1];
+                    V.next_N = Lc.N;
+                    V.agg_ptr = Lc.agg_ptr.as<int>();
+                    V.agg_nodes = Lc.agg_nodes.as<int>();
+                    V.fine_agg = Lc.fine_agg.as<int>();
+                    V.next_r = w.amg_lv[l + 1][2];
+                    V.next_z = w.amg_lv[l + 1][5];
+                } else {
+                    V.nc = H.coarse_n;
+                    V.dense = w.amg_dense;
+                }
+            }
+            PFEM_HIP(hipMemcpyAsync(d_amg.as<AmgView<double>>() + (size_t)slot * kAmgMaxLevels, av,
+                                    sizeof(AmgView<double>) * H.levels, hipMemcpyHostToDevice, s));
+        }
 
         KrylovView<float>& f = h_k32[slot];
         f = KrylovView<float>{};
@@ -492,23 +597,24 @@ void Engine::Impl::load_views(Req* r) {
     }
 }
 
-void Engine::Impl::reset_kscal(int slot, int which, double rtol, int maxit, double atol) {
+void Engine::Impl::reset_kscal(int slot, int which, double rtol, int maxit, double atol, double repl2) {
     KScal& k = h_reset[slot].k[which];
     std::memset(&k, 0, sizeof(KScal));
     k.rtol = rtol;
     k.atol2 = atol > 0 ? atol * atol : 0.0;
+    k.repl2 = repl2;
     k.maxit = maxit;
     k.req_init = 1;
     PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
 }
 
-// Restart of a finished solve from its current iterate (the device copy equals the last
-// readback: no kernel writes a finished solve's scalars).
-void Engine::Impl::restart_kscal(int slot, int which) {
+// Restart (mode 1) or residual replacement (mode 2) of a solve from its current iterate.
+// Between ticks no kernel runs, so the device scalars equal the last readback.
+void Engine::Impl::restart_kscal(int slot, int which, int mode) {
     KScal& k = h_reset[slot].k[which];
     k = h_scal[slot].k[which];
     k.done = 0;
-    k.req_init = 1;
+    k.req_init = mode;
     PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
 }
 
@@ -522,6 +628,29 @@ double step_atol(const Req* r) {
     return (r->kind == RK_STEP && d.residual == PFEM_RES_AX_F && d.min_res > 0) ? 0.1 * d.min_res : 0.0;
 }
 
+// FP32 values in the SpMVs: the recurrence residual is replaced by the FP64 one each time
+// it has dropped by a factor 10 (1e-2 in ||r||^2) since the last true residual.
+double repl_factor(const Req* r) { return r->c->d.precision == PFEM_PREC_MIXED ? 1e-2 : 0.0; }
+
+// The FP32 inner solves stopped helping (no progress, a diverging inner solve, or a
+// non-finite iterate): finish with FP64 BiCGSTAB from the current iterate (from zero when
+// it is not finite).  The rest of the request's solves then run in FP64 directly.
+void start_fallback(Engine::Impl& I, Req* r, bool zero_x) {
+    pfem_case* c = r->c;
+    if (zero_x) {
+        r->ops |= OP_COPY;
+        r->copy |= COPY_ZERO_X;
+    }
+    r->fallback = true;
+    r->fp64_only = true;
+    r->verifying = false;
+    r->restarts = 0;
+    r->kit_prev = -1;
+    I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r), repl_factor(r));
+    r->ops |= OP_K64_INIT | OP_K64_RUN;
+    r->wait = W_K64;
+}
+
 // Operations of a solve start (FP64 BiCGSTAB or the first refinement residual).
 void solve_start(Engine::Impl& I, Req* r) {
     pfem_case* c = r->c;
@@ -530,14 +659,14 @@ void solve_start(Engine::Impl& I, Req* r) {
     r->fallback = false;
     r->restarts = 0;
     r->kit_prev = -1;
-    if (c->d.precision == PFEM_PREC_MIXED) {
+    if (c->d.precision == PFEM_PREC_MIXED_IR && !r->fp64_only) {
         r->ref = 0;
         r->stall = 0;
         r->best = std::numeric_limits<double>::infinity();
         r->ops |= OP_IRR;
         r->wait = W_IR;
     } else {
-        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
+        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r), repl_factor(r));
         r->ops |= OP_K64_INIT | OP_K64_RUN;
         r->wait = W_K64;
     }
@@ -655,6 +784,7 @@ void Engine::tick(std::vector<Req*>& done) {
         r->iter = 0;
         r->res = std::numeric_limits<double>::max();
         r->first_asm = true;
+        r->fp64_only = false;
         r->st = pfem_step_stats{};
         I.load_views(r);
         switch (r->kind) {
@@ -723,6 +853,27 @@ void Engine::tick(std::vector<Req*>& done) {
         if (op & OP_K32_RUN) append_rows(tab[T_K32 + li], sl, c->n);
         if (op & OP_K64_INIT) tab[T_F64 + li].push_back(BlockTask{sl, 0, 1, 0});
         if (op & OP_K32_INIT) tab[T_F32 + li].push_back(BlockTask{sl, 0, 1, 0});
+        const AmgHierarchy& H = c->amg;
+        if (c->d.kind == PFEM_KIND_PSPG && H.levels > 1) {
+            const int base = sl * kAmgMaxLevels;  // AmgView index of (slot, level 0)
+            if (op & OP_ASM) {
+                for (int l = 1; l < H.levels; ++l) {
+                    append(tab[T_AG + l], base + l, (long long)H.lev[l].N * H.lev[l].lanes, kElemCap);
+                    append(tab[T_AB + l], base + l, H.lev[l].N, kElemCap);
+                }
+                tab[T_AD].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
+            }
+            if (op & OP_K64_RUN) {
+                for (int l = 0; l + 1 < H.levels; ++l) {
+                    const int Nl = l == 0 ? c->N : H.lev[l].N;
+                    const int lanes = l == 0 ? c->lanes : H.lev[l].lanes;
+                    append(tab[T_VN + l], base + l, Nl, kElemCap);
+                    append(tab[T_VS + l], base + l, (long long)Nl * lanes, kElemCap);
+                    append(tab[T_VR + l], base + l, H.lev[l + 1].N, kElemCap);
+                }
+                tab[T_VC].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
+            }
+        }
         const int wi = c->d.wc_state == PFEM_STATE_FP32 ? 1 : 0;
         if (op & OP_WC) {
             append(tab[T_WCN + wi], sl, c->N, kElemCap);
@@ -766,6 +917,13 @@ void Engine::tick(std::vector<Req*>& done) {
     if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
     if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
     if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
+    const AmgView<double>* vamg = I.d_amg.as<AmgView<double>>();
+    for (int l = 1; l < kAmgMaxLevels; ++l) {  // level l's Galerkin sums read level l-1
+        if (!NT(T_AG + l)) continue;
+        launch_amg_galerkin(vamg, T(T_AG + l), NT(T_AG + l), s);
+        launch_amg_bjac(vamg, T(T_AB + l), NT(T_AB + l), s);
+    }
+    if (NT(T_AD)) launch_amg_dense(vamg, T(T_AD), NT(T_AD), s);
     mark(2);
     for (int li = 0; li < 3; ++li)
         if (NT(T_RES + li)) launch_res_axf(kLanes[li], vpic, T(T_RES + li), NT(T_RES + li), s);
@@ -795,13 +953,35 @@ void Engine::tick(std::vector<Req*>& done) {
                 prof_before = init ? I.h_reset[r->slot].k[prof_w].iters : I.h_scal[r->slot].k[prof_w].iters;
             }
     }
-    for (int li = 0; li < 3; ++li) {
-        if (NT(T_K64 + li))
-            launch_kry_iters<double>(kLanes[li], chunk, vk64, T(T_K64 + li), NT(T_K64 + li), I.par, s,
-                                     prof_w == 0 ? I.pev : nullptr);
-        if (NT(T_K32 + li))
-            launch_kry_iters<float>(kLanes[li], chunk, vk32, T(T_K32 + li), NT(T_K32 + li), I.par, s,
-                                    prof_w == 1 ? I.pev : nullptr);
+    // multigrid V-cycle of the cases that have a hierarchy: which = 0 (ph = M^-1 p) or 1 (sh = M^-1 s)
+    auto vcycle = [&](int which) {
+        for (int l = 0; l < kAmgMaxLevels; ++l) {
+            if (!NT(T_VN + l)) break;
+            launch_amg_pre(vamg, T(T_VN + l), NT(T_VN + l), which, s);
+            launch_amg_res(vamg, T(T_VS + l), NT(T_VS + l), which, s);
+            launch_amg_restrict(vamg, T(T_VR + l), NT(T_VR + l), s);
+        }
+        if (NT(T_VC)) launch_amg_coarsest(vamg, T(T_VC), NT(T_VC), s);
+        for (int l = kAmgMaxLevels - 1; l >= 0; --l) {
+            if (!NT(T_VN + l)) continue;
+            launch_amg_prolong(vamg, T(T_VN + l), NT(T_VN + l), s);
+            launch_amg_post(vamg, T(T_VS + l), NT(T_VS + l), which, s);
+        }
+    };
+    for (int it = 0; it < chunk; ++it) {
+        const int p = (I.par + it) & 1;
+        for (int phase = 0; phase < 5; ++phase) {
+            for (int li = 0; li < 3; ++li) {
+                if (NT(T_K64 + li))
+                    launch_kry_phase<double>(phase, kLanes[li], vk64, T(T_K64 + li), NT(T_K64 + li), p, s,
+                                             (phase == 1 && prof_w == 0) ? I.pev + 2 * it : nullptr);
+                if (NT(T_K32 + li))
+                    launch_kry_phase<float>(phase, kLanes[li], vk32, T(T_K32 + li), NT(T_K32 + li), p, s,
+                                            (phase == 1 && prof_w == 1) ? I.pev + 2 * it : nullptr);
+            }
+            if (phase == 0) vcycle(0);
+            if (phase == 2) vcycle(1);
+        }
     }
     if (nrun) I.par = (I.par + chunk) & 1;
     const WcView<double>* vw64 = I.d_wc64.as<WcView<double>>();
@@ -861,18 +1041,25 @@ void Engine::tick(std::vector<Req*>& done) {
         if (!r->final_) {
             const SlotScal& h = I.h_scal[r->slot];
             switch (r->wait) {
-                case W_K64: {  // FP64 BiCGSTAB, convergence confirmed on the true residual
+                case W_K64: {  // FP64 BiCGSTAB; convergence is accepted on a true residual only
                     const KScal& k = h.k[0];
+                    r->verifying = false;
                     if (k.done == 0) {
                         r->ops = OP_K64_RUN;
-                    } else if (k.done == 1 && !r->verifying) {
+                    } else if (k.done == 4) {
+                        // FP32 values: the device stopped for a residual replacement
+                        I.restart_kscal(r->slot, 0, 2);
+                        r->ops = OP_K64_INIT | OP_K64_RUN;
+                    } else if (k.done == 1 && !k.true_rr) {
+                        // converged on the recurrence residual: confirm on the true one.  FP32
+                        // values keep the search directions (the recurrence drifts from the FP64
+                        // residual by design); FP64 restarts from the iterate
                         r->verifying = true;
-                        I.restart_kscal(r->slot, 0);
+                        I.restart_kscal(r->slot, 0, c->d.precision == PFEM_PREC_MIXED ? 2 : 1);
                         r->ops = OP_K64_INIT | OP_K64_RUN;
                     } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
                         r->restarts++;
-                        r->verifying = false;
-                        I.restart_kscal(r->slot, 0);
+                        I.restart_kscal(r->slot, 0, 1);
                         r->ops = OP_K64_INIT | OP_K64_RUN;
                     } else {
                         int stt = PFEM_OK;
@@ -891,14 +1078,21 @@ void Engine::tick(std::vector<Req*>& done) {
                         r->ops = OP_K32_RUN;
                     } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
                         r->restarts++;
-                        I.restart_kscal(r->slot, 1);
+                        I.restart_kscal(r->slot, 1, 1);
                         r->ops = OP_K32_INIT | OP_K32_RUN;
                     } else {
                         r->ss.krylov_iters += k.iters;
                         r->ss.restarts += r->restarts;
-                        r->ref++;
-                        r->ops = OP_IRU | OP_IRR;
-                        r->wait = W_IR;
+                        // a correction is applied only when the inner solve at least halved
+                        // its residual; otherwise it would corrupt the FP64 iterate
+                        const bool inner_ok = std::isfinite(k.rr) && k.bb > 0 && k.rr <= 0.25 * k.bb;
+                        if (inner_ok) {
+                            r->ref++;
+                            r->ops = OP_IRU | OP_IRR;
+                            r->wait = W_IR;
+                        } else {
+                            start_fallback(I, r, false);
+                        }
                     }
                     break;
                 }
@@ -917,19 +1111,7 @@ void Engine::tick(std::vector<Req*>& done) {
                     if (std::isfinite(relres)) r->best = std::min(r->best, relres);
                     if (r->stall >= 3 || !std::isfinite(relres) || r->ref >= 40 ||
                         r->ss.krylov_iters >= c->d.krylov_max_iter) {
-                        // the FP32 recurrence stopped helping: FP64 BiCGSTAB from the iterate
-                        // (from zero when the iterate is not finite)
-                        if (!std::isfinite(relres)) {
-                            r->ops |= OP_COPY;
-                            r->copy |= COPY_ZERO_X;
-                        }
-                        r->fallback = true;
-                        r->verifying = false;
-                        r->restarts = 0;
-                        r->kit_prev = -1;
-                        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
-                        r->ops |= OP_K64_INIT | OP_K64_RUN;
-                        r->wait = W_K64;
+                        start_fallback(I, r, !std::isfinite(relres));
                     } else {
                         r->restarts = 0;
                         r->kit_prev = -1;

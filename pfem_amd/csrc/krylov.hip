@@ -15,6 +15,10 @@
 // the kernel boundary, so no device-scope fence (an L2 write-back on gfx950) is needed.
 // Scalars: two slots used alternately by iteration parity (KScal in views.hpp).
 //
+// Mixed precision (use32): K_v and K_t read an FP32 copy of the values while every vector,
+// product and sum stays FP64; the init kernel's residual replacement recomputes
+// r = b - A x with the FP64 values, so the solve converges on the FP64 system's residual.
+//
 // Work split: workgroups are dispatched round-robin over the 8 XCDs; the workgroups of a
 // case that land on one XCD own one contiguous range of nodes (every field of those
 // nodes), so the SpMV's x gathers (neighbour nodes) and the vectors handed from one
@@ -118,37 +122,47 @@ __global__ __launch_bounds__(kBlock) void k_spmv(NodeMat<T> A, const T* __restri
     const int lane = threadIdx.x % K;
     for (long long i = (long long)blockIdx.x * NPB + threadIdx.x / K; i < A.N; i += (long long)gridDim.x * NPB) {
         T yv[kMaxNf];
-        node_spmv<T, K>(A, x, i, lane, yv);
+        node_spmv<K>(A, x, i, lane, yv);
         if (lane < A.nf) y[i + (long long)lane * A.N] = pick(yv, lane);
     }
 }
 
-// (Re)start: r = b - A x (or b when x_zero), rh = r, p = v = 0.
+// Mode 1 (restart): r = b - A x (or b when x_zero), rh = r, p = v = 0.
+// Mode 2 (residual replacement, van der Vorst & Ye 2000): r = b - A x with the system's
+// own values, keeping rh, p, v and the iteration scalars, so an iteration whose SpMVs read
+// FP32 values converges on the true residual of the FP64 system.
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __restrict__ views,
                                                      const BlockTask* __restrict__ tasks) {
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
-    if (!V.sc->req_init) return;
+    const int mode = V.sc->req_init;
+    if (!mode) return;
     constexpr int NPB = kBlock / L;
     const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
+    const bool restart = mode == 1;
+    const bool from_zero = restart && V.x_zero;
     double part[3] = {0.0, 0.0, 0.0};
     for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
         const long long i = sg.n0 + l;
         T ax[kMaxNf] = {T(0), T(0), T(0), T(0)};
-        if (!V.x_zero) node_spmv<T, L>(V.A, V.x, i, lane, ax);
+        if (!from_zero) node_spmv<L>(V.A, V.x, i, lane, ax);
         if (lane < V.nf) {
             const long long row = i + (long long)lane * V.nodes;
             const T bi = V.b[row];
             const T r = bi - pick(ax, lane);
             V.r[row] = r;
-            V.rh[row] = r;
-            V.p[row] = T(0);
-            V.v[row] = T(0);
-            if (V.x_zero) V.x[row] = T(0);
-            part[0] += (double)r * (double)r;
+            if (restart) {
+                V.rh[row] = r;
+                V.p[row] = T(0);
+                V.v[row] = T(0);
+                if (from_zero) V.x[row] = T(0);
+                part[0] += (double)r * (double)r;
+            } else {
+                part[0] += (double)V.rh[row] * (double)r;
+            }
             part[1] += (double)r * (double)r;
             part[2] += (double)bi * (double)bi;
         }
@@ -157,14 +171,17 @@ __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __rest
     write_partials<3>(part, V.partials, task.block, slots);
 }
 
-// One workgroup per listed case: finalises a (re)start and writes slot `par` as the
-// "previous" slot of the next iteration (which runs with parity par ^ 1).
+// One workgroup per listed case: finalises a restart or a residual replacement.  Slot
+// `par` becomes the "previous" slot of the next iteration (which runs with parity par ^ 1):
+// a restart fills it with fresh scalars, a replacement copies the last completed
+// iteration's scalars into it.
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __restrict__ views,
                                                          const BlockTask* __restrict__ ids, int count, int par) {
     for (int c = blockIdx.x; c < count; c += gridDim.x) {
         const KrylovView<T>& V = views[ids[c].case_id];
-        if (!V.sc->req_init) continue;
+        const int mode = V.sc->req_init;
+        if (!mode) continue;
         constexpr int slots[3] = {P_RHR, P_RR, P_BB};
         double tot[3];
         reduce_partials<3>(V.partials, V.nblk, slots, tot);
@@ -174,9 +191,16 @@ __global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __
             S.tol2 = S.rtol * S.rtol * tot[2];
             if (S.atol2 > 0.0 && S.atol2 < S.tol2) S.tol2 = S.atol2;
             S.rr = tot[1];
-            S.slot[par].rho = tot[0];
-            S.slot[par].alpha = 1.0;
-            S.slot[par].omega = 1.0;
+            if (mode == 1) {
+                S.slot[par].rho = tot[0];
+                S.slot[par].alpha = 1.0;
+                S.slot[par].omega = 1.0;
+            } else if (S.last != par) {
+                S.slot[par] = S.slot[S.last];
+            }
+            S.last = par;
+            S.true_rr = 1;
+            S.rr_ref = tot[1];
             S.req_init = 0;
             if (tot[2] == 0.0 || tot[1] <= S.tol2) S.done = 1;
             else if (!isfinite(tot[1])) S.done = 2;
@@ -204,6 +228,9 @@ __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restric
     if (tot[1] <= S.tol2) stop = 1;
     else if (!isfinite(tot[1]) || tot[0] == 0.0 || !isfinite(beta)) stop = 2;
     else if (S.iters >= S.maxit) stop = 3;
+    else if (S.repl2 > 0.0 && tot[1] <= S.repl2 * S.rr_ref) stop = 4;  // decided on the device:
+                                                                       // the same iteration alone
+                                                                       // or in any batch
     if (stop) {
         if (task.block == 0 && threadIdx.x == 0) {
             S.rr = tot[1];
@@ -226,6 +253,7 @@ __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restric
                 V.p[q] = pv[f];
             }
         }
+        if (V.amg) continue;  // ph = M^-1 p comes from the V-cycle
         block_apply<T>(V.binv, V.nodes, nf, i, pv, ph);
 #pragma unroll
         for (int f = 0; f < kMaxNf; ++f)
@@ -237,21 +265,15 @@ __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restric
     }
 }
 
-template <typename T, int L>
-__global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restrict__ views,
-                                                  const BlockTask* __restrict__ tasks) {
+// v = A ph over the workgroup's nodes; accumulates (rh, v).
+template <int L, typename MT, typename T>
+__device__ __forceinline__ void kry_v_nodes(const KrylovView<T>& V, const NodeMat<MT>& A, const Seg& sg, int lane,
+                                            double (&part)[1]) {
     constexpr int NPB = kBlock / L;
-    const BlockTask task = tasks[blockIdx.x];
-    if (task.case_id < 0) return;
-    const KrylovView<T>& V = views[task.case_id];
-    if (V.sc->done) return;
-    const Seg sg = xcd_segment(task, V.nodes);
-    const int lane = threadIdx.x % L;
-    double part[1] = {0.0};
     for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
         const long long i = sg.n0 + l;
         T y[kMaxNf];
-        node_spmv<T, L>(V.A, V.ph, i, lane, y);
+        node_spmv<L>(A, V.ph, i, lane, y);
         if (lane < V.nf) {
             const long long row = i + (long long)lane * V.nodes;
             const T yv = pick(y, lane);
@@ -259,6 +281,20 @@ __global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restric
             part[0] += (double)V.rh[row] * (double)yv;
         }
     }
+}
+
+template <typename T, int L>
+__global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restrict__ views,
+                                                  const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const KrylovView<T>& V = views[task.case_id];
+    if (V.sc->done) return;
+    const Seg sg = xcd_segment(task, V.nodes);
+    const int lane = threadIdx.x % L;
+    double part[1] = {0.0};
+    if (V.use32) kry_v_nodes<L>(V, V.A32, sg, lane, part);
+    else kry_v_nodes<L>(V, V.A, sg, lane, part);
     constexpr int slots[1] = {P_RV};
     write_partials<1>(part, V.partials, task.block, slots);
 }
@@ -296,6 +332,7 @@ __global__ __launch_bounds__(kBlock) void k_kry_s(const KrylovView<T>* __restric
                 part[0] += (double)sv[f] * (double)sv[f];
             }
         }
+        if (V.amg) continue;  // sh = M^-1 s comes from the V-cycle
         block_apply<T>(V.binv, V.nodes, nf, i, sv, sh);
 #pragma unroll
         for (int f = 0; f < kMaxNf; ++f)
@@ -306,10 +343,28 @@ __global__ __launch_bounds__(kBlock) void k_kry_s(const KrylovView<T>* __restric
     if (task.block == 0 && threadIdx.x == 0) S.slot[par].alpha = alpha;
 }
 
+// t = A sh over the workgroup's nodes; accumulates (t, s) and (t, t).
+template <int L, typename MT, typename T>
+__device__ __forceinline__ void kry_t_nodes(const KrylovView<T>& V, const NodeMat<MT>& A, const Seg& sg, int lane,
+                                            double (&part)[2]) {
+    constexpr int NPB = kBlock / L;
+    for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
+        const long long i = sg.n0 + l;
+        T y[kMaxNf];
+        node_spmv<L>(A, V.sh, i, lane, y);
+        if (lane < V.nf) {
+            const long long row = i + (long long)lane * V.nodes;
+            const T yv = pick(y, lane);
+            V.t[row] = yv;
+            part[0] += (double)yv * (double)V.s[row];
+            part[1] += (double)yv * (double)yv;
+        }
+    }
+}
+
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restrict__ views,
                                                   const BlockTask* __restrict__ tasks) {
-    constexpr int NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
@@ -321,18 +376,8 @@ __global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restric
     const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
     double part[2] = {0.0, 0.0};
-    for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
-        const long long i = sg.n0 + l;
-        T y[kMaxNf];
-        node_spmv<T, L>(V.A, V.sh, i, lane, y);
-        if (lane < V.nf) {
-            const long long row = i + (long long)lane * V.nodes;
-            const T yv = pick(y, lane);
-            V.t[row] = yv;
-            part[0] += (double)yv * (double)V.s[row];
-            part[1] += (double)yv * (double)yv;
-        }
-    }
+    if (V.use32) kry_t_nodes<L>(V, V.A32, sg, lane, part);
+    else kry_t_nodes<L>(V, V.A, sg, lane, part);
     constexpr int slots[2] = {P_TS, P_TT};
     write_partials<2>(part, V.partials, task.block, slots);
 }
@@ -374,6 +419,8 @@ __global__ __launch_bounds__(kBlock) void k_kry_x(const KrylovView<T>* __restric
     if (task.block == 0 && threadIdx.x == 0) {
         S.slot[par].omega = omega;
         S.iters += 1;
+        S.last = par;
+        S.true_rr = 0;
     }
 }
 
@@ -389,7 +436,7 @@ __global__ __launch_bounds__(kBlock) void k_ir_residual(const RefineView* __rest
     double part[2] = {0.0, 0.0};
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < V.A.N; i += (long long)task.nblocks * NPB) {
         double ax[kMaxNf];
-        node_spmv<double, L>(V.A, V.x, i, lane, ax);
+        node_spmv<L>(V.A, V.x, i, lane, ax);
         if (lane < V.A.nf) {
             const long long row = i + (long long)lane * V.A.N;
             const double bi = V.b[row];
@@ -453,6 +500,24 @@ void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockT
         PFEM_DISPATCH_L(lanes, (iter_launch<T, LL>(v, t, nt, s, (par0 + it) & 1, ev ? ev + 2 * it : nullptr)));
 }
 
+// One kernel of an iteration (phase 0 K_p, 1 K_v, 2 K_s, 3 K_t, 4 K_x) with parity par, so
+// that a preconditioner's launches can sit between them; ev (optional) brackets K_v.
+template <typename T>
+void launch_kry_phase(int phase, int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int par,
+                      hipStream_t s, hipEvent_t* ev) {
+    switch (phase) {
+        case 0: k_kry_p<T><<<nt, kBlock, 0, s>>>(v, t, par); break;
+        case 1:
+            if (ev) (void)hipEventRecord(ev[0], s);
+            PFEM_DISPATCH_L(lanes, (k_kry_v<T, LL><<<nt, kBlock, 0, s>>>(v, t)));
+            if (ev) (void)hipEventRecord(ev[1], s);
+            break;
+        case 2: k_kry_s<T><<<nt, kBlock, 0, s>>>(v, t, par); break;
+        case 3: PFEM_DISPATCH_L(lanes, (k_kry_t<T, LL><<<nt, kBlock, 0, s>>>(v, t))); break;
+        default: k_kry_x<T><<<nt, kBlock, 0, s>>>(v, t, par); break;
+    }
+}
+
 template <typename T>
 void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s) {
     PFEM_DISPATCH_L(lanes, (k_spmv<T, LL><<<nblocks, kBlock, 0, s>>>(A, x, y)));
@@ -473,6 +538,10 @@ template void launch_kry_init<float>(int, const KrylovView<float>*, const BlockT
 template void launch_kry_iters<double>(int, int, const KrylovView<double>*, const BlockTask*, int, int, hipStream_t,
                                        hipEvent_t*);
 template void launch_kry_iters<float>(int, int, const KrylovView<float>*, const BlockTask*, int, int, hipStream_t,
+                                      hipEvent_t*);
+template void launch_kry_phase<double>(int, int, const KrylovView<double>*, const BlockTask*, int, int, hipStream_t,
+                                       hipEvent_t*);
+template void launch_kry_phase<float>(int, int, const KrylovView<float>*, const BlockTask*, int, int, hipStream_t,
                                       hipEvent_t*);
 template void launch_spmv<double>(int, const NodeMat<double>&, const double*, double*, int, hipStream_t);
 template void launch_spmv<float>(int, const NodeMat<float>&, const float*, float*, int, hipStream_t);

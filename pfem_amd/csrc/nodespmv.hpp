@@ -21,9 +21,10 @@ namespace pfem {
 
 constexpr int kMaxNf = 4;
 
-// y[s] = (A x) of row (i, s), s < nf; valid in every lane of the group.
-template <typename T, int L>
-__device__ __forceinline__ void node_spmv(const NodeMat<T>& M, const T* __restrict__ x, long long i, int lane,
+// y[s] = (A x) of row (i, s), s < nf; valid in every lane of the group.  The values may
+// be stored in a narrower type VT than the vectors (FP32 values, FP64 vectors and sums).
+template <int L, typename VT, typename T>
+__device__ __forceinline__ void node_spmv(const NodeMat<VT>& M, const T* __restrict__ x, long long i, int lane,
                                           T (&y)[kMaxNf]) {
     const int4 inf = M.info[i];
     const int nb = M.nbr_ptr[i];
@@ -45,7 +46,7 @@ __device__ __forceinline__ void node_spmv(const NodeMat<T>& M, const T* __restri
         for (int s = 0; s < kMaxNf; ++s)
 #pragma unroll
             for (int t = 0; t < kMaxNf; ++t)
-                a[s][t] = (full[s] && t < nf) ? M.vals[rs[s] + k + (long long)t * deg] : T(0);
+                a[s][t] = (full[s] && t < nf) ? (T)M.vals[rs[s] + k + (long long)t * deg] : T(0);
         const long long j = M.nbr_idx[nb + k];
         T xv[kMaxNf];
 #pragma unroll
@@ -61,7 +62,7 @@ __device__ __forceinline__ void node_spmv(const NodeMat<T>& M, const T* __restri
     if (lane == 0) {
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s)
-            if (s < nf && !full[s]) y[s] = M.vals[rs[s]] * x[i + s * N];
+            if (s < nf && !full[s]) y[s] = (T)M.vals[rs[s]] * x[i + s * N];
     }
 #pragma unroll
     for (int s = 0; s < kMaxNf; ++s) {

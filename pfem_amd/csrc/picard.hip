@@ -64,7 +64,7 @@ __global__ __launch_bounds__(kBlock) void k_res_axf(const PicardView* __restrict
     double part[1] = {0.0};
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < V.N; i += (long long)task.nblocks * NPB) {
         double ax[kMaxNf];
-        node_spmv<double, L>(V.A, V.x, i, lane, ax);
+        node_spmv<L>(V.A, V.x, i, lane, ax);
         if (lane < V.A.nf) {
             const double r = pick(ax, lane) - V.b[i + (long long)lane * V.N];
             part[0] += r * r;

@@ -23,6 +23,7 @@
 #include <vector>
 
 #include "../../include/pfem_gpu.h"
+#include "amg.hpp"
 #include "hostutil.hpp"
 #include "launch.hpp"
 #include "pattern.hpp"
@@ -53,6 +54,12 @@ struct Workspace {
     float* vals32 = nullptr;
     float* binv32 = nullptr;
     float* kv32 = nullptr;  // 10 n: b, x, r, rh, p, v, s, t, ph, sh
+    // multigrid: level-0 smoother vectors, coarse levels' values / inverses / vectors,
+    // coarsest dense inverse
+    double* amg_w0 = nullptr;
+    double* amg_t0 = nullptr;
+    double* amg_dense = nullptr;
+    double* amg_lv[kAmgMaxLevels][6] = {};  // {vals, binv, r, w, t, z} of level l >= 1
     // Forgets the arrays (and frees an owned allocation).
     void clear() {
         own.release();
@@ -62,6 +69,9 @@ struct Workspace {
         off = 0;
         vals = b = binv = x = x_prev = pos_save = kv64 = nullptr;
         vals32 = binv32 = kv32 = nullptr;
+        amg_w0 = amg_t0 = amg_dense = nullptr;
+        for (auto& l : amg_lv)
+            for (auto& p : l) p = nullptr;
     }
 };
 
@@ -115,6 +125,7 @@ struct Req {
     int iter = 0;
     double res = 0;
     bool verifying = false, fallback = false;
+    bool fp64_only = false;  // mixed precision gave up on FP32 for the rest of the request
     int restarts = 0;
     int kit_prev = -1;
     double krr_prev = 0;
@@ -182,6 +193,7 @@ struct pfem_case {
     pfem::DevBuf partials, counters;
     pfem::DevBuf wstate, wacc;
     pfem::Workspace ws;
+    pfem::AmgHierarchy amg;  // levels == 0: block-Jacobi preconditioning
     int lanes = 8;
     std::atomic<int> busy{0};  // a serving request holds the case
     std::unique_ptr<pfem::Engine> engine;  // direct calls

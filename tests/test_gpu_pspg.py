@@ -121,16 +121,17 @@ def test_spmv_matches_scipy(ctx, name):
     assert np.abs(y - ref).max() <= 1e-12 * np.abs(A).sum(axis=1).max() * np.abs(x).max()
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precond", ["jacobi", "amg"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_ir"])
 @pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
-def test_solve_relres_on_oracle_system(ctx, name, precision):
+def test_solve_relres_on_oracle_system(ctx, name, precision, precond):
     mesh = small_meshes()[name]
-    say(f"[solve] {name} {precision}")
+    say(f"[solve] {name} {precision} {precond}")
     d, N = mesh.dim, mesh.n_nodes
     q = rand_state(mesh, 4)
     dt = 1e-3 if d == 2 else 5e-4
     ref = O.pspg_assemble(mesh, q, q, dt=dt)
-    c = make_case(ctx, mesh, precision=precision, krylov_rtol=1e-10)
+    c = make_case(ctx, mesh, precision=precision, krylov_rtol=1e-10, precond=precond)
     c.assemble(dt, q_prev=q, q_iter=q)
     rc, x, st = c.solve(x0=np.zeros_like(q))
     assert rc == 0, st
@@ -141,6 +142,30 @@ def test_solve_relres_on_oracle_system(ctx, name, precision):
 
     xs = spla.spsolve(A.tocsc(), ref.b)
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-6
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_amg_cuts_iterations(ctx, dim):
+    """On a mesh large enough for a multi-level hierarchy, the multigrid-preconditioned solve
+    reaches the same true relative residual on the oracle's system in a fraction of the
+    block-Jacobi iterations."""
+    mesh = mg.column_2d_nodes(20000, h=0.01, seed=2) if dim == 2 else mg.column_3d_nodes(30000, h=0.01, seed=2)
+    say(f"[amg] dim={dim} N={mesh.n_nodes}")
+    q = mg.pspg_hydrostatic(mesh)
+    dt = 1e-3 if dim == 2 else 5e-4
+    ref = O.pspg_assemble(mesh, q, q, dt=dt)
+    A = ref.to_scipy()
+    its = {}
+    for precond in ("jacobi", "amg"):
+        c = make_case(ctx, mesh, krylov_rtol=1e-8, precond=precond)
+        c.assemble(dt, q_prev=q, q_iter=q)
+        rc, x, st = c.solve(x0=np.zeros_like(q))
+        relres = np.linalg.norm(ref.b - A @ x) / np.linalg.norm(ref.b)
+        say(f"  {precond}: rc={rc} iters={st['krylov_iters']} relres={relres:.2e}")
+        assert rc == 0 and relres <= 1e-8, (precond, relres, st)
+        its[precond] = st["krylov_iters"]
+        c.close()
+    assert its["amg"] * (5 if dim == 2 else 2) <= its["jacobi"], its
 
 
 @pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])

@@ -27,23 +27,24 @@ def say(msg):
 
 
 def specs():
-    """(kind, dim, precision / state, mesh) of a small mixed population."""
+    """(kind, dim, precision / state, mesh, preconditioner) of a small mixed population."""
     return [
-        ("PSPG", 2, "fp64", mg.dambreak_2d()),
-        ("PSPG", 2, "mixed", mg.column_2d_nodes(900, h=0.02, seed=4)),
-        ("PSPG", 3, "fp64", mg.dambreak_3d(h=0.05, col=(0.5, 0.5, 1.0), seed=1)),
-        ("PSPG", 3, "mixed", mg.column_3d_nodes(3000, h=0.02, seed=5)),
-        ("WC", 2, "fp64", mg.tank_2d(h=0.05, size=(1.0, 0.5), fill=0.6, seed=2)),
-        ("WC", 3, "fp32", mg.tank_3d(h=0.05, size=(0.4, 0.4, 0.3), fill=0.7, seed=3)),
-        ("PSPG", 2, "fp64", mg.column_2d_nodes(2500, h=0.01, seed=6)),
-        ("WC", 3, "fp64", mg.tank_nodes(3, 4000, h=0.02, seed=7)),
+        ("PSPG", 2, "fp64", mg.dambreak_2d(), "jacobi"),
+        ("PSPG", 2, "mixed", mg.column_2d_nodes(900, h=0.02, seed=4), "jacobi"),
+        ("PSPG", 3, "fp64", mg.dambreak_3d(h=0.05, col=(0.5, 0.5, 1.0), seed=1), "amg"),
+        ("PSPG", 3, "mixed", mg.column_3d_nodes(3000, h=0.02, seed=5), "jacobi"),
+        ("WC", 2, "fp64", mg.tank_2d(h=0.05, size=(1.0, 0.5), fill=0.6, seed=2), None),
+        ("WC", 3, "fp32", mg.tank_3d(h=0.05, size=(0.4, 0.4, 0.3), fill=0.7, seed=3), None),
+        ("PSPG", 2, "fp64", mg.column_2d_nodes(2500, h=0.01, seed=6), "amg"),
+        ("WC", 3, "fp64", mg.tank_nodes(3, 4000, h=0.02, seed=7), None),
+        ("PSPG", 2, "mixed", mg.column_2d_nodes(6000, h=0.01, seed=8), "amg"),
     ]
 
 
 def make(ctx, spec):
-    kind, dim, prec, mesh = spec
+    kind, dim, prec, mesh, precond = spec
     if kind == "PSPG":
-        c = pfem_amd.Case(ctx, dim, kind="PSPG", precision=prec, krylov_rtol=1e-9)
+        c = pfem_amd.Case(ctx, dim, kind="PSPG", precision=prec, krylov_rtol=1e-9, precond=precond)
         c.set_mesh(mesh)
         c.set_state(mg.pspg_hydrostatic(mesh))
     else:
