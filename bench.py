@@ -324,6 +324,8 @@ def main():
     ap.add_argument("--dambreak-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--only", choices=["batched", "plain"], default=None,
+                    help="run one path once (saturated) and print its result only (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (cases run whole on one GPU; "
@@ -360,6 +362,14 @@ def main():
         reset(warm, batched, meshes, states)
         run_plain(warm, plain, poisson=False)
         run_batched(ctx, warm, batched, poisson=False, budget=budget, slots=args.slots)
+
+    if args.only:
+        cases = batched if args.only == "batched" else plain
+        reset(stream, cases, meshes, states)
+        res = (run_batched(ctx, stream, cases, False, budget, args.slots) if args.only == "batched" else
+               run_plain(stream, cases, False))
+        print(json.dumps({"path": args.only, "requests": n_req, **res}), flush=True)
+        return
 
     results = {}
     for name, poisson in (("saturated", False), ("poisson", True)):

@@ -55,9 +55,10 @@ extern "C" {
 #define PFEM_RES_AX_F 2 /* "Ax_f" */
 
 #define PFEM_PREC_FP64 0  /* FP64 assembly, SpMV and Krylov */
-#define PFEM_PREC_MIXED 1    /* BiCGSTAB whose SpMVs read an FP32 copy of A's values (vectors,
-                                products and sums in FP64), with residual replacement by the
-                                FP64 residual b - A x; converges on the FP64 system */
+#define PFEM_PREC_MIXED 1    /* FP64 BiCGSTAB (converges on the FP64 system) preconditioned by
+                                the multigrid hierarchy in FP32 (values, inverses, vectors and
+                                SpMVs; the coarsest dense inverse stays FP64) for 3D cases; 2D
+                                hierarchies and block-Jacobi cases run as FP64 */
 #define PFEM_PREC_MIXED_IR 2 /* FP32 inner BiCGSTAB (FP32 vectors) on the FP32 copy of A inside
                                 FP64 iterative refinement; falls back to FP64 BiCGSTAB when the
                                 FP32 corrections stop reducing the FP64 residual */
@@ -98,7 +99,7 @@ typedef struct pfem_case_desc {
 } pfem_case_desc;
 
 /* BiCGSTAB preconditioner of a PSPG case (chosen at pfem_case_set_mesh). */
-#define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 5,000 nodes and 3D cases of >= 150,000 */
+#define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 5,000 nodes and 3D cases of >= 400,000 */
 #define PFEM_PRECOND_JACOBI 1 /* block Jacobi over each node's (dim+1) x (dim+1) diagonal block */
 #define PFEM_PRECOND_AMG 2    /* aggregation multigrid V-cycle, block-Jacobi smoothing */
 

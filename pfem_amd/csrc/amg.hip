@@ -27,9 +27,10 @@ __device__ __forceinline__ long long row_start_of(const int4& inf, int s, int D,
     return (s < D) ? (long long)s * mom_nnz + inf.x : (long long)D * mom_nnz + inf.y;
 }
 
+// Entry q of the level's input (level 0: the FP64 Krylov vector p or s).
 template <typename T>
-__device__ __forceinline__ const T* level_in(const AmgView<T>& V, int which) {
-    return V.level == 0 ? V.r0[which] : V.r;
+__device__ __forceinline__ T in_at(const AmgView<T>& V, int which, long long q) {
+    return V.level == 0 ? (T)V.r0[which][q] : V.r[q];
 }
 
 // Invert the nf x nf block m (nf <= 4; rows/cols >= nf padded with identity) by
@@ -199,7 +200,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(const AmgView<T>* __restri
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < n * n; e += kBlock) V.dense[e] = (T)a[e];
+    for (int e = threadIdx.x; e < n * n; e += kBlock) V.dense[e] = a[e];
 }
 
 // ---- V-cycle -----------------------------------------------------------------------------
@@ -211,13 +212,12 @@ __global__ __launch_bounds__(kBlock) void k_amg_pre(const AmgView<T>* __restrict
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
-    const T* in = level_in(V, which);
     const int nf = V.A.nf, N = V.A.N;
     const T om = (T)V.omega;
     for (long long i = (long long)task.block * kBlock + threadIdx.x; i < N; i += (long long)task.nblocks * kBlock) {
         T x[kMaxNf];
 #pragma unroll
-        for (int t = 0; t < kMaxNf; ++t) x[t] = (t < nf) ? in[i + (long long)t * N] : T(0);
+        for (int t = 0; t < kMaxNf; ++t) x[t] = (t < nf) ? in_at(V, which, i + (long long)t * N) : T(0);
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s) {
             if (s >= nf) continue;
@@ -234,14 +234,13 @@ template <typename T, int L>
 __device__ __forceinline__ void res_body(const AmgView<T>& V, const BlockTask& task, int which) {
     constexpr int NPB = kBlock / L;
     const int lane = threadIdx.x % L;
-    const T* in = level_in(V, which);
     const int N = V.A.N;
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
         T y[kMaxNf];
         node_spmv<L>(V.A, (const T*)V.w, i, lane, y);
         if (lane < V.A.nf) {
             const long long q = i + (long long)lane * N;
-            V.t[q] = in[q] - pick(y, lane);
+            V.t[q] = in_at(V, which, q) - pick(y, lane);
         }
     }
 }
@@ -293,7 +292,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __res
     const int n = V.nc;
     for (int row = threadIdx.x; row < n; row += kBlock) {
         double acc = 0.0;
-        for (int j = 0; j < n; ++j) acc += (double)V.dense[row * n + j] * (double)V.r[j];
+        for (int j = 0; j < n; ++j) acc += V.dense[row * n + j] * (double)V.r[j];
         V.z[row] = (T)acc;
     }
 }
@@ -322,7 +321,6 @@ template <typename T, int L>
 __device__ __forceinline__ void post_body(const AmgView<T>& V, const BlockTask& task, int which) {
     constexpr int NPB = kBlock / L;
     const int lane = threadIdx.x % L;
-    const T* in = level_in(V, which);
     const int N = V.A.N, nf = V.A.nf;
     const T om = (T)V.omega;
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
@@ -331,14 +329,15 @@ __device__ __forceinline__ void post_body(const AmgView<T>& V, const BlockTask& 
         if (lane < nf) {
             T d[kMaxNf];
 #pragma unroll
-            for (int t = 0; t < kMaxNf; ++t) d[t] = (t < nf) ? in[i + (long long)t * N] - pick(y, t) : T(0);
+            for (int t = 0; t < kMaxNf; ++t)
+                d[t] = (t < nf) ? in_at(V, which, i + (long long)t * N) - pick(y, t) : T(0);
             T acc = 0;
 #pragma unroll
             for (int t = 0; t < kMaxNf; ++t)
                 if (t < nf) acc += V.binv[(long long)(lane * nf + t) * N + i] * d[t];
             const long long q = i + (long long)lane * N;
             const T out = V.w[q] + om * acc;
-            if (V.level == 0) V.out0[which][q] = out;
+            if (V.level == 0) V.out0[which][q] = (double)out;
             else V.z[q] = out;
         }
     }
@@ -357,32 +356,54 @@ __global__ __launch_bounds__(kBlock) void k_amg_post(const AmgView<T>* __restric
 }
 
 // ---- launchers ---------------------------------------------------------------------------
-void launch_amg_galerkin(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_galerkin<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_galerkin(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_galerkin<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_bjac(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_bjac<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_bjac(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_bjac<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_dense(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_dense<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_dense(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_dense<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_pre(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
-    k_amg_pre<double><<<nt, kBlock, 0, s>>>(v, t, which);
+template <typename T>
+void launch_amg_pre(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_pre<T><<<nt, kBlock, 0, s>>>(v, t, which);
 }
-void launch_amg_res(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
-    k_amg_res<double><<<nt, kBlock, 0, s>>>(v, t, which);
+template <typename T>
+void launch_amg_res(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_res<T><<<nt, kBlock, 0, s>>>(v, t, which);
 }
-void launch_amg_restrict(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_restrict<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_restrict(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_restrict<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_coarsest(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_coarsest<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_coarsest(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_coarsest<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_prolong(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_prolong<double><<<nt, kBlock, 0, s>>>(v, t);
+template <typename T>
+void launch_amg_prolong(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_prolong<T><<<nt, kBlock, 0, s>>>(v, t);
 }
-void launch_amg_post(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
-    k_amg_post<double><<<nt, kBlock, 0, s>>>(v, t, which);
+template <typename T>
+void launch_amg_post(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
+    k_amg_post<T><<<nt, kBlock, 0, s>>>(v, t, which);
 }
+
+#define PFEM_AMG_INSTANTIATE(T)                                                                    \
+    template void launch_amg_galerkin<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);   \
+    template void launch_amg_bjac<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);       \
+    template void launch_amg_dense<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);      \
+    template void launch_amg_pre<T>(const AmgView<T>*, const BlockTask*, int, int, hipStream_t);   \
+    template void launch_amg_res<T>(const AmgView<T>*, const BlockTask*, int, int, hipStream_t);   \
+    template void launch_amg_restrict<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);   \
+    template void launch_amg_coarsest<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);   \
+    template void launch_amg_prolong<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);    \
+    template void launch_amg_post<T>(const AmgView<T>*, const BlockTask*, int, int, hipStream_t);
+PFEM_AMG_INSTANTIATE(double)
+PFEM_AMG_INSTANTIATE(float)
 
 }  // namespace pfem

@@ -62,12 +62,18 @@ enum Tab : int {
     T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
     T_RES, T_IRR = T_RES + 3, T_K64 = T_IRR + 3, T_K32 = T_K64 + 3, T_F64 = T_K32 + 3, T_F32 = T_F64 + 3,
     T_WCN = T_F32 + 3, T_WCE2 = T_WCN + 2, T_WCE3 = T_WCE2 + 2, T_CFE2 = T_WCE3 + 2, T_CFE3 = T_CFE2 + 2,
-    // multigrid setup per level (Galerkin, block Jacobi), coarsest dense inverse
-    T_AG = T_CFE3 + 2, T_AB = T_AG + kAmgMaxLevels, T_AD = T_AB + kAmgMaxLevels,
-    // V-cycle per level: node ops, SpMV ops, restriction; coarsest solve
-    T_VN = T_AD + 1, T_VS = T_VN + kAmgMaxLevels, T_VR = T_VS + kAmgMaxLevels, T_VC = T_VR + kAmgMaxLevels,
-    T_NTAB = T_VC + 1
+    T_AMG = T_CFE3 + 2,  // multigrid tables: FP64 hierarchies, then FP32 ones (A_N each)
 };
+
+// Multigrid tables of one hierarchy type, relative to T_AMG + type * A_N: setup per level
+// (Galerkin, block Jacobi), coarsest dense inverse; V-cycle per level (node ops, SpMV ops,
+// restriction), coarsest solve.
+enum AmgTab : int {
+    A_G = 0, A_B = A_G + kAmgMaxLevels, A_D = A_B + kAmgMaxLevels,
+    A_VN = A_D + 1, A_VS = A_VN + kAmgMaxLevels, A_VR = A_VS + kAmgMaxLevels, A_VC = A_VR + kAmgMaxLevels,
+    A_N = A_VC + 1
+};
+constexpr int T_NTAB = T_AMG + 2 * A_N;
 
 void append(std::vector<BlockTask>& t, int slot, long long work, int cap, int pad = 0) {
     long long nb = (work + kBlock - 1) / kBlock;
@@ -137,6 +143,13 @@ struct WsLayout {
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
 };
+// The multigrid hierarchy runs in FP32 for PFEM_PREC_MIXED 3D cases (under FP64 BiCGSTAB).
+// 2D hierarchies stay FP64: there an FP32 V-cycle raised the BiCGSTAB iteration count by
+// 20-40 % (more than the bandwidth it saves), while 3D counts did not change.
+bool amg_fp32(const pfem_case* c) {
+    return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 3;
+}
+
 WsLayout ws_layout(const pfem_case* c) {
     WsLayout L{};
     const size_t n = (size_t)c->n, nnz = (size_t)c->pat.nnz, nf = (size_t)c->dim + 1, N = (size_t)c->N;
@@ -153,23 +166,25 @@ WsLayout ws_layout(const pfem_case* c) {
     L.x_prev = take(8 * n);
     L.pos_save = take(8 * (size_t)c->dim * N);
     L.kv64 = take(8 * 8 * n);
-    if (c->d.precision == PFEM_PREC_MIXED || c->d.precision == PFEM_PREC_MIXED_IR) L.vals32 = take(4 * nnz);
-    if (c->d.precision == PFEM_PREC_MIXED_IR) {
+    const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
+    if (ir || amg_fp32(c)) {  // FP32 copies of the values and of the block-Jacobi inverse
+        L.vals32 = take(4 * nnz);
         L.binv32 = take(4 * nf * nf * N);
-        L.kv32 = take(4 * 10 * n);
     }
+    if (ir) L.kv32 = take(4 * 10 * n);
     const AmgHierarchy& H = c->amg;
     if (H.levels > 1) {
-        L.amg_w0 = take(8 * n);
-        L.amg_t0 = take(8 * n);
+        const size_t es = amg_fp32(c) ? 4 : 8;
+        L.amg_w0 = take(es * n);
+        L.amg_t0 = take(es * n);
         for (int l = 1; l < H.levels; ++l) {
             const AmgLevel& lv = H.lev[l];
             const size_t nl = (size_t)nf * lv.N;
-            L.amg_lv[l][0] = take(8 * (size_t)lv.nnz);
-            L.amg_lv[l][1] = take(8 * nf * nf * (size_t)lv.N);
-            for (int k = 2; k < 6; ++k) L.amg_lv[l][k] = take(8 * nl);
+            L.amg_lv[l][0] = take(es * (size_t)lv.nnz);
+            L.amg_lv[l][1] = take(es * nf * nf * (size_t)lv.N);
+            for (int k = 2; k < 6; ++k) L.amg_lv[l][k] = take(es * nl);
         }
-        L.amg_dense = take(8 * (size_t)H.coarse_n * H.coarse_n);
+        L.amg_dense = take(8 * (size_t)H.coarse_n * H.coarse_n);  // FP64 in both types
     }
     L.total = o;
     return L;
@@ -194,20 +209,20 @@ void carve_workspace(pfem_case* c, char* base) {
     w.pos_save = reinterpret_cast<double*>(base + L.pos_save);
     w.kv64 = reinterpret_cast<double*>(base + L.kv64);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
-    const bool v32 = ir || c->d.precision == PFEM_PREC_MIXED;
+    const bool v32 = ir || amg_fp32(c);
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
-    w.binv32 = ir ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
+    w.binv32 = v32 ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
     w.kv32 = ir ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
     const AmgHierarchy& H = c->amg;
     w.amg_w0 = w.amg_t0 = w.amg_dense = nullptr;
     for (auto& lv : w.amg_lv)
         for (auto& p : lv) p = nullptr;
     if (H.levels > 1) {
-        w.amg_w0 = reinterpret_cast<double*>(base + L.amg_w0);
-        w.amg_t0 = reinterpret_cast<double*>(base + L.amg_t0);
-        w.amg_dense = reinterpret_cast<double*>(base + L.amg_dense);
+        w.amg_w0 = base + L.amg_w0;
+        w.amg_t0 = base + L.amg_t0;
+        w.amg_dense = base + L.amg_dense;
         for (int l = 1; l < H.levels; ++l)
-            for (int k = 0; k < 6; ++k) w.amg_lv[l][k] = reinterpret_cast<double*>(base + L.amg_lv[l][k]);
+            for (int k = 0; k < 6; ++k) w.amg_lv[l][k] = base + L.amg_lv[l][k];
     }
 }
 
@@ -269,8 +284,9 @@ struct Engine::Impl {
     hipStream_t s = nullptr;
     int S = 0;
     Arena* arena = nullptr;
-    DevBuf d_pv, d_pic, d_k64, d_k32, d_ref, d_wc64, d_wc32, d_scal, d_tab, d_amg;
-    AmgView<double>* h_amg = nullptr;  // [S * kAmgMaxLevels]
+    DevBuf d_pv, d_pic, d_k64, d_k32, d_ref, d_wc64, d_wc32, d_scal, d_tab, d_amg64, d_amg32;
+    AmgView<double>* h_amg64 = nullptr;  // [S * kAmgMaxLevels]
+    AmgView<float>* h_amg32 = nullptr;
     double amg_omega = 0.9;  // fewest BiCGSTAB iterations among 0.5 / 0.7 / 0.9 on 2D and 3D PSPG systems
     PspgView* h_pv = nullptr;
     PicardView* h_pic = nullptr;
@@ -295,7 +311,8 @@ struct Engine::Impl {
         for (auto& e : pev)
             if (e) (void)hipEventDestroy(e);
         for (void* p : {(void*)h_pv, (void*)h_pic, (void*)h_k64, (void*)h_k32, (void*)h_ref, (void*)h_wc64,
-                        (void*)h_wc32, (void*)h_scal, (void*)h_reset, (void*)h_tab, (void*)h_amg})
+                        (void*)h_wc32, (void*)h_scal, (void*)h_reset, (void*)h_tab, (void*)h_amg64,
+                        (void*)h_amg32})
             if (p) (void)hipHostFree(p);
     }
 
@@ -309,9 +326,11 @@ struct Engine::Impl {
         d_wc64.reserve(sizeof(WcView<double>) * n);
         d_wc32.reserve(sizeof(WcView<float>) * n);
         d_scal.reserve(sizeof(SlotScal) * n);
-        d_amg.reserve(sizeof(AmgView<double>) * n * kAmgMaxLevels);
+        d_amg64.reserve(sizeof(AmgView<double>) * n * kAmgMaxLevels);
+        d_amg32.reserve(sizeof(AmgView<float>) * n * kAmgMaxLevels);
         PFEM_HIP(hipMemsetAsync(d_scal.p, 0, sizeof(SlotScal) * n, s));
-        pinned_alloc(h_amg, (size_t)n * kAmgMaxLevels);
+        pinned_alloc(h_amg64, (size_t)n * kAmgMaxLevels);
+        pinned_alloc(h_amg32, (size_t)n * kAmgMaxLevels);
         pinned_alloc(h_pv, n);
         pinned_alloc(h_pic, n);
         pinned_alloc(h_k64, n);
@@ -334,8 +353,11 @@ struct Engine::Impl {
     }
 
     void load_views(Req* r);
-    void reset_kscal(int slot, int which, double rtol, int maxit, double atol = 0.0, double repl2 = 0.0);
-    void restart_kscal(int slot, int which, int mode);
+    template <class T>
+    void load_amg(pfem_case* c, int slot, const KrylovView<double>& k, AmgView<T>* h, DevBuf& d,
+                  const NodeMat<T>& A0, const T* binv0);
+    void reset_kscal(int slot, int which, double rtol, int maxit, double atol = 0.0);
+    void restart_kscal(int slot, int which);
 };
 
 Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl) {
@@ -378,7 +400,6 @@ void Engine::Impl::load_views(Req* r) {
         const Pattern& P = c->pat;
         const Workspace& w = c->ws;
         const bool mixed = c->d.precision == PFEM_PREC_MIXED_IR;  // FP32 inner solves
-        const bool v32 = mixed || c->d.precision == PFEM_PREC_MIXED;  // FP32 copy of the values
         PspgView& pv = h_pv[slot];
         pv = PspgView{};
         pv.dim = dim;
@@ -397,10 +418,10 @@ void Engine::Impl::load_views(Req* r) {
         pv.dir_vals = c->dir_vals.as<double>();
         pv.vals = w.vals;
         pv.b = w.b;
-        pv.vals32 = v32 ? w.vals32 : nullptr;
+        pv.vals32 = w.vals32;  // FP32 copies, when the precision mode has them
         pv.diag_pos = P.diag_pos.as<int>();
         pv.binv = w.binv;
-        pv.binv32 = mixed ? w.binv32 : nullptr;
+        pv.binv32 = w.binv32;
         pv.nnz = P.nnz;
         pv.rho = c->d.rho;
         pv.mu = c->d.mu;
@@ -434,8 +455,6 @@ void Engine::Impl::load_views(Req* r) {
         k.nf = dim + 1;
         k.nblk = row_blocks(n);
         k.A = node_mat<double>(P, dim, w.vals);
-        k.use32 = c->d.precision == PFEM_PREC_MIXED ? 1 : 0;
-        k.A32 = node_mat<float>(P, dim, k.use32 ? w.vals32 : nullptr);
         k.binv = w.binv;
         k.b = w.b;
         k.x = w.x;
@@ -451,67 +470,12 @@ void Engine::Impl::load_views(Req* r) {
         k.partials = c->partials.as<double>();
         k.counters = c->counters.as<unsigned>();
         k.sc = &ds->k[0];
-        const AmgHierarchy& H = c->amg;
-        k.amg = H.levels > 1 ? 1 : 0;
+        k.amg = c->amg.levels > 1 ? 1 : 0;
         if (k.amg) {
-            AmgView<double>* av = h_amg + (size_t)slot * kAmgMaxLevels;
-            for (int l = 0; l < H.levels; ++l) {
-                AmgView<double>& V = av[l];
-                V = AmgView<double>{};
-                V.level = l;
-                V.omega = amg_omega;
-                V.done_flag = &ds->k[0].done;
-                if (l == 0) {
-                    V.A = node_mat<double>(P, dim, w.vals);
-                    V.binv = w.binv;
-                    V.lanes = c->lanes;
-                    V.mask = 1;
-                    V.r0[0] = k.p;
-                    V.r0[1] = k.s;
-                    V.out0[0] = k.ph;
-                    V.out0[1] = k.sh;
-                    V.w = w.amg_w0;
-                    V.t = w.amg_t0;
-                } else {
-                    const AmgLevel& L = H.lev[l];
-                    NodeMat<double> A{};
-                    A.N = L.N;
-                    A.nf = dim + 1;
-                    A.mom_nnz = L.mom_nnz;
-                    A.info = L.node_info.as<int4>();
-                    A.nbr_ptr = L.nbr_ptr.as<int>();
-                    A.nbr_idx = L.nbr_idx.as<int>();
-                    A.vals = w.amg_lv[l][0];
-                    V.A = A;
-                    V.binv = w.amg_lv[l][1];
-                    V.lanes = L.lanes;
-                    V.r = w.amg_lv[l][2];
-                    V.w = w.amg_lv[l][3];
-                    V.t = w.amg_lv[l][4];
-                    V.z = w.amg_lv[l][5];
-                    V.F = av[l - 1].A;
-                    V.fmask = l == 1 ? 1 : 0;
-                    V.gal_ptr = L.gal_ptr.as<int>();
-                    V.gal_src = L.gal_src.as<int2>();
-                    V.self_rank = L.self_rank.as<int>();
-                    V.vals_out = w.amg_lv[l][0];
-                    V.binv_out = w.amg_lv[l][1];
-                }
-                if (l + 1 < H.levels) {
-                    const AmgLevel& Lc = H.lev[l + 1];
-                    V.next_N = Lc.N;
-                    V.agg_ptr = Lc.agg_ptr.as<int>();
-                    V.agg_nodes = Lc.agg_nodes.as<int>();
-                    V.fine_agg = Lc.fine_agg.as<int>();
-                    V.next_r = w.amg_lv[l + 1][2];
-                    V.next_z = w.amg_lv[l + 1][5];
-                } else {
-                    V.nc = H.coarse_n;
-                    V.dense = w.amg_dense;
-                }
-            }
-            PFEM_HIP(hipMemcpyAsync(d_amg.as<AmgView<double>>() + (size_t)slot * kAmgMaxLevels, av,
-                                    sizeof(AmgView<double>) * H.levels, hipMemcpyHostToDevice, s));
+            if (amg_fp32(c))
+                load_amg<float>(c, slot, k, h_amg32, d_amg32, node_mat<float>(P, dim, w.vals32), w.binv32);
+            else
+                load_amg<double>(c, slot, k, h_amg64, d_amg64, node_mat<double>(P, dim, w.vals), w.binv);
         }
 
         KrylovView<float>& f = h_k32[slot];
@@ -597,25 +561,93 @@ void Engine::Impl::load_views(Req* r) {
     }
 }
 
-void Engine::Impl::reset_kscal(int slot, int which, double rtol, int maxit, double atol, double repl2) {
+void Engine::Impl::reset_kscal(int slot, int which, double rtol, int maxit, double atol) {
     KScal& k = h_reset[slot].k[which];
     std::memset(&k, 0, sizeof(KScal));
     k.rtol = rtol;
     k.atol2 = atol > 0 ? atol * atol : 0.0;
-    k.repl2 = repl2;
     k.maxit = maxit;
     k.req_init = 1;
     PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
 }
 
-// Restart (mode 1) or residual replacement (mode 2) of a solve from its current iterate.
-// Between ticks no kernel runs, so the device scalars equal the last readback.
-void Engine::Impl::restart_kscal(int slot, int which, int mode) {
+// Restart of a solve from its current iterate.  Between ticks no kernel runs, so the device
+// scalars equal the last readback.
+void Engine::Impl::restart_kscal(int slot, int which) {
     KScal& k = h_reset[slot].k[which];
     k = h_scal[slot].k[which];
     k.done = 0;
-    k.req_init = mode;
+    k.req_init = 1;
     PFEM_HIP(hipMemcpyAsync(&dslot(slot)->k[which], &k, sizeof(KScal), hipMemcpyHostToDevice, s));
+}
+
+// Views of every level of the case's multigrid in arithmetic type T (slot * kAmgMaxLevels
+// + level); A0 / binv0 are the system's values and block-Jacobi inverse in type T.
+template <class T>
+void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k, AmgView<T>* h, DevBuf& d,
+                            const NodeMat<T>& A0, const T* binv0) {
+    const AmgHierarchy& H = c->amg;
+    const Workspace& w = c->ws;
+    SlotScal* ds = dslot(slot);
+    AmgView<T>* av = h + (size_t)slot * kAmgMaxLevels;
+    auto arr = [](char* p) { return reinterpret_cast<T*>(p); };
+    for (int l = 0; l < H.levels; ++l) {
+        AmgView<T>& V = av[l];
+        V = AmgView<T>{};
+        V.level = l;
+        V.omega = amg_omega;
+        V.done_flag = &ds->k[0].done;
+        if (l == 0) {
+            V.A = A0;
+            V.binv = binv0;
+            V.lanes = c->lanes;
+            V.mask = 1;
+            V.r0[0] = k.p;
+            V.r0[1] = k.s;
+            V.out0[0] = k.ph;
+            V.out0[1] = k.sh;
+            V.w = arr(w.amg_w0);
+            V.t = arr(w.amg_t0);
+        } else {
+            const AmgLevel& L = H.lev[l];
+            NodeMat<T> A{};
+            A.N = L.N;
+            A.nf = c->dim + 1;
+            A.mom_nnz = L.mom_nnz;
+            A.info = L.node_info.as<int4>();
+            A.nbr_ptr = L.nbr_ptr.as<int>();
+            A.nbr_idx = L.nbr_idx.as<int>();
+            A.vals = arr(w.amg_lv[l][0]);
+            V.A = A;
+            V.binv = arr(w.amg_lv[l][1]);
+            V.lanes = L.lanes;
+            V.r = arr(w.amg_lv[l][2]);
+            V.w = arr(w.amg_lv[l][3]);
+            V.t = arr(w.amg_lv[l][4]);
+            V.z = arr(w.amg_lv[l][5]);
+            V.F = av[l - 1].A;
+            V.fmask = l == 1 ? 1 : 0;
+            V.gal_ptr = L.gal_ptr.as<int>();
+            V.gal_src = L.gal_src.as<int2>();
+            V.self_rank = L.self_rank.as<int>();
+            V.vals_out = arr(w.amg_lv[l][0]);
+            V.binv_out = arr(w.amg_lv[l][1]);
+        }
+        if (l + 1 < H.levels) {
+            const AmgLevel& Lc = H.lev[l + 1];
+            V.next_N = Lc.N;
+            V.agg_ptr = Lc.agg_ptr.as<int>();
+            V.agg_nodes = Lc.agg_nodes.as<int>();
+            V.fine_agg = Lc.fine_agg.as<int>();
+            V.next_r = arr(w.amg_lv[l + 1][2]);
+            V.next_z = arr(w.amg_lv[l + 1][5]);
+        } else {
+            V.nc = H.coarse_n;
+            V.dense = reinterpret_cast<double*>(w.amg_dense);
+        }
+    }
+    PFEM_HIP(hipMemcpyAsync(d.as<AmgView<T>>() + (size_t)slot * kAmgMaxLevels, av, sizeof(AmgView<T>) * H.levels,
+                            hipMemcpyHostToDevice, s));
 }
 
 namespace {
@@ -627,10 +659,6 @@ double step_atol(const Req* r) {
     const pfem_case_desc& d = r->c->d;
     return (r->kind == RK_STEP && d.residual == PFEM_RES_AX_F && d.min_res > 0) ? 0.1 * d.min_res : 0.0;
 }
-
-// FP32 values in the SpMVs: the recurrence residual is replaced by the FP64 one each time
-// it has dropped by a factor 10 (1e-2 in ||r||^2) since the last true residual.
-double repl_factor(const Req* r) { return r->c->d.precision == PFEM_PREC_MIXED ? 1e-2 : 0.0; }
 
 // The FP32 inner solves stopped helping (no progress, a diverging inner solve, or a
 // non-finite iterate): finish with FP64 BiCGSTAB from the current iterate (from zero when
@@ -646,7 +674,7 @@ void start_fallback(Engine::Impl& I, Req* r, bool zero_x) {
     r->verifying = false;
     r->restarts = 0;
     r->kit_prev = -1;
-    I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r), repl_factor(r));
+    I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
     r->ops |= OP_K64_INIT | OP_K64_RUN;
     r->wait = W_K64;
 }
@@ -666,7 +694,7 @@ void solve_start(Engine::Impl& I, Req* r) {
         r->ops |= OP_IRR;
         r->wait = W_IR;
     } else {
-        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r), repl_factor(r));
+        I.reset_kscal(r->slot, 0, c->d.krylov_rtol, c->d.krylov_max_iter, step_atol(r));
         r->ops |= OP_K64_INIT | OP_K64_RUN;
         r->wait = W_K64;
     }
@@ -856,22 +884,23 @@ void Engine::tick(std::vector<Req*>& done) {
         const AmgHierarchy& H = c->amg;
         if (c->d.kind == PFEM_KIND_PSPG && H.levels > 1) {
             const int base = sl * kAmgMaxLevels;  // AmgView index of (slot, level 0)
+            std::vector<BlockTask>* A = tab + T_AMG + (amg_fp32(c) ? A_N : 0);
             if (op & OP_ASM) {
                 for (int l = 1; l < H.levels; ++l) {
-                    append(tab[T_AG + l], base + l, (long long)H.lev[l].N * H.lev[l].lanes, kElemCap);
-                    append(tab[T_AB + l], base + l, H.lev[l].N, kElemCap);
+                    append(A[A_G + l], base + l, (long long)H.lev[l].N * H.lev[l].lanes, kElemCap);
+                    append(A[A_B + l], base + l, H.lev[l].N, kElemCap);
                 }
-                tab[T_AD].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
+                A[A_D].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
             }
             if (op & OP_K64_RUN) {
                 for (int l = 0; l + 1 < H.levels; ++l) {
                     const int Nl = l == 0 ? c->N : H.lev[l].N;
                     const int lanes = l == 0 ? c->lanes : H.lev[l].lanes;
-                    append(tab[T_VN + l], base + l, Nl, kElemCap);
-                    append(tab[T_VS + l], base + l, (long long)Nl * lanes, kElemCap);
-                    append(tab[T_VR + l], base + l, H.lev[l + 1].N, kElemCap);
+                    append(A[A_VN + l], base + l, Nl, kElemCap);
+                    append(A[A_VS + l], base + l, (long long)Nl * lanes, kElemCap);
+                    append(A[A_VR + l], base + l, H.lev[l + 1].N, kElemCap);
                 }
-                tab[T_VC].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
+                A[A_VC].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
             }
         }
         const int wi = c->d.wc_state == PFEM_STATE_FP32 ? 1 : 0;
@@ -917,13 +946,19 @@ void Engine::tick(std::vector<Req*>& done) {
     if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
     if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
     if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
-    const AmgView<double>* vamg = I.d_amg.as<AmgView<double>>();
-    for (int l = 1; l < kAmgMaxLevels; ++l) {  // level l's Galerkin sums read level l-1
-        if (!NT(T_AG + l)) continue;
-        launch_amg_galerkin(vamg, T(T_AG + l), NT(T_AG + l), s);
-        launch_amg_bjac(vamg, T(T_AB + l), NT(T_AB + l), s);
-    }
-    if (NT(T_AD)) launch_amg_dense(vamg, T(T_AD), NT(T_AD), s);
+    const AmgView<double>* vamg64 = I.d_amg64.as<AmgView<double>>();
+    const AmgView<float>* vamg32 = I.d_amg32.as<AmgView<float>>();
+    // multigrid setup of both hierarchy types (TA: first table of the type)
+    auto amg_setup = [&](auto* v, int TA) {
+        for (int l = 1; l < kAmgMaxLevels; ++l) {  // level l's Galerkin sums read level l-1
+            if (!NT(TA + A_G + l)) continue;
+            launch_amg_galerkin(v, T(TA + A_G + l), NT(TA + A_G + l), s);
+            launch_amg_bjac(v, T(TA + A_B + l), NT(TA + A_B + l), s);
+        }
+        if (NT(TA + A_D)) launch_amg_dense(v, T(TA + A_D), NT(TA + A_D), s);
+    };
+    amg_setup(vamg64, T_AMG);
+    amg_setup(vamg32, T_AMG + A_N);
     mark(2);
     for (int li = 0; li < 3; ++li)
         if (NT(T_RES + li)) launch_res_axf(kLanes[li], vpic, T(T_RES + li), NT(T_RES + li), s);
@@ -954,19 +989,23 @@ void Engine::tick(std::vector<Req*>& done) {
             }
     }
     // multigrid V-cycle of the cases that have a hierarchy: which = 0 (ph = M^-1 p) or 1 (sh = M^-1 s)
-    auto vcycle = [&](int which) {
+    auto vcycle_type = [&](auto* v, int TA, int which) {
         for (int l = 0; l < kAmgMaxLevels; ++l) {
-            if (!NT(T_VN + l)) break;
-            launch_amg_pre(vamg, T(T_VN + l), NT(T_VN + l), which, s);
-            launch_amg_res(vamg, T(T_VS + l), NT(T_VS + l), which, s);
-            launch_amg_restrict(vamg, T(T_VR + l), NT(T_VR + l), s);
+            if (!NT(TA + A_VN + l)) break;
+            launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
+            launch_amg_res(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
+            launch_amg_restrict(v, T(TA + A_VR + l), NT(TA + A_VR + l), s);
         }
-        if (NT(T_VC)) launch_amg_coarsest(vamg, T(T_VC), NT(T_VC), s);
+        if (NT(TA + A_VC)) launch_amg_coarsest(v, T(TA + A_VC), NT(TA + A_VC), s);
         for (int l = kAmgMaxLevels - 1; l >= 0; --l) {
-            if (!NT(T_VN + l)) continue;
-            launch_amg_prolong(vamg, T(T_VN + l), NT(T_VN + l), s);
-            launch_amg_post(vamg, T(T_VS + l), NT(T_VS + l), which, s);
+            if (!NT(TA + A_VN + l)) continue;
+            launch_amg_prolong(v, T(TA + A_VN + l), NT(TA + A_VN + l), s);
+            launch_amg_post(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
         }
+    };
+    auto vcycle = [&](int which) {
+        vcycle_type(vamg64, T_AMG, which);
+        vcycle_type(vamg32, T_AMG + A_N, which);
     };
     for (int it = 0; it < chunk; ++it) {
         const int p = (I.par + it) & 1;
@@ -1046,20 +1085,15 @@ void Engine::tick(std::vector<Req*>& done) {
                     r->verifying = false;
                     if (k.done == 0) {
                         r->ops = OP_K64_RUN;
-                    } else if (k.done == 4) {
-                        // FP32 values: the device stopped for a residual replacement
-                        I.restart_kscal(r->slot, 0, 2);
-                        r->ops = OP_K64_INIT | OP_K64_RUN;
                     } else if (k.done == 1 && !k.true_rr) {
-                        // converged on the recurrence residual: confirm on the true one.  FP32
-                        // values keep the search directions (the recurrence drifts from the FP64
-                        // residual by design); FP64 restarts from the iterate
+                        // converged on the recurrence residual: restart from the iterate, which
+                        // confirms (or not) on the true residual b - A x
                         r->verifying = true;
-                        I.restart_kscal(r->slot, 0, c->d.precision == PFEM_PREC_MIXED ? 2 : 1);
+                        I.restart_kscal(r->slot, 0);
                         r->ops = OP_K64_INIT | OP_K64_RUN;
                     } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
                         r->restarts++;
-                        I.restart_kscal(r->slot, 0, 1);
+                        I.restart_kscal(r->slot, 0);
                         r->ops = OP_K64_INIT | OP_K64_RUN;
                     } else {
                         int stt = PFEM_OK;
@@ -1078,7 +1112,7 @@ void Engine::tick(std::vector<Req*>& done) {
                         r->ops = OP_K32_RUN;
                     } else if (k.done == 2 && r->restarts < 8 && k.iters < k.maxit) {
                         r->restarts++;
-                        I.restart_kscal(r->slot, 1, 1);
+                        I.restart_kscal(r->slot, 1);
                         r->ops = OP_K32_INIT | OP_K32_RUN;
                     } else {
                         r->ss.krylov_iters += k.iters;

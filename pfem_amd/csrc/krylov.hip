@@ -15,10 +15,6 @@
 // the kernel boundary, so no device-scope fence (an L2 write-back on gfx950) is needed.
 // Scalars: two slots used alternately by iteration parity (KScal in views.hpp).
 //
-// Mixed precision (use32): K_v and K_t read an FP32 copy of the values while every vector,
-// product and sum stays FP64; the init kernel's residual replacement recomputes
-// r = b - A x with the FP64 values, so the solve converges on the FP64 system's residual.
-//
 // Work split: workgroups are dispatched round-robin over the 8 XCDs; the workgroups of a
 // case that land on one XCD own one contiguous range of nodes (every field of those
 // nodes), so the SpMV's x gathers (neighbour nodes) and the vectors handed from one
@@ -127,42 +123,32 @@ __global__ __launch_bounds__(kBlock) void k_spmv(NodeMat<T> A, const T* __restri
     }
 }
 
-// Mode 1 (restart): r = b - A x (or b when x_zero), rh = r, p = v = 0.
-// Mode 2 (residual replacement, van der Vorst & Ye 2000): r = b - A x with the system's
-// own values, keeping rh, p, v and the iteration scalars, so an iteration whose SpMVs read
-// FP32 values converges on the true residual of the FP64 system.
+// (Re)start: r = b - A x (or b when x_zero), rh = r, p = v = 0.
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __restrict__ views,
                                                      const BlockTask* __restrict__ tasks) {
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
-    const int mode = V.sc->req_init;
-    if (!mode) return;
+    if (!V.sc->req_init) return;
     constexpr int NPB = kBlock / L;
     const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
-    const bool restart = mode == 1;
-    const bool from_zero = restart && V.x_zero;
     double part[3] = {0.0, 0.0, 0.0};
     for (long long l = (long long)sg.lb * NPB + threadIdx.x / L; l < sg.ns; l += (long long)sg.cnt * NPB) {
         const long long i = sg.n0 + l;
         T ax[kMaxNf] = {T(0), T(0), T(0), T(0)};
-        if (!from_zero) node_spmv<L>(V.A, V.x, i, lane, ax);
+        if (!V.x_zero) node_spmv<L>(V.A, V.x, i, lane, ax);
         if (lane < V.nf) {
             const long long row = i + (long long)lane * V.nodes;
             const T bi = V.b[row];
             const T r = bi - pick(ax, lane);
             V.r[row] = r;
-            if (restart) {
-                V.rh[row] = r;
-                V.p[row] = T(0);
-                V.v[row] = T(0);
-                if (from_zero) V.x[row] = T(0);
-                part[0] += (double)r * (double)r;
-            } else {
-                part[0] += (double)V.rh[row] * (double)r;
-            }
+            V.rh[row] = r;
+            V.p[row] = T(0);
+            V.v[row] = T(0);
+            if (V.x_zero) V.x[row] = T(0);
+            part[0] += (double)r * (double)r;
             part[1] += (double)r * (double)r;
             part[2] += (double)bi * (double)bi;
         }
@@ -171,17 +157,14 @@ __global__ __launch_bounds__(kBlock) void k_kry_init(const KrylovView<T>* __rest
     write_partials<3>(part, V.partials, task.block, slots);
 }
 
-// One workgroup per listed case: finalises a restart or a residual replacement.  Slot
-// `par` becomes the "previous" slot of the next iteration (which runs with parity par ^ 1):
-// a restart fills it with fresh scalars, a replacement copies the last completed
-// iteration's scalars into it.
+// One workgroup per listed case: finalises a (re)start and writes slot `par` as the
+// "previous" slot of the next iteration (which runs with parity par ^ 1).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __restrict__ views,
                                                          const BlockTask* __restrict__ ids, int count, int par) {
     for (int c = blockIdx.x; c < count; c += gridDim.x) {
         const KrylovView<T>& V = views[ids[c].case_id];
-        const int mode = V.sc->req_init;
-        if (!mode) continue;
+        if (!V.sc->req_init) continue;
         constexpr int slots[3] = {P_RHR, P_RR, P_BB};
         double tot[3];
         reduce_partials<3>(V.partials, V.nblk, slots, tot);
@@ -191,16 +174,10 @@ __global__ __launch_bounds__(kBlock) void k_kry_init_fin(const KrylovView<T>* __
             S.tol2 = S.rtol * S.rtol * tot[2];
             if (S.atol2 > 0.0 && S.atol2 < S.tol2) S.tol2 = S.atol2;
             S.rr = tot[1];
-            if (mode == 1) {
-                S.slot[par].rho = tot[0];
-                S.slot[par].alpha = 1.0;
-                S.slot[par].omega = 1.0;
-            } else if (S.last != par) {
-                S.slot[par] = S.slot[S.last];
-            }
-            S.last = par;
+            S.slot[par].rho = tot[0];
+            S.slot[par].alpha = 1.0;
+            S.slot[par].omega = 1.0;
             S.true_rr = 1;
-            S.rr_ref = tot[1];
             S.req_init = 0;
             if (tot[2] == 0.0 || tot[1] <= S.tol2) S.done = 1;
             else if (!isfinite(tot[1])) S.done = 2;
@@ -228,9 +205,6 @@ __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restric
     if (tot[1] <= S.tol2) stop = 1;
     else if (!isfinite(tot[1]) || tot[0] == 0.0 || !isfinite(beta)) stop = 2;
     else if (S.iters >= S.maxit) stop = 3;
-    else if (S.repl2 > 0.0 && tot[1] <= S.repl2 * S.rr_ref) stop = 4;  // decided on the device:
-                                                                       // the same iteration alone
-                                                                       // or in any batch
     if (stop) {
         if (task.block == 0 && threadIdx.x == 0) {
             S.rr = tot[1];
@@ -293,8 +267,7 @@ __global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restric
     const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
     double part[1] = {0.0};
-    if (V.use32) kry_v_nodes<L>(V, V.A32, sg, lane, part);
-    else kry_v_nodes<L>(V, V.A, sg, lane, part);
+    kry_v_nodes<L>(V, V.A, sg, lane, part);
     constexpr int slots[1] = {P_RV};
     write_partials<1>(part, V.partials, task.block, slots);
 }
@@ -376,8 +349,7 @@ __global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restric
     const Seg sg = xcd_segment(task, V.nodes);
     const int lane = threadIdx.x % L;
     double part[2] = {0.0, 0.0};
-    if (V.use32) kry_t_nodes<L>(V, V.A32, sg, lane, part);
-    else kry_t_nodes<L>(V, V.A, sg, lane, part);
+    kry_t_nodes<L>(V, V.A, sg, lane, part);
     constexpr int slots[2] = {P_TS, P_TT};
     write_partials<2>(part, V.partials, task.block, slots);
 }
@@ -419,7 +391,6 @@ __global__ __launch_bounds__(kBlock) void k_kry_x(const KrylovView<T>* __restric
     if (task.block == 0 && threadIdx.x == 0) {
         S.slot[par].omega = omega;
         S.iters += 1;
-        S.last = par;
         S.true_rr = 0;
     }
 }

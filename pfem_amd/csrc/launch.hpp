@@ -24,16 +24,25 @@ void launch_kry_phase(int phase, int lanes, const KrylovView<T>* v, const BlockT
 template <typename T>
 void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s);
 
-// aggregation multigrid (FP64 hierarchy)
-void launch_amg_galerkin(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_bjac(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_dense(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_pre(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s);
-void launch_amg_res(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s);
-void launch_amg_restrict(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_coarsest(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_prolong(const AmgView<double>* v, const BlockTask* t, int nt, hipStream_t s);
-void launch_amg_post(const AmgView<double>* v, const BlockTask* t, int nt, int which, hipStream_t s);
+// aggregation multigrid (T: FP64 or FP32 hierarchy)
+template <typename T>
+void launch_amg_galerkin(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_bjac(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_dense(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_pre(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s);
+template <typename T>
+void launch_amg_res(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s);
+template <typename T>
+void launch_amg_restrict(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_coarsest(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_prolong(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+template <typename T>
+void launch_amg_post(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s);
 void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_ir_update(const RefineView* v, const BlockTask* t, int nt, hipStream_t s);
 

@@ -54,12 +54,12 @@ struct Workspace {
     float* vals32 = nullptr;
     float* binv32 = nullptr;
     float* kv32 = nullptr;  // 10 n: b, x, r, rh, p, v, s, t, ph, sh
-    // multigrid: level-0 smoother vectors, coarse levels' values / inverses / vectors,
-    // coarsest dense inverse
-    double* amg_w0 = nullptr;
-    double* amg_t0 = nullptr;
-    double* amg_dense = nullptr;
-    double* amg_lv[kAmgMaxLevels][6] = {};  // {vals, binv, r, w, t, z} of level l >= 1
+    // multigrid (FP64 or FP32 arrays): level-0 smoother vectors, coarse levels' values /
+    // inverses / vectors, coarsest dense inverse
+    char* amg_w0 = nullptr;
+    char* amg_t0 = nullptr;
+    char* amg_dense = nullptr;
+    char* amg_lv[kAmgMaxLevels][6] = {};  // {vals, binv, r, w, t, z} of level l >= 1
     // Forgets the arrays (and frees an owned allocation).
     void clear() {
         own.release();

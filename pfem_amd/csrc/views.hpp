@@ -59,16 +59,12 @@ struct KScal {
     double tol2;  // absolute squared tolerance on ||r||^2
     double rtol;
     double atol2; // > 0: ||r||^2 must also fall below it (absolute cap), 0: relative only
-    double repl2;   // > 0: stop for a residual replacement once ||r||^2 <= repl2 * rr_ref
-    double rr_ref;  // ||r||^2 of the last true residual (set by init)
-    int done;     // 0 running, 1 converged, 2 breakdown or non-finite, 3 iteration limit,
-                  // 4 residual replacement due (the host answers with req_init = 2)
+    int done;     // 0 running, 1 converged, 2 breakdown or non-finite, 3 iteration limit
     int iters;
     int maxit;
-    int req_init;  // set by the host: 1 the next init launch restarts this case, 2 it replaces
-                   // the recurrence residual by the true one b - A x (search directions kept)
-    int true_rr;   // 1 when rr is a true residual (set by init, cleared by each iteration)
-    int last;      // slot holding the scalars of the last completed iteration
+    int req_init;  // set by the host: the next init launch (re)starts this case
+    int true_rr;   // 1 when rr is a true residual b - A x (set by init, cleared by each iteration)
+    int pad;
 };
 
 // Slots of the per-workgroup partial sums (kPartStride doubles per workgroup).
@@ -81,11 +77,8 @@ struct KrylovView {
     int n, x_zero;
     int nodes, nf;
     int nblk;       // workgroups of this case in the Krylov launches (partials producers)
-    int use32;      // 1: the iteration SpMVs read A32 (FP32 values; vectors and sums stay T)
     int amg;        // 1: ph and sh come from the multigrid V-cycle (K_p / K_s leave them)
-    int pad2;
-    NodeMat<T> A;         // the system (init and residual replacement always use it)
-    NodeMat<float> A32;   // FP32 copy of the values (use32)
+    NodeMat<T> A;
     const T* binv;  // [nf*nf][nodes] inverse of each node's diagonal (nf x nf) block, SoA
     const T* b;
     T* x;
@@ -103,12 +96,13 @@ struct KrylovView {
 };
 
 // One level of a case's aggregation multigrid (amg.hpp) as seen by the setup and V-cycle
-// kernels; T is the hierarchy's arithmetic type.
+// kernels; T is the hierarchy's arithmetic type (FP64, or FP32 under FP64 BiCGSTAB: the
+// preconditioner only shapes the search directions, the solve converges on the FP64 system).
 constexpr int kAmgMaxDense = 88;  // coarsest-level unknowns inverted densely in LDS
 
 template <typename T>
 struct AmgView {
-    NodeMat<T> A;          // this level's matrix (level 0: the system)
+    NodeMat<T> A;          // this level's matrix (level 0: the system or its FP32 copy)
     const T* binv;         // block-Jacobi inverse, SoA [(nf*nf)][N]
     int level;
     int lanes;             // SpMV lanes per node on this level
@@ -116,8 +110,8 @@ struct AmgView {
     int nc;                // coarsest level: unknowns of the dense inverse
     double omega;          // block-Jacobi damping of the smoother
     const int* done_flag;  // the case's BiCGSTAB done flag (finished solves skip the cycle)
-    const T* r0[2];        // level 0 inputs (p, s)
-    T* out0[2];            // level 0 outputs (ph, sh)
+    const double* r0[2];   // level 0 inputs (p, s of the FP64 BiCGSTAB)
+    double* out0[2];       // level 0 outputs (ph, sh)
     T* r;                  // level >= 1 input
     T* w;                  // smoothed iterate
     T* t;                  // residual
@@ -137,7 +131,8 @@ struct AmgView {
     const int* self_rank;
     T* vals_out;
     T* binv_out;
-    T* dense;  // coarsest: [nc * nc] inverse, row major over unknowns I + s N
+    double* dense;  // coarsest: [nc * nc] inverse, row major over unknowns I + s N (FP64 in
+                    // either type: the coarse operators are the worst conditioned)
 };
 
 // Mixed-precision iterative refinement (FP64 outer residual, FP32 inner solve).
