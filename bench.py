@@ -84,12 +84,13 @@ def initial_state(req, mesh):
     return np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)])
 
 
-def make_case(ctx, req, mesh, variant):
+def make_case(ctx, req, mesh, variant, batched_precision="mixed"):
     import pfem_amd
 
     if req["kind"] == "PSPG":
         c = pfem_amd.Case(ctx, req["dim"], kind="PSPG", rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6,
-                          max_iter=10, precision="mixed" if variant == "batched" else "fp64", krylov_rtol=1e-8,
+                          max_iter=10, precision=batched_precision if variant == "batched" else "fp64",
+                          krylov_rtol=1e-8,
                           krylov_max_iter=20000, inner_rtol=1e-3, inner_max_iter=300)
     else:
         c = pfem_amd.Case(ctx, req["dim"], kind="WC", mu=1e-3, K0=K0, K0p=K0P, rho_star=RHO_STAR,
@@ -324,6 +325,8 @@ def main():
     ap.add_argument("--dambreak-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batched-precision", choices=["fp64", "mixed", "mixed_ir"], default="mixed",
+                    help="PSPG solve precision of the batched path")
     ap.add_argument("--only", choices=["batched", "plain"], default=None,
                     help="run one path once (saturated) and print its result only (for profilers)")
     args = ap.parse_args()
@@ -343,7 +346,7 @@ def main():
         f"{sum(m.n_elems for m in meshes.values())} elements; meshes in {time.perf_counter() - t0:.1f} s")
     t0 = time.perf_counter()
     plain = {r["i"]: make_case(ctx, r, meshes[r["i"]], "plain") for r in stream}
-    batched = {r["i"]: make_case(ctx, r, meshes[r["i"]], "batched") for r in stream}
+    batched = {r["i"]: make_case(ctx, r, meshes[r["i"]], "batched", args.batched_precision) for r in stream}
     for r in stream:
         plain[r["i"]].reserve_workspace()
         if r["kind"] == "PSPG":

@@ -8,7 +8,8 @@
 //   t = r - A w                           (k_amg_res)
 //   r_{l+1} = P^T t                       (k_amg_restrict, gather over aggregate members)
 //   z_{l+1} = V-cycle(l+1)  /  A_c^-1 r_c (k_amg_coarsest, dense inverse on the coarsest)
-//   w += P z_{l+1}                        (k_amg_prolong)
+//   w += alpha P z_{l+1}                  (k_amg_prolong, alpha > 1 over-corrects the
+//                                          piecewise-constant interpolation)
 //   z = w + omega B^-1 (r - A w)          (k_amg_post, post-smoothing)
 // Every reduction is a fixed-order gather, so a case's V-cycle does not depend on the batch.
 #include "common.hpp"
@@ -297,7 +298,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __res
     }
 }
 
-// w(i, s) += z_{l+1}(agg(i), s) (level 0: full rows only).
+// w(i, s) += alpha z_{l+1}(agg(i), s) (level 0: full rows only).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amg_prolong(const AmgView<T>* __restrict__ views,
                                                         const BlockTask* __restrict__ tasks) {
@@ -313,7 +314,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_prolong(const AmgView<T>* __rest
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s)
             if (s < nf && (!V.mask || row_full_flags(f, s, D)))
-                V.w[i + (long long)s * N] += V.next_z[I + (long long)s * Nc];
+                V.w[i + (long long)s * N] += (T)V.alpha * V.next_z[I + (long long)s * Nc];
     }
 }
 

@@ -288,6 +288,7 @@ struct Engine::Impl {
     AmgView<double>* h_amg64 = nullptr;  // [S * kAmgMaxLevels]
     AmgView<float>* h_amg32 = nullptr;
     double amg_omega = 0.9;  // fewest BiCGSTAB iterations among 0.5 / 0.7 / 0.9 on 2D and 3D PSPG systems
+    double amg_alpha = 1.6;  // fewest iterations among 1.0 ... 2.2 on 2D (20k, 100k nodes) and 3D systems
     PspgView* h_pv = nullptr;
     PicardView* h_pic = nullptr;
     KrylovView<double>* h_k64 = nullptr;
@@ -369,6 +370,11 @@ Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl
     if (const char* e = std::getenv("PFEM_AMG_OMEGA")) {
         const double w = std::atof(e);
         if (w > 0.0 && w <= 1.5) p_->amg_omega = w;
+    }
+    // PFEM_AMG_ALPHA overrides the coarse-correction scaling (tuning)
+    if (const char* e = std::getenv("PFEM_AMG_ALPHA")) {
+        const double a = std::atof(e);
+        if (a > 0.0 && a <= 3.0) p_->amg_alpha = a;
     }
     p_->alloc_slots(std::max(1, slots));
 }
@@ -596,6 +602,7 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         V = AmgView<T>{};
         V.level = l;
         V.omega = amg_omega;
+        V.alpha = amg_alpha;
         V.done_flag = &ds->k[0].done;
         if (l == 0) {
             V.A = A0;

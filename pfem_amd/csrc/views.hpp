@@ -109,6 +109,8 @@ struct AmgView {
     int mask;              // level 0: transfers and Galerkin sums use full rows only
     int nc;                // coarsest level: unknowns of the dense inverse
     double omega;          // block-Jacobi damping of the smoother
+    double alpha;          // scaling of the coarse-grid correction (over-correction of the
+                           // piecewise-constant prolongation)
     const int* done_flag;  // the case's BiCGSTAB done flag (finished solves skip the cycle)
     const double* r0[2];   // level 0 inputs (p, s of the FP64 BiCGSTAB)
     double* out0[2];       // level 0 outputs (ph, sh)
