@@ -240,6 +240,114 @@ def cpu_baseline(stream, meshes, states, threads, budget_s=20.0):
 
 
 # ------------------------------------------------------------------------------------------
+# the 3D weakly compressible sloshing tank and the mixed-precision 3D column field
+# ------------------------------------------------------------------------------------------
+def wc_step_bytes(dim, nodes, elems, s_state):
+    """Algorithmic bytes of one WC step plus computeNextDT: connectivity read by the element
+    passes (continuity, momentum, CFL), the node state v, a (dim each), p, rho read and
+    written once, positions read and written once, the FP64 accumulators (continuity mass and
+    rhs, momentum mass, dim force components) written and read once, and the CFL pass's read of
+    v, p, rho."""
+    conn = 3 * elems * (dim + 1) * 4
+    state = 2 * nodes * (2 * dim + 2) * s_state
+    pos = 2 * nodes * dim * 8
+    acc = 2 * nodes * (3 + dim) * 8
+    cfl = nodes * (dim + 2) * s_state
+    return conn + state + pos + acc + cfl
+
+
+def run_wc3d(ctx, steps, warmup=3):
+    """3D weakly compressible sloshing: CDS_dpdt steps with computeNextDT, FP32 node state and
+    FP64 accumulators, on the 2 x 2 x 1 m tank half filled at h = 0.01 m."""
+    import pfem_amd
+    from pfem_amd import meshgen as mg
+
+    mesh = mg.tank_3d(h=0.01)
+    v, p, rho, a = mg.wc_tait_hydrostatic(mesh, K0=K0, K0p=K0P, rho_star=RHO_STAR)
+    out = {}
+    for state in ("fp32", "fp64"):
+        case = pfem_amd.WCompNewtonStep(ctx, 3, mu=1e-3, K0=K0, K0p=K0P, rho_star=RHO_STAR, wc_state=state)
+        case.set_mesh(mesh)
+        case.set_state(np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)]))
+        dt = case.computeNextDT(WC_SC, WC_MAXDT)
+        for _ in range(warmup):
+            case.solveOneTimeStep(dt)
+            dt = case.computeNextDT(WC_SC, WC_MAXDT)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t1 = time.perf_counter()
+            case.solveOneTimeStep(dt)
+            dt = case.computeNextDT(WC_SC, WC_MAXDT)
+            lat.append((time.perf_counter() - t1) * 1e3)
+        total = time.perf_counter() - t0
+        sb = 4 if state == "fp32" else 8
+        b = wc_step_bytes(3, mesh.n_nodes, mesh.n_elems, sb)
+        out[state] = dict(steps_per_s=steps / total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99),
+                          elements_per_s=mesh.n_elems * steps / total, dt_last=dt,
+                          algorithmic_bytes_per_step=b, achieved_GBps=b * steps / total / 1e9,
+                          frac_hbm_peak=b * steps / total / 1e9 / HBM_PEAK_GBPS)
+        case.close()
+    return dict(workload=f"3D WC sloshing tank 2x2x1 m half filled, h=0.01 m ({mesh.n_nodes} nodes, "
+                         f"{mesh.n_elems} tets), CDS_dpdt + computeNextDT (securityCoeff {WC_SC})",
+                steps=steps, **out)
+
+
+def run_precision3d(ctx, steps):
+    """3D PSPG column field (seeded random-walk heights, ~1.07M nodes): FP32 inner BiCGSTAB +
+    block Jacobi inside FP64 iterative refinement to relres 1e-10, against the FP64 BiCGSTAB
+    solve to the same relres; per-step largest error of u, p and the free-surface positions,
+    relative to each field's largest magnitude."""
+    import pfem_amd
+    from pfem_amd import meshgen as mg
+
+    mesh = mg.randomwalk_3d(h=0.01, ncol=100, mean_cells=100, seed=0)
+    q0 = mg.pspg_hydrostatic(mesh)
+    N = mesh.n_nodes
+    dt = 5e-4
+    runs = {}
+    for prec in ("fp64", "mixed_ir"):
+        c = pfem_amd.MomContEqIncompNewton(ctx, 3, rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6, max_iter=10,
+                                           precision=prec, precond="jacobi", krylov_rtol=1e-10,
+                                           krylov_max_iter=40000, inner_rtol=1e-3, inner_max_iter=400)
+        c.set_mesh(mesh)
+        c.set_state(q0)
+        qs, ps, lat, st = [], [], [], []
+        for k in range(steps):
+            t1 = time.perf_counter()
+            ok = c.solve(dt)
+            lat.append((time.perf_counter() - t1) * 1e3)
+            if not ok:
+                raise SystemExit(f"precision case {prec} step {k} failed: {c.last_stats}")
+            st.append(c.last_stats)
+            qs.append(c.get_state())
+            ps.append(c.get_positions())
+        runs[prec] = dict(q=qs, pos=ps, lat=lat, st=st)
+        c.close()
+    surf = np.zeros(N, bool)
+    top = mesh.surface - mesh.pos[2] < 0.5 * mesh.h
+    surf[top] = True
+    err = []
+    for k in range(steps):
+        q64, q32 = runs["fp64"]["q"][k], runs["mixed_ir"]["q"][k]
+        u64, u32 = q64[:3 * N], q32[:3 * N]
+        p64, p32 = q64[3 * N:], q32[3 * N:]
+        x64 = runs["fp64"]["pos"][k].reshape(3, N)[:, surf]
+        x32 = runs["mixed_ir"]["pos"][k].reshape(3, N)[:, surf]
+        err.append(dict(velocity=float(np.abs(u32 - u64).max() / max(np.abs(u64).max(), 1e-300)),
+                        pressure=float(np.abs(p32 - p64).max() / max(np.abs(p64).max(), 1e-300)),
+                        free_surface_position=float(np.abs(x32 - x64).max() / mesh.h)))
+    summ = lambda r: dict(p50_ms=pct(r["lat"], 50),
+                          krylov_iters_per_step=float(np.mean([s["krylov_iters"] for s in r["st"]])),
+                          max_relres=float(max(s["max_relres"] for s in r["st"])))
+    return dict(workload=f"3D PSPG random-walk column field, {N} nodes, {mesh.n_elems} tets, dt {dt}, "
+                         f"linear solves to relres 1e-10",
+                fp64=summ(runs["fp64"]), mixed_ir=summ(runs["mixed_ir"]), error_vs_fp64_per_step=err,
+                error_note="velocity and pressure: max abs difference / field max; free-surface position: "
+                           "max abs difference of the top-layer nodes in units of h")
+
+
+# ------------------------------------------------------------------------------------------
 # the 3D dam break drop-in and the SpMV roofline
 # ------------------------------------------------------------------------------------------
 def spmv_fused_bytes(nodes, n, nnz, graph_nnz, vbytes=8):
@@ -283,6 +391,9 @@ def run_dambreak3d(ctx, steps, warmup):
             raise SystemExit(f"dam break step {k} failed: {case.last_stats}")
     total = time.perf_counter() - t0
     prof = case.profile_spmv(False)
+    # ceilings on the same device memory: a streaming read of the values and a copy kernel
+    # (also the known-size kernel the HBM byte counters are scaled by)
+    kb = case.bench_kernels(20)
     gnnz = graph_nnz_from_csr(rp, 3, mesh.n_nodes)
     bytes_spmv = spmv_fused_bytes(mesh.n_nodes, n, nnz, gnnz)
     spmv_ms = prof["avg_ms_fp64"]
@@ -309,6 +420,9 @@ def run_dambreak3d(ctx, steps, warmup):
         "avg_launch_ms": spmv_ms,
         "launches_timed": prof["launches_fp64"],
         "algorithmic_bytes_per_launch": bytes_spmv,
+        "ceilings_GBps": {"stream_read": nnz * 8 / kb["stream_read_us"] / 1e3,
+                          "copy": kb["copy_bytes"] / kb["copy_us"] / 1e3},
+        "spmv_back_to_back_us": kb["spmv64_us"],
     }
     case.close()
     return out, roof
@@ -327,8 +441,10 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--batched-precision", choices=["fp64", "mixed", "mixed_ir"], default="mixed",
                     help="PSPG solve precision of the batched path")
-    ap.add_argument("--only", choices=["batched", "plain"], default=None,
-                    help="run one path once (saturated) and print its result only (for profilers)")
+    ap.add_argument("--wc-steps", type=int, default=200, help="steps of the 3D WC sloshing tank")
+    ap.add_argument("--precision-steps", type=int, default=3, help="steps of the 3D mixed-precision column field")
+    ap.add_argument("--only", choices=["batched", "plain", "dambreak", "wc3d", "precision"], default=None,
+                    help="run one workload once and print its result only (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (cases run whole on one GPU; "
@@ -337,6 +453,16 @@ def main():
     import pfem_amd
 
     ctx = pfem_amd.Context(0)
+    if args.only == "dambreak":
+        db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
+        print(json.dumps({"dambreak3d_fp64": db, "roofline": roof}), flush=True)
+        return
+    if args.only == "wc3d":
+        print(json.dumps({"wc3d": run_wc3d(ctx, args.wc_steps)}), flush=True)
+        return
+    if args.only == "precision":
+        print(json.dumps({"precision3d": run_precision3d(ctx, args.precision_steps)}), flush=True)
+        return
     n_req = max(1, args.steps)
     stream = build_stream(n_req)
     t0 = time.perf_counter()
@@ -389,6 +515,10 @@ def main():
 
     db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
     log(f"[bench] dam break 3D: {db}")
+    wc3d = run_wc3d(ctx, args.wc_steps) if args.wc_steps > 0 else None
+    log(f"[bench] WC sloshing 3D: {wc3d}")
+    prec3d = run_precision3d(ctx, args.precision_steps) if args.precision_steps > 0 else None
+    log(f"[bench] precision 3D: {prec3d}")
     bs, ps = results["batched_saturated"], results["plain_saturated"]
     bp, pp = results["batched_poisson"], results["plain_poisson"]
     n2 = sum(r["dim"] == 2 for r in stream)
@@ -429,6 +559,8 @@ def main():
         "max_relres": max(bs["max_relres"], bp["max_relres"], ps["max_relres"], pp["max_relres"]),
         "accuracy_vs_fp64": acc,
         "dambreak3d_fp64": db,
+        "wc3d": wc3d,
+        "precision3d": prec3d,
         "roofline": roof,
     }
     if not args.no_cpu_baseline:

@@ -220,6 +220,15 @@ class Case:
         check(lib().pfem_profile_spmv(self._h, 1 if enable else 0, _dp(out)))
         return dict(avg_ms_fp64=out[0], launches_fp64=int(out[1]), avg_ms_fp32=out[2], launches_fp32=int(out[3]))
 
+    def bench_kernels(self, reps=20):
+        """Back-to-back timings (microseconds per launch) on the assembled system: node-block
+        SpMV FP64 and FP32, a streaming read of the nnz FP64 values and a copy kernel, with the
+        copy's bytes, nnz, node-graph nnz and lanes per node."""
+        out = np.zeros(8)
+        check(lib().pfem_bench_kernels(self._h, int(reps), _dp(out)))
+        keys = ("spmv64_us", "spmv32_us", "stream_read_us", "copy_us", "copy_bytes", "nnz", "graph_nnz", "lanes")
+        return dict(zip(keys, out.tolist()))
+
     def wc_next_dt(self, security_coeff, max_dt):
         out = C.c_double(0)
         check(lib().pfem_wc_next_dt(self._h, security_coeff, max_dt, C.byref(out)))

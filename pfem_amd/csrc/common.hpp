@@ -30,6 +30,34 @@ struct BlockTask {
 // Node flag bits (PFEM_NODE_* in include/pfem_gpu.h).
 enum : uint8_t { NODE_BOUND = 1, NODE_FREE = 2, NODE_ONFS = 4, NODE_FIXED = 8, NODE_DIRICHLET = 16 };
 
+// Lane permutations inside 16-lane DPP rows (VALU moves; no LDS round trip as __shfl_xor's
+// ds_bpermute takes).  CTRL is a DPP control: 0xB1 / 0x4E quad_perm [1,0,3,2] / [2,3,0,1],
+// 0x141 row_half_mirror (lane i of 8 reads 7-i), 0x140 row_mirror (lane i of 16 reads 15-i).
+template <int CTRL>
+__device__ __forceinline__ float dpp_mov(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xF, 0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xF, 0xF, false);
+    return __longlong_as_double((long long)(((unsigned long long)(unsigned)hi << 32) | (unsigned)lo));
+}
+
+// Sum over each aligned group of L lanes (L = 4, 8 or 16), returned in every lane of the
+// group.  The partner sums are symmetric, so all lanes of a group hold the same bits.  Every
+// lane of a group must be active (groups never straddle a DPP row, so other groups may not be).
+template <int L, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+    static_assert(L == 4 || L == 8 || L == 16, "group width");
+    v += dpp_mov<0xB1>(v);
+    v += dpp_mov<0x4E>(v);
+    if (L >= 8) v += dpp_mov<0x141>(v);
+    if (L >= 16) v += dpp_mov<0x140>(v);
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);

@@ -1,5 +1,5 @@
-// K3 (CSR SpMV) and K5 (block-Jacobi-preconditioned BiCGSTAB) for a batch of cases, plus
-// the FP64 iterative-refinement kernels of the mixed-precision solve.
+// Node-block SpMV and right-preconditioned BiCGSTAB for a batch of cases, plus the FP64
+// iterative-refinement kernels of the mixed-precision solve.
 //
 // Replaces SparseLU<SparseMatrix<double>, COLAMDOrdering<int>> (analyzePattern / factorize /
 // solve at MomContEquationPSPG.inl:281-290) with right-preconditioned BiCGSTAB
@@ -19,8 +19,7 @@
 // case that land on one XCD own one contiguous range of nodes (every field of those
 // nodes), so the SpMV's x gathers (neighbour nodes) and the vectors handed from one
 // kernel to the next stay in that XCD's L2.  Vector kernels are node-major (a thread owns
-// the nf DOFs of a node); SpMV kernels use L lanes per row (L | 64) reading consecutive
-// values and column indices and reducing with cross-lane shuffles.
+// the nf DOFs of a node); SpMV kernels use a group of L lanes per node (nodespmv.hpp).
 #include "common.hpp"
 #include "nodespmv.hpp"
 #include "views.hpp"
@@ -448,7 +447,7 @@ static void iter_launch(const KrylovView<T>* v, const BlockTask* t, int nt, hipS
     k_kry_x<T><<<nt, kBlock, 0, s>>>(v, t, par);
 }
 
-// lanes = K, the lanes per field of the node-block SpMV (a node takes kMaxNf * K lanes)
+// lanes: the lanes per node of the node-block SpMV (4, 8 or 16)
 #define PFEM_DISPATCH_L(lanes, CALL)                      \
     switch (lanes) {                                       \
         case 4: { constexpr int LL = 4; CALL; } break;     \

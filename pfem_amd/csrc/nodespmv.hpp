@@ -1,4 +1,4 @@
-// Node-block SpMV of the PSPG matrix (the K3 kernel body).
+// Node-block SpMV of the PSPG matrix (the body of every SpMV-type kernel).
 //
 // The L lanes of a group compute the nf rows of one node i.  Lane k takes neighbour
 // nbr_k of i: it loads the nf*nf values of rows (i, s) at columns (nbr_k, t) — a full
@@ -7,11 +7,15 @@
 // The value loads do not depend on the neighbour index and are issued before the
 // index -> x gather chain, so each lane keeps nf*nf + nf loads in flight.  Rows holding
 // only their diagonal (momentum rows of boundary nodes, every row of a free node) are
-// computed by lane 0.  Compared with a row-wise CSR SpMV this reads the column structure
-// from the node graph (deg indices per node instead of nf*nf*deg) and gathers each x
-// entry once per node instead of once per row.  (Measured on gfx950: one node per
-// 64-lane wavefront, with the fields spread over lanes, keeps fewer bytes in flight per
-// wave and runs ~1.6x slower.)
+// computed by lane 0.  The group's nf sums are DPP butterflies (group_sum), which keep
+// the reduction in the VALU instead of ds_bpermute round trips through the LDS crossbar
+// (the same speed on the C2 matrix, no LDS traffic).  Compared with a row-wise CSR SpMV
+// this reads the column structure from the node graph (deg indices per node instead of
+// nf*nf*deg) and gathers each x entry once per node instead of once per row.  (Measured on
+// gfx950, C2 matrix of 516 MB: this kernel 115 us per launch; one node per 64-lane
+// wavefront with the fields spread over lanes ~1.6x slower; a wave-sliced layout with fully
+// coalesced value loads 118 us; a node-major sliced ELL layout with one lane per node and no
+// cross-lane sum 98 us; a plain linear read of the same bytes 75 us.)
 #pragma once
 
 #include "common.hpp"
@@ -65,12 +69,8 @@ __device__ __forceinline__ void node_spmv(const NodeMat<VT>& M, const T* __restr
             if (s < nf && !full[s]) y[s] = (T)M.vals[rs[s]] * x[i + s * N];
     }
 #pragma unroll
-    for (int s = 0; s < kMaxNf; ++s) {
-        T acc = y[s];
-#pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, L);
-        y[s] = acc;
-    }
+    for (int s = 0; s < kMaxNf; ++s)
+        if (s < nf) y[s] = group_sum<L>(y[s]);
 }
 
 // y[s] for a runtime s without dynamic register indexing.

@@ -241,13 +241,10 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
             bv[D] += G.tau * vol * hg + G.tau / V.dt * vol_ne * cv;
         }
 #pragma unroll
-        for (int o = L / 2; o > 0; o >>= 1) {
+        for (int s = 0; s < NE; ++s) {
+            bv[s] = group_sum<L>(bv[s]);
 #pragma unroll
-            for (int s = 0; s < NE; ++s) {
-                bv[s] += __shfl_xor(bv[s], o, L);
-#pragma unroll
-                for (int t = 0; t < NE; ++t) blk[s][t] += __shfl_xor(blk[s][t], o, L);
-            }
+            for (int t = 0; t < NE; ++t) blk[s][t] = group_sum<L>(blk[s][t]);
         }
         if (lane == 0) {
 #pragma unroll
