@@ -535,6 +535,8 @@ void Engine::Impl::load_views(Req* r) {
             V.E = c->E;
             V.conn = c->pat.conn.as<int>();
             V.flags = c->pat.flags.as<uint8_t>();
+            V.elem_ptr = c->pat.elem_ptr.as<int>();
+            V.elem_list = c->pat.elem_list.as<uint32_t>();
             V.pos = c->pos.as<double>();
             // layout of pfem_case_set_state: v (dim x N), p (N), rho (N), a (dim x N)
             V.v = base;
@@ -913,7 +915,8 @@ void Engine::tick(std::vector<Req*>& done) {
         const int wi = c->d.wc_state == PFEM_STATE_FP32 ? 1 : 0;
         if (op & OP_WC) {
             append(tab[T_WCN + wi], sl, c->N, kElemCap);
-            append(tab[(c->dim == 2 ? T_WCE2 : T_WCE3) + wi], sl, c->E, kElemCap);
+            // node gathers over the incident elements: kWcLanes lanes per node
+            append(tab[(c->dim == 2 ? T_WCE2 : T_WCE3) + wi], sl, (long long)c->N * kWcLanes[c->dim - 2], kElemCap);
         }
         if (op & OP_CFL) {
             append(tab[(c->dim == 2 ? T_CFE2 : T_CFE3) + wi], sl, c->E, kElemCap);

@@ -16,9 +16,6 @@ template <typename T>
 void launch_kry_init(int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, const BlockTask* ids, int nids,
                      int par, hipStream_t s);
 template <typename T>
-void launch_kry_iters(int lanes, int iters, const KrylovView<T>* v, const BlockTask* t, int nt, int par0,
-                      hipStream_t s, hipEvent_t* ev = nullptr);
-template <typename T>
 void launch_kry_phase(int phase, int lanes, const KrylovView<T>* v, const BlockTask* t, int nt, int par,
                       hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
@@ -51,9 +48,10 @@ void launch_update_pos(const PicardView* v, const BlockTask* t, int nt, hipStrea
 void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_res_u(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 
+// node_tasks: one lane per node; gather2 / gather3: kWcLanes lanes per node of 2D / 3D cases
 template <typename S>
-void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_tasks, const BlockTask* elem2,
-                    int n_elem2, const BlockTask* elem3, int n_elem3, hipStream_t s);
+void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_tasks, const BlockTask* gather2,
+                    int n_gather2, const BlockTask* gather3, int n_gather3, hipStream_t s);
 template <typename S>
 void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, const BlockTask* elem3, int n_elem3,
                    hipStream_t s);

@@ -235,6 +235,17 @@ __global__ void k_pei_fill(int E, const int* __restrict__ soa, const uint4* __re
     }
 }
 
+// Node-element incidence entries e << 2 | a at node conn[e][a] (order fixed afterwards by k_pei_sort).
+__global__ void k_elem_fill(int E, int ne, const int* __restrict__ soa, const int* __restrict__ ptr,
+                            int* __restrict__ cursor, uint32_t* __restrict__ list) {
+    for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x; e < E; e += (long long)gridDim.x * blockDim.x)
+        for (int a = 0; a < ne; ++a) {
+            const int nd = soa[(long long)a * E + e];
+            list[ptr[nd] + atomicAdd(&cursor[nd], 1)] = ((uint32_t)e << 2) | (uint32_t)a;
+        }
+}
+
+// Sorts each list m (entries ptr[m] .. ptr[m+1]) in increasing order.
 __global__ void k_pei_sort(int M, const int* __restrict__ pei_ptr, uint32_t* __restrict__ pei) {
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
         const int beg = pei_ptr[m], end = pei_ptr[m + 1];
@@ -269,12 +280,14 @@ void build_mesh_basic(int dim, int N, int E, const int* conn_aos, const uint8_t*
     pat.N = N;
     pat.E = E;
     pat.n = (dim + 1) * N;
-    DevBuf count, err;
+    PFEM_REQUIRE((long long)E * ne < (1LL << 31) && E < (1 << 30), E_INVALID,
+                 "too many elements for int32 incidence indexing");
+    DevBuf count, err, cursor, tmp;
     pat.conn.reserve(sizeof(int) * (size_t)E * ne);
     pat.flags.reserve((size_t)N);
-    count.reserve(sizeof(int) * (size_t)N);
+    count.reserve(sizeof(int) * ((size_t)N + 1));  // count[N] = 0 closes the scan
     err.reserve(sizeof(int));
-    PFEM_HIP(hipMemsetAsync(count.p, 0, sizeof(int) * (size_t)N, stream));
+    PFEM_HIP(hipMemsetAsync(count.p, 0, sizeof(int) * ((size_t)N + 1), stream));
     PFEM_HIP(hipMemsetAsync(err.p, 0, sizeof(int), stream));
     k_conn_soa<<<grid_for(E, kBlock, 65536), kBlock, 0, stream>>>(E, ne, N, conn_aos, pat.conn.as<int>(),
                                                                   count.as<int>(), err.as<int>());
@@ -283,6 +296,18 @@ void build_mesh_basic(int dim, int N, int E, const int* conn_aos, const uint8_t*
     PFEM_HIP(hipMemcpyAsync(&herr, err.p, sizeof(int), hipMemcpyDeviceToHost, stream));
     PFEM_HIP(hipStreamSynchronize(stream));
     PFEM_REQUIRE(herr == 0, E_INVALID, "connectivity references a node index outside [0, N)");
+    // node-element incidence: counts -> offsets, fill through per-node cursors, then sort each
+    // node's list by element so that every sum over it has a fixed order
+    pat.elem_ptr.reserve(sizeof(int) * ((size_t)N + 1));
+    pat.elem_list.reserve(sizeof(uint32_t) * (size_t)E * ne);
+    cursor.reserve(sizeof(int) * (size_t)N);
+    PFEM_HIP(hipMemsetAsync(cursor.p, 0, sizeof(int) * (size_t)N, stream));
+    exclusive_sum(count.as<int>(), pat.elem_ptr.as<int>(), N + 1, stream, tmp);
+    k_elem_fill<<<grid_for(E, kBlock, 65536), kBlock, 0, stream>>>(E, ne, pat.conn.as<int>(), pat.elem_ptr.as<int>(),
+                                                                   cursor.as<int>(), pat.elem_list.as<uint32_t>());
+    k_pei_sort<<<grid_for(N, kBlock, 65536), kBlock, 0, stream>>>(N, pat.elem_ptr.as<int>(),
+                                                                  pat.elem_list.as<uint32_t>());
+    PFEM_HIP(hipStreamSynchronize(stream));
 }
 
 void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,

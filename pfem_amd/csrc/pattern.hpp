@@ -36,6 +36,10 @@ struct Pattern {
     // and conn[e][b] = j
     DevBuf pei_ptr;    // int  [graph_nnz + 1]
     DevBuf pei;        // u32  [E * ne * ne]
+    // node-element incidence (weakly compressible cases): for node i, entries
+    // elem_ptr[i] .. elem_ptr[i+1] hold e << 2 | a with conn[e][a] = i, sorted by element
+    DevBuf elem_ptr;   // int  [N + 1]
+    DevBuf elem_list;  // u32  [E * ne]
 };
 
 // Builds every array of `pat` on `stream` (synchronises the stream before returning,
@@ -48,8 +52,8 @@ void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* fl
 // only: the kernels address values through the node graph).
 void build_col_idx(const Pattern& pat, int* col_idx, hipStream_t stream);
 
-// Only the structure-of-arrays connectivity and the derived node flags (what the
-// weakly compressible step needs; no matrix).
+// The structure-of-arrays connectivity, the derived node flags and the node-element
+// incidence (what the weakly compressible step needs; no matrix).
 void build_mesh_basic(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,
                       Pattern& pat);
 

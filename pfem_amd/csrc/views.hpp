@@ -151,22 +151,28 @@ struct RefineView {
     double* out;         // [4] {||r||^2, ||b||^2, ...}
 };
 
+// Lanes per node of the weakly compressible node-gather passes (2D, 3D): a P1 node has
+// ~6 incident triangles in 2D and ~24 tetrahedra in 3D.
+constexpr int kWcLanes[2] = {8, 16};
+
 // Weakly compressible explicit step (WCompNewton/Solver.cpp:236-276).
 template <typename S>
 struct WcView {
     int dim, N, E, pad;
     const int* conn;          // [ne][E]
     const uint8_t* flags;     // [N] with NODE_FREE derived
+    const int* elem_ptr;      // [N+1] node-element incidence (Pattern::elem_ptr / elem_list)
+    const uint32_t* elem_list;
     double* pos;              // [dim][N]
     S* v;                     // [dim][N]
     S* a;                     // [dim][N]
     S* p;                     // [N]
     S* rho;                   // [N]
     const double* dir_vals;   // [dim][N]
-    double* m_cont;           // [N] lumped continuity mass accumulator
-    double* f_cont;           // [N] continuity rhs accumulator
-    double* m_mom;            // [N] lumped momentum mass accumulator (same for every component)
-    double* f_mom;            // [dim][N] momentum force accumulator
+    double* m_cont;           // [N] lumped continuity mass of each node
+    double* f_cont;           // [N] continuity rhs
+    double* m_mom;            // [N] lumped momentum mass (same for every component)
+    double* f_mom;            // [dim][N] momentum force
     unsigned long long* dt2;  // CFL min accumulator (bits of a positive double)
     double mu, K0, K0p, rho_star;
     double g[3];
