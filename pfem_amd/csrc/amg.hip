@@ -1,7 +1,8 @@
 // Aggregation multigrid for the PSPG systems: per-assembly setup (Galerkin coarse
 // operators, block-Jacobi inverses, dense inverse of the coarsest level) and the V-cycle
 // used as BiCGSTAB's right preconditioner, each batched over cases and levels through work
-// tables.  The hierarchy's structure comes from amg.cpp; see amg.hpp for the transfer.
+// tables.  The hierarchy's structure comes from amg_hierarchy.cpp; see amg.hpp for the
+// transfer.
 //
 // V-cycle on level l (input r_l, output z_l; level 0 reads p or s and writes ph or sh):
 //   w = omega B^-1 r                      (k_amg_pre, block-Jacobi pre-smoothing)

@@ -145,7 +145,9 @@ struct WsLayout {
 };
 // The multigrid hierarchy runs in FP32 for PFEM_PREC_MIXED 3D cases (under FP64 BiCGSTAB).
 // 2D hierarchies stay FP64: there an FP32 V-cycle raised the BiCGSTAB iteration count by
-// 20-40 % (more than the bandwidth it saves), while 3D counts did not change.
+// 20-40 %, and FP32 operators with FP64 vectors (iteration counts unchanged) still served
+// the stream 2-3 % slower than FP64, while the 3D counts did not change and the all-FP32 3D
+// hierarchy served it 8 % faster than FP32 operators with FP64 vectors.
 bool amg_fp32(const pfem_case* c) {
     return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 3;
 }
