@@ -357,6 +357,21 @@ def spmv_fused_bytes(nodes, n, nnz, graph_nnz, vbytes=8):
     return nnz * vbytes + graph_nnz * 4 + (nodes + 1) * 4 + nodes * 16 + 3 * n * vbytes
 
 
+def pmc_traffic(path=os.path.join(ROOT, "profiles", "spmv_kry_v_dambreak3d_262k_pmc_traffic.md")):
+    """HBM bytes per launch of the dam-break SpMV from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE passes (scaled by the counters' readings for a copy kernel of known size);
+    None when the summary is absent.  The counters need the profiler, so they cannot be read
+    inside this process."""
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def graph_nnz_from_csr(rp, dim, nodes):
     """Node-graph entries: a node's continuity row holds (dim+1)*deg entries (1 for a free node)."""
     cont = np.diff(rp[dim * nodes:])
@@ -416,7 +431,9 @@ def run_dambreak3d(ctx, steps, warmup):
         "peak": HBM_PEAK_GBPS,
         "unit": "GB/s",
         "frac": achieved / HBM_PEAK_GBPS,
-        "traffic": None,
+        "traffic": pmc_traffic(),
+        "traffic_source": "profiles/spmv_kry_v_dambreak3d_262k_pmc_traffic.md (rocprofv3 FETCH_SIZE x 1.9988 + "
+                          "WRITE_SIZE x 1.0, factors from a copy kernel of known size)",
         "avg_launch_ms": spmv_ms,
         "launches_timed": prof["launches_fp64"],
         "algorithmic_bytes_per_launch": bytes_spmv,
