@@ -9,9 +9,10 @@
 //   t = r - A w                           (k_amg_res)
 //   r_{l+1} = P^T t                       (k_amg_restrict, gather over aggregate members)
 //   z_{l+1} = V-cycle(l+1)  /  A_c^-1 r_c (k_amg_coarsest, dense inverse on the coarsest)
-//   w += alpha P z_{l+1}                  (k_amg_prolong, alpha > 1 over-corrects the
-//                                          piecewise-constant interpolation)
-//   z = w + omega B^-1 (r - A w)          (k_amg_post, post-smoothing)
+//   w' = w + alpha P z_{l+1}              (alpha > 1 over-corrects the piecewise-constant
+//                                          interpolation)
+//   z = w' + omega B^-1 (r - A w')        (k_amg_post, post-smoothing; r - A w' is formed
+//       = w' + omega B^-1 (t - alpha (A P) z_{l+1})   from t with the set-up product A P)
 // Every reduction is a fixed-order gather, so a case's V-cycle does not depend on the batch.
 #include "common.hpp"
 #include "nodespmv.hpp"
@@ -154,6 +155,69 @@ __global__ __launch_bounds__(kBlock) void k_amg_bjac(const AmgView<T>* __restric
         for (int s = 0; s < nf; ++s)
             for (int t = 0; t < nf; ++t) V.binv_out[(long long)(s * nf + t) * N + I] = (T)m[s][4 + t];
     }
+}
+
+// A P of a level with a coarser one: entry block (i, J) sums the blocks (i, j) of i's
+// neighbours j in aggregate J (gather list, fixed order).  Level 0 keeps only full rows
+// (i, s) and full columns (j, t): P prolongates to full rows only, and a row holding only
+// its diagonal has no entry off it.
+template <typename T, int L>
+__device__ __forceinline__ void ap_body(const AmgView<T>& V, const BlockTask& task) {
+    constexpr int NPB = kBlock / L;
+    const int lane = threadIdx.x % L;
+    const int nf = V.A.nf, D = nf - 1;
+    const NodeMat<T>& A = V.A;
+    const NodeMat<T>& P = V.AP;
+    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < P.N; i += (long long)task.nblocks * NPB) {
+        const int4 pinf = P.info[i];
+        const int pdeg = pinf.z;
+        const int4 ainf = A.info[i];
+        const int anb = A.nbr_ptr[i];
+        const uint8_t fi = (uint8_t)ainf.w;
+        for (int m = lane; m < pdeg; m += L) {
+            const int e = P.nbr_ptr[i] + m;
+            double blk[kMaxNf][kMaxNf];
+#pragma unroll
+            for (int s = 0; s < kMaxNf; ++s)
+#pragma unroll
+                for (int t = 0; t < kMaxNf; ++t) blk[s][t] = 0.0;
+            for (int q = V.ap_gptr[e]; q < V.ap_gptr[e + 1]; ++q) {
+                const int k = V.ap_gsrc[q];
+                const uint8_t fj = V.mask ? (uint8_t)A.info[A.nbr_idx[anb + k]].w : (uint8_t)0;
+#pragma unroll
+                for (int s = 0; s < kMaxNf; ++s) {
+                    if (s >= nf) continue;
+                    if (V.mask && !row_full_flags(fi, s, D)) continue;
+                    const long long rs = row_start_of(ainf, s, D, A.mom_nnz) + k;
+#pragma unroll
+                    for (int t = 0; t < kMaxNf; ++t) {
+                        if (t >= nf) continue;
+                        if (V.mask && !row_full_flags(fj, t, D)) continue;
+                        blk[s][t] += (double)A.vals[rs + (long long)t * ainf.z];
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < kMaxNf; ++s) {
+                if (s >= nf) continue;
+                const long long rs = row_start_of(pinf, s, D, P.mom_nnz) + m;
+#pragma unroll
+                for (int t = 0; t < kMaxNf; ++t)
+                    if (t < nf) V.ap_out[rs + (long long)t * pdeg] = (T)blk[s][t];
+            }
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_amg_ap(const AmgView<T>* __restrict__ views,
+                                                   const BlockTask* __restrict__ tasks) {
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const AmgView<T>& V = views[task.case_id];
+    if (V.ap_lanes == 4) ap_body<T, 4>(V, task);
+    else if (V.ap_lanes == 8) ap_body<T, 8>(V, task);
+    else ap_body<T, 16>(V, task);
 }
 
 // Dense inverse of the coarsest level (one workgroup per case; Gauss-Jordan in LDS without
@@ -299,46 +363,31 @@ __global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __res
     }
 }
 
-// w(i, s) += alpha z_{l+1}(agg(i), s) (level 0: full rows only).
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_amg_prolong(const AmgView<T>* __restrict__ views,
-                                                        const BlockTask* __restrict__ tasks) {
-    const BlockTask task = tasks[blockIdx.x];
-    if (task.case_id < 0) return;
-    const AmgView<T>& V = views[task.case_id];
-    if (*V.done_flag) return;
-    const int nf = V.A.nf, D = nf - 1, N = V.A.N, Nc = V.next_N;
-    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < N; i += (long long)task.nblocks * kBlock) {
-        const int I = V.fine_agg[i];
-        if (I < 0) continue;
-        const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
-#pragma unroll
-        for (int s = 0; s < kMaxNf; ++s)
-            if (s < nf && (!V.mask || row_full_flags(f, s, D)))
-                V.w[i + (long long)s * N] += (T)V.alpha * V.next_z[I + (long long)s * Nc];
-    }
-}
-
+// z = w' + omega B^-1 (t - alpha (A P) z_{l+1}),  w' = w + alpha P z_{l+1} (level 0: full
+// rows only); L lanes per node for the A P product.
 template <typename T, int L>
 __device__ __forceinline__ void post_body(const AmgView<T>& V, const BlockTask& task, int which) {
     constexpr int NPB = kBlock / L;
     const int lane = threadIdx.x % L;
-    const int N = V.A.N, nf = V.A.nf;
-    const T om = (T)V.omega;
+    const int N = V.A.N, nf = V.A.nf, D = nf - 1, Nc = V.next_N;
+    const T om = (T)V.omega, al = (T)V.alpha;
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
         T y[kMaxNf];
-        node_spmv<L>(V.A, (const T*)V.w, i, lane, y);
+        node_spmv<L>(V.AP, V.next_z, i, lane, y);
         if (lane < nf) {
             T d[kMaxNf];
 #pragma unroll
-            for (int t = 0; t < kMaxNf; ++t)
-                d[t] = (t < nf) ? in_at(V, which, i + (long long)t * N) - pick(y, t) : T(0);
+            for (int t = 0; t < kMaxNf; ++t) d[t] = (t < nf) ? V.t[i + (long long)t * N] - al * pick(y, t) : T(0);
             T acc = 0;
 #pragma unroll
             for (int t = 0; t < kMaxNf; ++t)
                 if (t < nf) acc += V.binv[(long long)(lane * nf + t) * N + i] * d[t];
             const long long q = i + (long long)lane * N;
-            const T out = V.w[q] + om * acc;
+            T wq = V.w[q];
+            const int I = V.fine_agg[i];
+            if (I >= 0 && (!V.mask || row_full_flags((uint8_t)V.A.info[i].w, lane, D)))
+                wq += al * V.next_z[I + (long long)lane * Nc];
+            const T out = wq + om * acc;
             if (V.level == 0) V.out0[which][q] = (double)out;
             else V.z[q] = out;
         }
@@ -352,8 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_post(const AmgView<T>* __restric
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
-    if (V.lanes == 4) post_body<T, 4>(V, task, which);
-    else if (V.lanes == 8) post_body<T, 8>(V, task, which);
+    if (V.ap_lanes == 4) post_body<T, 4>(V, task, which);
+    else if (V.ap_lanes == 8) post_body<T, 8>(V, task, which);
     else post_body<T, 16>(V, task, which);
 }
 
@@ -387,8 +436,8 @@ void launch_amg_coarsest(const AmgView<T>* v, const BlockTask* t, int nt, hipStr
     k_amg_coarsest<T><<<nt, kBlock, 0, s>>>(v, t);
 }
 template <typename T>
-void launch_amg_prolong(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
-    k_amg_prolong<T><<<nt, kBlock, 0, s>>>(v, t);
+void launch_amg_ap(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s) {
+    k_amg_ap<T><<<nt, kBlock, 0, s>>>(v, t);
 }
 template <typename T>
 void launch_amg_post(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s) {
@@ -403,7 +452,7 @@ void launch_amg_post(const AmgView<T>* v, const BlockTask* t, int nt, int which,
     template void launch_amg_res<T>(const AmgView<T>*, const BlockTask*, int, int, hipStream_t);   \
     template void launch_amg_restrict<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);   \
     template void launch_amg_coarsest<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);   \
-    template void launch_amg_prolong<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);    \
+    template void launch_amg_ap<T>(const AmgView<T>*, const BlockTask*, int, hipStream_t);         \
     template void launch_amg_post<T>(const AmgView<T>*, const BlockTask*, int, int, hipStream_t);
 PFEM_AMG_INSTANTIATE(double)
 PFEM_AMG_INSTANTIATE(float)

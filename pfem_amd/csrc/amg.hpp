@@ -10,6 +10,9 @@
 // to the smoother).  Coarse operators are Galerkin products A_{l+1} = P^T A_l P; in the
 // node-block layout every coarse entry block (I, J) is the sum of the fine blocks (i, j)
 // with i in I and j in J, gathered through a precomputed list (no atomics, fixed order).
+// The post-smoothing uses A P (finer rows, coarse columns), built the same way, so that the
+// prolongated correction's residual A (w + P z) = A w + (A P) z costs a product with A P
+// (a node's neighbours fall in a few aggregates) instead of a second product with A.
 #pragma once
 
 #include <vector>
@@ -38,6 +41,14 @@ struct AmgLevel {
     DevBuf fine_agg;    // int  [N_finer] aggregate of each finer node (-1: not aggregated)
     DevBuf gal_ptr;     // int  [graph_nnz+1] Galerkin gather list of each coarse entry
     DevBuf gal_src;     // int2 [finer graph entries] {finer row node i, rank k in i's list}
+    // A_finer P: rows of the finer level, columns of this level's aggregates
+    int ap_nnz = 0;     // entries (finer node, aggregate)
+    int ap_lanes = 4;
+    DevBuf ap_ptr;      // int  [N_finer+1]
+    DevBuf ap_idx;      // int  [ap_nnz] aggregates adjacent to each finer node, sorted
+    DevBuf ap_info;     // int4 [N_finer] {nf*ap_ptr, nf*ap_ptr, degree, 0}
+    DevBuf ap_gptr;     // int  [ap_nnz+1] finer neighbours of each entry
+    DevBuf ap_gsrc;     // int  [finer graph entries] rank k in the finer node's list
 };
 
 struct AmgHierarchy {

@@ -157,6 +157,35 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
             for (int i = 0; i < G.N; ++i)
                 if (active[i]) anodes[cur[agg[i]]++] = i;
         }
+        // A P structure: for each finer node i, the distinct aggregates of its active
+        // neighbours (sorted) and, per (i, J), the ranks k of those neighbours in i's list
+        std::vector<int> ap_ptr((size_t)G.N + 1, 0), ap_idx, ap_gptr(1, 0), ap_gsrc;
+        std::vector<int4> ap_info((size_t)G.N);
+        {
+            std::vector<std::pair<int, int>> nb;  // (aggregate, rank)
+            int apmax = 0;
+            for (int i = 0; i < G.N; ++i) {
+                nb.clear();
+                if (active[i])
+                    for (int m = G.ptr[i]; m < G.ptr[i + 1]; ++m) {
+                        const int j = G.idx[m];
+                        if (active[j] && agg[j] >= 0) nb.emplace_back(agg[j], m - G.ptr[i]);
+                    }
+                std::sort(nb.begin(), nb.end());
+                for (size_t a = 0; a < nb.size();) {
+                    size_t b = a;
+                    while (b < nb.size() && nb[b].first == nb[a].first) ap_gsrc.push_back(nb[b++].second);
+                    ap_idx.push_back(nb[a].first);
+                    ap_gptr.push_back((int)ap_gsrc.size());
+                    a = b;
+                }
+                ap_ptr[i + 1] = (int)ap_idx.size();
+                const int deg = ap_ptr[i + 1] - ap_ptr[i];
+                apmax = std::max(apmax, deg);
+                ap_info[i] = int4{nf * ap_ptr[i], nf * ap_ptr[i], deg, 0};
+            }
+            PFEM_REQUIRE(apmax <= 255, E_UNSUPPORTED, "a node's neighbours span more than 255 aggregates");
+        }
         // node descriptors of the coarse level (every row full)
         std::vector<int4> cinfo((size_t)na);
         std::vector<int> self((size_t)na);
@@ -185,6 +214,13 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
         upload(L.fine_agg, agg, stream);
         upload(L.gal_ptr, gcount, stream);
         upload(L.gal_src, gsrc, stream);
+        L.ap_nnz = (int)ap_idx.size();
+        L.ap_lanes = lanes_for(G.N ? (double)L.ap_nnz / G.N : 0.0);
+        upload(L.ap_ptr, ap_ptr, stream);
+        upload(L.ap_idx, ap_idx, stream);
+        upload(L.ap_info, ap_info, stream);
+        upload(L.ap_gptr, ap_gptr, stream);
+        upload(L.ap_gsrc, ap_gsrc, stream);
         PFEM_HIP(hipStreamSynchronize(stream));
         levels.push_back(std::move(L));
         G = std::move(C);

@@ -69,9 +69,9 @@ enum Tab : int {
 // (Galerkin, block Jacobi), coarsest dense inverse; V-cycle per level (node ops, SpMV ops,
 // restriction), coarsest solve.
 enum AmgTab : int {
-    A_G = 0, A_B = A_G + kAmgMaxLevels, A_D = A_B + kAmgMaxLevels,
-    A_VN = A_D + 1, A_VS = A_VN + kAmgMaxLevels, A_VR = A_VS + kAmgMaxLevels, A_VC = A_VR + kAmgMaxLevels,
-    A_N = A_VC + 1
+    A_G = 0, A_B = A_G + kAmgMaxLevels, A_D = A_B + kAmgMaxLevels, A_AP = A_D + 1,
+    A_VN = A_AP + kAmgMaxLevels, A_VS = A_VN + kAmgMaxLevels, A_VR = A_VS + kAmgMaxLevels,
+    A_VC = A_VR + kAmgMaxLevels, A_N = A_VC + 1
 };
 constexpr int T_NTAB = T_AMG + 2 * A_N;
 
@@ -142,6 +142,7 @@ struct WsLayout {
     size_t vals, b, binv, x, x_prev, pos_save, kv64, vals32, binv32, kv32, total;
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
+    size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
 };
 // The multigrid hierarchy runs in FP32 for PFEM_PREC_MIXED 3D cases (under FP64 BiCGSTAB).
 // 2D hierarchies stay FP64: there an FP32 V-cycle raised the BiCGSTAB iteration count by
@@ -186,6 +187,7 @@ WsLayout ws_layout(const pfem_case* c) {
             L.amg_lv[l][1] = take(es * nf * nf * (size_t)lv.N);
             for (int k = 2; k < 6; ++k) L.amg_lv[l][k] = take(es * nl);
         }
+        for (int l = 0; l + 1 < H.levels; ++l) L.amg_ap[l] = take(es * nf * nf * (size_t)H.lev[l + 1].ap_nnz);
         L.amg_dense = take(8 * (size_t)H.coarse_n * H.coarse_n);  // FP64 in both types
     }
     L.total = o;
@@ -219,12 +221,14 @@ void carve_workspace(pfem_case* c, char* base) {
     w.amg_w0 = w.amg_t0 = w.amg_dense = nullptr;
     for (auto& lv : w.amg_lv)
         for (auto& p : lv) p = nullptr;
+    for (auto& p : w.amg_ap) p = nullptr;
     if (H.levels > 1) {
         w.amg_w0 = base + L.amg_w0;
         w.amg_t0 = base + L.amg_t0;
         w.amg_dense = base + L.amg_dense;
         for (int l = 1; l < H.levels; ++l)
             for (int k = 0; k < 6; ++k) w.amg_lv[l][k] = base + L.amg_lv[l][k];
+        for (int l = 0; l + 1 < H.levels; ++l) w.amg_ap[l] = base + L.amg_ap[l];
     }
 }
 
@@ -646,6 +650,20 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         }
         if (l + 1 < H.levels) {
             const AmgLevel& Lc = H.lev[l + 1];
+            NodeMat<T> AP{};
+            AP.N = V.A.N;
+            AP.nf = c->dim + 1;
+            AP.mom_nnz = AP.nf * Lc.ap_nnz;
+            AP.ncol = Lc.N;
+            AP.info = Lc.ap_info.as<int4>();
+            AP.nbr_ptr = Lc.ap_ptr.as<int>();
+            AP.nbr_idx = Lc.ap_idx.as<int>();
+            AP.vals = arr(w.amg_ap[l]);
+            V.AP = AP;
+            V.ap_lanes = Lc.ap_lanes;
+            V.ap_gptr = Lc.ap_gptr.as<int>();
+            V.ap_gsrc = Lc.ap_gsrc.as<int>();
+            V.ap_out = arr(w.amg_ap[l]);
             V.next_N = Lc.N;
             V.agg_ptr = Lc.agg_ptr.as<int>();
             V.agg_nodes = Lc.agg_nodes.as<int>();
@@ -902,6 +920,10 @@ void Engine::tick(std::vector<Req*>& done) {
                     append(A[A_B + l], base + l, H.lev[l].N, kElemCap);
                 }
                 A[A_D].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
+                for (int l = 0; l + 1 < H.levels; ++l) {
+                    const int Nl = l == 0 ? c->N : H.lev[l].N;
+                    append(A[A_AP + l], base + l, (long long)Nl * H.lev[l + 1].ap_lanes, kElemCap);
+                }
             }
             if (op & OP_K64_RUN) {
                 for (int l = 0; l + 1 < H.levels; ++l) {
@@ -968,6 +990,8 @@ void Engine::tick(std::vector<Req*>& done) {
             launch_amg_bjac(v, T(TA + A_B + l), NT(TA + A_B + l), s);
         }
         if (NT(TA + A_D)) launch_amg_dense(v, T(TA + A_D), NT(TA + A_D), s);
+        for (int l = 0; l < kAmgMaxLevels; ++l)  // A P reads level l's final values
+            if (NT(TA + A_AP + l)) launch_amg_ap(v, T(TA + A_AP + l), NT(TA + A_AP + l), s);
     };
     amg_setup(vamg64, T_AMG);
     amg_setup(vamg32, T_AMG + A_N);
@@ -1009,11 +1033,8 @@ void Engine::tick(std::vector<Req*>& done) {
             launch_amg_restrict(v, T(TA + A_VR + l), NT(TA + A_VR + l), s);
         }
         if (NT(TA + A_VC)) launch_amg_coarsest(v, T(TA + A_VC), NT(TA + A_VC), s);
-        for (int l = kAmgMaxLevels - 1; l >= 0; --l) {
-            if (!NT(TA + A_VN + l)) continue;
-            launch_amg_prolong(v, T(TA + A_VN + l), NT(TA + A_VN + l), s);
-            launch_amg_post(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
-        }
+        for (int l = kAmgMaxLevels - 1; l >= 0; --l)
+            if (NT(TA + A_VS + l)) launch_amg_post(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
     };
     auto vcycle = [&](int which) {
         vcycle_type(vamg64, T_AMG, which);

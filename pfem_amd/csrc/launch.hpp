@@ -37,7 +37,7 @@ void launch_amg_restrict(const AmgView<T>* v, const BlockTask* t, int nt, hipStr
 template <typename T>
 void launch_amg_coarsest(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
 template <typename T>
-void launch_amg_prolong(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
+void launch_amg_ap(const AmgView<T>* v, const BlockTask* t, int nt, hipStream_t s);
 template <typename T>
 void launch_amg_post(const AmgView<T>* v, const BlockTask* t, int nt, int which, hipStream_t s);
 void launch_ir_residual(int lanes, const RefineView* v, const BlockTask* t, int nt, hipStream_t s);

@@ -33,7 +33,7 @@ __device__ __forceinline__ void node_spmv(const NodeMat<VT>& M, const T* __restr
     const int4 inf = M.info[i];
     const int nb = M.nbr_ptr[i];
     const int deg = inf.z, nf = M.nf, D = nf - 1;
-    const long long N = M.N;
+    const long long N = M.N, Nx = M.ncol ? M.ncol : M.N;
     const uint8_t f = (uint8_t)inf.w;
     const bool mom_full = (f & (NODE_BOUND | NODE_FREE)) == 0, cont_full = (f & NODE_FREE) == 0;
     long long rs[kMaxNf];
@@ -54,7 +54,7 @@ __device__ __forceinline__ void node_spmv(const NodeMat<VT>& M, const T* __restr
         const long long j = M.nbr_idx[nb + k];
         T xv[kMaxNf];
 #pragma unroll
-        for (int t = 0; t < kMaxNf; ++t) xv[t] = (t < nf) ? x[j + t * N] : T(0);
+        for (int t = 0; t < kMaxNf; ++t) xv[t] = (t < nf) ? x[j + t * Nx] : T(0);
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s) {
             T acc = y[s];

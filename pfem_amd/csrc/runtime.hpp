@@ -60,6 +60,7 @@ struct Workspace {
     char* amg_t0 = nullptr;
     char* amg_dense = nullptr;
     char* amg_lv[kAmgMaxLevels][6] = {};  // {vals, binv, r, w, t, z} of level l >= 1
+    char* amg_ap[kAmgMaxLevels] = {};     // A P values of level l against level l+1
     // Forgets the arrays (and frees an owned allocation).
     void clear() {
         own.release();
@@ -72,6 +73,7 @@ struct Workspace {
         amg_w0 = amg_t0 = amg_dense = nullptr;
         for (auto& l : amg_lv)
             for (auto& p : l) p = nullptr;
+        for (auto& p : amg_ap) p = nullptr;
     }
 };
 

@@ -38,7 +38,8 @@ struct PspgView {
 // holds only its diagonal.
 template <typename T>
 struct NodeMat {
-    int N, nf, mom_nnz, pad;
+    int N, nf, mom_nnz;
+    int ncol;            // nodes of the column space when it differs from the rows' (0: N)
     const int4* info;    // [N] {mom_ptr, cont_ptr, deg, flags}
     const int* nbr_ptr;  // [N+1]
     const int* nbr_idx;  // [graph_nnz]
@@ -126,6 +127,15 @@ struct AmgView {
     const int* fine_agg;   // next-level aggregate of each node of this level (-1: none)
     T* next_r;
     const T* next_z;
+    // A P: this level's rows, the next coarser level's columns (every row full; the entry
+    // block (i, J) sums the blocks (i, j) of i's neighbours j in aggregate J, level 0: over
+    // full rows (j, t) only, and is zero in rows (i, s) that hold only their diagonal)
+    NodeMat<T> AP;
+    int ap_lanes;          // SpMV lanes per node of AP
+    int pad1;
+    const int* ap_gptr;    // [AP entries + 1] gather list of each AP entry
+    const int* ap_gsrc;    // rank k in i's neighbour list of each gathered block
+    T* ap_out;             // AP values (written at set-up)
     // setup of this level (>= 1) from the finer level
     NodeMat<T> F;
     const int* gal_ptr;
