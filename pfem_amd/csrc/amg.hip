@@ -14,7 +14,10 @@
 //   z = w' + omega B^-1 (r - A w')        (k_amg_post, post-smoothing; r - A w' is formed
 //       = w' + omega B^-1 (t - alpha (A P) z_{l+1})   from t with the set-up product A P)
 // Every reduction is a fixed-order gather, so a case's V-cycle does not depend on the batch.
+#include <type_traits>
+
 #include "common.hpp"
+#include "ellspmv.hpp"
 #include "nodespmv.hpp"
 #include "views.hpp"
 
@@ -311,6 +314,28 @@ __device__ __forceinline__ void res_body(const AmgView<T>& V, const BlockTask& t
     }
 }
 
+// Level 0 of an FP64 hierarchy with the system's ELL copy: one lane per node, whole
+// 64-node slices per wavefront (a workgroup's waves stride over the case's slices).
+template <int NF>
+__device__ __forceinline__ void res_ell(const AmgView<double>& V, const BlockTask& task, int which) {
+    const EllMat& E = V.E0;
+    const int lane = threadIdx.x & (kEllSlice - 1);
+    const long long N = E.N;
+    for (long long c = (long long)task.block * kWaves + (threadIdx.x >> 6); c < E.nsl;
+         c += (long long)task.nblocks * kWaves) {
+        double y[kMaxNf];
+        ell_rows<NF>(E, V.w, c, lane, y);
+        const long long i = c * kEllSlice + lane;
+        if (i < N) {
+#pragma unroll
+            for (int s = 0; s < NF; ++s) {
+                const long long q = i + (long long)s * N;
+                V.t[q] = V.r0[which][q] - y[s];
+            }
+        }
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict__ views,
                                                     const BlockTask* __restrict__ tasks, int which) {
@@ -318,6 +343,13 @@ __global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
+    if constexpr (std::is_same<T, double>::value) {
+        if (V.level == 0 && V.E0.vals) {
+            if (V.A.nf == 4) res_ell<4>(V, task, which);
+            else res_ell<3>(V, task, which);
+            return;
+        }
+    }
     if (V.lanes == 4) res_body<T, 4>(V, task, which);
     else if (V.lanes == 8) res_body<T, 8>(V, task, which);
     else res_body<T, 16>(V, task, which);

@@ -95,7 +95,7 @@ size_t persistent_bytes(pfem_case* c) {
     for (DevBuf* p : {&c->pos, &c->dir_vals, &c->q_prev, &c->partials, &c->counters, &c->wstate, &c->wacc,
                       &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr, &c->pat.nbr_idx, &c->pat.node_info,
                       &c->pat.row_ptr, &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei, &c->pat.elem_ptr,
-                      &c->pat.elem_list})
+                      &c->pat.elem_list, &c->pat.ell_soff, &c->pat.ell_idx})
         b += p->bytes;
     for (auto& L : c->amg.lev)
         for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,
@@ -333,6 +333,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         PFEM_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned) * 8, c->stream));
         if (c->d.kind == PFEM_KIND_PSPG) {
             pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
+            pfem::build_ell(c->pat, c->stream);
             const size_t n = (size_t)c->n;
             c->q_prev.reserve(sizeof(double) * n);
             PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * n, c->stream));

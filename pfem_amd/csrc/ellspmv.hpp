@@ -1,0 +1,45 @@
+// SpMV over the sliced ELL copy of a PSPG system (EllMat): lane r of a wavefront computes
+// the NF rows of node 64 c + r of slice c.  Each rank k is one column node j (one index
+// load, NF gathers of x) and NF*NF value loads, each 64 consecutive doubles per wavefront.
+#pragma once
+
+#include "common.hpp"
+#include "nodespmv.hpp"
+#include "views.hpp"
+
+namespace pfem {
+
+// y[s] = (A x) of row (64 c + lane, s) for s < NF (y[s] = 0 above); lanes past the last
+// node compute zeros.
+template <int NF>
+__device__ __forceinline__ void ell_rows(const EllMat& E, const double* __restrict__ x, long long c, int lane,
+                                         double (&y)[kMaxNf]) {
+    const long long b = E.soff[c];
+    const int W = E.soff[c + 1] - (int)b;
+    const double* __restrict__ vb = E.vals + b * (NF * NF * kEllSlice) + lane;
+    const int* __restrict__ ib = E.idx + b * kEllSlice + lane;
+    const long long N = E.N;
+    double acc[NF];
+#pragma unroll
+    for (int s = 0; s < NF; ++s) acc[s] = 0.0;
+#pragma unroll 2
+    for (int k = 0; k < W; ++k) {
+        const long long j = ib[(long long)k * kEllSlice];
+        double a[NF][NF];
+#pragma unroll
+        for (int s = 0; s < NF; ++s)
+#pragma unroll
+            for (int t = 0; t < NF; ++t) a[s][t] = vb[((long long)k * NF * NF + s * NF + t) * kEllSlice];
+        double xv[NF];
+#pragma unroll
+        for (int t = 0; t < NF; ++t) xv[t] = x[j + t * N];
+#pragma unroll
+        for (int s = 0; s < NF; ++s)
+#pragma unroll
+            for (int t = 0; t < NF; ++t) acc[s] += a[s][t] * xv[t];
+    }
+#pragma unroll
+    for (int s = 0; s < kMaxNf; ++s) y[s] = s < NF ? acc[s] : 0.0;
+}
+
+}  // namespace pfem

@@ -16,6 +16,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <limits>
+#include <type_traits>
 
 namespace pfem {
 
@@ -99,6 +100,17 @@ NodeMat<T> node_mat(const Pattern& P, int dim, const T* vals) {
     return A;
 }
 
+EllMat ell_mat(const Pattern& P, int dim, const double* vals) {
+    EllMat E{};
+    E.N = P.N;
+    E.nf = dim + 1;
+    E.nsl = P.ell_nsl;
+    E.soff = P.ell_soff.as<int>();
+    E.idx = P.ell_idx.as<int>();
+    E.vals = vals;
+    return E;
+}
+
 template <class T>
 void pinned_alloc(T*& p, size_t count) {
     if (p) (void)hipHostFree(p);
@@ -139,7 +151,7 @@ int pick_lanes(double avg_deg) {
 // ---------------------------------------------------------------------------------------
 namespace {
 struct WsLayout {
-    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals32, binv32, kv32, total;
+    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32, binv32, kv32, total;
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
     size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
@@ -169,6 +181,7 @@ WsLayout ws_layout(const pfem_case* c) {
     L.x_prev = take(8 * n);
     L.pos_save = take(8 * (size_t)c->dim * N);
     L.kv64 = take(8 * 8 * n);
+    L.vals_ell = take(8 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     if (ir || amg_fp32(c)) {  // FP32 copies of the values and of the block-Jacobi inverse
         L.vals32 = take(4 * nnz);
@@ -212,6 +225,7 @@ void carve_workspace(pfem_case* c, char* base) {
     w.x_prev = reinterpret_cast<double*>(base + L.x_prev);
     w.pos_save = reinterpret_cast<double*>(base + L.pos_save);
     w.kv64 = reinterpret_cast<double*>(base + L.kv64);
+    w.vals_ell = reinterpret_cast<double*>(base + L.vals_ell);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     const bool v32 = ir || amg_fp32(c);
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
@@ -432,6 +446,9 @@ void Engine::Impl::load_views(Req* r) {
         pv.b = w.b;
         pv.vals32 = w.vals32;  // FP32 copies, when the precision mode has them
         pv.diag_pos = P.diag_pos.as<int>();
+        pv.ell_nsl = P.ell_nsl;
+        pv.ell_soff = P.ell_soff.as<int>();
+        pv.vals_ell = w.vals_ell;
         pv.binv = w.binv;
         pv.binv32 = w.binv32;
         pv.nnz = P.nnz;
@@ -482,6 +499,7 @@ void Engine::Impl::load_views(Req* r) {
         k.partials = c->partials.as<double>();
         k.counters = c->counters.as<unsigned>();
         k.sc = &ds->k[0];
+        k.E = ell_mat(P, dim, w.vals_ell);
         k.amg = c->amg.levels > 1 ? 1 : 0;
         if (k.amg) {
             if (amg_fp32(c))
@@ -615,6 +633,7 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         if (l == 0) {
             V.A = A0;
             V.binv = binv0;
+            if (std::is_same<T, double>::value) V.E0 = ell_mat(c->pat, c->dim, w.vals_ell);
             V.lanes = c->lanes;
             V.mask = 1;
             V.r0[0] = k.p;
@@ -979,7 +998,10 @@ void Engine::tick(std::vector<Req*>& done) {
     if (NT(T_ASM2)) launch_pspg_assemble(2, vpv, T(T_ASM2), NT(T_ASM2), s);
     if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
     if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
-    if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
+    if (NT(T_PRE)) {
+        launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
+        launch_ell_fill(vpv, T(T_PRE), NT(T_PRE), s);
+    }
     const AmgView<double>* vamg64 = I.d_amg64.as<AmgView<double>>();
     const AmgView<float>* vamg32 = I.d_amg32.as<AmgView<float>>();
     // multigrid setup of both hierarchy types (TA: first table of the type)

@@ -22,7 +22,10 @@
 // the nf DOFs of a node); SpMV kernels use a group of L lanes per node (nodespmv.hpp).
 #include <hip/hip_ext.h>
 
+#include <type_traits>
+
 #include "common.hpp"
+#include "ellspmv.hpp"
 #include "nodespmv.hpp"
 #include "views.hpp"
 
@@ -258,6 +261,23 @@ __device__ __forceinline__ void kry_v_nodes(const KrylovView<T>& V, const NodeMa
     }
 }
 
+// The ELL form of an SpMV pass over a case (FP64): each wavefront of the workgroups on this
+// XCD takes whole 64-node slices of the XCD's slice range; per node, emit(i, y) sees the
+// node's NF row values.
+template <int NF, typename F>
+__device__ __forceinline__ void ell_pass(const EllMat& E, const double* __restrict__ x, const BlockTask& task,
+                                         F&& emit) {
+    const Seg sg = xcd_segment(task, E.nsl);
+    const int lane = threadIdx.x & (kEllSlice - 1);
+    for (long long l = (long long)sg.lb * kWaves + (threadIdx.x >> 6); l < sg.ns; l += (long long)sg.cnt * kWaves) {
+        const long long c = sg.n0 + l;
+        double y[kMaxNf];
+        ell_rows<NF>(E, x, c, lane, y);
+        const long long i = c * kEllSlice + lane;
+        if (i < E.N) emit(i, y);
+    }
+}
+
 template <typename T, int L>
 __global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restrict__ views,
                                                   const BlockTask* __restrict__ tasks) {
@@ -265,10 +285,29 @@ __global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restric
     if (task.case_id < 0) return;
     const KrylovView<T>& V = views[task.case_id];
     if (V.sc->done) return;
-    const Seg sg = xcd_segment(task, V.nodes);
-    const int lane = threadIdx.x % L;
     double part[1] = {0.0};
-    kry_v_nodes<L>(V, V.A, sg, lane, part);
+    bool done = false;
+    if constexpr (std::is_same<T, double>::value) {
+        if (V.E.vals) {
+            auto emit = [&](long long i, const double (&y)[kMaxNf]) {
+#pragma unroll
+                for (int s = 0; s < kMaxNf; ++s)
+                    if (s < V.nf) {
+                        const long long row = i + (long long)s * V.nodes;
+                        V.v[row] = y[s];
+                        part[0] += V.rh[row] * y[s];
+                    }
+            };
+            if (V.nf == 4) ell_pass<4>(V.E, V.ph, task, emit);
+            else ell_pass<3>(V.E, V.ph, task, emit);
+            done = true;
+        }
+    }
+    if (!done) {
+        const Seg sg = xcd_segment(task, V.nodes);
+        const int lane = threadIdx.x % L;
+        kry_v_nodes<L>(V, V.A, sg, lane, part);
+    }
     constexpr int slots[1] = {P_RV};
     write_partials<1>(part, V.partials, task.block, slots);
 }
@@ -347,10 +386,30 @@ __global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restric
     double ss[1];
     reduce_partials<1>(V.partials, V.nblk, ss_slot, ss);
     if (ss[0] <= V.sc->tol2) return;  // ||s|| converged: K_x finishes the iteration
-    const Seg sg = xcd_segment(task, V.nodes);
-    const int lane = threadIdx.x % L;
     double part[2] = {0.0, 0.0};
-    kry_t_nodes<L>(V, V.A, sg, lane, part);
+    bool done = false;
+    if constexpr (std::is_same<T, double>::value) {
+        if (V.E.vals) {
+            auto emit = [&](long long i, const double (&y)[kMaxNf]) {
+#pragma unroll
+                for (int s = 0; s < kMaxNf; ++s)
+                    if (s < V.nf) {
+                        const long long row = i + (long long)s * V.nodes;
+                        V.t[row] = y[s];
+                        part[0] += y[s] * V.s[row];
+                        part[1] += y[s] * y[s];
+                    }
+            };
+            if (V.nf == 4) ell_pass<4>(V.E, V.sh, task, emit);
+            else ell_pass<3>(V.E, V.sh, task, emit);
+            done = true;
+        }
+    }
+    if (!done) {
+        const Seg sg = xcd_segment(task, V.nodes);
+        const int lane = threadIdx.x % L;
+        kry_t_nodes<L>(V, V.A, sg, lane, part);
+    }
     constexpr int slots[2] = {P_TS, P_TT};
     write_partials<2>(part, V.partials, task.block, slots);
 }

@@ -40,6 +40,11 @@ struct Pattern {
     // elem_ptr[i] .. elem_ptr[i+1] hold e << 2 | a with conn[e][a] = i, sorted by element
     DevBuf elem_ptr;   // int  [N + 1]
     DevBuf elem_list;  // u32  [E * ne]
+    // node-major sliced ELL structure (EllMat in views.hpp; PSPG cases)
+    int ell_nsl = 0;          // slices of 64 nodes
+    long long ell_ranks = 0;  // sum of the slice widths
+    DevBuf ell_soff;          // int [ell_nsl + 1]
+    DevBuf ell_idx;           // int [ell_ranks * 64]
 };
 
 // Builds every array of `pat` on `stream` (synchronises the stream before returning,
@@ -47,6 +52,10 @@ struct Pattern {
 // flags_in: device, N bytes (NODE_FREE is recomputed).
 void build_pattern(int dim, int N, int E, const int* conn_aos, const uint8_t* flags_in, hipStream_t stream,
                    Pattern& pat);
+
+// Builds the sliced ELL structure of `pat` (slice widths and offsets, padded column nodes)
+// from its node graph (synchronises the stream).
+void build_ell(Pattern& pat, hipStream_t stream);
 
 // Writes the nnz CSR column indices of `pat` into col_idx (device) on `stream` (export
 // only: the kernels address values through the node graph).

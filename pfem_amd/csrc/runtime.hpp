@@ -51,6 +51,7 @@ struct Workspace {
     double* x_prev = nullptr;
     double* pos_save = nullptr;
     double* kv64 = nullptr;  // 8 n: r, rh, p, v, s, t, ph, sh
+    double* vals_ell = nullptr;  // sliced ELL copy of the values (EllMat)
     float* vals32 = nullptr;
     float* binv32 = nullptr;
     float* kv32 = nullptr;  // 10 n: b, x, r, rh, p, v, s, t, ph, sh
@@ -68,7 +69,7 @@ struct Workspace {
         bytes = 0;
         arena = nullptr;
         off = 0;
-        vals = b = binv = x = x_prev = pos_save = kv64 = nullptr;
+        vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = nullptr;
         vals32 = binv32 = kv32 = nullptr;
         amg_w0 = amg_t0 = amg_dense = nullptr;
         for (auto& l : amg_lv)

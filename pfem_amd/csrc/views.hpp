@@ -24,6 +24,9 @@ struct PspgView {
     double* b;                // [(dim+1)N]
     float* vals32;            // [nnz] FP32 copy for the mixed-precision inner solve (may be null)
     const int* diag_pos;      // [n]
+    int ell_nsl, pad_ell;     // ELL copy of the values (EllMat): slices,
+    const int* ell_soff;      // slice offsets,
+    double* vals_ell;         // values written after assembly and BC (null: none)
     double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
     float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
@@ -44,6 +47,23 @@ struct NodeMat {
     const int* nbr_ptr;  // [N+1]
     const int* nbr_idx;  // [graph_nnz]
     const T* vals;       // [nnz]
+};
+
+// Node-major sliced ELL copy of a case's FP64 system (rebuilt from the values after every
+// assembly) for the hot SpMV-type kernels: nodes in slices of 64, one wavefront lane per
+// node, each slice padded to its largest node degree W_c.  A node's d+1 rows are computed
+// by its own lane: no cross-lane sums, and every value load is 64 consecutive doubles.
+// Rows holding only their diagonal keep it at (rank of the node itself, t = s) and zeros
+// elsewhere, so every row is computed the same way.
+constexpr int kEllSlice = 64;  // nodes per slice: one wavefront
+
+struct EllMat {
+    int N, nf, nsl, pad;  // nodes, fields, slices
+    const int* soff;      // [nsl + 1] sum of the widths of the earlier slices
+    const int* idx;       // [soff[nsl] * 64] column node of lane r at rank k: idx[(soff[c] + k) * 64 + r]
+                          // (padding ranks: the node itself, with zero values)
+    const double* vals;   // [soff[nsl] * nf * nf * 64] value (s, t) of lane r at rank k:
+                          // vals[((soff[c] + k) * nf * nf + s * nf + t) * 64 + r]; null: no copy
 };
 
 // Scalars of one BiCGSTAB solve.  The per-iteration scalars live in two slots used
@@ -80,6 +100,7 @@ struct KrylovView {
     int nblk;       // workgroups of this case in the Krylov launches (partials producers)
     int amg;        // 1: ph and sh come from the multigrid V-cycle (K_p / K_s leave them)
     NodeMat<T> A;
+    EllMat E;       // FP64 solves: the ELL copy of A used by K_v / K_t (vals null: node-block A)
     const T* binv;  // [nf*nf][nodes] inverse of each node's diagonal (nf x nf) block, SoA
     const T* b;
     T* x;
@@ -104,6 +125,7 @@ constexpr int kAmgMaxDense = 88;  // coarsest-level unknowns inverted densely in
 template <typename T>
 struct AmgView {
     NodeMat<T> A;          // this level's matrix (level 0: the system or its FP32 copy)
+    EllMat E0;             // level 0 of an FP64 hierarchy: the system's ELL copy (vals null: none)
     const T* binv;         // block-Jacobi inverse, SoA [(nf*nf)][N]
     int level;
     int lanes;             // SpMV lanes per node on this level
