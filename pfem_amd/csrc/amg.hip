@@ -355,18 +355,21 @@ __global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict
     else res_body<T, 16>(V, task, which);
 }
 
-// r_{l+1}(I, s) = sum over the members i of aggregate I of t(i, s) (level 0: full rows only).
+// r_{l+1}(I, s) = sum over the members i of aggregate I of t(i, s) (level 0: full rows only);
+// kRestrictLanes lanes per aggregate share its members (aggregates hold 5-30 nodes).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __restrict__ views,
                                                          const BlockTask* __restrict__ tasks) {
+    constexpr int L = kRestrictLanes, NPB = kBlock / L;
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
     const int nf = V.A.nf, D = nf - 1, N = V.A.N, Nc = V.next_N;
-    for (long long I = (long long)task.block * kBlock + threadIdx.x; I < Nc; I += (long long)task.nblocks * kBlock) {
+    const int lane = threadIdx.x % L;
+    for (long long I = (long long)task.block * NPB + threadIdx.x / L; I < Nc; I += (long long)task.nblocks * NPB) {
         double acc[kMaxNf] = {0.0, 0.0, 0.0, 0.0};
-        for (int m = V.agg_ptr[I]; m < V.agg_ptr[I + 1]; ++m) {
+        for (int m = V.agg_ptr[I] + lane; m < V.agg_ptr[I + 1]; m += L) {
             const int i = V.agg_nodes[m];
             const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
 #pragma unroll
@@ -375,7 +378,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __res
         }
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s)
-            if (s < nf) V.next_r[I + (long long)s * Nc] = (T)acc[s];
+            if (s < nf) acc[s] = group_sum<L>(acc[s]);
+        if (lane < nf) V.next_r[I + (long long)lane * Nc] = (T)pick(acc, lane);
     }
 }
 

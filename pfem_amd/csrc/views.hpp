@@ -121,6 +121,7 @@ struct KrylovView {
 // kernels; T is the hierarchy's arithmetic type (FP64, or FP32 under FP64 BiCGSTAB: the
 // preconditioner only shapes the search directions, the solve converges on the FP64 system).
 constexpr int kAmgMaxDense = 88;  // coarsest-level unknowns inverted densely in LDS
+constexpr int kRestrictLanes = 8;  // lanes per aggregate of the restriction
 
 template <typename T>
 struct AmgView {
