@@ -163,28 +163,31 @@ __global__ __launch_bounds__(kBlock) void k_amg_bjac(const AmgView<T>* __restric
 // A P of a level with a coarser one: entry block (i, J) sums the blocks (i, j) of i's
 // neighbours j in aggregate J (gather list, fixed order).  Level 0 keeps only full rows
 // (i, s) and full columns (j, t): P prolongates to full rows only, and a row holding only
-// its diagonal has no entry off it.
+// its diagonal has no entry off it.  Written in the sliced ELL layout, padding ranks of
+// each node's slice as zeros.
 template <typename T, int L>
 __device__ __forceinline__ void ap_body(const AmgView<T>& V, const BlockTask& task) {
     constexpr int NPB = kBlock / L;
     const int lane = threadIdx.x % L;
     const int nf = V.A.nf, D = nf - 1;
     const NodeMat<T>& A = V.A;
-    const NodeMat<T>& P = V.AP;
+    const EllMatT<T>& P = V.AP;
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < P.N; i += (long long)task.nblocks * NPB) {
-        const int4 pinf = P.info[i];
-        const int pdeg = pinf.z;
+        const int e0 = V.ap_ptr[i], pdeg = V.ap_ptr[i + 1] - e0;
+        const long long c = i / kEllSlice, r = i % kEllSlice;
+        const long long b = P.soff[c];
+        const int W = P.soff[c + 1] - (int)b;
         const int4 ainf = A.info[i];
         const int anb = A.nbr_ptr[i];
         const uint8_t fi = (uint8_t)ainf.w;
-        for (int m = lane; m < pdeg; m += L) {
-            const int e = P.nbr_ptr[i] + m;
+        for (int m = lane; m < W; m += L) {
             double blk[kMaxNf][kMaxNf];
 #pragma unroll
             for (int s = 0; s < kMaxNf; ++s)
 #pragma unroll
                 for (int t = 0; t < kMaxNf; ++t) blk[s][t] = 0.0;
-            for (int q = V.ap_gptr[e]; q < V.ap_gptr[e + 1]; ++q) {
+            const int qe = m < pdeg ? V.ap_gptr[e0 + m + 1] : 0;
+            for (int q = m < pdeg ? V.ap_gptr[e0 + m] : 0; q < qe; ++q) {
                 const int k = V.ap_gsrc[q];
                 const uint8_t fj = V.mask ? (uint8_t)A.info[A.nbr_idx[anb + k]].w : (uint8_t)0;
 #pragma unroll
@@ -201,13 +204,10 @@ __device__ __forceinline__ void ap_body(const AmgView<T>& V, const BlockTask& ta
                 }
             }
 #pragma unroll
-            for (int s = 0; s < kMaxNf; ++s) {
-                if (s >= nf) continue;
-                const long long rs = row_start_of(pinf, s, D, P.mom_nnz) + m;
+            for (int s = 0; s < kMaxNf; ++s)
 #pragma unroll
                 for (int t = 0; t < kMaxNf; ++t)
-                    if (t < nf) V.ap_out[rs + (long long)t * pdeg] = (T)blk[s][t];
-            }
+                    if (s < nf && t < nf) V.ap_out[((b + m) * nf * nf + s * nf + t) * kEllSlice + r] = (T)blk[s][t];
         }
     }
 }
@@ -400,29 +400,31 @@ __global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __res
 }
 
 // z = w' + omega B^-1 (t - alpha (A P) z_{l+1}),  w' = w + alpha P z_{l+1} (level 0: full
-// rows only); L lanes per node for the A P product.
-template <typename T, int L>
-__device__ __forceinline__ void post_body(const AmgView<T>& V, const BlockTask& task, int which) {
-    constexpr int NPB = kBlock / L;
-    const int lane = threadIdx.x % L;
-    const int N = V.A.N, nf = V.A.nf, D = nf - 1, Nc = V.next_N;
+// rows only): one lane per node, whole 64-node slices per wavefront.
+template <int NF, typename T>
+__device__ __forceinline__ void post_ell(const AmgView<T>& V, const BlockTask& task, int which) {
+    const int N = V.A.N, D = NF - 1, Nc = V.next_N;
     const T om = (T)V.omega, al = (T)V.alpha;
-    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
+    const int lane = threadIdx.x & (kEllSlice - 1);
+    for (long long c = (long long)task.block * kWaves + (threadIdx.x >> 6); c < V.AP.nsl;
+         c += (long long)task.nblocks * kWaves) {
         T y[kMaxNf];
-        node_spmv<L>(V.AP, V.next_z, i, lane, y);
-        if (lane < nf) {
-            T d[kMaxNf];
+        ell_rows<NF>(V.AP, V.next_z, c, lane, y);
+        const long long i = c * kEllSlice + lane;
+        if (i >= N) continue;
+        T d[NF];
 #pragma unroll
-            for (int t = 0; t < kMaxNf; ++t) d[t] = (t < nf) ? V.t[i + (long long)t * N] - al * pick(y, t) : T(0);
+        for (int t = 0; t < NF; ++t) d[t] = V.t[i + (long long)t * N] - al * y[t];
+        const int I = V.fine_agg[i];
+        const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
+#pragma unroll
+        for (int s = 0; s < NF; ++s) {
             T acc = 0;
 #pragma unroll
-            for (int t = 0; t < kMaxNf; ++t)
-                if (t < nf) acc += V.binv[(long long)(lane * nf + t) * N + i] * d[t];
-            const long long q = i + (long long)lane * N;
+            for (int t = 0; t < NF; ++t) acc += V.binv[(long long)(s * NF + t) * N + i] * d[t];
+            const long long q = i + (long long)s * N;
             T wq = V.w[q];
-            const int I = V.fine_agg[i];
-            if (I >= 0 && (!V.mask || row_full_flags((uint8_t)V.A.info[i].w, lane, D)))
-                wq += al * V.next_z[I + (long long)lane * Nc];
+            if (I >= 0 && (!V.mask || row_full_flags(f, s, D))) wq += al * V.next_z[I + (long long)s * Nc];
             const T out = wq + om * acc;
             if (V.level == 0) V.out0[which][q] = (double)out;
             else V.z[q] = out;
@@ -437,9 +439,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_post(const AmgView<T>* __restric
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
-    if (V.ap_lanes == 4) post_body<T, 4>(V, task, which);
-    else if (V.ap_lanes == 8) post_body<T, 8>(V, task, which);
-    else post_body<T, 16>(V, task, which);
+    if (V.A.nf == 4) post_ell<4>(V, task, which);
+    else post_ell<3>(V, task, which);
 }
 
 // ---- launchers ---------------------------------------------------------------------------

@@ -44,11 +44,15 @@ struct AmgLevel {
     // A_finer P: rows of the finer level, columns of this level's aggregates
     int ap_nnz = 0;     // entries (finer node, aggregate)
     int ap_lanes = 4;
-    DevBuf ap_ptr;      // int  [N_finer+1]
-    DevBuf ap_idx;      // int  [ap_nnz] aggregates adjacent to each finer node, sorted
-    DevBuf ap_info;     // int4 [N_finer] {nf*ap_ptr, nf*ap_ptr, degree, 0}
+    DevBuf ap_ptr;      // int  [N_finer+1] entries of each finer node (its distinct adjacent
+                        //      aggregates, sorted; ELL ranks 0 .. degree-1)
     DevBuf ap_gptr;     // int  [ap_nnz+1] finer neighbours of each entry
     DevBuf ap_gsrc;     // int  [finer graph entries] rank k in the finer node's list
+    // sliced ELL layout of A_finer P (EllMatT): slices of 64 finer nodes
+    int ap_nsl = 0;
+    long long ap_ranks = 0;
+    DevBuf ap_soff;     // int [ap_nsl + 1]
+    DevBuf ap_eidx;     // int [ap_ranks * 64] aggregate of each rank (padding: 0, zero values)
 };
 
 struct AmgHierarchy {

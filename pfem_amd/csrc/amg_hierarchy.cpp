@@ -160,7 +160,6 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
         // A P structure: for each finer node i, the distinct aggregates of its active
         // neighbours (sorted) and, per (i, J), the ranks k of those neighbours in i's list
         std::vector<int> ap_ptr((size_t)G.N + 1, 0), ap_idx, ap_gptr(1, 0), ap_gsrc;
-        std::vector<int4> ap_info((size_t)G.N);
         {
             std::vector<std::pair<int, int>> nb;  // (aggregate, rank)
             int apmax = 0;
@@ -180,9 +179,7 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
                     a = b;
                 }
                 ap_ptr[i + 1] = (int)ap_idx.size();
-                const int deg = ap_ptr[i + 1] - ap_ptr[i];
-                apmax = std::max(apmax, deg);
-                ap_info[i] = int4{nf * ap_ptr[i], nf * ap_ptr[i], deg, 0};
+                apmax = std::max(apmax, ap_ptr[i + 1] - ap_ptr[i]);
             }
             PFEM_REQUIRE(apmax <= 255, E_UNSUPPORTED, "a node's neighbours span more than 255 aggregates");
         }
@@ -216,9 +213,31 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
         upload(L.gal_src, gsrc, stream);
         L.ap_nnz = (int)ap_idx.size();
         L.ap_lanes = lanes_for(G.N ? (double)L.ap_nnz / G.N : 0.0);
+        {  // sliced ELL structure of A P
+            const int nsl = (G.N + kEllSlice - 1) / kEllSlice;
+            std::vector<int> soff((size_t)nsl + 1, 0);
+            for (int c = 0; c < nsl; ++c) {
+                int w = 0;
+                for (int r = 0; r < kEllSlice; ++r) {
+                    const int i = c * kEllSlice + r;
+                    if (i < G.N) w = std::max(w, ap_ptr[i + 1] - ap_ptr[i]);
+                }
+                soff[c + 1] = soff[c] + w;
+            }
+            std::vector<int> eidx((size_t)soff[nsl] * kEllSlice, 0);
+            for (int c = 0; c < nsl; ++c)
+                for (int r = 0; r < kEllSlice; ++r) {
+                    const int i = c * kEllSlice + r;
+                    if (i >= G.N) continue;
+                    for (int m = 0; m < ap_ptr[i + 1] - ap_ptr[i]; ++m)
+                        eidx[((size_t)soff[c] + m) * kEllSlice + r] = ap_idx[(size_t)ap_ptr[i] + m];
+                }
+            L.ap_nsl = nsl;
+            L.ap_ranks = soff[nsl];
+            upload(L.ap_soff, soff, stream);
+            upload(L.ap_eidx, eidx, stream);
+        }
         upload(L.ap_ptr, ap_ptr, stream);
-        upload(L.ap_idx, ap_idx, stream);
-        upload(L.ap_info, ap_info, stream);
         upload(L.ap_gptr, ap_gptr, stream);
         upload(L.ap_gsrc, ap_gsrc, stream);
         PFEM_HIP(hipStreamSynchronize(stream));

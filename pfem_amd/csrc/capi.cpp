@@ -99,7 +99,8 @@ size_t persistent_bytes(pfem_case* c) {
         b += p->bytes;
     for (auto& L : c->amg.lev)
         for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,
-                          &L.gal_ptr, &L.gal_src, &L.ap_ptr, &L.ap_idx, &L.ap_info, &L.ap_gptr, &L.ap_gsrc})
+                          &L.gal_ptr, &L.gal_src, &L.ap_ptr, &L.ap_gptr, &L.ap_gsrc, &L.ap_soff,
+                          &L.ap_eidx})
             b += p->bytes;
     return b;
 }

@@ -9,37 +9,38 @@
 
 namespace pfem {
 
-// y[s] = (A x) of row (64 c + lane, s) for s < NF (y[s] = 0 above); lanes past the last
-// node compute zeros.
-template <int NF>
-__device__ __forceinline__ void ell_rows(const EllMat& E, const double* __restrict__ x, long long c, int lane,
-                                         double (&y)[kMaxNf]) {
+// y[s] = (A x) of row (64 c + lane, s) for s < NF (y[s] = 0 above), with values of type VT
+// and vectors and sums of type XT; lanes past the last node compute values that the
+// callers drop.
+template <int NF, typename VT, typename XT>
+__device__ __forceinline__ void ell_rows(const EllMatT<VT>& E, const XT* __restrict__ x, long long c, int lane,
+                                         XT (&y)[kMaxNf]) {
     const long long b = E.soff[c];
     const int W = E.soff[c + 1] - (int)b;
-    const double* __restrict__ vb = E.vals + b * (NF * NF * kEllSlice) + lane;
+    const VT* __restrict__ vb = E.vals + b * (NF * NF * kEllSlice) + lane;
     const int* __restrict__ ib = E.idx + b * kEllSlice + lane;
-    const long long N = E.N;
-    double acc[NF];
+    const long long Nx = E.ncol ? E.ncol : E.N;
+    XT acc[NF];
 #pragma unroll
-    for (int s = 0; s < NF; ++s) acc[s] = 0.0;
+    for (int s = 0; s < NF; ++s) acc[s] = XT(0);
 #pragma unroll 2
     for (int k = 0; k < W; ++k) {
         const long long j = ib[(long long)k * kEllSlice];
-        double a[NF][NF];
+        XT a[NF][NF];
 #pragma unroll
         for (int s = 0; s < NF; ++s)
 #pragma unroll
-            for (int t = 0; t < NF; ++t) a[s][t] = vb[((long long)k * NF * NF + s * NF + t) * kEllSlice];
-        double xv[NF];
+            for (int t = 0; t < NF; ++t) a[s][t] = (XT)vb[((long long)k * NF * NF + s * NF + t) * kEllSlice];
+        XT xv[NF];
 #pragma unroll
-        for (int t = 0; t < NF; ++t) xv[t] = x[j + t * N];
+        for (int t = 0; t < NF; ++t) xv[t] = x[j + t * Nx];
 #pragma unroll
         for (int s = 0; s < NF; ++s)
 #pragma unroll
             for (int t = 0; t < NF; ++t) acc[s] += a[s][t] * xv[t];
     }
 #pragma unroll
-    for (int s = 0; s < kMaxNf; ++s) y[s] = s < NF ? acc[s] : 0.0;
+    for (int s = 0; s < kMaxNf; ++s) y[s] = s < NF ? acc[s] : XT(0);
 }
 
 }  // namespace pfem

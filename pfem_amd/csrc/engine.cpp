@@ -200,7 +200,8 @@ WsLayout ws_layout(const pfem_case* c) {
             L.amg_lv[l][1] = take(es * nf * nf * (size_t)lv.N);
             for (int k = 2; k < 6; ++k) L.amg_lv[l][k] = take(es * nl);
         }
-        for (int l = 0; l + 1 < H.levels; ++l) L.amg_ap[l] = take(es * nf * nf * (size_t)H.lev[l + 1].ap_nnz);
+        for (int l = 0; l + 1 < H.levels; ++l)
+            L.amg_ap[l] = take(es * nf * nf * kEllSlice * (size_t)H.lev[l + 1].ap_ranks);
         L.amg_dense = take(8 * (size_t)H.coarse_n * H.coarse_n);  // FP64 in both types
     }
     L.total = o;
@@ -673,17 +674,17 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         }
         if (l + 1 < H.levels) {
             const AmgLevel& Lc = H.lev[l + 1];
-            NodeMat<T> AP{};
+            EllMatT<T> AP{};
             AP.N = V.A.N;
             AP.nf = c->dim + 1;
-            AP.mom_nnz = AP.nf * Lc.ap_nnz;
+            AP.nsl = Lc.ap_nsl;
             AP.ncol = Lc.N;
-            AP.info = Lc.ap_info.as<int4>();
-            AP.nbr_ptr = Lc.ap_ptr.as<int>();
-            AP.nbr_idx = Lc.ap_idx.as<int>();
+            AP.soff = Lc.ap_soff.as<int>();
+            AP.idx = Lc.ap_eidx.as<int>();
             AP.vals = arr(w.amg_ap[l]);
             V.AP = AP;
             V.ap_lanes = Lc.ap_lanes;
+            V.ap_ptr = Lc.ap_ptr.as<int>();
             V.ap_gptr = Lc.ap_gptr.as<int>();
             V.ap_gsrc = Lc.ap_gsrc.as<int>();
             V.ap_out = arr(w.amg_ap[l]);

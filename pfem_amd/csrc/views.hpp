@@ -57,14 +57,17 @@ struct NodeMat {
 // elsewhere, so every row is computed the same way.
 constexpr int kEllSlice = 64;  // nodes per slice: one wavefront
 
-struct EllMat {
-    int N, nf, nsl, pad;  // nodes, fields, slices
+template <typename VT>
+struct EllMatT {
+    int N, nf, nsl;       // row nodes, fields, slices
+    int ncol;             // column nodes when they differ from the rows' (0: N)
     const int* soff;      // [nsl + 1] sum of the widths of the earlier slices
     const int* idx;       // [soff[nsl] * 64] column node of lane r at rank k: idx[(soff[c] + k) * 64 + r]
-                          // (padding ranks: the node itself, with zero values)
-    const double* vals;   // [soff[nsl] * nf * nf * 64] value (s, t) of lane r at rank k:
+                          // (padding ranks: the node itself or node 0, with zero values)
+    const VT* vals;       // [soff[nsl] * nf * nf * 64] value (s, t) of lane r at rank k:
                           // vals[((soff[c] + k) * nf * nf + s * nf + t) * 64 + r]; null: no copy
 };
+using EllMat = EllMatT<double>;
 
 // Scalars of one BiCGSTAB solve.  The per-iteration scalars live in two slots used
 // alternately (the host passes the slot parity of each iteration): a kernel only writes
@@ -150,15 +153,16 @@ struct AmgView {
     const int* fine_agg;   // next-level aggregate of each node of this level (-1: none)
     T* next_r;
     const T* next_z;
-    // A P: this level's rows, the next coarser level's columns (every row full; the entry
-    // block (i, J) sums the blocks (i, j) of i's neighbours j in aggregate J, level 0: over
-    // full rows (j, t) only, and is zero in rows (i, s) that hold only their diagonal)
-    NodeMat<T> AP;
-    int ap_lanes;          // SpMV lanes per node of AP
+    // A P in the sliced ELL layout: this level's rows, the next coarser level's columns; the
+    // entry block (i, J) sums the blocks (i, j) of i's neighbours j in aggregate J (level 0:
+    // over full rows (j, t) only; zero in rows (i, s) that hold only their diagonal)
+    EllMatT<T> AP;
+    int ap_lanes;          // lanes per node of the set-up kernel
     int pad1;
+    const int* ap_ptr;     // [N + 1] AP entries of each node (ranks 0 .. degree-1 of its ELL row)
     const int* ap_gptr;    // [AP entries + 1] gather list of each AP entry
     const int* ap_gsrc;    // rank k in i's neighbour list of each gathered block
-    T* ap_out;             // AP values (written at set-up)
+    T* ap_out;             // AP values (written at set-up, padding ranks included)
     // setup of this level (>= 1) from the finer level
     NodeMat<T> F;
     const int* gal_ptr;
