@@ -316,21 +316,21 @@ __device__ __forceinline__ void res_body(const AmgView<T>& V, const BlockTask& t
 
 // Level 0 of an FP64 hierarchy with the system's ELL copy: one lane per node, whole
 // 64-node slices per wavefront (a workgroup's waves stride over the case's slices).
-template <int NF>
-__device__ __forceinline__ void res_ell(const AmgView<double>& V, const BlockTask& task, int which) {
-    const EllMat& E = V.E0;
+template <int NF, typename T>
+__device__ __forceinline__ void res_ell(const AmgView<T>& V, const BlockTask& task, int which) {
+    const EllMatT<T>& E = V.E0;
     const int lane = threadIdx.x & (kEllSlice - 1);
     const long long N = E.N;
     for (long long c = (long long)task.block * kWaves + (threadIdx.x >> 6); c < E.nsl;
          c += (long long)task.nblocks * kWaves) {
-        double y[kMaxNf];
-        ell_rows<NF>(E, V.w, c, lane, y);
+        T y[kMaxNf];
+        ell_rows<NF>(E, (const T*)V.w, c, lane, y);
         const long long i = c * kEllSlice + lane;
         if (i < N) {
 #pragma unroll
             for (int s = 0; s < NF; ++s) {
                 const long long q = i + (long long)s * N;
-                V.t[q] = V.r0[which][q] - y[s];
+                V.t[q] = (T)V.r0[which][q] - y[s];
             }
         }
     }
@@ -343,12 +343,10 @@ __global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
-    if constexpr (std::is_same<T, double>::value) {
-        if (V.level == 0 && V.E0.vals) {
-            if (V.A.nf == 4) res_ell<4>(V, task, which);
-            else res_ell<3>(V, task, which);
-            return;
-        }
+    if (V.level == 0 && V.E0.vals) {
+        if (V.A.nf == 4) res_ell<4>(V, task, which);
+        else res_ell<3>(V, task, which);
+        return;
     }
     if (V.lanes == 4) res_body<T, 4>(V, task, which);
     else if (V.lanes == 8) res_body<T, 8>(V, task, which);

@@ -109,7 +109,9 @@ __global__ __launch_bounds__(kBlock) void k_ell_fill(const PspgView* __restrict_
             for (int p = wave; p < W * nf; p += kWaves) {
                 const int k = p / nf, t = p - k * nf;
                 const double v = (i < N) ? ell_value(row, full, inf.z, self, s, t, k) : 0.0;
-                V.vals_ell[((b + k) * nf * nf + s * nf + t) * kSlice + lane] = v;
+                const long long q = ((b + k) * nf * nf + s * nf + t) * kSlice + lane;
+                V.vals_ell[q] = v;
+                if (V.vals32_ell) V.vals32_ell[q] = (float)v;
             }
         }
     }

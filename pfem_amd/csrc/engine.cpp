@@ -100,8 +100,9 @@ NodeMat<T> node_mat(const Pattern& P, int dim, const T* vals) {
     return A;
 }
 
-EllMat ell_mat(const Pattern& P, int dim, const double* vals) {
-    EllMat E{};
+template <class T>
+EllMatT<T> ell_mat(const Pattern& P, int dim, const T* vals) {
+    EllMatT<T> E{};
     E.N = P.N;
     E.nf = dim + 1;
     E.nsl = P.ell_nsl;
@@ -151,7 +152,7 @@ int pick_lanes(double avg_deg) {
 // ---------------------------------------------------------------------------------------
 namespace {
 struct WsLayout {
-    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32, binv32, kv32, total;
+    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32_ell, vals32, binv32, kv32, total;
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
     size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
@@ -188,6 +189,7 @@ WsLayout ws_layout(const pfem_case* c) {
         L.binv32 = take(4 * nf * nf * N);
     }
     if (ir) L.kv32 = take(4 * 10 * n);
+    if (amg_fp32(c)) L.vals32_ell = take(4 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
     const AmgHierarchy& H = c->amg;
     if (H.levels > 1) {
         const size_t es = amg_fp32(c) ? 4 : 8;
@@ -232,6 +234,7 @@ void carve_workspace(pfem_case* c, char* base) {
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
     w.binv32 = v32 ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
     w.kv32 = ir ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
+    w.vals32_ell = amg_fp32(c) ? reinterpret_cast<float*>(base + L.vals32_ell) : nullptr;
     const AmgHierarchy& H = c->amg;
     w.amg_w0 = w.amg_t0 = w.amg_dense = nullptr;
     for (auto& lv : w.amg_lv)
@@ -454,6 +457,7 @@ void Engine::Impl::load_views(Req* r) {
         pv.ell_nsl = P.ell_nsl;
         pv.ell_soff = P.ell_soff.as<int>();
         pv.vals_ell = w.vals_ell;
+        pv.vals32_ell = w.vals32_ell;
         pv.binv = w.binv;
         pv.binv32 = w.binv32;
         pv.nnz = P.nnz;
@@ -504,7 +508,7 @@ void Engine::Impl::load_views(Req* r) {
         k.partials = c->partials.as<double>();
         k.counters = c->counters.as<unsigned>();
         k.sc = &ds->k[0];
-        k.E = ell_mat(P, dim, w.vals_ell);
+        k.E = ell_mat<double>(P, dim, w.vals_ell);
         k.amg = c->amg.levels > 1 ? 1 : 0;
         if (k.amg) {
             if (amg_fp32(c))
@@ -638,7 +642,8 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         if (l == 0) {
             V.A = A0;
             V.binv = binv0;
-            if (std::is_same<T, double>::value) V.E0 = ell_mat(c->pat, c->dim, w.vals_ell);
+            if constexpr (std::is_same<T, double>::value) V.E0 = ell_mat<double>(c->pat, c->dim, w.vals_ell);
+            else V.E0 = ell_mat<float>(c->pat, c->dim, w.vals32_ell);
             V.lanes = c->lanes;
             V.mask = 1;
             V.r0[0] = k.p;

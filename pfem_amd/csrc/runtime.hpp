@@ -52,6 +52,7 @@ struct Workspace {
     double* pos_save = nullptr;
     double* kv64 = nullptr;  // 8 n: r, rh, p, v, s, t, ph, sh
     double* vals_ell = nullptr;  // sliced ELL copy of the values (EllMat)
+    float* vals32_ell = nullptr;  // its FP32 copy (FP32 multigrid hierarchies)
     float* vals32 = nullptr;
     float* binv32 = nullptr;
     float* kv32 = nullptr;  // 10 n: b, x, r, rh, p, v, s, t, ph, sh
@@ -70,7 +71,7 @@ struct Workspace {
         arena = nullptr;
         off = 0;
         vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = nullptr;
-        vals32 = binv32 = kv32 = nullptr;
+        vals32 = binv32 = kv32 = vals32_ell = nullptr;
         amg_w0 = amg_t0 = amg_dense = nullptr;
         for (auto& l : amg_lv)
             for (auto& p : l) p = nullptr;

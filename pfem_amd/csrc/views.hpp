@@ -27,6 +27,7 @@ struct PspgView {
     int ell_nsl, pad_ell;     // ELL copy of the values (EllMat): slices,
     const int* ell_soff;      // slice offsets,
     double* vals_ell;         // values written after assembly and BC (null: none)
+    float* vals32_ell;        // FP32 ELL copy for an FP32 multigrid hierarchy (null: none)
     double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
     float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
@@ -129,7 +130,7 @@ constexpr int kRestrictLanes = 8;  // lanes per aggregate of the restriction
 template <typename T>
 struct AmgView {
     NodeMat<T> A;          // this level's matrix (level 0: the system or its FP32 copy)
-    EllMat E0;             // level 0 of an FP64 hierarchy: the system's ELL copy (vals null: none)
+    EllMatT<T> E0;         // level 0: the system's ELL copy in type T (vals null: none)
     const T* binv;         // block-Jacobi inverse, SoA [(nf*nf)][N]
     int level;
     int lanes;             // SpMV lanes per node on this level
