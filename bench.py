@@ -406,12 +406,14 @@ def run_dambreak3d(ctx, steps, warmup):
             raise SystemExit(f"dam break step {k} failed: {case.last_stats}")
     total = time.perf_counter() - t0
     prof = case.profile_spmv(False)
-    # ceilings on the same device memory: a streaming read of the values and a copy kernel
-    # (also the known-size kernel the HBM byte counters are scaled by)
-    kb = case.bench_kernels(20)
+    # the same SpMV kernel replayed back to back on the last step's system (its duration
+    # without the event packets that the per-dispatch timing inside the solve adds), and
+    # the ceilings on the same device memory: a streaming read of the values and a copy
+    # kernel (also the known-size kernel the HBM byte counters are scaled by)
+    kb = case.bench_kernels(50)
     gnnz = graph_nnz_from_csr(rp, 3, mesh.n_nodes)
     bytes_spmv = spmv_fused_bytes(mesh.n_nodes, n, nnz, gnnz)
-    spmv_ms = prof["avg_ms_fp64"]
+    spmv_ms = kb["kry_v_us"] * 1e-3
     achieved = bytes_spmv / (spmv_ms * 1e-3) / 1e9 if spmv_ms > 0 else 0.0
     t_asm = float(np.mean([s["t_assemble"] for s in stats]))
     out = dict(
@@ -425,7 +427,8 @@ def run_dambreak3d(ctx, steps, warmup):
         elements_per_s=mesh.n_elems / t_asm if t_asm > 0 else None,
     )
     roof = {
-        "kernel": "k_kry_v<double,16> (node-block SpMV v = A ph with fused (rh, v)), 3D dam break 262,701 nodes",
+        "kernel": "k_kry_v<double,16> (BiCGSTAB SpMV v = A ph over the sliced ELL copy, fused (rh, v)), "
+                  "3D dam break 262,701 nodes",
         "bound": "hbm",
         "achieved": achieved,
         "peak": HBM_PEAK_GBPS,
@@ -435,11 +438,13 @@ def run_dambreak3d(ctx, steps, warmup):
         "traffic_source": "profiles/spmv_kry_v_dambreak3d_262k_pmc_traffic.md (rocprofv3 FETCH_SIZE x 1.9988 + "
                           "WRITE_SIZE x 1.0, factors from a copy kernel of known size)",
         "avg_launch_ms": spmv_ms,
-        "launches_timed": prof["launches_fp64"],
+        "launch_timing": "HIP events around 50 back-to-back launches of the kernel on the case's stream, "
+                         "after the timed steps",
+        "in_solve_event_ms": prof["avg_ms_fp64"],
+        "in_solve_launches_timed": prof["launches_fp64"],
         "algorithmic_bytes_per_launch": bytes_spmv,
         "ceilings_GBps": {"stream_read": nnz * 8 / kb["stream_read_us"] / 1e3,
                           "copy": kb["copy_bytes"] / kb["copy_us"] / 1e3},
-        "spmv_back_to_back_us": kb["spmv64_us"],
     }
     case.close()
     return out, roof
