@@ -7,11 +7,12 @@ weakly compressible sloshing tanks; each request is one time step of one case (a
 step with Ax_f minRes 1e-6, maxIter 10 and linear solves to a true relative residual of
 1e-8; or a CDS_dpdt step followed by computeNextDT).  Two paths serve the same requests:
 
-* plain FP64: one request at a time through the direct C-ABI calls (FP64 BiCGSTAB, FP64
-  WC state) — the drop-in replacement of the reference's per-case step;
+* plain FP64: one request at a time through the direct C-ABI calls (FP64 BiCGSTAB with FP64
+  multigrid or block Jacobi, FP64 WC state) — the drop-in replacement of the reference's
+  per-case step;
 * batched mixed: the continuous-batching server (requests join the running batch at every
-  engine tick, admitted by step-workspace footprint), FP32 inner BiCGSTAB with FP64
-  iterative refinement, FP32 WC state with FP64 accumulators.
+  engine tick, admitted by step-workspace footprint), PFEM_PREC_MIXED (FP64 BiCGSTAB; the
+  multigrid of 3D cases in FP32, from 20,000 nodes), FP32 WC state with FP64 sums.
 
 Each path runs the stream twice: saturated (every request queued at t0: throughput, the
 headline steps/s) and with seeded Poisson arrivals at 200 requests/s (p50/p99 latency from
@@ -565,9 +566,12 @@ def main():
                          f"{npspg} PSPG dam-break columns / {n_req - npspg} WC sloshing tanks, nodes log-uniform "
                          f"2k-300k, one time step per request; Poisson arrivals {RATE:.0f}/s for latency, "
                          f"saturated for throughput"),
-            "picard": "Ax_f, minRes 1e-6, maxIter 10", "linear_solve": "BiCGSTAB + block Jacobi, relres <= 1e-8",
-            "batched": "continuous batching, FP32 inner BiCGSTAB + FP64 refinement, FP32 WC state",
-            "plain": "one request at a time, FP64 BiCGSTAB, FP64 WC state",
+            "picard": "Ax_f, minRes 1e-6, maxIter 10",
+            "linear_solve": "BiCGSTAB to true relres <= 1e-8 (and ||r|| <= 0.1 minRes), right-preconditioned by "
+                            "aggregation multigrid (2D >= 5k nodes; 3D >= 400k FP64 / >= 20k mixed) or block Jacobi",
+            "batched": "continuous batching (64 slots), PFEM_PREC_MIXED: FP64 BiCGSTAB, FP32 3D multigrid, "
+                       "FP32 WC state",
+            "plain": "one request at a time through the direct calls, FP64 throughout",
             "slots": args.slots, "arena_GiB": args.budget_gb,
         },
         "speedup_vs_plain_fp64": bs["steps_per_s"] / ps["steps_per_s"],
