@@ -95,7 +95,7 @@ size_t persistent_bytes(pfem_case* c) {
     for (DevBuf* p : {&c->pos, &c->dir_vals, &c->q_prev, &c->partials, &c->counters, &c->wstate, &c->wacc,
                       &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr, &c->pat.nbr_idx, &c->pat.node_info,
                       &c->pat.row_ptr, &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei, &c->pat.elem_ptr,
-                      &c->pat.elem_list, &c->pat.ell_soff, &c->pat.ell_idx})
+                      &c->pat.elem_list, &c->pat.elem_nodes, &c->pat.ell_soff, &c->pat.ell_idx})
         b += p->bytes;
     for (auto& L : c->amg.lev)
         for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,

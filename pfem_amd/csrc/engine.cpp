@@ -570,6 +570,7 @@ void Engine::Impl::load_views(Req* r) {
             V.flags = c->pat.flags.as<uint8_t>();
             V.elem_ptr = c->pat.elem_ptr.as<int>();
             V.elem_list = c->pat.elem_list.as<uint32_t>();
+            V.elem_nodes = c->pat.elem_nodes.as<int4>();
             V.pos = c->pos.as<double>();
             // layout of pfem_case_set_state: v (dim x N), p (N), rho (N), a (dim x N)
             V.v = base;

@@ -245,6 +245,20 @@ __global__ void k_elem_fill(int E, int ne, const int* __restrict__ soa, const in
         }
 }
 
+// Nodes of the element of each node-element incidence entry (padded with -1 in 2D).
+__global__ void k_elem_nodes(long long total, int E, int ne, const int* __restrict__ soa,
+                             const uint32_t* __restrict__ list, int4* __restrict__ nodes) {
+    for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < total; q += (long long)gridDim.x * blockDim.x) {
+        const long long e = list[q] >> 2;
+        int4 v;
+        v.x = soa[e];
+        v.y = soa[(long long)E + e];
+        v.z = soa[2LL * E + e];
+        v.w = ne > 3 ? soa[3LL * E + e] : -1;
+        nodes[q] = v;
+    }
+}
+
 // Sorts each list m (entries ptr[m] .. ptr[m+1]) in increasing order.
 __global__ void k_pei_sort(int M, const int* __restrict__ pei_ptr, uint32_t* __restrict__ pei) {
     for (int m = blockIdx.x * blockDim.x + threadIdx.x; m < M; m += gridDim.x * blockDim.x) {
@@ -307,6 +321,11 @@ void build_mesh_basic(int dim, int N, int E, const int* conn_aos, const uint8_t*
                                                                    cursor.as<int>(), pat.elem_list.as<uint32_t>());
     k_pei_sort<<<grid_for(N, kBlock, 65536), kBlock, 0, stream>>>(N, pat.elem_ptr.as<int>(),
                                                                   pat.elem_list.as<uint32_t>());
+    const long long total = (long long)E * ne;
+    pat.elem_nodes.reserve(sizeof(int4) * (size_t)total);
+    k_elem_nodes<<<grid_for(total, kBlock, 65536), kBlock, 0, stream>>>(total, E, ne, pat.conn.as<int>(),
+                                                                        pat.elem_list.as<uint32_t>(),
+                                                                        pat.elem_nodes.as<int4>());
     PFEM_HIP(hipStreamSynchronize(stream));
 }
 

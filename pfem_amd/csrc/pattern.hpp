@@ -40,6 +40,8 @@ struct Pattern {
     // elem_ptr[i] .. elem_ptr[i+1] hold e << 2 | a with conn[e][a] = i, sorted by element
     DevBuf elem_ptr;   // int  [N + 1]
     DevBuf elem_list;  // u32  [E * ne]
+    DevBuf elem_nodes; // int4 [E * ne] the element's nodes (local order) of each entry, so that
+                       //      a gather over a node's elements needs no connectivity lookup
     // node-major sliced ELL structure (EllMat in views.hpp; PSPG cases)
     int ell_nsl = 0;          // slices of 64 nodes
     long long ell_ranks = 0;  // sum of the slice widths

@@ -199,8 +199,9 @@ struct WcView {
     int dim, N, E, pad;
     const int* conn;          // [ne][E]
     const uint8_t* flags;     // [N] with NODE_FREE derived
-    const int* elem_ptr;      // [N+1] node-element incidence (Pattern::elem_ptr / elem_list)
-    const uint32_t* elem_list;
+    const int* elem_ptr;      // [N+1] node-element incidence (Pattern::elem_ptr / elem_list /
+    const uint32_t* elem_list;  //       elem_nodes)
+    const int4* elem_nodes;
     double* pos;              // [dim][N]
     S* v;                     // [dim][N]
     S* a;                     // [dim][N]

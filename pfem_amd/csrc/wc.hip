@@ -117,7 +117,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const WcView<S>& V = views[task.case_id];
-    const int N = V.N, E = V.E;
+    const int N = V.N;
     const double dt = V.dt;
     const double c2 = 1.0 / (double)((D + 1) * (D + 2));
     const int lane = threadIdx.x % L;
@@ -125,12 +125,13 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
         const int q1 = V.elem_ptr[i + 1];
         double ml = 0.0, fc = 0.0;
         for (int q = V.elem_ptr[i] + lane; q < q1; q += L) {
-            const uint32_t code = V.elem_list[q];
-            const long long e = code >> 2;
-            const int me = (int)(code & 3u);
+            const int me = (int)(V.elem_list[q] & 3u);
+            const int4 en = V.elem_nodes[q];
             int nd[NE];
-#pragma unroll
-            for (int a = 0; a < NE; ++a) nd[a] = V.conn[(long long)a * E + e];
+            nd[0] = en.x;
+            nd[1] = en.y;
+            nd[2] = en.z;
+            if constexpr (NE > 3) nd[NE - 1] = en.w;
             double x[D][NE];
             WGeom<D> G;
             wc_geometry<D>(V.pos, N, nd, x, G);
@@ -186,7 +187,7 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_gather(const WcView<S>* __res
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const WcView<S>& V = views[task.case_id];
-    const int N = V.N, E = V.E;
+    const int N = V.N;
     const double c2 = 1.0 / (double)((D + 1) * (D + 2));
     const int lane = threadIdx.x % L;
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
@@ -195,12 +196,13 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_gather(const WcView<S>* __res
 #pragma unroll
         for (int s = 0; s < D; ++s) fm[s] = 0.0;
         for (int q = V.elem_ptr[i] + lane; q < q1; q += L) {
-            const uint32_t code = V.elem_list[q];
-            const long long e = code >> 2;
-            const int me = (int)(code & 3u);
+            const int me = (int)(V.elem_list[q] & 3u);
+            const int4 en = V.elem_nodes[q];
             int nd[NE];
-#pragma unroll
-            for (int a = 0; a < NE; ++a) nd[a] = V.conn[(long long)a * E + e];
+            nd[0] = en.x;
+            nd[1] = en.y;
+            nd[2] = en.z;
+            if constexpr (NE > 3) nd[NE - 1] = en.w;
             double x[D][NE];
             WGeom<D> G;
             wc_geometry<D>(V.pos, N, nd, x, G);
