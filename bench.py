@@ -241,6 +241,59 @@ def cpu_baseline(stream, meshes, states, threads, budget_s=20.0):
 
 
 # ------------------------------------------------------------------------------------------
+# the 2D dam break that the reference itself runs on the CPU
+# ------------------------------------------------------------------------------------------
+def run_dambreak2d(ctx, steps=10, threads=1, with_cpu=True):
+    """2D PSPG dam break (1.0 x 2.0 m column in a 4.0 x 3.0 m tank, h = 0.05 m, 1,001 nodes):
+    `steps` Picard steps of dt 1e-3 through the plain FP64 drop-in, and the same steps through
+    the oracle on the host (BiCGSTAB + ILU(0) to relres 1e-12 standing in for SparseLU; its
+    dense LU takes ~9 s a step here); the largest state and position differences after the
+    last step."""
+    import pfem_amd
+    from pfem_amd import meshgen as mg
+
+    dt = 1e-3
+    mesh = mg.dambreak_2d(h=0.05)
+    q0 = mg.pspg_hydrostatic(mesh)
+    eq = pfem_amd.MomContEqIncompNewton(ctx, 2, rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6, max_iter=10,
+                                        precision="fp64", krylov_rtol=1e-12, krylov_max_iter=20000)
+    eq.set_mesh(mesh)
+    eq.set_state(q0)
+    eq.solve(dt)  # warm-up step (kernel loading), then restart from the initial state
+    eq.set_state(q0)
+    eq.set_positions(mesh.pos)
+    lat = []
+    t0 = time.perf_counter()
+    for k in range(steps):
+        t1 = time.perf_counter()
+        if not eq.solve(dt):
+            raise SystemExit(f"2D dam break step {k} failed: {eq.last_stats}")
+        lat.append((time.perf_counter() - t1) * 1e3)
+    total = time.perf_counter() - t0
+    q_gpu, pos_gpu = eq.get_state(), eq.get_positions().reshape(-1)
+    eq.close()
+    out = dict(workload=f"2D PSPG dam break, plain FP64 drop-in ({mesh.n_nodes} nodes, {mesh.n_elems} triangles, "
+                        f"dt {dt}, {steps} steps)",
+               steps_per_s=steps / total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99))
+    if with_cpu:
+        from oracle import oracle as O
+
+        pos = mesh.pos.reshape(-1).copy()
+        q = q0.copy()
+        t0 = time.perf_counter()
+        for k in range(steps):
+            st, _ = O.pspg_step(mesh, pos, q, dt=dt, residual="Ax_f", min_res=1e-6, max_iter=10, solver_kind=2,
+                                rtol=1e-12, nthreads=threads)
+            if st != 0:
+                raise SystemExit(f"oracle 2D dam break step {k} failed")
+        cpu = time.perf_counter() - t0
+        out.update(oracle_steps_per_s=steps / cpu, oracle_threads=threads,
+                   max_state_diff_vs_oracle=float(np.abs(q_gpu - q).max() / np.abs(q).max()),
+                   max_position_diff_vs_oracle=float(np.abs(pos_gpu - pos).max()))
+    return out
+
+
+# ------------------------------------------------------------------------------------------
 # the 3D weakly compressible sloshing tank and the mixed-precision 3D column field
 # ------------------------------------------------------------------------------------------
 def wc_step_bytes(dim, nodes, elems, s_state):
@@ -468,7 +521,8 @@ def main():
                     help="PSPG solve precision of the batched path")
     ap.add_argument("--wc-steps", type=int, default=200, help="steps of the 3D WC sloshing tank")
     ap.add_argument("--precision-steps", type=int, default=3, help="steps of the 3D mixed-precision column field")
-    ap.add_argument("--only", choices=["batched", "plain", "dambreak", "wc3d", "precision"], default=None,
+    ap.add_argument("--only", choices=["batched", "plain", "dambreak", "dambreak2d", "wc3d", "precision"],
+                    default=None,
                     help="run one workload once and print its result only (for profilers)")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -481,6 +535,9 @@ def main():
     if args.only == "dambreak":
         db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
         print(json.dumps({"dambreak3d_fp64": db, "roofline": roof}), flush=True)
+        return
+    if args.only == "dambreak2d":
+        print(json.dumps({"dambreak2d_fp64": run_dambreak2d(ctx)}), flush=True)
         return
     if args.only == "wc3d":
         print(json.dumps({"wc3d": run_wc3d(ctx, args.wc_steps)}), flush=True)
@@ -540,6 +597,8 @@ def main():
 
     db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
     log(f"[bench] dam break 3D: {db}")
+    db2 = run_dambreak2d(ctx, 10, with_cpu=not args.no_cpu_baseline)
+    log(f"[bench] dam break 2D: {db2}")
     wc3d = run_wc3d(ctx, args.wc_steps) if args.wc_steps > 0 else None
     log(f"[bench] WC sloshing 3D: {wc3d}")
     prec3d = run_precision3d(ctx, args.precision_steps) if args.precision_steps > 0 else None
@@ -586,6 +645,7 @@ def main():
                           "peak_inflight": bs["peak_inflight"], "arena_peak_bytes": bs["arena_peak"]},
         "max_relres": max(bs["max_relres"], bp["max_relres"], ps["max_relres"], pp["max_relres"]),
         "accuracy_vs_fp64": acc,
+        "dambreak2d_fp64": db2,
         "dambreak3d_fp64": db,
         "wc3d": wc3d,
         "precision3d": prec3d,
