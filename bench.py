@@ -510,10 +510,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=512, help="requests in the serving stream")
     ap.add_argument("--warmup", type=int, default=2, help="untimed warm-up requests per path")
-    # slots: 64 / 128 / 192 served the 512-request stream at 144 / 156 / 161 steps/s (larger
-    # launches, shorter tail); the arena peaks near 50 GiB at 192
-    ap.add_argument("--slots", type=int, default=192, help="server requests in flight at once")
-    ap.add_argument("--budget-gb", type=float, default=96.0, help="server workspace arena (GiB)")
+    # slots: 64 / 128 / 192 / 256 / 384 served the 512-request stream at 144 / 156 / 161 / 166 /
+    # 166 steps/s (larger launches, shorter tail); the arena peaks near 59 GiB from 256 on
+    ap.add_argument("--slots", type=int, default=256, help="server requests in flight at once")
+    ap.add_argument("--budget-gb", type=float, default=120.0, help="server workspace arena (GiB)")
     ap.add_argument("--dambreak-steps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
