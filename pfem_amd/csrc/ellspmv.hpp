@@ -1,6 +1,9 @@
 // SpMV over the sliced ELL copy of a PSPG system (EllMat): lane r of a wavefront computes
 // the NF rows of node 64 c + r of slice c.  Each rank k is one column node j (one index
 // load, NF gathers of x) and NF*NF value loads, each 64 consecutive doubles per wavefront.
+// The values are loaded non-temporally: they are read once per launch, and without the hint
+// they evict the x entries and the results from the caches (measured on gfx950, 516 MB
+// matrix: 98.7 -> 90.0 us per launch; 2D 900k nodes: 104 -> 94 us).
 #pragma once
 
 #include "common.hpp"
@@ -30,7 +33,8 @@ __device__ __forceinline__ void ell_rows(const EllMatT<VT>& E, const XT* __restr
 #pragma unroll
         for (int s = 0; s < NF; ++s)
 #pragma unroll
-            for (int t = 0; t < NF; ++t) a[s][t] = (XT)vb[((long long)k * NF * NF + s * NF + t) * kEllSlice];
+            for (int t = 0; t < NF; ++t)
+                a[s][t] = (XT)ld_stream(vb + ((long long)k * NF * NF + s * NF + t) * kEllSlice);
         XT xv[NF];
 #pragma unroll
         for (int t = 0; t < NF; ++t) xv[t] = x[j + t * Nx];

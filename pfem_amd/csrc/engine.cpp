@@ -82,8 +82,8 @@ void append(std::vector<BlockTask>& t, int slot, long long work, int cap, int pa
     for (int b = 0; b < nb; ++b) t.push_back(BlockTask{slot, b, (int)nb, pad});
 }
 
-void append_rows(std::vector<BlockTask>& t, int slot, long long n) {
-    const int nb = row_blocks(n);
+void append_rows(std::vector<BlockTask>& t, int slot, long long n, int nsl) {
+    const int nb = row_blocks(n, nsl);
     for (int b = 0; b < nb; ++b) t.push_back(BlockTask{slot, b, nb, 0});
 }
 
@@ -128,8 +128,20 @@ int pow2ceil(int v) {
 
 }  // namespace
 
-int row_blocks(long long n) {
-    return (int)std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, kRowCap));
+int row_blocks(long long n, int nsl) {
+    long long nb = std::max<long long>(1, std::min<long long>((n + kBlock - 1) / kBlock, kRowCap));
+    if (nsl > 0) {
+        // The ELL passes give each wavefront one 64-node slice at a time, over kXcd segments
+        // of up to ceil(nsl / kXcd) slices: enough wavefronts per segment that every one takes
+        // the same number of slices (one wavefront left with an extra slice runs it alone
+        // after the others have finished).
+        const long long seg = (nsl + kXcd - 1) / kXcd;
+        const long long most = (long long)kMaxCaseBlocks / kXcd * kWaves;  // wavefronts per segment
+        const long long rounds = (seg + most - 1) / most;
+        const long long waves = (seg + rounds - 1) / rounds;
+        nb = std::max<long long>(nb, kXcd * ((waves + kWaves - 1) / kWaves));
+    }
+    return (int)nb;
 }
 
 // lanes per field of the node-block SpMV: the node-graph degree (incl. the node itself)
@@ -491,7 +503,7 @@ void Engine::Impl::load_views(Req* r) {
         k.x_zero = 0;
         k.nodes = N;
         k.nf = dim + 1;
-        k.nblk = row_blocks(n);
+        k.nblk = row_blocks(n, P.ell_nsl);
         k.A = node_mat<double>(P, dim, w.vals);
         k.binv = w.binv;
         k.b = w.b;
@@ -527,7 +539,7 @@ void Engine::Impl::load_views(Req* r) {
             f.x_zero = 1;
             f.nodes = N;
             f.nf = dim + 1;
-            f.nblk = row_blocks(n);
+            f.nblk = row_blocks(n, P.ell_nsl);
             f.A = node_mat<float>(P, dim, w.vals32);
             f.binv = w.binv32;
             f.b = u;
@@ -928,16 +940,16 @@ void Engine::tick(std::vector<Req*>& done) {
         if (op & OP_ASM) {
             append(tab[c->dim == 2 ? T_ASM2 : T_ASM3], sl, (long long)c->N * 16, kElemCap);
             append(tab[T_BC], sl, c->N, kElemCap);
-            append_rows(tab[T_PRE], sl, c->n);
+            append_rows(tab[T_PRE], sl, c->n, c->pat.ell_nsl);
         }
         if (op & OP_RES) {
-            if (c->d.residual == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n);
-            else append_rows(tab[T_RESU], sl, c->n);
+            if (c->d.residual == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n, c->pat.ell_nsl);
+            else append_rows(tab[T_RESU], sl, c->n, c->pat.ell_nsl);
         }
-        if (op & OP_IRU) append_rows(tab[T_IRU], sl, c->n);
-        if (op & OP_IRR) append_rows(tab[T_IRR + li], sl, c->n);
-        if (op & OP_K64_RUN) append_rows(tab[T_K64 + li], sl, c->n);
-        if (op & OP_K32_RUN) append_rows(tab[T_K32 + li], sl, c->n);
+        if (op & OP_IRU) append_rows(tab[T_IRU], sl, c->n, c->pat.ell_nsl);
+        if (op & OP_IRR) append_rows(tab[T_IRR + li], sl, c->n, c->pat.ell_nsl);
+        if (op & OP_K64_RUN) append_rows(tab[T_K64 + li], sl, c->n, c->pat.ell_nsl);
+        if (op & OP_K32_RUN) append_rows(tab[T_K32 + li], sl, c->n, c->pat.ell_nsl);
         if (op & OP_K64_INIT) tab[T_F64 + li].push_back(BlockTask{sl, 0, 1, 0});
         if (op & OP_K32_INIT) tab[T_F32 + li].push_back(BlockTask{sl, 0, 1, 0});
         const AmgHierarchy& H = c->amg;

@@ -31,8 +31,6 @@
 
 namespace pfem {
 
-constexpr int kXcd = 8;
-
 // The node range and the stride over it owned by this workgroup.
 struct Seg {
     int n0;   // first node of the range

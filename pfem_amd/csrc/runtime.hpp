@@ -173,7 +173,9 @@ private:
 };
 
 int64_t now_ns();
-int row_blocks(long long n);
+// workgroups per case of the row launches (Krylov, residual, refinement); nsl = the
+// case's ELL slices (0: none)
+int row_blocks(long long n, int nsl = 0);
 int pick_lanes(double avg_deg);
 
 }  // namespace pfem
