@@ -207,7 +207,11 @@ __device__ __forceinline__ void ap_body(const AmgView<T>& V, const BlockTask& ta
             for (int s = 0; s < kMaxNf; ++s)
 #pragma unroll
                 for (int t = 0; t < kMaxNf; ++t)
-                    if (s < nf && t < nf) V.ap_out[((b + m) * nf * nf + s * nf + t) * kEllSlice + r] = (T)blk[s][t];
+                    if (s < nf && t < nf) {
+                        const long long q = ((b + m) * nf * nf + s * nf + t) * kEllSlice + r;
+                        if (V.ap_out32) V.ap_out32[q] = (float)blk[s][t];
+                        else V.ap_out[q] = (T)blk[s][t];
+                    }
         }
     }
 }
@@ -314,8 +318,9 @@ __device__ __forceinline__ void res_body(const AmgView<T>& V, const BlockTask& t
     }
 }
 
-// Level 0 of an FP64 hierarchy with the system's ELL copy: one lane per node, whole
-// 64-node slices per wavefront (a workgroup's waves stride over the case's slices).
+// Level 0 with the system's ELL copy: one lane per node, whole 64-node slices per
+// wavefront (a workgroup's waves stride over the case's slices); an FP64 hierarchy reads
+// the FP32 copy of the values when the view has one (E0f).
 template <int NF, typename T>
 __device__ __forceinline__ void res_ell(const AmgView<T>& V, const BlockTask& task, int which) {
     const EllMatT<T>& E = V.E0;
@@ -324,7 +329,8 @@ __device__ __forceinline__ void res_ell(const AmgView<T>& V, const BlockTask& ta
     for (long long c = (long long)task.block * kWaves + (threadIdx.x >> 6); c < E.nsl;
          c += (long long)task.nblocks * kWaves) {
         T y[kMaxNf];
-        ell_rows<NF>(E, (const T*)V.w, c, lane, y);
+        if (V.E0f.vals) ell_rows<NF>(V.E0f, (const T*)V.w, c, lane, y);
+        else ell_rows<NF>(E, (const T*)V.w, c, lane, y);
         const long long i = c * kEllSlice + lane;
         if (i < N) {
 #pragma unroll
@@ -398,7 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_amg_coarsest(const AmgView<T>* __res
 }
 
 // z = w' + omega B^-1 (t - alpha (A P) z_{l+1}),  w' = w + alpha P z_{l+1} (level 0: full
-// rows only): one lane per node, whole 64-node slices per wavefront.
+// rows only): one lane per node, whole 64-node slices per wavefront; A P in FP32 when the
+// view holds it so (APf).
 template <int NF, typename T>
 __device__ __forceinline__ void post_ell(const AmgView<T>& V, const BlockTask& task, int which) {
     const int N = V.A.N, D = NF - 1, Nc = V.next_N;
@@ -407,7 +414,8 @@ __device__ __forceinline__ void post_ell(const AmgView<T>& V, const BlockTask& t
     for (long long c = (long long)task.block * kWaves + (threadIdx.x >> 6); c < V.AP.nsl;
          c += (long long)task.nblocks * kWaves) {
         T y[kMaxNf];
-        ell_rows<NF>(V.AP, V.next_z, c, lane, y);
+        if (V.APf.vals) ell_rows<NF>(V.APf, V.next_z, c, lane, y);
+        else ell_rows<NF>(V.AP, V.next_z, c, lane, y);
         const long long i = c * kEllSlice + lane;
         if (i >= N) continue;
         T d[NF];

@@ -131,6 +131,7 @@ template <typename T>
 struct AmgView {
     NodeMat<T> A;          // this level's matrix (level 0: the system or its FP32 copy)
     EllMatT<T> E0;         // level 0: the system's ELL copy in type T (vals null: none)
+    EllMatT<float> E0f;    // level 0 of an FP64 hierarchy: FP32 values read by the residual (vals null: none)
     const T* binv;         // block-Jacobi inverse, SoA [(nf*nf)][N]
     int level;
     int lanes;             // SpMV lanes per node on this level
@@ -164,6 +165,8 @@ struct AmgView {
     const int* ap_gptr;    // [AP entries + 1] gather list of each AP entry
     const int* ap_gsrc;    // rank k in i's neighbour list of each gathered block
     T* ap_out;             // AP values (written at set-up, padding ranks included)
+    EllMatT<float> APf;    // level 0 of an FP64 hierarchy: AP held in FP32 instead (vals null: none)
+    float* ap_out32;       // its values (written at set-up in place of ap_out)
     // setup of this level (>= 1) from the finer level
     NodeMat<T> F;
     const int* gal_ptr;
