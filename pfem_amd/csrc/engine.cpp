@@ -997,7 +997,7 @@ void Engine::tick(std::vector<Req*>& done) {
                     const int Nl = l == 0 ? c->N : H.lev[l].N;
                     const int lanes = l == 0 ? c->lanes : H.lev[l].lanes;
                     append(A[A_VN + l], base + l, Nl, kElemCap);
-                    append(A[A_VS + l], base + l, (long long)Nl * lanes, kElemCap);
+                    if (l > 0) append(A[A_VS + l], base + l, (long long)Nl * lanes, kElemCap);
                     append(A[A_VR + l], base + l, (long long)H.lev[l + 1].N * kRestrictLanes, kElemCap);
                 }
                 A[A_VC].push_back(BlockTask{base + H.levels - 1, 0, 1, 0});
@@ -1094,17 +1094,21 @@ void Engine::tick(std::vector<Req*>& done) {
                 prof_before = init ? I.h_reset[r->slot].k[prof_w].iters : I.h_scal[r->slot].k[prof_w].iters;
             }
     }
-    // multigrid V-cycle of the cases that have a hierarchy: which = 0 (ph = M^-1 p) or 1 (sh = M^-1 s)
+    // multigrid V-cycle of the cases that have a hierarchy: which = 0 (ph = M^-1 p) or 1 (sh = M^-1 s).
+    // Level 0's residual and every level's post-smoothing run one lane per node over ELL
+    // slices (table A_VN: N_l / kBlock workgroups per case); the residual of the coarser
+    // levels runs `lanes` lanes per node (A_VS).
     auto vcycle_type = [&](auto* v, int TA, int which) {
         for (int l = 0; l < kAmgMaxLevels; ++l) {
             if (!NT(TA + A_VN + l)) break;
+            const int tr = TA + (l == 0 ? A_VN : A_VS) + l;
             launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
-            launch_amg_res(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
+            launch_amg_res(v, T(tr), NT(tr), which, s);
             launch_amg_restrict(v, T(TA + A_VR + l), NT(TA + A_VR + l), s);
         }
         if (NT(TA + A_VC)) launch_amg_coarsest(v, T(TA + A_VC), NT(TA + A_VC), s);
         for (int l = kAmgMaxLevels - 1; l >= 0; --l)
-            if (NT(TA + A_VS + l)) launch_amg_post(v, T(TA + A_VS + l), NT(TA + A_VS + l), which, s);
+            if (NT(TA + A_VN + l)) launch_amg_post(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
     };
     auto vcycle = [&](int which) {
         vcycle_type(vamg64, T_AMG, which);
