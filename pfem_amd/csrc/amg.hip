@@ -5,7 +5,8 @@
 // transfer.
 //
 // V-cycle on level l (input r_l, output z_l; level 0 reads p or s and writes ph or sh):
-//   w = omega B^-1 r                      (k_amg_pre, block-Jacobi pre-smoothing)
+//   w = omega B^-1 r                      (k_amg_pre at level 0, block-Jacobi pre-smoothing;
+//                                          on coarser levels done by the restriction above)
 //   t = r - A w                           (k_amg_res)
 //   r_{l+1} = P^T t                       (k_amg_restrict, gather over aggregate members)
 //   z_{l+1} = V-cycle(l+1)  /  A_c^-1 r_c (k_amg_coarsest, dense inverse on the coarsest)
@@ -360,7 +361,9 @@ __global__ __launch_bounds__(kBlock) void k_amg_res(const AmgView<T>* __restrict
 }
 
 // r_{l+1}(I, s) = sum over the members i of aggregate I of t(i, s) (level 0: full rows only);
-// kRestrictLanes lanes per aggregate share its members (aggregates hold 5-30 nodes).
+// kRestrictLanes lanes per aggregate share its members (aggregates hold 5-30 nodes).  When
+// level l+1 is not the coarsest, its pre-smoothing w_{l+1}(I) = omega B^-1(I) r_{l+1}(I) is
+// local to I and is applied here as well (level l+1 has no k_amg_pre launch).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __restrict__ views,
                                                          const BlockTask* __restrict__ tasks) {
@@ -369,6 +372,9 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __res
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
     if (*V.done_flag) return;
+    const AmgView<T>& Vn = views[task.case_id + 1];  // level l+1 (views of a case are consecutive)
+    const bool smooth = Vn.dense == nullptr;
+    const T om = (T)Vn.omega;
     const int nf = V.A.nf, D = nf - 1, N = V.A.N, Nc = V.next_N;
     const int lane = threadIdx.x % L;
     for (long long I = (long long)task.block * NPB + threadIdx.x / L; I < Nc; I += (long long)task.nblocks * NPB) {
@@ -383,7 +389,16 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __res
 #pragma unroll
         for (int s = 0; s < kMaxNf; ++s)
             if (s < nf) acc[s] = group_sum<L>(acc[s]);
-        if (lane < nf) V.next_r[I + (long long)lane * Nc] = (T)pick(acc, lane);
+        if (lane < nf) {
+            V.next_r[I + (long long)lane * Nc] = (T)pick(acc, lane);
+            if (smooth) {
+                T w = 0;
+#pragma unroll
+                for (int t = 0; t < kMaxNf; ++t)
+                    if (t < nf) w += Vn.binv[(long long)(lane * nf + t) * Nc + I] * (T)acc[t];
+                Vn.w[I + (long long)lane * Nc] = om * w;
+            }
+        }
     }
 }
 

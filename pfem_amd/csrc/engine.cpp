@@ -1102,7 +1102,8 @@ void Engine::tick(std::vector<Req*>& done) {
         for (int l = 0; l < kAmgMaxLevels; ++l) {
             if (!NT(TA + A_VN + l)) break;
             const int tr = TA + (l == 0 ? A_VN : A_VS) + l;
-            launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
+            // the pre-smoothing of levels >= 1 is part of the restriction from the finer level
+            if (l == 0) launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
             launch_amg_res(v, T(tr), NT(tr), which, s);
             launch_amg_restrict(v, T(TA + A_VR + l), NT(TA + A_VR + l), s);
         }
