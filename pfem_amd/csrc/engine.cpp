@@ -210,7 +210,7 @@ WsLayout ws_layout(const pfem_case* c) {
         L.binv32 = take(4 * nf * nf * N);
     }
     if (ir) L.kv32 = take(4 * 10 * n);
-    if (amg_fp32(c) || amg_l0_fp32(c)) L.vals32_ell = take(4 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
+    if (ir || amg_fp32(c) || amg_l0_fp32(c)) L.vals32_ell = take(4 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
     const AmgHierarchy& H = c->amg;
     if (H.levels > 1) {
         const size_t es = amg_fp32(c) ? 4 : 8;
@@ -257,7 +257,7 @@ void carve_workspace(pfem_case* c, char* base) {
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
     w.binv32 = v32 ? reinterpret_cast<float*>(base + L.binv32) : nullptr;
     w.kv32 = ir ? reinterpret_cast<float*>(base + L.kv32) : nullptr;
-    w.vals32_ell = (amg_fp32(c) || amg_l0_fp32(c)) ? reinterpret_cast<float*>(base + L.vals32_ell) : nullptr;
+    w.vals32_ell = (ir || amg_fp32(c) || amg_l0_fp32(c)) ? reinterpret_cast<float*>(base + L.vals32_ell) : nullptr;
     const AmgHierarchy& H = c->amg;
     w.amg_w0 = w.amg_t0 = w.amg_dense = nullptr;
     for (auto& lv : w.amg_lv)
@@ -552,6 +552,7 @@ void Engine::Impl::load_views(Req* r) {
             f.nf = dim + 1;
             f.nblk = row_blocks(n, P.ell_nsl);
             f.A = node_mat<float>(P, dim, w.vals32);
+            f.E = ell_mat<float>(P, dim, w.vals32_ell);  // the inner solve's SpMVs run over the FP32 ELL copy
             f.binv = w.binv32;
             f.b = u;
             f.x = u + n;

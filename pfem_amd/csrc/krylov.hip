@@ -259,17 +259,17 @@ __device__ __forceinline__ void kry_v_nodes(const KrylovView<T>& V, const NodeMa
     }
 }
 
-// The ELL form of an SpMV pass over a case (FP64): each wavefront of the workgroups on this
-// XCD takes whole 64-node slices of the XCD's slice range; per node, emit(i, y) sees the
-// node's NF row values.
-template <int NF, typename F>
-__device__ __forceinline__ void ell_pass(const EllMat& E, const double* __restrict__ x, const BlockTask& task,
+// The ELL form of an SpMV pass over a case: each wavefront of the workgroups on this XCD
+// takes whole 64-node slices of the XCD's slice range; per node, emit(i, y) sees the node's
+// NF row values.
+template <int NF, typename T, typename F>
+__device__ __forceinline__ void ell_pass(const EllMatT<T>& E, const T* __restrict__ x, const BlockTask& task,
                                          F&& emit) {
     const Seg sg = xcd_segment(task, E.nsl);
     const int lane = threadIdx.x & (kEllSlice - 1);
     for (long long l = (long long)sg.lb * kWaves + (threadIdx.x >> 6); l < sg.ns; l += (long long)sg.cnt * kWaves) {
         const long long c = sg.n0 + l;
-        double y[kMaxNf];
+        T y[kMaxNf];
         ell_rows<NF>(E, x, c, lane, y);
         const long long i = c * kEllSlice + lane;
         if (i < E.N) emit(i, y);
@@ -285,21 +285,19 @@ __global__ __launch_bounds__(kBlock) void k_kry_v(const KrylovView<T>* __restric
     if (V.sc->done) return;
     double part[1] = {0.0};
     bool done = false;
-    if constexpr (std::is_same<T, double>::value) {
-        if (V.E.vals) {
-            auto emit = [&](long long i, const double (&y)[kMaxNf]) {
+    if (V.E.vals) {
+        auto emit = [&](long long i, const T (&y)[kMaxNf]) {
 #pragma unroll
-                for (int s = 0; s < kMaxNf; ++s)
-                    if (s < V.nf) {
-                        const long long row = i + (long long)s * V.nodes;
-                        V.v[row] = y[s];
-                        part[0] += V.rh[row] * y[s];
-                    }
-            };
-            if (V.nf == 4) ell_pass<4>(V.E, V.ph, task, emit);
-            else ell_pass<3>(V.E, V.ph, task, emit);
-            done = true;
-        }
+            for (int s = 0; s < kMaxNf; ++s)
+                if (s < V.nf) {
+                    const long long row = i + (long long)s * V.nodes;
+                    V.v[row] = y[s];
+                    part[0] += (double)V.rh[row] * (double)y[s];
+                }
+        };
+        if (V.nf == 4) ell_pass<4>(V.E, (const T*)V.ph, task, emit);
+        else ell_pass<3>(V.E, (const T*)V.ph, task, emit);
+        done = true;
     }
     if (!done) {
         const Seg sg = xcd_segment(task, V.nodes);
@@ -386,22 +384,20 @@ __global__ __launch_bounds__(kBlock) void k_kry_t(const KrylovView<T>* __restric
     if (ss[0] <= V.sc->tol2) return;  // ||s|| converged: K_x finishes the iteration
     double part[2] = {0.0, 0.0};
     bool done = false;
-    if constexpr (std::is_same<T, double>::value) {
-        if (V.E.vals) {
-            auto emit = [&](long long i, const double (&y)[kMaxNf]) {
+    if (V.E.vals) {
+        auto emit = [&](long long i, const T (&y)[kMaxNf]) {
 #pragma unroll
-                for (int s = 0; s < kMaxNf; ++s)
-                    if (s < V.nf) {
-                        const long long row = i + (long long)s * V.nodes;
-                        V.t[row] = y[s];
-                        part[0] += y[s] * V.s[row];
-                        part[1] += y[s] * y[s];
-                    }
-            };
-            if (V.nf == 4) ell_pass<4>(V.E, V.sh, task, emit);
-            else ell_pass<3>(V.E, V.sh, task, emit);
-            done = true;
-        }
+            for (int s = 0; s < kMaxNf; ++s)
+                if (s < V.nf) {
+                    const long long row = i + (long long)s * V.nodes;
+                    V.t[row] = y[s];
+                    part[0] += (double)y[s] * (double)V.s[row];
+                    part[1] += (double)y[s] * (double)y[s];
+                }
+        };
+        if (V.nf == 4) ell_pass<4>(V.E, (const T*)V.sh, task, emit);
+        else ell_pass<3>(V.E, (const T*)V.sh, task, emit);
+        done = true;
     }
     if (!done) {
         const Seg sg = xcd_segment(task, V.nodes);

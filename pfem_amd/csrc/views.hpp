@@ -104,7 +104,7 @@ struct KrylovView {
     int nblk;       // workgroups of this case in the Krylov launches (partials producers)
     int amg;        // 1: ph and sh come from the multigrid V-cycle (K_p / K_s leave them)
     NodeMat<T> A;
-    EllMat E;       // FP64 solves: the ELL copy of A used by K_v / K_t (vals null: node-block A)
+    EllMatT<T> E;   // the ELL copy of A in type T used by K_v / K_t (vals null: node-block A)
     const T* binv;  // [nf*nf][nodes] inverse of each node's diagonal (nf x nf) block, SoA
     const T* b;
     T* x;
