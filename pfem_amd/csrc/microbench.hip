@@ -5,12 +5,14 @@
 
 namespace pfem {
 
-// Sum of n doubles (one partial per workgroup): the HBM read ceiling for the matrix values.
+// Sum of n doubles (one partial per workgroup): the HBM read ceiling for the matrix values,
+// read with the non-temporal loads the SpMVs use for them (its known byte count also
+// calibrates the FETCH_SIZE counter for such loads).
 __global__ __launch_bounds__(kBlock) void k_stream_read(const double* __restrict__ a, long long n,
                                                         double* __restrict__ out) {
     double acc = 0.0;
     for (long long i = (long long)blockIdx.x * kBlock + threadIdx.x; i < n; i += (long long)gridDim.x * kBlock)
-        acc += a[i];
+        acc += ld_stream(a + i);
     double v[1] = {acc};
     block_sum<1>(v);
     if (threadIdx.x == 0) out[blockIdx.x] = v[0];
