@@ -144,20 +144,22 @@ def test_solve_relres_on_oracle_system(ctx, name, precision, precond):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-6
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_amg_cuts_iterations(ctx, dim):
+def test_amg_cuts_iterations(ctx, dim, precision):
     """On a mesh large enough for a multi-level hierarchy, the multigrid-preconditioned solve
     reaches the same true relative residual on the oracle's system in a fraction of the
-    block-Jacobi iterations."""
+    block-Jacobi iterations (mixed: FP32 level-0 values under the FP64 2D hierarchy, FP32 3D
+    hierarchy)."""
     mesh = mg.column_2d_nodes(20000, h=0.01, seed=2) if dim == 2 else mg.column_3d_nodes(30000, h=0.01, seed=2)
-    say(f"[amg] dim={dim} N={mesh.n_nodes}")
+    say(f"[amg] dim={dim} {precision} N={mesh.n_nodes}")
     q = mg.pspg_hydrostatic(mesh)
     dt = 1e-3 if dim == 2 else 5e-4
     ref = O.pspg_assemble(mesh, q, q, dt=dt)
     A = ref.to_scipy()
     its = {}
     for precond in ("jacobi", "amg"):
-        c = make_case(ctx, mesh, krylov_rtol=1e-8, precond=precond)
+        c = make_case(ctx, mesh, precision=precision, krylov_rtol=1e-8, precond=precond)
         c.assemble(dt, q_prev=q, q_iter=q)
         rc, x, st = c.solve(x0=np.zeros_like(q))
         relres = np.linalg.norm(ref.b - A @ x) / np.linalg.norm(ref.b)
