@@ -607,6 +607,7 @@ void Engine::Impl::load_views(Req* r) {
             V.f_cont = acc + N;
             V.m_mom = acc + 2 * (size_t)N;
             V.f_mom = acc + 3 * (size_t)N;
+            V.rec = c->wrec.as<double>();
             V.dt2 = &ds->dt2;
             V.mu = c->d.mu;
             V.K0 = c->d.K0;

@@ -198,6 +198,7 @@ struct pfem_case {
     pfem::DevBuf pos, dir_vals, q_prev;
     pfem::DevBuf partials, counters;
     pfem::DevBuf wstate, wacc;
+    pfem::DevBuf wrec;  // WC: [N][8] node records read by the element gathers (WcView::rec)
     pfem::Workspace ws;
     pfem::AmgHierarchy amg;  // levels == 0: block-Jacobi preconditioning
     int lanes = 8;

@@ -6,7 +6,10 @@
 // arithmetic and the lumped-mass / force sums are FP64.  The element passes are node
 // gathers: a group of kWcLanes lanes per node walks the node's incident elements (the
 // node-element incidence, sorted by element), each lane evaluating its elements' share for
-// that node, and a DPP group sum adds the shares.  Each element is evaluated once per node
+// that node, and a DPP group sum adds the shares.  The gathers read each node's coordinates,
+// pressure, velocity and density from one 64-byte record (WcView::rec, written by the half
+// step and the continuity node pass) instead of from d+3 separate field arrays, so an
+// element visit touches d+1 cache lines rather than (d+1)(2d+2).  Each element is evaluated once per node
 // (d+1 times in all) instead of scattering its d+1 shares with FP64 atomics: device-scope
 // FP64 atomics on gfx950 ran at ~36 G/s, which held the 3D step at 1.6 % of the HBM
 // roofline; the gather needs none, and every sum has a fixed order.  The P1 element
@@ -29,13 +32,9 @@ struct WGeom {
     double vol, detJ;
 };
 
+// Element geometry from the coordinates x[k][a] of its nodes.
 template <int D>
-__device__ __forceinline__ void wc_geometry(const double* __restrict__ pos, int N, const int (&nd)[D + 1],
-                                            double (&x)[D][D + 1], WGeom<D>& G) {
-#pragma unroll
-    for (int k = 0; k < D; ++k)
-#pragma unroll
-        for (int a = 0; a <= D; ++a) x[k][a] = pos[(long long)k * N + nd[a]];
+__device__ __forceinline__ void wc_geom(const double (&x)[D][D + 1], WGeom<D>& G) {
     double J[D][D];
 #pragma unroll
     for (int r = 0; r < D; ++r)
@@ -77,6 +76,32 @@ __device__ __forceinline__ void wc_geometry(const double* __restrict__ pos, int 
     }
 }
 
+template <int D>
+__device__ __forceinline__ void wc_geometry(const double* __restrict__ pos, int N, const int (&nd)[D + 1],
+                                            double (&x)[D][D + 1], WGeom<D>& G) {
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int a = 0; a <= D; ++a) x[k][a] = pos[(long long)k * N + nd[a]];
+    wc_geom<D>(x, G);
+}
+
+// Node j's record {x, y, z, p, vx, vy, vz, rho}: two 32-byte loads from one 64-byte line.
+template <int D>
+__device__ __forceinline__ void rec_load(const double* __restrict__ rec, int j, double (&x)[D], double& p,
+                                         double (&v)[D], double& rho) {
+    const double4* R = reinterpret_cast<const double4*>(rec) + 2 * (long long)j;
+    const double4 r0 = R[0], r1 = R[1];
+    x[0] = r0.x;
+    x[1] = r0.y;
+    if constexpr (D == 3) x[D - 1] = r0.z;
+    p = r0.w;
+    v[0] = r1.x;
+    v[1] = r1.y;
+    if constexpr (D == 3) v[D - 1] = r1.z;
+    rho = r1.w;
+}
+
 // v1/2 = v + dt/2 a (all nodes); x += dt v1/2 (non-fixed).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_wc_half(const WcView<S>* __restrict__ views,
@@ -88,12 +113,25 @@ __global__ __launch_bounds__(kBlock) void k_wc_half(const WcView<S>* __restrict_
     const double dt = V.dt;
     for (int i = task.block * kBlock + threadIdx.x; i < N; i += task.nblocks * kBlock) {
         const bool fixed = V.flags[i] & NODE_FIXED;
-        for (int k = 0; k < V.dim; ++k) {
+        double xr[3] = {0.0, 0.0, 0.0}, vr[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            if (k >= V.dim) continue;
             const long long q = (long long)k * N + i;
             const double vh = (double)V.v[q] + 0.5 * dt * (double)V.a[q];
-            V.v[q] = (S)vh;
-            if (!fixed) V.pos[q] += vh * dt;
+            const S vs = (S)vh;
+            V.v[q] = vs;
+            double x = V.pos[q];
+            if (!fixed) {
+                x += vh * dt;
+                V.pos[q] = x;
+            }
+            xr[k] = x;
+            vr[k] = (double)vs;
         }
+        double4* R = reinterpret_cast<double4*>(V.rec) + 2 * (long long)i;
+        R[0] = make_double4(xr[0], xr[1], xr[2], (double)V.p[i]);
+        R[1] = make_double4(vr[0], vr[1], vr[2], (double)V.rho[i]);
     }
 }
 
@@ -132,16 +170,25 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
             nd[1] = en.y;
             nd[2] = en.z;
             if constexpr (NE > 3) nd[NE - 1] = en.w;
-            double x[D][NE];
-            WGeom<D> G;
-            wc_geometry<D>(V.pos, N, nd, x, G);
-            double P[NE], psum = 0, divv = 0;
+            double x[D][NE], vel[D][NE], P[NE];
 #pragma unroll
             for (int a = 0; a < NE; ++a) {
-                P[a] = (double)V.p[nd[a]];
+                double xa[D], va[D], ra;
+                rec_load<D>(V.rec, nd[a], xa, P[a], va, ra);
+#pragma unroll
+                for (int k = 0; k < D; ++k) {
+                    x[k][a] = xa[k];
+                    vel[k][a] = va[k];
+                }
+            }
+            WGeom<D> G;
+            wc_geom<D>(x, G);
+            double psum = 0, divv = 0;
+#pragma unroll
+            for (int a = 0; a < NE; ++a) {
                 psum += P[a];
 #pragma unroll
-                for (int k = 0; k < D; ++k) divv += G.gN[k][a] * (double)V.v[(long long)k * N + nd[a]];
+                for (int k = 0; k < D; ++k) divv += G.gN[k][a] * vel[k][a];
             }
             const double vol = G.vol;
             const double Pm = at_local<NE>(P, me);
@@ -173,8 +220,11 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_node(const WcView<S>* __rest
         double p;
         if (V.flags[i] & NODE_FREE) p = 0.0;
         else p = (1.0 / V.m_cont[i]) * V.f_cont[i];
-        V.p[i] = (S)p;
-        V.rho[i] = (S)(pow((V.K0p / V.K0) * p + 1.0, 1.0 / V.K0p) * V.rho_star);
+        const S ps = (S)p, rs = (S)(pow((V.K0p / V.K0) * p + 1.0, 1.0 / V.K0p) * V.rho_star);
+        V.p[i] = ps;
+        V.rho[i] = rs;
+        V.rec[8 * (long long)i + 3] = (double)ps;  // the momentum gather reads the new p and rho
+        V.rec[8 * (long long)i + 7] = (double)rs;
     }
 }
 
@@ -203,17 +253,24 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_gather(const WcView<S>* __res
             nd[1] = en.y;
             nd[2] = en.z;
             if constexpr (NE > 3) nd[NE - 1] = en.w;
-            double x[D][NE];
-            WGeom<D> G;
-            wc_geometry<D>(V.pos, N, nd, x, G);
-            double vel[D][NE], R[NE], rsum = 0, psum = 0;
+            double x[D][NE], vel[D][NE], R[NE], P[NE];
 #pragma unroll
             for (int a = 0; a < NE; ++a) {
+                double xa[D], va[D];
+                rec_load<D>(V.rec, nd[a], xa, P[a], va, R[a]);
 #pragma unroll
-                for (int k = 0; k < D; ++k) vel[k][a] = (double)V.v[(long long)k * N + nd[a]];
-                R[a] = (double)V.rho[nd[a]];
+                for (int k = 0; k < D; ++k) {
+                    x[k][a] = xa[k];
+                    vel[k][a] = va[k];
+                }
+            }
+            WGeom<D> G;
+            wc_geom<D>(x, G);
+            double rsum = 0, psum = 0;
+#pragma unroll
+            for (int a = 0; a < NE; ++a) {
                 rsum += R[a];
-                psum += (double)V.p[nd[a]];
+                psum += P[a];
             }
             // velocity gradient L[k][m] = sum_a dN_a/dx_m v_k(a)
             double Lg[D][D];
