@@ -12,7 +12,7 @@ step with Ax_f minRes 1e-6, maxIter 10 and linear solves to a true relative resi
   per-case step;
 * batched mixed: the continuous-batching server (requests join the running batch at every
   engine tick, admitted by step-workspace footprint), PFEM_PREC_MIXED (FP64 BiCGSTAB; the
-  multigrid of 3D cases in FP32, from 20,000 nodes), FP32 WC state with FP64 sums.
+  multigrid of 3D cases in FP32, from 5,000 nodes), FP32 WC state with FP64 sums.
 
 Each path runs the stream twice: saturated (every request queued at t0: throughput, the
 headline steps/s) and with seeded Poisson arrivals at 200 requests/s (p50/p99 latency from
@@ -635,7 +635,7 @@ def main():
                          f"saturated for throughput"),
             "picard": "Ax_f, minRes 1e-6, maxIter 10",
             "linear_solve": "BiCGSTAB to true relres <= 1e-8 (and ||r|| <= 0.1 minRes), right-preconditioned by "
-                            "aggregation multigrid (2D >= 5k nodes; 3D >= 50k FP64 / >= 20k mixed) or block Jacobi",
+                            "aggregation multigrid (2D >= 5k nodes; 3D >= 50k FP64 / >= 5k mixed) or block Jacobi",
             "batched": f"continuous batching ({args.slots} slots), PFEM_PREC_MIXED: FP64 BiCGSTAB, FP32 3D "
                        f"multigrid, FP32 WC state",
             "plain": "one request at a time through the direct calls, FP64 throughout",

@@ -111,7 +111,8 @@ size_t persistent_bytes(pfem_case* c) {
 // 262k-node dam break steps 14.5 times per second with it against 11.7 without, and the
 // one-at-a-time serving stream is fastest with 50,000 among 5k / 20k / 50k / 100k / 400k).
 // PFEM_PREC_MIXED 3D cases run the hierarchy in FP32, which halves the V-cycle's bytes:
-// they take it from 20,000 nodes.
+// they take it from 5,000 nodes (the batched stream: 217.6 / 219.2 / 220.4 steps/s from
+// 20k / 10k / 5k).
 // PFEM_AMG_MIN_NODES_2D / _3D / _3D_MIXED override the thresholds.
 bool wants_amg(const pfem_case* c) {
     if (c->d.precond == PFEM_PRECOND_JACOBI) return false;
@@ -121,7 +122,7 @@ bool wants_amg(const pfem_case* c) {
         return e ? std::atoi(e) : dflt;
     };
     static const int min2 = env("PFEM_AMG_MIN_NODES_2D", 5000), min3 = env("PFEM_AMG_MIN_NODES_3D", 50000),
-                     min3m = env("PFEM_AMG_MIN_NODES_3D_MIXED", 20000);
+                     min3m = env("PFEM_AMG_MIN_NODES_3D_MIXED", 5000);
     if (c->dim == 2) return c->N >= min2;
     return c->N >= (c->d.precision == PFEM_PREC_MIXED ? min3m : min3);
 }
