@@ -494,8 +494,8 @@ def run_dambreak3d(ctx, steps, warmup):
         "frac": achieved / HBM_PEAK_GBPS,
         "traffic": pmc_traffic(),
         "traffic_source": "profiles/spmv_kry_v_dambreak3d_262k_pmc_traffic.md (rocprofv3 FETCH_SIZE x 1.9988 + "
-                          "WRITE_SIZE x 1.0, factors from a copy kernel and a non-temporal streaming read of "
-                          "known size)",
+                          "WRITE_SIZE x 1.0 over the replay launches, factors from a copy kernel and a "
+                          "non-temporal streaming read of known size)",
         "avg_launch_ms": spmv_ms,
         "launch_timing": "HIP events around 50 back-to-back launches of the kernel on the case's stream, "
                          "after the timed steps; median of 5 such groups",
