@@ -169,11 +169,11 @@ struct WsLayout {
     size_t amg_lv[kAmgMaxLevels][6];
     size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
 };
-// The multigrid hierarchy runs in FP32 for PFEM_PREC_MIXED 3D cases (under FP64 BiCGSTAB).
-// 2D hierarchies stay FP64: there an FP32 V-cycle raised the BiCGSTAB iteration count by
-// 20-40 %, and FP32 operators with FP64 vectors (iteration counts unchanged) still served
-// the stream 2-3 % slower than FP64, while the 3D counts did not change and the all-FP32 3D
-// hierarchy served it 8 % faster than FP32 operators with FP64 vectors.
+// The multigrid hierarchy runs in FP32 for PFEM_PREC_MIXED 3D cases (under FP64 BiCGSTAB):
+// the 3D iteration counts did not change, and the all-FP32 3D hierarchy served the stream
+// 8 % faster than FP32 operators with FP64 vectors.  2D hierarchies keep FP64 vectors: an
+// FP32 V-cycle raised the 2D BiCGSTAB iteration count by 20-40 % (their level 0 reads FP32
+// values instead, amg_l0_fp32).
 bool amg_fp32(const pfem_case* c) {
     return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 3;
 }
