@@ -1,5 +1,6 @@
 """GPU parity of the weakly compressible CDS_dpdt step and the CFL time step against the
 CPU oracle (FP64 state: 1e-10 normwise per field; FP32 state: 2e-5)."""
+import dataclasses
 import sys
 
 import numpy as np
@@ -60,3 +61,44 @@ def test_wc_step_parity(ctx, name, state):
         # positions are FP64 in both states; with FP32 velocities they drift by dt * |dv|
         np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0,
                                    atol=1e-12 if state == "fp64" else 1e-9)
+
+
+@pytest.mark.parametrize("state", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["2d_tank", "3d_tank"])
+def test_wc_step_parity_dirichlet_without_stabilization(ctx, name, state):
+    """Wall nodes on the x = min side flagged Dirichlet with random values (the reference's
+    'a = v_bc' rule of MomEquation.inl:358-364) and ContEq.stabilization = "None" (lumped
+    continuity mass instead of Meduri's consistent one), against the oracle."""
+    base = meshes()[name]
+    d, N = base.dim, base.n_nodes
+    flags = base.flags.copy()
+    left = ((flags & mg.BOUND) != 0) & (base.pos[0] < base.pos[0].min() + 0.5 * base.h)
+    flags[left] |= mg.DIRICHLET
+    mesh = dataclasses.replace(base, flags=flags)
+    say(f"[wc-dir] {name} {state} N={N} dirichlet={int(left.sum())}")
+    assert left.sum() > 0
+    dv = np.random.default_rng(7).normal(scale=0.05, size=d * N)
+    v, p, rho, a = mg.wc_tait_hydrostatic(mesh, K0=K0, K0p=K0P, rho_star=RS)
+    c = pfem_amd.Case(ctx, d, kind="WC", mu=MU, K0=K0, K0p=K0P, rho_star=RS, wc_state=state,
+                      stabilization="None")
+    c.set_mesh(mesh)
+    c.set_state(np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)]))
+    c.set_dirichlet(dv)
+    pos_o = mesh.pos.reshape(-1).copy()
+    vo, po, ro, ao = v.reshape(-1).copy(), p.copy(), rho.copy(), a.reshape(-1).copy()
+    tol = 1e-10 if state == "fp64" else 2e-5
+    for step in range(3):
+        dt = 2e-5
+        O.wc_step(mesh, pos_o, vo, po, ro, ao, dt, dir_vals=dv, mu=MU, K0=K0, K0p=K0P, rho_star=RS, meduri=False)
+        c.wc_step(dt)
+        s = c.get_state()
+        vg, pg, rg, ag = s[:d * N], s[d * N:(d + 1) * N], s[(d + 1) * N:(d + 2) * N], s[(d + 2) * N:]
+        say(f"  step {step}: dv={rel(vo, vg):.2e} dp={rel(po, pg):.2e} drho={rel(ro, rg):.2e} da={rel(ao, ag):.2e}")
+        assert rel(vo, vg) <= tol
+        assert rel(po, pg) <= tol
+        assert rel(ro, rg) <= tol
+        assert np.abs(ag - ao).max() <= tol * max(np.abs(ao).max(), 9.81)
+        # Dirichlet nodes: a holds the prescribed values (as the reference's solve leaves it)
+        for k in range(d):
+            np.testing.assert_allclose(ag[k * N:(k + 1) * N][left], dv[k * N:(k + 1) * N][left],
+                                       rtol=0, atol=1e-12 if state == "fp64" else 1e-7)
