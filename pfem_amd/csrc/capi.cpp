@@ -316,8 +316,11 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
     return guarded([&] {
         require_case(c);
         PFEM_REQUIRE(N > 0 && E > 0 && conn && flags && pos, PFEM_E_INVALID, "invalid mesh arguments");
-        PFEM_HIP(hipSetDevice(c->ctx->device));
         const int dim = c->dim, ne = dim + 1;
+        // unknowns, and the pair-element incidence (E (d+1)^2 entries), are indexed in int32
+        PFEM_REQUIRE((long long)ne * N <= INT32_MAX && (long long)ne * ne * E <= INT32_MAX, PFEM_E_INVALID,
+                     "mesh too large: (d+1) N and (d+1)^2 E must fit 32-bit indices");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
         c->has_mesh = false;
         c->ws.clear();
         c->N = N;

@@ -241,6 +241,19 @@ def test_invalid_arguments_raise(ctx):
     c = pfem_amd.Case(ctx, 2)
     with pytest.raises(pfem_amd.PfemError):
         c.set_mesh(bad)
+    # empty meshes, and meshes whose unknowns or pair-element incidence overflow 32-bit
+    # indices, are rejected before anything is read from the arrays
+    import ctypes
+
+    lib = pfem_amd.lib()
+    conn = np.zeros(3, np.int32)
+    flags = np.zeros(1, np.uint8)
+    pos = np.zeros(2)
+    ptr = lambda a, t: a.ctypes.data_as(ctypes.POINTER(t))
+    for n_nodes, n_elems in ((0, 1), (1, 0), (2**30, 1), (1, 2**28)):
+        rc = lib.pfem_case_set_mesh(c._h, n_nodes, n_elems, ptr(conn, ctypes.c_int32), ptr(flags, ctypes.c_uint8),
+                                    ptr(pos, ctypes.c_double))
+        assert rc == pfem_amd._lib.PFEM_E_INVALID, (n_nodes, n_elems, rc)
 
 
 def test_assembly_is_deterministic(ctx):
