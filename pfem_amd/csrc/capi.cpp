@@ -92,7 +92,7 @@ size_t wc_state_size(pfem_case* c) { return c->d.wc_state == PFEM_STATE_FP32 ? s
 
 size_t persistent_bytes(pfem_case* c) {
     size_t b = 0;
-    for (DevBuf* p : {&c->pos, &c->dir_vals, &c->q_prev, &c->partials, &c->counters, &c->wstate, &c->wacc, &c->wrec,
+    for (DevBuf* p : {&c->pos, &c->dir_vals, &c->q_prev, &c->partials, &c->counters, &c->wstate, &c->wacc, &c->wrec, &c->werec,
                       &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr, &c->pat.nbr_idx, &c->pat.node_info,
                       &c->pat.row_ptr, &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei, &c->pat.elem_ptr,
                       &c->pat.elem_list, &c->pat.elem_nodes, &c->pat.ell_soff, &c->pat.ell_idx})
@@ -354,6 +354,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             PFEM_HIP(hipMemsetAsync(c->wstate.p, 0, ss * (size_t)(2 * dim + 2) * N, c->stream));
             c->wacc.reserve(sizeof(double) * (size_t)(3 + dim) * N);
             c->wrec.reserve(sizeof(double) * 8 * (size_t)N);
+            c->werec.reserve(sizeof(double) * 4 * (size_t)E);
             PFEM_HIP(hipMemsetAsync(c->wrec.p, 0, sizeof(double) * 8 * (size_t)N, c->stream));
         }
         PFEM_HIP(hipStreamSynchronize(c->stream));

@@ -63,7 +63,8 @@ enum Tab : int {
     T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
     T_RES, T_IRR = T_RES + 3, T_K64 = T_IRR + 3, T_K32 = T_K64 + 3, T_F64 = T_K32 + 3, T_F32 = T_F64 + 3,
     T_WCN = T_F32 + 3, T_WCE2 = T_WCN + 2, T_WCE3 = T_WCE2 + 2, T_CFE2 = T_WCE3 + 2, T_CFE3 = T_CFE2 + 2,
-    T_AMG = T_CFE3 + 2,  // multigrid tables: FP64 hierarchies, then FP32 ones (A_N each)
+    T_WEL2 = T_CFE3 + 2, T_WEL3 = T_WEL2 + 2,  // WC continuity element pass (one lane per element)
+    T_AMG = T_WEL3 + 2,  // multigrid tables: FP64 hierarchies, then FP32 ones (A_N each)
 };
 
 // Multigrid tables of one hierarchy type, relative to T_AMG + type * A_N: setup per level
@@ -608,6 +609,7 @@ void Engine::Impl::load_views(Req* r) {
             V.m_mom = acc + 2 * (size_t)N;
             V.f_mom = acc + 3 * (size_t)N;
             V.rec = c->wrec.as<double>();
+            V.erec = c->werec.as<double>();
             V.dt2 = &ds->dt2;
             V.mu = c->d.mu;
             V.K0 = c->d.K0;
@@ -1010,6 +1012,7 @@ void Engine::tick(std::vector<Req*>& done) {
             append(tab[T_WCN + wi], sl, c->N, kElemCap);
             // node gathers over the incident elements: kWcLanes lanes per node
             append(tab[(c->dim == 2 ? T_WCE2 : T_WCE3) + wi], sl, (long long)c->N * kWcLanes[c->dim - 2], kElemCap);
+            append(tab[(c->dim == 2 ? T_WEL2 : T_WEL3) + wi], sl, c->E, kElemCap);
         }
         if (op & OP_CFL) {
             append(tab[(c->dim == 2 ? T_CFE2 : T_CFE3) + wi], sl, c->E, kElemCap);
@@ -1136,10 +1139,11 @@ void Engine::tick(std::vector<Req*>& done) {
     const WcView<double>* vw64 = I.d_wc64.as<WcView<double>>();
     const WcView<float>* vw32 = I.d_wc32.as<WcView<float>>();
     if (NT(T_WCN + 0))
-        launch_wc_step<double>(vw64, T(T_WCN), NT(T_WCN), T(T_WCE2), NT(T_WCE2), T(T_WCE3), NT(T_WCE3), s);
+        launch_wc_step<double>(vw64, T(T_WCN), NT(T_WCN), T(T_WCE2), NT(T_WCE2), T(T_WCE3), NT(T_WCE3), T(T_WEL2),
+                               NT(T_WEL2), T(T_WEL3), NT(T_WEL3), s);
     if (NT(T_WCN + 1))
         launch_wc_step<float>(vw32, T(T_WCN + 1), NT(T_WCN + 1), T(T_WCE2 + 1), NT(T_WCE2 + 1), T(T_WCE3 + 1),
-                              NT(T_WCE3 + 1), s);
+                              NT(T_WCE3 + 1), T(T_WEL2 + 1), NT(T_WEL2 + 1), T(T_WEL3 + 1), NT(T_WEL3 + 1), s);
     if (NT(T_CFE2) + NT(T_CFE3)) launch_wc_cfl<double>(vw64, T(T_CFE2), NT(T_CFE2), T(T_CFE3), NT(T_CFE3), s);
     if (NT(T_CFE2 + 1) + NT(T_CFE3 + 1))
         launch_wc_cfl<float>(vw32, T(T_CFE2 + 1), NT(T_CFE2 + 1), T(T_CFE3 + 1), NT(T_CFE3 + 1), s);

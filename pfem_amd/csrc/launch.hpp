@@ -50,10 +50,12 @@ void launch_update_pos(const PicardView* v, const BlockTask* t, int nt, hipStrea
 void launch_res_axf(int lanes, const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 void launch_res_u(const PicardView* v, const BlockTask* t, int nt, hipStream_t s);
 
-// node_tasks: one lane per node; gather2 / gather3: kWcLanes lanes per node of 2D / 3D cases
+// node_tasks: one lane per node; gather2 / gather3: kWcLanes lanes per node of 2D / 3D cases;
+// elem2 / elem3: one lane per element of 2D / 3D cases
 template <typename S>
 void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_tasks, const BlockTask* gather2,
-                    int n_gather2, const BlockTask* gather3, int n_gather3, hipStream_t s);
+                    int n_gather2, const BlockTask* gather3, int n_gather3, const BlockTask* elem2, int n_elem2,
+                    const BlockTask* elem3, int n_elem3, hipStream_t s);
 template <typename S>
 void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, const BlockTask* elem3, int n_elem3,
                    hipStream_t s);

@@ -198,7 +198,8 @@ struct pfem_case {
     pfem::DevBuf pos, dir_vals, q_prev;
     pfem::DevBuf partials, counters;
     pfem::DevBuf wstate, wacc;
-    pfem::DevBuf wrec;  // WC: [N][8] node records read by the element gathers (WcView::rec)
+    pfem::DevBuf wrec;   // WC: [N][8] node records read by the element gathers (WcView::rec)
+    pfem::DevBuf werec;  // WC: [E][4] continuity element records (WcView::erec)
     pfem::Workspace ws;
     pfem::AmgHierarchy amg;  // levels == 0: block-Jacobi preconditioning
     int lanes = 8;

@@ -217,6 +217,7 @@ struct WcView {
     double* f_mom;            // [dim][N] momentum force
     double* rec;              // [N][8] node records {x, y, z, p, vx, vy, vz, rho} (FP64; 2D: z = vz = 0),
                               // written by the node passes, read by the element gathers
+    double* erec;             // [E][4] continuity element records {|e|, div v, sum P, 0} (FP64)
     unsigned long long* dt2;  // CFL min accumulator (bits of a positive double)
     double mu, K0, K0p, rho_star;
     double g[3];

@@ -6,7 +6,9 @@
 // arithmetic and the lumped-mass / force sums are FP64.  The element passes are node
 // gathers: a group of kWcLanes lanes per node walks the node's incident elements (the
 // node-element incidence, sorted by element), each lane evaluating its elements' share for
-// that node, and a DPP group sum adds the shares.  The gathers read each node's coordinates,
+// that node, and a DPP group sum adds the shares (the continuity pass first evaluates each
+// element once, k_wc_cont_elem, and its gather reads one 32-byte element record per incident
+// element).  The gathers read each node's coordinates,
 // pressure, velocity and density from one 64-byte record (WcView::rec, written by the half
 // step and the continuity node pass) instead of from d+3 separate field arrays, so an
 // element visit touches d+1 cache lines rather than (d+1)(2d+2).  Each element is evaluated once per node
@@ -145,9 +147,43 @@ __device__ __forceinline__ double at_local(const double (&a)[NE], int me) {
     return r;
 }
 
-// Continuity (m_buildSystemdpdt, ContEquation.inl:353-413): lumped mass lump2(M(1)) and rhs
-// F0 = -dt D(K0 + K0' N.P) V + M(1) P (Meduri) or lump2(M(1)) P of node i, summed over its
-// elements.
+// Continuity (m_buildSystemdpdt, ContEquation.inl:353-413), element part: |e|, div_e(V) and
+// sum P of each element, from its nodes' records, once per element.
+template <int D, typename S>
+__global__ __launch_bounds__(kBlock) void k_wc_cont_elem(const WcView<S>* __restrict__ views,
+                                                         const BlockTask* __restrict__ tasks) {
+    constexpr int NE = D + 1;
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const WcView<S>& V = views[task.case_id];
+    const int E = V.E;
+    for (long long e = (long long)task.block * kBlock + threadIdx.x; e < E; e += (long long)task.nblocks * kBlock) {
+        double x[D][NE], vel[D][NE], P[NE];
+#pragma unroll
+        for (int a = 0; a < NE; ++a) {
+            double xa[D], va[D], ra;
+            rec_load<D>(V.rec, V.conn[(long long)a * E + e], xa, P[a], va, ra);
+#pragma unroll
+            for (int k = 0; k < D; ++k) {
+                x[k][a] = xa[k];
+                vel[k][a] = va[k];
+            }
+        }
+        WGeom<D> G;
+        wc_geom<D>(x, G);
+        double psum = 0, divv = 0;
+#pragma unroll
+        for (int a = 0; a < NE; ++a) {
+            psum += P[a];
+#pragma unroll
+            for (int k = 0; k < D; ++k) divv += G.gN[k][a] * vel[k][a];
+        }
+        reinterpret_cast<double4*>(V.erec)[e] = make_double4(G.vol, divv, psum, 0.0);
+    }
+}
+
+// Continuity, node part: lumped mass lump2(M(1)) and rhs F0 = -dt D(K0 + K0' N.P) V + M(1) P
+// (Meduri) or lump2(M(1)) P of node i, summed over its elements' records.
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __restrict__ views,
                                                            const BlockTask* __restrict__ tasks) {
@@ -159,39 +195,14 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
     const double dt = V.dt;
     const double c2 = 1.0 / (double)((D + 1) * (D + 2));
     const int lane = threadIdx.x % L;
+    const double4* __restrict__ erec = reinterpret_cast<const double4*>(V.erec);
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
         const int q1 = V.elem_ptr[i + 1];
+        const double Pm = (double)V.p[i];
         double ml = 0.0, fc = 0.0;
         for (int q = V.elem_ptr[i] + lane; q < q1; q += L) {
-            const int me = (int)(V.elem_list[q] & 3u);
-            const int4 en = V.elem_nodes[q];
-            int nd[NE];
-            nd[0] = en.x;
-            nd[1] = en.y;
-            nd[2] = en.z;
-            if constexpr (NE > 3) nd[NE - 1] = en.w;
-            double x[D][NE], vel[D][NE], P[NE];
-#pragma unroll
-            for (int a = 0; a < NE; ++a) {
-                double xa[D], va[D], ra;
-                rec_load<D>(V.rec, nd[a], xa, P[a], va, ra);
-#pragma unroll
-                for (int k = 0; k < D; ++k) {
-                    x[k][a] = xa[k];
-                    vel[k][a] = va[k];
-                }
-            }
-            WGeom<D> G;
-            wc_geom<D>(x, G);
-            double psum = 0, divv = 0;
-#pragma unroll
-            for (int a = 0; a < NE; ++a) {
-                psum += P[a];
-#pragma unroll
-                for (int k = 0; k < D; ++k) divv += G.gN[k][a] * vel[k][a];
-            }
-            const double vol = G.vol;
-            const double Pm = at_local<NE>(P, me);
+            const double4 er = erec[V.elem_list[q] >> 2];
+            const double vol = er.x, divv = er.y, psum = er.z;
             const double ci = V.K0 / NE + V.K0p * (Pm + psum) * c2;
             const double mlump = vol / NE;
             const double mp = V.meduri ? vol * (Pm + psum) * c2 : mlump * Pm;
@@ -411,8 +422,11 @@ __global__ __launch_bounds__(kBlock) void k_wc_cfl(const WcView<S>* __restrict__
 // Launchers ---------------------------------------------------------------------------
 template <typename S>
 void launch_wc_step(const WcView<S>* v, const BlockTask* node_tasks, int n_node_tasks, const BlockTask* gather2,
-                    int n_gather2, const BlockTask* gather3, int n_gather3, hipStream_t s) {
+                    int n_gather2, const BlockTask* gather3, int n_gather3, const BlockTask* elem2, int n_elem2,
+                    const BlockTask* elem3, int n_elem3, hipStream_t s) {
     k_wc_half<S><<<n_node_tasks, kBlock, 0, s>>>(v, node_tasks);
+    if (n_elem2) k_wc_cont_elem<2, S><<<n_elem2, kBlock, 0, s>>>(v, elem2);
+    if (n_elem3) k_wc_cont_elem<3, S><<<n_elem3, kBlock, 0, s>>>(v, elem3);
     if (n_gather2) k_wc_cont_gather<2, S><<<n_gather2, kBlock, 0, s>>>(v, gather2);
     if (n_gather3) k_wc_cont_gather<3, S><<<n_gather3, kBlock, 0, s>>>(v, gather3);
     k_wc_cont_node<S><<<n_node_tasks, kBlock, 0, s>>>(v, node_tasks);
@@ -429,9 +443,10 @@ void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, cons
 }
 
 template void launch_wc_step<double>(const WcView<double>*, const BlockTask*, int, const BlockTask*, int,
-                                     const BlockTask*, int, hipStream_t);
+                                     const BlockTask*, int, const BlockTask*, int, const BlockTask*, int,
+                                     hipStream_t);
 template void launch_wc_step<float>(const WcView<float>*, const BlockTask*, int, const BlockTask*, int,
-                                    const BlockTask*, int, hipStream_t);
+                                    const BlockTask*, int, const BlockTask*, int, const BlockTask*, int, hipStream_t);
 template void launch_wc_cfl<double>(const WcView<double>*, const BlockTask*, int, const BlockTask*, int, hipStream_t);
 template void launch_wc_cfl<float>(const WcView<float>*, const BlockTask*, int, const BlockTask*, int, hipStream_t);
 
