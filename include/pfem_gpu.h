@@ -182,6 +182,16 @@ int pfem_wc_step(pfem_case* c, double dt, pfem_step_stats* stats);
 /* computeNextDT (WCompNewton/Solver.cpp:192-234): dt = min(sqrt(min_e sc^2 he^2 / S), max_dt). */
 int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* dt_out);
 
+/* ---- extractors ------------------------------------------------------------------------
+ * The problem's global "mass", as MassExtractor::update (extractors/MassExtractor.cpp:27-38)
+ * reads it: PSPG, the sum of the element sizes (IncompNewton/Problem.cpp:173-190); WC, each
+ * size times the mean density of the element's nodes (WCompNewton/Problem.cpp:216-240). */
+int pfem_case_mass(pfem_case* c, double* mass);
+/* MinMaxExtractor::update (extractors/MinMaxExtractor.cpp:28-62): the smallest (want_max = 0)
+ * or largest coordinate `coordinate` (0 .. dim-1) over the nodes that are neither bound nor
+ * free; the largest starts from 0, as the reference's does. */
+int pfem_case_minmax(pfem_case* c, int32_t coordinate, int32_t want_max, double* value);
+
 /* ---- batched serving (no counterpart in the reference: many independent cases) -------
  * Runs one time step of every listed case as one batch: each operation of the cases'
  * steps (assembly, BC, BiCGSTAB iterations, residuals, WC passes) is one launch over a

@@ -64,6 +64,10 @@ def lib():
         L.orc_wc_next_dt.restype = C.c_double
         L.orc_wc_next_dt.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, C.c_double,
                                      C.c_double, C.c_double, C.c_double, C.c_int]
+        L.orc_mass.restype = C.c_double
+        L.orc_mass.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp]
+        L.orc_minmax.restype = C.c_double
+        L.orc_minmax.argtypes = [C.c_int, C.c_int64, _u8p, _dp, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -232,3 +236,23 @@ def wc_next_dt(mesh, pos, v, p, rho, mu=1e-3, K0=2.2e9, K0p=7.0, security_coeff=
     conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
     return L.orc_wc_next_dt(mesh.dim, mesh.n_nodes, mesh.n_elems, _i(conn), _d(pos), _d(v), _d(p), _d(rho), mu, K0,
                             K0p, security_coeff, max_dt, nthreads)
+
+
+def mass(mesh, pos, rho=None):
+    """Global "mass" (IncompNewton/Problem.cpp:173-190: sum of element sizes; with rho,
+    WCompNewton/Problem.cpp:216-240: sizes weighted by the element's mean node density)."""
+    L = lib()
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    r = None if rho is None else np.ascontiguousarray(rho, dtype=np.float64)
+    return L.orc_mass(mesh.dim, mesh.n_nodes, mesh.n_elems, _i(conn), _d(pos),
+                      _d(r) if r is not None else C.cast(None, _dp))
+
+
+def minmax(mesh, pos, coordinate, kind="max", flags=None):
+    """MinMaxExtractor (MinMaxExtractor.cpp:35-55) over nodes neither bound nor free; the
+    library derives FREE from connectivity, so pass those flags when comparing with it."""
+    L = lib()
+    fl = np.ascontiguousarray(mesh.flags if flags is None else flags, dtype=np.uint8)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    return L.orc_minmax(mesh.dim, mesh.n_nodes, _u8(fl), _d(pos), coordinate, 1 if kind == "max" else 0)

@@ -1345,4 +1345,49 @@ double orc_wc_next_dt(int dim, int64_t N, int64_t E, const int* conn, const doub
                     : wcNextDT<3>(W, pos, v, p, rho, security_coeff, max_dt);
 }
 
+// Global "mass" of a problem: ProbIncompNewton::getGlobalWrittableData
+// (IncompNewton/Problem.cpp:173-190) sums the element sizes Element::getSize = detJ * refSize
+// (Element.cpp:159-162); ProbWCompNewton::getGlobalWrittableData (WCompNewton/Problem.cpp:216-240)
+// weights each size by the mean density of the element's nodes (rho non-null).  Elements in
+// index order, as the reference loops.
+double orc_mass(int dim, int64_t N, int64_t E, const int* conn, const double* pos, const double* rho) {
+    const int ne = dim + 1;
+    double mass = 0;
+    for (int64_t e = 0; e < E; ++e) {
+        const int* nodes = conn + e * ne;
+        Geo g;
+        double size;
+        if (dim == 2) {
+            computeGeo<2>(pos, N, nodes, g);
+            size = g.detJ * Quad<2>::refSize;
+        } else {
+            computeGeo<3>(pos, N, nodes, g);
+            size = g.detJ * Quad<3>::refSize;
+        }
+        if (rho) {
+            double middle = 0;
+            for (int n = 0; n < ne; ++n) middle += rho[nodes[n]];
+            middle /= static_cast<double>(ne);
+            mass += middle * size;
+        } else {
+            mass += size;
+        }
+    }
+    return mass;
+}
+
+// MinMaxExtractor::update (MinMaxExtractor.cpp:35-55): the smallest (want_max = 0) or largest
+// coordinate `coordinate` over the nodes that are neither bound nor free; the largest starts
+// from 0, as the reference's does, the smallest from DBL_MAX.
+double orc_minmax(int dim, int64_t N, const uint8_t* flags, const double* pos, int coordinate, int want_max) {
+    (void)dim;
+    double v = want_max ? 0.0 : std::numeric_limits<double>::max();
+    for (int64_t n = 0; n < N; ++n) {
+        if ((flags[n] & BOUND) || (flags[n] & FREE)) continue;
+        const double x = pos[n + coordinate * N];
+        v = want_max ? std::max(v, x) : std::min(v, x);
+    }
+    return v;
+}
+
 }  // extern "C"

@@ -235,6 +235,20 @@ class Case:
         check(lib().pfem_wc_next_dt(self._h, security_coeff, max_dt, C.byref(out)))
         return out.value
 
+    def mass(self) -> float:
+        """The problem's global "mass" (MassExtractor; Problem::getGlobalWrittableData)."""
+        out = C.c_double(0)
+        check(lib().pfem_case_mass(self._h, C.byref(out)))
+        return out.value
+
+    def minmax(self, coordinate: int, kind: str = "max") -> float:
+        """MinMaxExtractor: smallest or largest coordinate of the nodes neither bound nor free."""
+        if kind not in ("min", "max"):
+            raise PfemError(_lib.PFEM_E_INVALID, "kind must be 'min' or 'max'")
+        out = C.c_double(0)
+        check(lib().pfem_case_minmax(self._h, int(coordinate), 1 if kind == "max" else 0, C.byref(out)))
+        return out.value
+
 
 class MomContEqIncompNewton(Case):
     """PSPG momentum-continuity equation (MomContEquation.inl) on the GPU.

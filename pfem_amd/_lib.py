@@ -36,7 +36,8 @@ EXPORTS = [
     "pfem_case_destroy", "pfem_case_set_mesh", "pfem_case_set_state", "pfem_case_get_state",
     "pfem_case_set_positions", "pfem_case_get_positions", "pfem_case_set_dirichlet", "pfem_case_sync",
     "pfem_pspg_assemble", "pfem_pspg_solve", "pfem_get_csr_sizes", "pfem_get_csr", "pfem_get_rhs", "pfem_spmv",
-    "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_batch_step", "pfem_case_footprint",
+    "pfem_pspg_step", "pfem_wc_step", "pfem_wc_next_dt", "pfem_case_mass", "pfem_case_minmax", "pfem_batch_step",
+    "pfem_case_footprint",
     "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd", "pfem_now_ns", "pfem_case_workspace_bytes",
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
     "pfem_case_reserve_workspace", "pfem_case_release_workspace",
@@ -126,6 +127,8 @@ def lib():
         "pfem_pspg_step": [_vp, C.c_double, C.POINTER(StepStats)],
         "pfem_wc_step": [_vp, C.c_double, C.POINTER(StepStats)],
         "pfem_wc_next_dt": [_vp, C.c_double, C.c_double, _dp],
+        "pfem_case_mass": [_vp, _dp],
+        "pfem_case_minmax": [_vp, C.c_int32, C.c_int32, _dp],
         "pfem_batch_step": [_vp, C.c_int32, C.POINTER(_vp), _dp, _ip, C.POINTER(StepStats)],
         "pfem_case_footprint": [_vp, C.POINTER(C.c_int64)],
         "pfem_profile_spmv": [_vp, C.c_int32, _dp],

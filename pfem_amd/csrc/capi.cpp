@@ -638,6 +638,49 @@ int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* 
     });
 }
 
+int pfem_case_mass(pfem_case* c, double* mass) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(mass != nullptr, PFEM_E_INVALID, "null output");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        const int nb = (int)std::min<long long>(((long long)c->E + pfem::kBlock - 1) / pfem::kBlock, pfem::kMaxCaseBlocks);
+        const void* rho = nullptr;
+        const bool fp32 = c->d.wc_state == PFEM_STATE_FP32;
+        if (c->d.kind == PFEM_KIND_WC)  // state layout v (dim x N), p (N), rho (N), a (dim x N)
+            rho = static_cast<const char*>(c->wstate.p) + wc_state_size(c) * (size_t)(c->dim + 1) * c->N;
+        pfem::launch_mass(c->dim, c->N, c->E, c->pat.conn.as<int>(), c->pos.as<double>(), rho,
+                          c->d.kind == PFEM_KIND_WC && fp32 ? 1 : 0, c->partials.as<double>(), nb, c->stream);
+        PFEM_HIP(hipGetLastError());
+        std::vector<double> h((size_t)nb);
+        PFEM_HIP(hipMemcpyAsync(h.data(), c->partials.p, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        double m = 0.0;
+        for (double v : h) m += v;
+        *mass = m;
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_minmax(pfem_case* c, int32_t coordinate, int32_t want_max, double* value) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(value != nullptr && coordinate >= 0 && coordinate < c->dim, PFEM_E_INVALID,
+                     "coordinate out of range or null output");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        const int nb = (int)std::min<long long>(((long long)c->N + pfem::kBlock - 1) / pfem::kBlock, pfem::kMaxCaseBlocks);
+        pfem::launch_minmax(c->N, c->pat.flags.as<uint8_t>(), c->pos.as<double>() + (size_t)coordinate * c->N,
+                            want_max ? 1 : 0, c->partials.as<double>(), nb, c->stream);
+        PFEM_HIP(hipGetLastError());
+        std::vector<double> h((size_t)nb);
+        PFEM_HIP(hipMemcpyAsync(h.data(), c->partials.p, sizeof(double) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        double v = want_max ? 0.0 : std::numeric_limits<double>::max();
+        for (double x : h) v = want_max ? std::max(v, x) : std::min(v, x);
+        *value = v;
+        return PFEM_OK;
+    });
+}
+
 int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4) {
     return guarded([&] {
         require_case(c);

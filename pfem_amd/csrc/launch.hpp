@@ -60,6 +60,13 @@ template <typename S>
 void launch_wc_cfl(const WcView<S>* v, const BlockTask* elem2, int n_elem2, const BlockTask* elem3, int n_elem3,
                    hipStream_t s);
 
+// Extractors: one partial per workgroup (nblocks of them) into partials.  rho: null (PSPG)
+// or the density array in FP64 / FP32 (rho_fp32).
+void launch_mass(int dim, int N, int E, const int* conn, const double* pos, const void* rho, int rho_fp32,
+                 double* partials, int nblocks, hipStream_t s);
+void launch_minmax(int N, const uint8_t* flags, const double* x, int want_max, double* partials, int nblocks,
+                   hipStream_t s);
+
 void launch_stream_read(const double* a, long long n, double* out, int nblocks, hipStream_t s);
 void launch_copy(const double* a, double* b, long long n, int nblocks, hipStream_t s);
 void launch_xcd_probe(int* out, int nblocks, hipStream_t s);

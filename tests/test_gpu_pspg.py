@@ -273,3 +273,39 @@ def test_assembly_is_deterministic(ctx):
         c.close()
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+def derived_flags(mesh):
+    """Node flags with FREE set on nodes that belong to no element, as the library derives them."""
+    fl = mesh.flags.copy()
+    used = np.zeros(mesh.n_nodes, bool)
+    used[mesh.conn.reshape(-1)] = True
+    fl[~used] |= mg.FREE
+    return fl
+
+
+@pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
+def test_extractors_match_oracle(ctx, name):
+    """MassExtractor / MinMaxExtractor quantities on the device against the oracle, at the
+    initial positions and after a Picard step has moved the nodes."""
+    mesh = small_meshes()[name]
+    say(f"[extract] {name}")
+    d = mesh.dim
+    c = pfem_amd.MomContEqIncompNewton(ctx, d, rho=1000.0, mu=1e-3, residual="Ax_f", min_res=1e-6, max_iter=10)
+    c.set_mesh(mesh)
+    c.set_state(mg.pspg_hydrostatic(mesh))
+    fl = derived_flags(mesh)
+    for step in range(2):
+        pos = c.get_positions().reshape(d, -1)
+        m_ref = O.mass(mesh, pos)
+        m_gpu = c.mass()
+        say(f"  step {step}: mass {m_gpu:.15g} vs {m_ref:.15g}")
+        assert m_gpu == pytest.approx(m_ref, rel=1e-12)
+        for k in range(d):
+            for kind in ("min", "max"):
+                assert c.minmax(k, kind) == O.minmax(mesh, pos, k, kind, flags=fl), (k, kind)
+        if step == 0:
+            assert c.solve(1e-3 if d == 2 else 5e-4)
+    with pytest.raises(pfem_amd.PfemError):
+        c.minmax(d, "max")
+    c.close()

@@ -102,3 +102,25 @@ def test_wc_step_parity_dirichlet_without_stabilization(ctx, name, state):
         for k in range(d):
             np.testing.assert_allclose(ag[k * N:(k + 1) * N][left], dv[k * N:(k + 1) * N][left],
                                        rtol=0, atol=1e-12 if state == "fp64" else 1e-7)
+
+
+@pytest.mark.parametrize("state", ["fp64", "fp32"])
+@pytest.mark.parametrize("name", ["2d_tank", "3d_tank"])
+def test_wc_mass_matches_oracle(ctx, name, state):
+    """The WC global mass (element sizes times mean nodal density) on the device against the
+    oracle, before and after steps (FP32 state: the oracle reads the same FP32-rounded rho)."""
+    mesh = meshes()[name]
+    d, N = mesh.dim, mesh.n_nodes
+    say(f"[wc-mass] {name} {state}")
+    v, p, rho, a = mg.wc_tait_hydrostatic(mesh, K0=K0, K0p=K0P, rho_star=RS)
+    c = pfem_amd.Case(ctx, d, kind="WC", mu=MU, K0=K0, K0p=K0P, rho_star=RS, wc_state=state)
+    c.set_mesh(mesh)
+    c.set_state(np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)]))
+    for step in range(3):
+        s = c.get_state()
+        rg = s[(d + 1) * N:(d + 2) * N]
+        m_ref = O.mass(mesh, c.get_positions().reshape(d, -1), rg)
+        m_gpu = c.mass()
+        say(f"  step {step}: mass {m_gpu:.15g} vs {m_ref:.15g}")
+        assert m_gpu == pytest.approx(m_ref, rel=1e-12)
+        c.wc_step(c.wc_next_dt(0.5, 1e-3))

@@ -349,3 +349,56 @@ def test_picard_failure_paths_restore():
     assert st == 1
     np.testing.assert_array_equal(pos, m.pos.reshape(-1))
     np.testing.assert_array_equal(q, q0)
+
+
+def unit_box_mesh(dim):
+    """The unit square in 2 triangles or the unit cube in 6 Kuhn tetrahedra, each positively
+    oriented (node order swapped where the determinant came out negative)."""
+    if dim == 2:
+        pos = np.array([[0.0, 1.0, 0.0, 1.0], [0.0, 0.0, 1.0, 1.0]])
+        conn = [[0, 1, 3], [0, 3, 2]]
+    else:
+        pts = [(x, y, z) for z in (0, 1) for y in (0, 1) for x in (0, 1)]
+        pos = np.array(pts, dtype=float).T
+        conn = []
+        for perm in itertools.permutations(range(3)):  # path 0 -> 7 along axes in this order
+            v, path = 0, [0]
+            for ax in perm:
+                v |= 1 << ax
+                path.append(v)
+            conn.append(path)
+    conn = np.array(conn, dtype=np.int32)
+    for e in range(len(conn)):
+        P = pos[:, conn[e]]
+        if np.linalg.det(P[:, 1:] - P[:, :1]) < 0:
+            conn[e, [0, 1]] = conn[e, [1, 0]]
+    flags = np.zeros(pos.shape[1], np.uint8)
+    return mg.PfemMesh(dim, pos, conn, flags, 1.0)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_mass_closed_forms(dim):
+    """Sum of element sizes = the box's measure; with a density linear in x the WC mass is the
+    sum over elements of |e| times the mean nodal density, which for a linear field is the
+    exact integral (the P1 mean is the centroid value): int_0^1 (2 + 3x) dx = 3.5."""
+    m = unit_box_mesh(dim)
+    assert O.mass(m, m.pos) == pytest.approx(1.0, abs=1e-14)
+    rho = 2.0 + 3.0 * m.pos[0]
+    assert O.mass(m, m.pos, rho) == pytest.approx(3.5, abs=1e-13)
+    # uniform density scales the measure; a scaled box scales it by s^dim
+    assert O.mass(m, m.pos, np.full(m.n_nodes, 1000.0)) == pytest.approx(1000.0, rel=1e-14)
+    assert O.mass(m, 2.0 * m.pos) == pytest.approx(2.0 ** dim, rel=1e-14)
+
+
+def test_minmax_matches_brute_force_and_keeps_the_zero_start():
+    m = mg.dambreak_2d()
+    fl = m.flags.copy()
+    keep = (fl & (mg.BOUND | mg.FREE)) == 0
+    assert keep.sum() > 0 and (~keep).sum() > 0
+    for k in range(2):
+        assert O.minmax(m, m.pos, k, "max") == max(0.0, m.pos[k][keep].max())
+        assert O.minmax(m, m.pos, k, "min") == m.pos[k][keep].min()
+    # the reference's largest value starts from 0: a domain below 0 reports 0
+    shifted = m.pos - 10.0
+    assert O.minmax(m, shifted, 1, "max") == 0.0
+    assert O.minmax(m, shifted, 1, "min") == shifted[1][keep].min()
