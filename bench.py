@@ -337,10 +337,22 @@ def run_wc3d(ctx, steps, warmup=3):
         total = time.perf_counter() - t0
         sb = 4 if state == "fp32" else 8
         b = wc_step_bytes(3, mesh.n_nodes, mesh.n_elems, sb)
+        # the extractors (MassExtractor, MinMaxExtractor) on the stepped tank, outside the clock
+        # of the steps: host wall time per call, including the readback of the partials
+        mass = case.mass()
+        t2 = time.perf_counter()
+        for _ in range(10):
+            case.mass()
+        mass_ms = (time.perf_counter() - t2) * 100.0
+        t3 = time.perf_counter()
+        for _ in range(10):
+            zmax = case.minmax(2, "max")
+        minmax_ms = (time.perf_counter() - t3) * 100.0
         out[state] = dict(steps_per_s=steps / total, p50_ms=pct(lat, 50), p99_ms=pct(lat, 99),
                           elements_per_s=mesh.n_elems * steps / total, dt_last=dt,
                           algorithmic_bytes_per_step=b, achieved_GBps=b * steps / total / 1e9,
-                          frac_hbm_peak=b * steps / total / 1e9 / HBM_PEAK_GBPS)
+                          frac_hbm_peak=b * steps / total / 1e9 / HBM_PEAK_GBPS,
+                          extract=dict(mass=mass, mass_ms=mass_ms, z_max=zmax, minmax_ms=minmax_ms))
         case.close()
     return dict(workload=f"3D WC sloshing tank 2x2x1 m half filled, h=0.01 m ({mesh.n_nodes} nodes, "
                          f"{mesh.n_elems} tets), CDS_dpdt + computeNextDT (securityCoeff {WC_SC})",
