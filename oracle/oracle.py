@@ -1,7 +1,7 @@
 """ctypes front-end of the CPU oracle (liboracle.so built from oracle/pfem_oracle.cpp).
 
 TEST INFRASTRUCTURE: imported only by tests/, __graft_entry__.smoke() and the CPU-baseline
-leg of bench.py, always as the checker or the timed CPU baseline — never by the product
+leg of bench.py, always as the parity reference or the timed CPU baseline — never by the product
 path (pfem_amd/).  The restatement it wraps follows the reference files cited in
 pfem_oracle.cpp; the pins that check it live in tests/test_oracle_*.py.
 """

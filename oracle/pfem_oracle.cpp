@@ -3,8 +3,9 @@
 // CPU FP64 restatement of the reference PFEM per-time-step path (ImperatorS79/PFEM):
 // P1 element matrices, PSPG triplet assembly, setFromTriplets-equivalent duplicate sum,
 // Dirichlet column elimination, the Picard loop, the weakly compressible CDS_dpdt step and
-// the CFL time step.  It is the checker for the HIP library and the timed CPU baseline; it
-// is never linked into or called by the product path.
+// the CFL time step, and the extractors' mass and min/max.  It is the parity reference for
+// the HIP library and the timed CPU baseline; it is never linked into or called by the
+// product path.
 //
 // Every function cites the reference file:line it restates (paths relative to the
 // reference repository root).  Third-party arithmetic the reference delegates to Eigen
