@@ -254,6 +254,11 @@ def test_invalid_arguments_raise(ctx):
         rc = lib.pfem_case_set_mesh(c._h, n_nodes, n_elems, ptr(conn, ctypes.c_int32), ptr(flags, ctypes.c_uint8),
                                     ptr(pos, ctypes.c_double))
         assert rc == pfem_amd._lib.PFEM_E_INVALID, (n_nodes, n_elems, rc)
+    # the extractors need a mesh
+    with pytest.raises(pfem_amd.PfemError):
+        c.mass()
+    with pytest.raises(pfem_amd.PfemError):
+        c.minmax(0, "max")
 
 
 def test_assembly_is_deterministic(ctx):
