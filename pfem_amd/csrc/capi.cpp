@@ -106,10 +106,12 @@ size_t persistent_bytes(pfem_case* c) {
 }
 
 // Multigrid for the systems whose block-Jacobi BiCGSTAB iteration count grows with the
-// number of nodes across the domain: 2D cases from 5,000 nodes, 3D cases from 50,000
+// number of nodes across the domain: 2D cases from 2,000 nodes, 3D cases from 50,000
 // (measured: below these the extra V-cycle work outweighs the saved iterations; the 3D
 // 262k-node dam break steps 14.5 times per second with it against 11.7 without, and the
-// one-at-a-time serving stream is fastest with 50,000 among 5k / 20k / 50k / 100k / 400k).
+// one-at-a-time serving stream is fastest with 50,000 among 5k / 20k / 50k / 100k / 400k;
+// in 2D, 1,000 / 2,500 / 5,000 served the stream at 77.2 / 75.3 / 73.9 steps/s one at a time
+// and 224.3 / 223.6 / 220.3 batched).
 // PFEM_PREC_MIXED 3D cases run the hierarchy in FP32, which halves the V-cycle's bytes:
 // they take it from 5,000 nodes (the batched stream: 217.6 / 219.2 / 220.4 steps/s from
 // 20k / 10k / 5k).
@@ -121,7 +123,7 @@ bool wants_amg(const pfem_case* c) {
         const char* e = std::getenv(k);
         return e ? std::atoi(e) : dflt;
     };
-    static const int min2 = env("PFEM_AMG_MIN_NODES_2D", 5000), min3 = env("PFEM_AMG_MIN_NODES_3D", 50000),
+    static const int min2 = env("PFEM_AMG_MIN_NODES_2D", 2000), min3 = env("PFEM_AMG_MIN_NODES_3D", 50000),
                      min3m = env("PFEM_AMG_MIN_NODES_3D_MIXED", 5000);
     if (c->dim == 2) return c->N >= min2;
     return c->N >= (c->d.precision == PFEM_PREC_MIXED ? min3m : min3);
