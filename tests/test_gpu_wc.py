@@ -124,3 +124,35 @@ def test_wc_mass_matches_oracle(ctx, name, state):
         say(f"  step {step}: mass {m_gpu:.15g} vs {m_ref:.15g}")
         assert m_gpu == pytest.approx(m_ref, rel=1e-12)
         c.wc_step(c.wc_next_dt(0.5, 1e-3))
+
+
+def test_wc_full_size_matches_oracle(ctx):
+    """The C3 sloshing tank at its full size (2x2x1 m half filled, h = 0.01 m: 2.06M nodes,
+    12M tets), FP64 state: two CDS_dpdt steps with computeNextDT against the oracle."""
+    mesh = mg.tank_3d(h=0.01)
+    d, N = mesh.dim, mesh.n_nodes
+    say(f"[wc-full] N={N} E={mesh.n_elems}")
+    assert N > 2_000_000 and mesh.n_elems == 12_000_000
+    v, p, rho, a = mg.wc_tait_hydrostatic(mesh, K0=2.2e9, K0p=7.0, rho_star=RS)
+    c = pfem_amd.Case(ctx, d, kind="WC", mu=MU, K0=2.2e9, K0p=7.0, rho_star=RS, wc_state="fp64")
+    c.set_mesh(mesh)
+    c.set_state(np.concatenate([v.reshape(-1), p, rho, a.reshape(-1)]))
+    pos_o = mesh.pos.reshape(-1).copy()
+    vo, po, ro, ao = v.reshape(-1).copy(), p.copy(), rho.copy(), a.reshape(-1).copy()
+    for step in range(2):
+        dt_o = O.wc_next_dt(mesh, pos_o, vo, po, ro, mu=MU, K0=2.2e9, K0p=7.0, security_coeff=0.5, max_dt=1e-3)
+        dt_g = c.wc_next_dt(0.5, 1e-3)
+        assert dt_g == pytest.approx(dt_o, rel=1e-12)
+        O.wc_step(mesh, pos_o, vo, po, ro, ao, dt_o, mu=MU, K0=2.2e9, K0p=7.0, rho_star=RS)
+        c.wc_step(dt_o)
+        s = c.get_state()
+        vg, pg, rg, ag = s[:d * N], s[d * N:(d + 1) * N], s[(d + 1) * N:(d + 2) * N], s[(d + 2) * N:]
+        say(f"  step {step}: dt={dt_g:.6e} dv={rel(vo, vg):.2e} dp={rel(po, pg):.2e} drho={rel(ro, rg):.2e} "
+            f"da={rel(ao, ag):.2e}")
+        assert rel(vo, vg) <= 1e-10
+        assert rel(po, pg) <= 1e-10
+        assert rel(ro, rg) <= 1e-10
+        assert np.abs(ag - ao).max() <= 1e-10 * max(np.abs(ao).max(), 9.81)
+        np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0, atol=1e-12)
+    assert c.mass() == pytest.approx(O.mass(mesh, pos_o.reshape(d, -1), ro), rel=1e-12)
+    c.close()
