@@ -99,7 +99,7 @@ typedef struct pfem_case_desc {
 } pfem_case_desc;
 
 /* BiCGSTAB preconditioner of a PSPG case (chosen at pfem_case_set_mesh). */
-#define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 2,000 nodes and 3D cases of >= 50,000 */
+#define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 2,000 nodes, 3D of >= 50,000 (MIXED: 5,000) */
 #define PFEM_PRECOND_JACOBI 1 /* block Jacobi over each node's (dim+1) x (dim+1) diagonal block */
 #define PFEM_PRECOND_AMG 2    /* aggregation multigrid V-cycle, block-Jacobi smoothing */
 
