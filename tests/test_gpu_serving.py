@@ -38,6 +38,8 @@ def specs():
         ("PSPG", 2, "fp64", mg.column_2d_nodes(2500, h=0.01, seed=6), "amg"),
         ("WC", 3, "fp64", mg.tank_nodes(3, 4000, h=0.02, seed=7), None),
         ("PSPG", 2, "mixed", mg.column_2d_nodes(6000, h=0.01, seed=8), "amg"),
+        ("PSPG", 3, "mixed_ir", mg.column_3d_nodes(3000, h=0.02, seed=9), "jacobi"),
+        ("PSPG", 3, "mixed", mg.column_3d_nodes(8000, h=0.02, seed=10), "amg"),
     ]
 
 
