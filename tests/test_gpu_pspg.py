@@ -290,9 +290,12 @@ def derived_flags(mesh):
 
 
 @pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
-def test_extractors_match_oracle(ctx, name):
+def test_extractors_match_oracle(ctx, name, tmp_path):
     """MassExtractor / MinMaxExtractor quantities on the device against the oracle, at the
-    initial positions and after a Picard step has moved the nodes."""
+    initial positions and after a Picard step has moved the nodes, and the CSV lines the
+    extractors write for them."""
+    from pfem_amd.extractors import MassExtractor, MinMaxExtractor
+
     mesh = small_meshes()[name]
     say(f"[extract] {name}")
     d = mesh.dim
@@ -300,6 +303,10 @@ def test_extractors_match_oracle(ctx, name):
     c.set_mesh(mesh)
     c.set_state(mg.pspg_hydrostatic(mesh))
     fl = derived_flags(mesh)
+    dt = 1e-3 if d == 2 else 5e-4
+    ex_mass = MassExtractor(c, str(tmp_path / "mass.csv"), dt)
+    ex_top = MinMaxExtractor(c, str(tmp_path / "top.csv"), dt, d - 1, "max")
+    expect_mass, expect_top = [], []
     for step in range(2):
         pos = c.get_positions().reshape(d, -1)
         m_ref = O.mass(mesh, pos)
@@ -309,8 +316,20 @@ def test_extractors_match_oracle(ctx, name):
         for k in range(d):
             for kind in ("min", "max"):
                 assert c.minmax(k, kind) == O.minmax(mesh, pos, k, kind, flags=fl), (k, kind)
+        t = step * dt
+        assert ex_mass.update(t) and ex_top.update(t)
+        expect_mass.append((t, m_ref))
+        expect_top.append("%f,%f" % (t, O.minmax(mesh, pos, d - 1, "max", flags=fl)))
         if step == 0:
-            assert c.solve(1e-3 if d == 2 else 5e-4)
+            assert c.solve(dt)
+    ex_mass.close()
+    ex_top.close()
+    lines = (tmp_path / "mass.csv").read_text().splitlines()
+    assert len(lines) == len(expect_mass)
+    for line, (t, m) in zip(lines, expect_mass):
+        ts, ms = line.split(",")
+        assert ts == "%f" % t and float(ms) == pytest.approx(m, rel=1e-12, abs=5e-7)
+    assert (tmp_path / "top.csv").read_text().splitlines() == expect_top
     with pytest.raises(pfem_amd.PfemError):
         c.minmax(d, "max")
     c.close()
