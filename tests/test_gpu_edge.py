@@ -180,3 +180,27 @@ def test_edge_wc_step_parity(ctx, name):
         assert np.abs(ag - ao).max() <= 1e-10 * max(np.abs(ao).max(), 9.81)
         np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0, atol=1e-12)
     c.close()
+
+
+def test_edge_picard_step_parity_ragged(ctx):
+    """Two Picard time steps on the ragged column field from its hydrostatic state, against
+    the oracle's Picard loop with a tight Krylov stand-in."""
+    mesh = ragged_mesh()
+    d, N = mesh.dim, mesh.n_nodes
+    say(f"[edge picard] ragged N={N}")
+    dt = 5e-4
+    q0 = mg.pspg_hydrostatic(mesh)
+    pos_o = mesh.pos.reshape(-1).copy()
+    q_o = q0.copy()
+    c = pfem_amd.MomContEqIncompNewton(ctx, d, krylov_rtol=1e-11)
+    c.set_mesh(mesh)
+    c.set_state(q0)
+    for step in range(2):
+        st_o, info = O.pspg_step(mesh, pos_o, q_o, dt=dt, solver_kind=2, rtol=1e-13)
+        rc, st = c.pspg_step(dt)
+        say(f"  step {step}: oracle iters={info['picard_iters']} gpu iters={st['picard_iters']}")
+        assert st_o == 0 and rc == 0
+        assert abs(st["picard_iters"] - info["picard_iters"]) <= 1
+        assert block_rel_err(d, N, q_o, c.get_state()) <= 1e-6
+        np.testing.assert_allclose(c.get_positions().reshape(-1), pos_o, rtol=0, atol=1e-9 * np.abs(pos_o).max())
+    c.close()

@@ -40,6 +40,8 @@ def specs():
         ("PSPG", 2, "mixed", mg.column_2d_nodes(6000, h=0.01, seed=8), "amg"),
         ("PSPG", 3, "mixed_ir", mg.column_3d_nodes(3000, h=0.02, seed=9), "jacobi"),
         ("PSPG", 3, "mixed", mg.column_3d_nodes(8000, h=0.02, seed=10), "amg"),
+        ("PSPG", 3, "fp64", mg.randomwalk_3d(h=0.05, ncol=8, mean_cells=20, seed=1), "jacobi"),
+        ("WC", 3, "fp64", mg.randomwalk_3d(h=0.05, ncol=8, mean_cells=20, seed=1), None),
     ]
 
 
