@@ -15,7 +15,9 @@ from dataclasses import dataclass
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# PFEM_ORACLE_LIB points the wrapper at another build of the same sources (the mutation test
+# builds deliberately broken copies to show that the pins catch them)
+_LIB_PATH = os.environ.get("PFEM_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)
@@ -68,6 +70,10 @@ def lib():
         L.orc_mass.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp]
         L.orc_minmax.restype = C.c_double
         L.orc_minmax.argtypes = [C.c_int, C.c_int64, _u8p, _dp, C.c_int, C.c_int]
+        L.orc_picard_residual.restype = C.c_double
+        L.orc_picard_residual.argtypes = [C.c_int, C.c_int64, _u8p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.orc_wc_parts.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, _dp,
+                                   C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         _lib = L
     return _lib
 
@@ -256,3 +262,39 @@ def minmax(mesh, pos, coordinate, kind="max", flags=None):
     fl = np.ascontiguousarray(mesh.flags if flags is None else flags, dtype=np.uint8)
     pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
     return L.orc_minmax(mesh.dim, mesh.n_nodes, _u8(fl), _d(pos), coordinate, 1 if kind == "max" else 0)
+
+
+def picard_residual(dim, flags, kind, q_iter, q_iter_prev, system=None):
+    """Residual callback of the Picard loop (MomContEquationPSPG.inl:314-372) on given iterates;
+    flags are used as given (FREE included).  kind: "U", "U_P" or "Ax_f" (needs system, a
+    CsrSystem)."""
+    L = lib()
+    k = RES_KIND[kind]
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    N = fl.size
+    qi = np.ascontiguousarray(q_iter, dtype=np.float64)
+    qp = np.ascontiguousarray(q_iter_prev, dtype=np.float64)
+    assert qi.size == (dim + 1) * N and qp.size == qi.size
+    if k == 2:
+        rp, ci, va, b = system.row_ptr, system.col_idx, system.vals, system.b
+        return L.orc_picard_residual(dim, N, _u8(fl), k, _i(rp), _i(ci), _d(va), _d(b), _d(qi), _d(qp))
+    nul_i, nul_d = C.cast(None, _ip), C.cast(None, _dp)
+    return L.orc_picard_residual(dim, N, _u8(fl), k, nul_i, nul_i, nul_d, nul_d, _d(qi), _d(qp))
+
+
+def wc_parts(mesh, pos, v, p, rho, dt=1e-4, mu=1e-3, g=None, K0=2.2e9, K0p=7.0, meduri=True, nthreads=0):
+    """The scattered WC systems of one step at the given state, before boundary conditions:
+    continuity lumped mass ML and right-hand side F0 (ContEquation.inl:353-413), momentum
+    lumped mass Mv and force F (MomEquation.inl:229-302).  Returns (ML, F0, Mv, F)."""
+    L = lib()
+    d, N = mesh.dim, mesh.n_nodes
+    g = _g3(g if g is not None else ([0.0, -9.81] if d == 2 else [0.0, 0.0, -9.81]))
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    pos = np.ascontiguousarray(pos, dtype=np.float64).reshape(-1)
+    v = np.ascontiguousarray(v, dtype=np.float64).reshape(-1)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    rho = np.ascontiguousarray(rho, dtype=np.float64)
+    ML, F0, Mv, F = np.empty(N), np.empty(N), np.empty(d * N), np.empty(d * N)
+    L.orc_wc_parts(d, N, mesh.n_elems, _i(conn), _d(pos), _d(v), _d(p), _d(rho), mu, _d(g), K0, K0p,
+                   1 if meduri else 0, dt, _d(ML), _d(F0), _d(Mv), _d(F), nthreads)
+    return ML, F0, Mv, F

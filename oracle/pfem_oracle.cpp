@@ -797,6 +797,43 @@ static void assembleAndBC(const Params& P, const std::vector<uint8_t>& fl, const
     applyBCPSPG<D>(P, fl, qPrev, dirVals, S);
 }
 
+// Residual callback of the PSPG Picard loop (MomContEquationPSPG.inl:314-372).  kind 2
+// ("Ax_f"): ||A q - b||_2 with the system re-assembled at the updated positions.  kind 0 ("U"):
+// sqrt(sum |u - u_prev|^2 / sum |u_prev|^2) over the velocity components of the nodes that are
+// not free, DBL_MAX when the denominator is 0; kind 1 ("U_P"): the larger of that and the same
+// ratio over the pressures.
+static double picardResidual(int D, int64_t N, const uint8_t* fl, int resKind, const CsrSys& S, const double* qIter,
+                             const double* qIterPrev) {
+    const int64_t n = (D + 1) * N;
+    if (resKind == 2) {
+        std::vector<double> Ax(n);
+        spmv(n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), qIter, Ax.data());
+        double s2 = 0;
+        for (int64_t i = 0; i < n; ++i) s2 += (Ax[i] - S.b[i]) * (Ax[i] - S.b[i]);
+        return std::sqrt(s2);
+    }
+    double num = 0, den = 0, resV, resP = 0;
+    for (int64_t nn = 0; nn < N; ++nn)
+        if (!(fl[nn] & FREE))
+            for (int d = 0; d < D; ++d) {
+                const double a = qIter[nn + d * N], b = qIterPrev[nn + d * N];
+                num += (a - b) * (a - b);
+                den += b * b;
+            }
+    resV = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+    if (resKind == 1) {
+        num = den = 0;
+        for (int64_t nn = 0; nn < N; ++nn)
+            if (!(fl[nn] & FREE)) {
+                const double a = qIter[nn + D * N], b = qIterPrev[nn + D * N];
+                num += (a - b) * (a - b);
+                den += b * b;
+            }
+        resP = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+    }
+    return std::max(resV, resP);
+}
+
 // One PSPG time step: PicardAlgo::solve (srcs/simulation/nonLinearAlgo/PicardAlgo.cpp:31-95)
 // with the PSPG callbacks (PSPG.inl:262-372).  On failure pos and q are left unchanged
 // (restoreNodesList restores positions and states, Mesh.cpp:1019-1043).
@@ -870,34 +907,7 @@ static int pspgStep(const Params& P, double* pos, double* q, const double* dirVa
         st.tBC += S.tBC;
         // residual callback (PSPG.inl:314-372)
         auto tr = clk::now();
-        if (resKind == 2) {
-            std::vector<double> Ax(n);
-            spmv(n, S.rowPtr.data(), S.colIdx.data(), S.vals.data(), qIter.data(), Ax.data());
-            double s2 = 0;
-            for (int64_t i = 0; i < n; ++i) s2 += (Ax[i] - S.b[i]) * (Ax[i] - S.b[i]);
-            res = std::sqrt(s2);
-        } else {
-            double num = 0, den = 0, resV, resP = 0;
-            for (int64_t nn = 0; nn < N; ++nn)
-                if (!(fl[nn] & FREE))
-                    for (int d = 0; d < D; ++d) {
-                        const double a = qIter[nn + d * N], b = qIterPrev[nn + d * N];
-                        num += (a - b) * (a - b);
-                        den += b * b;
-                    }
-            resV = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
-            if (resKind == 1) {
-                num = den = 0;
-                for (int64_t nn = 0; nn < N; ++nn)
-                    if (!(fl[nn] & FREE)) {
-                        const double a = qIter[nn + D * N], b = qIterPrev[nn + D * N];
-                        num += (a - b) * (a - b);
-                        den += b * b;
-                    }
-                resP = (den == 0) ? std::numeric_limits<double>::max() : std::sqrt(num / den);
-            }
-            res = std::max(resV, resP);
-        }
+        res = picardResidual(D, N, fl.data(), resKind, S, qIter.data(), qIterPrev.data());
         st.tResidual += secs(tr, clk::now());
         qIterPrev = qIter;
         if (std::isnan(res)) {
@@ -928,27 +938,16 @@ struct WCParams {
     int meduri;
 };
 
+// Continuity system of one CDS_dpdt step (m_buildSystemdpdt, ContEquation.inl:353-413), scattered
+// in element order: ML = the lump2 row sums of the consistent mass (factor 1,
+// MatricesBuilder.hpp:87-102) and F0 = sum_e (-dt D(K0 + K0' N.P) V + M P) (Meduri) or
+// (-dt D V + M_L P) (None).  Before m_applyBCdpdt and the inversion of ML.
 template <int D>
-static void wcStep(const WCParams& W, double dt, double* pos, double* v, double* p, double* rho, double* a,
-                   const double* dirVals, double* tCont, double* tMom) {
+static void wcContAssemble(const WCParams& W, double dt, const double* pos, const double* v, const double* p,
+                           double* MLout, double* F0out) {
     constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
     const int64_t N = W.N, E = W.E;
-    Params P{};
-    P.dim = D;
-    P.N = N;
-    P.E = E;
-    P.conn = W.conn;
-    P.flags = W.flags;
-    auto fl = derivedFlags(P);
     MB<D> mb;
-    // v1/2 = v + dt/2 a ; x += dt v1/2 (non-fixed)   Solver.cpp:249-254, Mesh.cpp:1101-1137
-    for (int64_t i = 0; i < D * N; ++i) v[i] = v[i] + 0.5 * dt * a[i];
-    for (int64_t nn = 0; nn < N; ++nn)
-        if (!(W.flags[nn] & FIXED))
-            for (int d = 0; d < D; ++d) pos[nn + d * N] += v[nn + d * N] * dt;
-    auto t0 = clk::now();
-    // ---- continuity (m_buildSystemdpdt, ContEquation.inl:353-413)
-    std::vector<double> invM(N, 0.0), F0(N, 0.0);
     std::vector<double> MLe(E * ne), F0e(E * ne);
 #pragma omp parallel for schedule(static)
     for (int64_t e = 0; e < E; ++e) {
@@ -988,24 +987,25 @@ static void wcStep(const WCParams& W, double dt, double* pos, double* v, double*
             MLe[e * ne + i] = ML[i];
         }
     }
+    std::fill(MLout, MLout + N, 0.0);
+    std::fill(F0out, F0out + N, 0.0);
     for (int64_t e = 0; e < E; ++e)
         for (int i = 0; i < ne; ++i) {
-            invM[W.conn[e * ne + i]] += MLe[e * ne + i];
-            F0[W.conn[e * ne + i]] += F0e[e * ne + i];
+            MLout[W.conn[e * ne + i]] += MLe[e * ne + i];
+            F0out[W.conn[e * ne + i]] += F0e[e * ne + i];
         }
-    for (int64_t nn = 0; nn < N; ++nn) invM[nn] = 1 / invM[nn];
-    for (int64_t nn = 0; nn < N; ++nn)  // m_applyBCdpdt: ContEquation.inl:334-350
-        if (fl[nn] & FREE) {
-            F0[nn] = 0;
-            invM[nn] = 1;
-        }
-    for (int64_t nn = 0; nn < N; ++nn) {
-        p[nn] = invM[nn] * F0[nn];
-        rho[nn] = std::pow((W.K0p / W.K0) * p[nn] + 1, 1 / W.K0p) * W.rhoStar;  // :319-332
-    }
-    auto t1 = clk::now();
-    // ---- momentum (m_buildSystem, MomEquation.inl:229-302)
-    std::vector<double> invMv(D * N, 0.0), F(D * N, 0.0);
+}
+
+// Momentum system of one step (m_buildSystem, MomEquation.inl:229-302), scattered in element
+// order: Mv = diagBlock + lump of M(rho_gp) (MatricesBuilder.hpp:104-129) per component, and
+// F = sum_e (-K V + D^T P + F(rho_gp)) (K with the deviatoric ddev, MomEquation.inl:63-83).
+// Before m_applyBC and the inversion of Mv.
+template <int D>
+static void wcMomAssemble(const WCParams& W, const double* pos, const double* v, const double* p, const double* rho,
+                          double* Mvout, double* Fout) {
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    const int64_t N = W.N, E = W.E;
+    MB<D> mb;
     std::vector<double> Mle(E * nd), Fte(E * nd);
     double dd[nv][nv];
     ddevWC<D>(dd);
@@ -1052,13 +1052,51 @@ static void wcStep(const WCParams& W, double dt, double* pos, double* v, double*
             Mle[e * nd + r] = lumped[r % ne];
         }
     }
+    std::fill(Mvout, Mvout + D * N, 0.0);
+    std::fill(Fout, Fout + D * N, 0.0);
     for (int64_t e = 0; e < E; ++e)
         for (int i = 0; i < ne; ++i)
             for (int d = 0; d < D; ++d) {
                 const int64_t gi = W.conn[e * ne + i] + d * N;
-                invMv[gi] += Mle[e * nd + i + d * ne];
-                F[gi] += Fte[e * nd + i + d * ne];
+                Mvout[gi] += Mle[e * nd + i + d * ne];
+                Fout[gi] += Fte[e * nd + i + d * ne];
             }
+}
+
+template <int D>
+static void wcStep(const WCParams& W, double dt, double* pos, double* v, double* p, double* rho, double* a,
+                   const double* dirVals, double* tCont, double* tMom) {
+    const int64_t N = W.N, E = W.E;
+    Params P{};
+    P.dim = D;
+    P.N = N;
+    P.E = E;
+    P.conn = W.conn;
+    P.flags = W.flags;
+    auto fl = derivedFlags(P);
+    // v1/2 = v + dt/2 a ; x += dt v1/2 (non-fixed)   Solver.cpp:249-254, Mesh.cpp:1101-1137
+    for (int64_t i = 0; i < D * N; ++i) v[i] = v[i] + 0.5 * dt * a[i];
+    for (int64_t nn = 0; nn < N; ++nn)
+        if (!(W.flags[nn] & FIXED))
+            for (int d = 0; d < D; ++d) pos[nn + d * N] += v[nn + d * N] * dt;
+    auto t0 = clk::now();
+    // ---- continuity (ContEquation.inl:353-413, m_applyBCdpdt :334-350, solve :123-166)
+    std::vector<double> invM(N), F0(N);
+    wcContAssemble<D>(W, dt, pos, v, p, invM.data(), F0.data());
+    for (int64_t nn = 0; nn < N; ++nn) invM[nn] = 1 / invM[nn];
+    for (int64_t nn = 0; nn < N; ++nn)  // m_applyBCdpdt: ContEquation.inl:334-350
+        if (fl[nn] & FREE) {
+            F0[nn] = 0;
+            invM[nn] = 1;
+        }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        p[nn] = invM[nn] * F0[nn];
+        rho[nn] = std::pow((W.K0p / W.K0) * p[nn] + 1, 1 / W.K0p) * W.rhoStar;  // :319-332
+    }
+    auto t1 = clk::now();
+    // ---- momentum (MomEquation.inl:229-302, m_applyBC :305-374, solve :201-226)
+    std::vector<double> invMv(D * N), F(D * N);
+    wcMomAssemble<D>(W, pos, v, p, rho, invMv.data(), F.data());
     for (int64_t i = 0; i < D * N; ++i) invMv[i] = 1 / invMv[i];
     for (int64_t nn = 0; nn < N; ++nn) {  // m_applyBC: MomEquation.inl:305-374 (gamma = 0)
         const uint8_t f = fl[nn];
@@ -1391,4 +1429,47 @@ double orc_minmax(int dim, int64_t N, const uint8_t* flags, const double* pos, i
     return v;
 }
 
+// Residual callback on given iterates (kind 0 U, 1 U_P, 2 Ax_f).  flags are used as given
+// (FREE included); for Ax_f the CSR system (rp, ci, vals, b) of n = (dim+1) N rows is used.
+double orc_picard_residual(int dim, int64_t N, const uint8_t* flags, int kind, const int* rp, const int* ci,
+                           const double* vals, const double* b, const double* q_iter, const double* q_iter_prev) {
+    CsrSys S;
+    const int64_t n = (dim + 1) * N;
+    if (kind == 2) {
+        S.n = n;
+        S.rowPtr.assign(rp, rp + n + 1);
+        S.colIdx.assign(ci, ci + rp[n]);
+        S.vals.assign(vals, vals + rp[n]);
+        S.b.assign(b, b + n);
+    }
+    return picardResidual(dim, N, flags, kind, S, q_iter, q_iter_prev);
+}
+
+// The two scattered WC systems of one step at the given state (no position update, no BC):
+// ML, F0 (N each) of the continuity equation and Mv, F (dim x N each) of the momentum
+// equation; rho is the density the momentum system reads.
+void orc_wc_parts(int dim, int64_t N, int64_t E, const int* conn, const double* pos, const double* v,
+                  const double* p, const double* rho, double mu, const double* g, double K0, double K0p, int meduri,
+                  double dt, double* ML, double* F0, double* Mv, double* F, int nthreads) {
+    setThreads(nthreads);
+    WCParams W{};
+    W.dim = dim;
+    W.N = N;
+    W.E = E;
+    W.conn = conn;
+    W.mu = mu;
+    for (int d = 0; d < 3; ++d) W.g[d] = d < dim ? g[d] : 0.0;
+    W.K0 = K0;
+    W.K0p = K0p;
+    W.meduri = meduri;
+    if (dim == 2) {
+        wcContAssemble<2>(W, dt, pos, v, p, ML, F0);
+        wcMomAssemble<2>(W, pos, v, p, rho, Mv, F);
+    } else {
+        wcContAssemble<3>(W, dt, pos, v, p, ML, F0);
+        wcMomAssemble<3>(W, pos, v, p, rho, Mv, F);
+    }
+}
+
 }  // extern "C"
+

@@ -1,0 +1,61 @@
+"""The oracle's pins catch plausible restatement mistakes.
+
+Each case plants one mistake into a copy of oracle/pfem_oracle.cpp (on the CPU, in a temporary
+directory), builds it, and runs the pin tests against that build: at least one pin must fail.
+The mistakes are the slips a restatement of the reference is most likely to make — a
+flipped sign, swapped factors, a dropped square, the wrong stabilisation or deviatoric
+matrix, an index or denominator mix-up.
+"""
+import os
+import subprocess
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "oracle", "pfem_oracle.cpp")
+PINS = ["tests/test_oracle_pins.py", "tests/test_oracle_wc_pins.py"]
+
+MUTATIONS = {
+    # WC momentum: sign of D^T P (MomEquation.inl:229-302)
+    "wc_dtp_sign": ("Fte[e * nd + r] = -kv + dp + Fe[r];", "Fte[e * nd + r] = -kv - dp + Fe[r];"),
+    # WC continuity: K0 and K0' swapped in the D factor (ContEquation.inl:80-86)
+    "wc_k0_swap": ("return W.K0 + W.K0p * pg;", "return W.K0p + W.K0 * pg;"),
+    # WC continuity: Meduri and lumped stabilisation swapped (ContEquation.inl:353-413)
+    "wc_meduri_swap": ("if (W.meduri)", "if (!W.meduri)"),
+    # WC deviatoric ddev replaced by the incompressible one (MomEquation.inl:63-83)
+    "wc_ddev": ("dd[r][c] = (r == c) ? 4.0 / 3 : -2.0 / 3;", "dd[r][c] = (r == c) ? 2.0 : 0.0;"),
+    # WC body force without the density factor (MomEquation.inl:135-139)
+    "wc_force_rho": ("mb.Fv(geo.detJ, rhoG, W.g, Fe);", "mb.Fv(geo.detJ, [](const double*) { return 1000.0; }, W.g, Fe);"),
+    # CFL: viscous term without its square (WCompNewton/Solver.cpp:214-217)
+    "cfl_alpha_square": ("maxS = std::max(maxS, 4 * alpha * alpha / (he * he));",
+                         "maxS = std::max(maxS, 4 * alpha / (he * he));"),
+    # CFL: c^2 without the K0' p term (ContEquation.inl:117-121)
+    "cfl_c2": ("const double c2 = (W.K0 + W.K0p * p[nn]) / rho[nn];", "const double c2 = W.K0 / rho[nn];"),
+    # Tait EOS exponent (ContEquation.inl:319-332)
+    "tait_exponent": ("rho[nn] = std::pow((W.K0p / W.K0) * p[nn] + 1, 1 / W.K0p) * W.rhoStar;",
+                      "rho[nn] = std::pow((W.K0p / W.K0) * p[nn] + 1, W.K0p) * W.rhoStar;"),
+    # Picard U residual: iterate instead of previous iterate in the denominator (PSPG.inl:333)
+    "res_den": ("const double a = qIter[nn + d * N], b = qIterPrev[nn + d * N];\n                num += (a - b) * (a - b);\n                den += b * b;",
+                "const double a = qIter[nn + d * N], b = qIterPrev[nn + d * N];\n                num += (a - b) * (a - b);\n                den += a * a;"),
+    # PSPG tau: viscous term without the factor 9 (PSPG.inl:257-258)
+    "tau_nine": ("9 * (4 * P.mu / (h * h * P.rho))", "(4 * P.mu / (h * h * P.rho))"),
+    # PSPG free-node rhs without dt g (PSPG.inl:193-204)
+    "bc_free_rhs": ("S.b[nn + d * N] = qPrev[nn + d * N] + P.dt * P.g[d];", "S.b[nn + d * N] = qPrev[nn + d * N];"),
+}
+
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS))
+def test_planted_mistake_fails_a_pin(name, tmp_path):
+    old, new = MUTATIONS[name]
+    src = open(SRC).read()
+    assert src.count(old) == 1, f"mutation site not found once: {old}"
+    mut = tmp_path / "pfem_oracle.cpp"
+    mut.write_text(src.replace(old, new))
+    lib = tmp_path / "liboracle_mut.so"
+    subprocess.run(["g++", "-O1", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-o", str(lib), str(mut)], check=True,
+                   capture_output=True)
+    env = dict(os.environ, PFEM_ORACLE_LIB=str(lib), OMP_NUM_THREADS="2")
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", *PINS], cwd=ROOT, env=env,
+                       capture_output=True, text=True, timeout=600)
+    assert r.returncode == 1, f"no pin caught the mutation {name}:\n{r.stdout[-2000:]}"
