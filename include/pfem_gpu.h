@@ -175,6 +175,45 @@ int pfem_spmv(pfem_case* c, const double* x, double* y);
  * (Mesh.cpp:1238-1277).  On failure (> 0) positions and state are restored
  * (Mesh::restoreNodesList). */
 int pfem_pspg_step(pfem_case* c, double dt, pfem_step_stats* stats);
+/* pfem_pspg_step, pfem_batch_step and pfem_server_submit take the Dirichlet values once per
+ * step, so they return PFEM_E_UNSUPPORTED for a mesh with a boundary Dirichlet node that is
+ * not PFEM_NODE_FIXED: its value must be re-evaluated at its new position before every
+ * re-assembly (MomContEquationPSPG.inl:206-214), which the callback-level entry points below
+ * allow. */
+
+/* ---- the Picard loop's callbacks one by one --------------------------------------------
+ * PicardAlgo::solve (PicardAlgo.cpp:31-95) takes three callbacks (NonLinearAlgo.hpp:18-22)
+ * that MomContEquationPSPG.inl:262-372 binds.  These entry points let the reference's own
+ * PicardAlgo drive the GPU through them, with every vector staying on the device:
+ *   prepare  -> pfem_save_positions; pfem_pspg_assemble(c, dt, NULL, NULL)
+ *   solve    -> pfem_pspg_solve(c, NULL, NULL, &st); pfem_update_positions(c, dt, NULL);
+ *               [pfem_case_set_dirichlet with the BC values at the new positions;]
+ *               pfem_pspg_assemble(c, dt, NULL, NULL)   (on failure: pfem_restore_positions)
+ *   residual -> pfem_residual(c, kind, NULL, NULL, &res)
+ * and, when PicardAlgo::solve returns true, pfem_pspg_commit (the next step's qPrev is read
+ * from the node states, MomContEquation.inl:287-298).  The device keeps the case's node
+ * states as the Picard iterate q (what tau reads and what pfem_pspg_solve writes) and the
+ * step's previous state q_prev. */
+/* Mesh::saveNodesList (Mesh.cpp:1047-1053) at the start of a step: saves the positions; the
+ * node states and the previous iterate are set to q_prev (PicardAlgo.cpp:37-42). */
+int pfem_save_positions(pfem_case* c);
+/* Mesh::restoreNodesList (Mesh.cpp:1019-1045): positions back to the saved ones, node states
+ * back to q_prev. */
+int pfem_restore_positions(pfem_case* c);
+/* setNodesStatesfromQ + Mesh::updateNodesPositionFromSave (MomContEquationPSPG.inl:293-295,
+ * Mesh.cpp:1238-1277): node states = q (host array of (dim+1) N, or NULL for the device
+ * iterate the last pfem_pspg_solve wrote), then x = x_saved + dt q_v for non-fixed nodes. */
+int pfem_update_positions(pfem_case* c, double dt, const double* q);
+/* Residual callback (MomContEquationPSPG.inl:314-372) of kind PFEM_RES_*: U / U_P from the
+ * iterate and the previous iterate, Ax_f = ||A q - b|| with the system last assembled.
+ * q_iter / q_iter_prev: host arrays, or NULL for the device copies.  Afterwards the device's
+ * previous iterate is the current one (PicardAlgo.cpp:72-73). */
+int pfem_residual(pfem_case* c, int32_t kind, const double* q_iter, const double* q_iter_prev, double* out);
+/* End of a converged step: q_prev = the node states (MomContEquation.inl:287-298 reads the
+ * next step's qPrev from them). */
+int pfem_pspg_commit(pfem_case* c);
+/* Copy of the device iterate (the node states) into q ((dim+1) N). */
+int pfem_get_iterate(pfem_case* c, double* q);
 
 /* ---- WC ------------------------------------------------------------------------------
  * One CDS_dpdt step (WCompNewton/Solver.cpp:236-276). */

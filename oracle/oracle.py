@@ -59,7 +59,7 @@ def lib():
         L.orc_pspg_step.restype = C.c_int
         L.orc_pspg_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, C.c_double, C.c_double,
                                     _dp, C.c_double, C.c_int, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, _dp,
-                                    C.c_int]
+                                    C.c_int, _dp]
         L.orc_wc_step.restype = C.c_int
         L.orc_wc_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, _dp, _dp, _dp, C.c_double,
                                   _dp, C.c_double, C.c_double, C.c_double, C.c_int, C.c_double, _dp, C.c_int]
@@ -200,8 +200,10 @@ RES_KIND = {"U": 0, "U_P": 1, "Ax_f": 2}
 
 
 def pspg_step(mesh, pos, q, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3, residual="Ax_f", min_res=1e-6,
-              max_iter=10, solver_kind=2, rtol=1e-10, krylov_maxit=20000, nthreads=0):
-    """One Picard time step (PicardAlgo.cpp:31-95 + PSPG.inl:262-372); pos, q updated in place."""
+              max_iter=10, solver_kind=2, rtol=1e-10, krylov_maxit=20000, nthreads=0, dir_affine=None):
+    """One Picard time step (PicardAlgo.cpp:31-95 + PSPG.inl:262-372); pos, q updated in place.
+    dir_affine (dim x (dim+1)): Dirichlet values v_bc(x) = A x + c re-evaluated at the current
+    positions before every assembly (instead of the constant dir_vals)."""
     L = lib()
     d = mesh.dim
     N = mesh.n_nodes
@@ -211,8 +213,10 @@ def pspg_step(mesh, pos, q, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3,
     conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
     flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
     assert pos.dtype == np.float64 and pos.flags.c_contiguous and q.dtype == np.float64 and q.flags.c_contiguous
+    aff = None if dir_affine is None else np.ascontiguousarray(dir_affine, dtype=np.float64).reshape(d, d + 1)
     st = L.orc_pspg_step(d, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(q), _d(dir_vals), rho, mu, _d(g), dt,
-                         RES_KIND[residual], min_res, max_iter, solver_kind, rtol, krylov_maxit, _d(stats), nthreads)
+                         RES_KIND[residual], min_res, max_iter, solver_kind, rtol, krylov_maxit, _d(stats), nthreads,
+                         _d(aff) if aff is not None else C.cast(None, _dp))
     keys = ["picard_iters", "krylov_iters", "assemblies", "final_res", "max_relres", "t_assemble", "t_solve",
             "t_update", "t_residual", "t_triplets", "t_push", "t_set_from_triplets", "t_vector", "t_bc"]
     return st, dict(zip(keys, stats.tolist()))

@@ -847,9 +847,13 @@ struct StepStats {
     double tTriplets = 0, tPush = 0, tSetFromTriplets = 0, tVector = 0, tBC = 0;
 };
 
+// dirAffine (nullable): the Dirichlet values are re-evaluated before every assembly at the
+// nodes' current positions, v_bc(x) = A x + c with A = dirAffine[d][0..D-1], c = dirAffine[d][D]
+// per component d: the Lua "<Tag>V"(pos, t + dt) call of each m_applyBCPSPG
+// (MomContEquationPSPG.inl:206-214) for a BC that depends on the position.
 template <int D>
 static int pspgStep(const Params& P, double* pos, double* q, const double* dirVals, int resKind, double minRes,
-                    int maxIter, const SolveCfg& cfg, StepStats& st) {
+                    int maxIter, const SolveCfg& cfg, StepStats& st, const double* dirAffine = nullptr) {
     const int64_t N = P.N;
     const int64_t n = (D + 1) * N;
     auto fl = derivedFlags(P);
@@ -857,9 +861,25 @@ static int pspgStep(const Params& P, double* pos, double* q, const double* dirVa
     std::vector<double> qPrev(q, q + n);
     std::vector<double> qIter(n, 0.0), qIterPrev(qPrev);
     std::vector<double> states(qPrev);  // node states (u, v, [w], p) used by tau
+    std::vector<double> dirEval;
+    if (dirAffine) {
+        dirEval.assign(dirVals, dirVals + D * N);
+        dirVals = dirEval.data();
+    }
+    auto evalBC = [&]() {
+        if (!dirAffine) return;
+        for (int64_t nn = 0; nn < N; ++nn)
+            if ((fl[nn] & BOUND) && (fl[nn] & DIRICHLET))
+                for (int d = 0; d < D; ++d) {
+                    double v = dirAffine[d * (D + 1) + D];
+                    for (int k = 0; k < D; ++k) v += dirAffine[d * (D + 1) + k] * pos[nn + k * N];
+                    dirEval[nn + d * N] = v;
+                }
+    };
     CsrSys S;
     auto restore = [&]() { std::copy(posSave.begin(), posSave.end(), pos); };
     auto t0 = clk::now();
+    evalBC();
     assembleAndBC<D>(P, fl, pos, qPrev.data(), states.data(), dirVals, S);  // prepare
     st.assemblies++;
     st.tAssemble += secs(t0, clk::now());
@@ -897,6 +917,7 @@ static int pspgStep(const Params& P, double* pos, double* q, const double* dirVa
         st.tUpdate += secs(tu, clk::now());
         auto ta = clk::now();
         S = CsrSys();
+        evalBC();
         assembleAndBC<D>(P, fl, pos, qPrev.data(), states.data(), dirVals, S);
         st.assemblies++;
         st.tAssemble += secs(ta, clk::now());
@@ -1313,7 +1334,7 @@ int orc_csr_bicgstab(int64_t n, const int* rp, const int* ci, const double* v, c
 int orc_pspg_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, double* pos, double* q,
                   const double* dir_vals, double rho, double mu, const double* g, double dt, int res_kind,
                   double min_res, int max_iter, int solver_kind, double rtol, int krylov_maxit, double* stats_out,
-                  int nthreads) {
+                  int nthreads, const double* dir_affine) {
     setThreads(nthreads);
     Params P{};
     P.dim = dim;
@@ -1327,8 +1348,8 @@ int orc_pspg_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t*
     for (int d = 0; d < 3; ++d) P.g[d] = d < dim ? g[d] : 0.0;
     SolveCfg cfg{solver_kind, rtol, krylov_maxit};
     StepStats st;
-    int s = (dim == 2) ? pspgStep<2>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st)
-                       : pspgStep<3>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st);
+    int s = (dim == 2) ? pspgStep<2>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st, dir_affine)
+                       : pspgStep<3>(P, pos, q, dir_vals, res_kind, min_res, max_iter, cfg, st, dir_affine);
     if (stats_out) {
         double o[14] = {(double)st.picardIters, (double)st.krylovIters, (double)st.assemblies, st.finalRes,
                         st.maxRelres,           st.tAssemble,           st.tSolve,          st.tUpdate,

@@ -22,7 +22,8 @@ from . import _lib
 from ._lib import PfemError, check, lib
 from .meshgen import BOUND, DIRICHLET, FIXED, FREE, ONFS, PfemMesh
 
-__all__ = ["Context", "Case", "MomContEqIncompNewton", "WCompNewtonStep", "batch_step", "Server", "now_ns",
+__all__ = ["Context", "Case", "MomContEqIncompNewton", "PicardAlgo", "WCompNewtonStep", "batch_step", "Server",
+           "now_ns",
            "PfemError", "PfemMesh", "BOUND", "FREE", "ONFS", "FIXED", "DIRICHLET"]
 
 
@@ -181,6 +182,13 @@ class Case:
                    allow_positive=True)
         return rc, x, st.as_dict()
 
+    def solve_device(self):
+        """pfem_pspg_solve with the device iterate as initial guess and result (nothing copied
+        to the host); returns (status, None, stats)."""
+        st = _lib.SolveStats()
+        rc = check(lib().pfem_pspg_solve(self._h, None, None, C.byref(st)), allow_positive=True)
+        return rc, None, st.as_dict()
+
     def csr(self):
         n = C.c_int64()
         nnz = C.c_int64()
@@ -201,6 +209,40 @@ class Case:
         y = np.empty_like(x)
         check(lib().pfem_spmv(self._h, _dp(x), _dp(y)))
         return y
+
+    # ---- the Picard loop's callbacks one by one (PicardAlgo.cpp:31-95) -----------------
+    def save_positions(self):
+        """Mesh::saveNodesList at the start of a step (node states = q_prev)."""
+        check(lib().pfem_save_positions(self._h))
+
+    def restore_positions(self):
+        """Mesh::restoreNodesList: saved positions, node states back to q_prev."""
+        check(lib().pfem_restore_positions(self._h))
+
+    def update_positions(self, dt, q=None):
+        """setNodesStatesfromQ + updateNodesPositionFromSave (q None: the device iterate)."""
+        qa = None if q is None else np.ascontiguousarray(q, dtype=np.float64)
+        check(lib().pfem_update_positions(self._h, dt, _dp(qa) if qa is not None else None))
+
+    def residual(self, kind=None, q_iter=None, q_iter_prev=None) -> float:
+        """Residual callback (kind "U", "U_P" or "Ax_f"; None: the case's); afterwards the
+        device's previous iterate is the current one."""
+        k = _lib.RES[kind] if kind is not None else self.desc.residual
+        qi = None if q_iter is None else np.ascontiguousarray(q_iter, dtype=np.float64)
+        qp = None if q_iter_prev is None else np.ascontiguousarray(q_iter_prev, dtype=np.float64)
+        out = C.c_double(0)
+        check(lib().pfem_residual(self._h, k, _dp(qi) if qi is not None else None,
+                                  _dp(qp) if qp is not None else None, C.byref(out)))
+        return out.value
+
+    def commit(self):
+        """q_prev = node states (the next step's qPrev)."""
+        check(lib().pfem_pspg_commit(self._h))
+
+    def iterate(self) -> np.ndarray:
+        q = np.empty((self.dim + 1) * self.N)
+        check(lib().pfem_get_iterate(self._h, _dp(q)))
+        return q
 
     # ---- time steps ---------------------------------------------------------------------
     def pspg_step(self, dt):
@@ -265,6 +307,69 @@ class MomContEqIncompNewton(Case):
         rc, st = self.pspg_step(dt)
         self.last_stats = st
         return rc == _lib.PFEM_OK
+
+
+class PicardAlgo:
+    """PicardAlgo::solve (nonLinearAlgo/PicardAlgo.cpp:31-95) on the host, driving a PSPG case
+    through the three callbacks of MomContEquationPSPG.inl:262-372 one C-ABI call at a time,
+    as the reference's own PicardAlgo would bind them:
+
+    * prepare: save the nodes, assemble and apply the BCs with qPrev;
+    * solve: solve, set the node states and move the nodes from the saved positions,
+      re-evaluate the Dirichlet values at the new positions (``dirichlet(pos)``, the Lua
+      "<Tag>V"(pos, t + dt) call of MomContEquationPSPG.inl:206-214), re-assemble;
+    * residual: the case's residual kind.
+
+    ``dirichlet`` (optional) maps the positions (dim x N) to the Dirichlet values (dim x N);
+    it is called before every assembly.  Returns True on success, False on numerical
+    failure (nodes restored), like the reference."""
+
+    def __init__(self, case: Case, max_iter: int | None = None, min_res: float | None = None, dirichlet=None):
+        self.case = case
+        self.max_iter = case.desc.max_iter if max_iter is None else max_iter
+        self.min_res = case.desc.min_res if min_res is None else min_res
+        self.dirichlet = dirichlet
+        self.iterations = 0
+        self.residuals = []
+
+    def _bc(self):
+        if self.dirichlet is not None:
+            self.case.set_dirichlet(self.dirichlet(self.case.get_positions()))
+
+    def solve(self, dt: float) -> bool:
+        c = self.case
+        self.residuals = []
+        # prepare
+        c.save_positions()
+        self._bc()
+        c.assemble(dt)
+        it = 0
+        res = float(np.finfo(np.float64).max)
+        while res > self.min_res:
+            if it > self.max_iter:
+                c.restore_positions()
+                self.iterations = it
+                return False
+            # solve callback
+            rc, _, _ = c.solve_device()
+            if rc != _lib.PFEM_OK:
+                c.restore_positions()
+                self.iterations = it
+                return False
+            c.update_positions(dt)
+            self._bc()
+            c.assemble(dt)
+            # residual callback
+            res = c.residual()
+            self.residuals.append(res)
+            if np.isnan(res):
+                c.restore_positions()
+                self.iterations = it + 1
+                return False
+            it += 1
+        c.commit()
+        self.iterations = it
+        return True
 
 
 class WCompNewtonStep(Case):

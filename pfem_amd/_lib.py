@@ -40,7 +40,8 @@ EXPORTS = [
     "pfem_case_footprint",
     "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd", "pfem_now_ns", "pfem_case_workspace_bytes",
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
-    "pfem_case_reserve_workspace", "pfem_case_release_workspace",
+    "pfem_case_reserve_workspace", "pfem_case_release_workspace", "pfem_save_positions", "pfem_restore_positions",
+    "pfem_update_positions", "pfem_residual", "pfem_pspg_commit", "pfem_get_iterate",
 ]
 
 
@@ -143,6 +144,12 @@ def lib():
         "pfem_server_stats": [_vp, C.POINTER(C.c_int64)],
         "pfem_case_reserve_workspace": [_vp],
         "pfem_case_release_workspace": [_vp],
+        "pfem_save_positions": [_vp],
+        "pfem_restore_positions": [_vp],
+        "pfem_update_positions": [_vp, C.c_double, _dp],
+        "pfem_residual": [_vp, C.c_int32, _dp, _dp, _dp],
+        "pfem_pspg_commit": [_vp],
+        "pfem_get_iterate": [_vp, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -61,6 +61,16 @@ void require_pspg(pfem_case* c) {
     require_mesh(c);
     PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG, PFEM_E_STATE, "not a PSPG case");
 }
+// The fused Picard step takes the Dirichlet values once per step; a Dirichlet node that
+// moves needs them re-evaluated at its new position before every re-assembly
+// (MomContEquationPSPG.inl:206-214), which the callback-level entry points allow.
+void require_static_dirichlet(pfem_case* c) {
+    PFEM_REQUIRE(!c->moving_dirichlet, PFEM_E_UNSUPPORTED,
+                 "the mesh has Dirichlet nodes that are not fixed: their values change with their positions at "
+                 "every Picard re-assembly; drive the step through pfem_save_positions / pfem_pspg_assemble / "
+                 "pfem_pspg_solve / pfem_update_positions / pfem_residual with pfem_case_set_dirichlet before each "
+                 "assembly");
+}
 void require_system(pfem_case* c) {
     require_pspg(c);
     PFEM_REQUIRE(c->ws.base != nullptr, PFEM_E_STATE, "no assembled system on this case (assemble first)");
@@ -325,6 +335,10 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         PFEM_HIP(hipSetDevice(c->ctx->device));
         c->has_mesh = false;
         c->ws.clear();
+        c->moving_dirichlet = false;
+        for (int32_t i = 0; i < N; ++i)
+            if ((flags[i] & PFEM_NODE_BOUND) && (flags[i] & PFEM_NODE_DIRICHLET) && !(flags[i] & PFEM_NODE_FIXED))
+                c->moving_dirichlet = true;
         c->N = N;
         c->E = E;
         c->n = (dim + 1) * N;
@@ -586,6 +600,7 @@ int pfem_spmv(pfem_case* c, const double* x, double* y) {
 int pfem_pspg_step(pfem_case* c, double dt, pfem_step_stats* stats) {
     return guarded([&] {
         require_pspg(c);
+        require_static_dirichlet(c);
         PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
         PFEM_HIP(hipSetDevice(c->ctx->device));
         Req r;
@@ -597,6 +612,88 @@ int pfem_pspg_step(pfem_case* c, double dt, pfem_step_stats* stats) {
             stats->status = rc;
         }
         return rc;
+    });
+}
+
+// ---- the Picard loop's callbacks one by one ----------------------------------------------
+namespace {
+int run_callback(pfem_case* c, unsigned ops, unsigned copy, double dt, int res_kind, double* res_out) {
+    Req r;
+    r.kind = pfem::RK_CALLBACK;
+    r.cb_ops = ops;
+    r.cb_copy = copy;
+    r.dt = dt;
+    r.res_kind = res_kind;
+    const int rc = run_direct(c, r);
+    if (res_out) *res_out = r.res;
+    return rc;
+}
+}  // namespace
+
+int pfem_save_positions(pfem_case* c) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        return run_callback(c, pfem::OP_COPY, pfem::COPY_PREP, 1.0, -1, nullptr);
+    });
+}
+
+int pfem_restore_positions(pfem_case* c) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        return run_callback(c, pfem::OP_COPY, pfem::COPY_RESTORE | pfem::COPY_RESET_X, 1.0, -1, nullptr);
+    });
+}
+
+int pfem_update_positions(pfem_case* c, double dt, const double* q) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        pfem::own_workspace(c, c->stream);
+        if (q)
+            PFEM_HIP(hipMemcpyAsync(c->ws.x, q, sizeof(double) * (size_t)c->n, hipMemcpyHostToDevice, c->stream));
+        return run_callback(c, pfem::OP_UPD, 0, dt, -1, nullptr);
+    });
+}
+
+int pfem_residual(pfem_case* c, int32_t kind, const double* q_iter, const double* q_iter_prev, double* out) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_REQUIRE(out != nullptr, PFEM_E_INVALID, "null output");
+        PFEM_REQUIRE(kind == PFEM_RES_U || kind == PFEM_RES_U_P || kind == PFEM_RES_AX_F, PFEM_E_INVALID,
+                     "unknown residual kind");
+        if (kind == PFEM_RES_AX_F) require_system(c);
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        pfem::own_workspace(c, c->stream);
+        const size_t nb = sizeof(double) * (size_t)c->n;
+        if (q_iter) PFEM_HIP(hipMemcpyAsync(c->ws.x, q_iter, nb, hipMemcpyHostToDevice, c->stream));
+        if (q_iter_prev) PFEM_HIP(hipMemcpyAsync(c->ws.x_prev, q_iter_prev, nb, hipMemcpyHostToDevice, c->stream));
+        double res = 0.0;
+        const int rc = run_callback(c, pfem::OP_RES, 0, 1.0, kind, &res);
+        *out = res;
+        return rc;
+    });
+}
+
+int pfem_pspg_commit(pfem_case* c) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        return run_callback(c, pfem::OP_COPY, pfem::COPY_COMMIT, 1.0, -1, nullptr);
+    });
+}
+
+int pfem_get_iterate(pfem_case* c, double* q) {
+    return guarded([&] {
+        require_pspg(c);
+        PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null output");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        pfem::own_workspace(c, c->stream);
+        PFEM_HIP(hipMemcpyAsync(q, c->ws.x, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
     });
 }
 
@@ -800,6 +897,7 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
         for (int i = 0; i < ncases; ++i) {
             pfem_case* c = cases[i];
             require_mesh(c);
+            if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
             PFEM_REQUIRE(c->ctx == ctx, PFEM_E_INVALID, "case belongs to another context");
             PFEM_REQUIRE(dts[i] > 0, PFEM_E_INVALID, "dt must be > 0");
             for (int j = 0; j < i; ++j) PFEM_REQUIRE(cases[j] != c, PFEM_E_INVALID, "a case is listed twice");
@@ -892,6 +990,7 @@ int pfem_server_submit(pfem_server* S, pfem_case* c, double dt, int64_t tag, int
         PFEM_REQUIRE(c->has_mesh, PFEM_E_STATE, "no mesh set on this case");
         PFEM_REQUIRE(c->ctx == S->ctx, PFEM_E_INVALID, "case belongs to another context");
         PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
         const size_t wb = c->ws.base ? 0 : pfem::workspace_bytes(c);
         PFEM_REQUIRE(wb <= S->arena.capacity(), PFEM_E_OOM, "the case's step workspace exceeds the server budget");
         int expected = 0;

@@ -42,22 +42,11 @@ size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 int lane_index(int lanes) { return lanes == 4 ? 0 : (lanes == 8 ? 1 : 2); }
 constexpr int kLanes[3] = {4, 8, 16};
 
-enum : unsigned {
-    OP_COPY = 1u << 0,
-    OP_UPD = 1u << 1,
-    OP_ASM = 1u << 2,
-    OP_RES = 1u << 3,
-    OP_IRU = 1u << 4,
-    OP_IRR = 1u << 5,
-    OP_K64_INIT = 1u << 6,
-    OP_K64_RUN = 1u << 7,
-    OP_K32_INIT = 1u << 8,
-    OP_K32_RUN = 1u << 9,
-    OP_WC = 1u << 10,
-    OP_CFL = 1u << 11,
-};
 
-enum : int { W_NONE = 0, W_K64 = 1, W_K32 = 2, W_IR = 3, W_POST = 4 };
+enum : int { W_NONE = 0, W_K64 = 1, W_K32 = 2, W_IR = 3, W_POST = 4, W_RESID = 5 };
+
+// residual kind of a request: its own (pfem_residual) or the case's
+int res_kind_of(const Req* r) { return r->res_kind >= 0 ? r->res_kind : r->c->d.residual; }
 
 enum Tab : int {
     T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
@@ -495,7 +484,7 @@ void Engine::Impl::load_views(Req* r) {
         pc.dim = dim;
         pc.N = N;
         pc.n = (int)n;
-        pc.res_kind = c->d.residual;
+        pc.res_kind = res_kind_of(r);
         pc.dt = r->dt;
         pc.flags = P.flags.as<uint8_t>();
         pc.pos = c->pos.as<double>();
@@ -841,6 +830,17 @@ void solve_done(Engine::Impl& I, Req* r, int status) {
     r->wait = W_POST;
 }
 
+// The Picard residual (MomContEquationPSPG.inl:314-372) from the sums the residual kernels
+// leave in the slot: Ax_f ||A q - b||; U sqrt(du^2 / u_prev^2) over non-free nodes, DBL_MAX
+// for a zero denominator; U_P the larger of that and the pressure ratio.
+double picard_residual(int kind, const double* o) {
+    if (kind == PFEM_RES_AX_F) return std::sqrt(o[0]);
+    const double big = std::numeric_limits<double>::max();
+    const double rv = (o[1] == 0) ? big : std::sqrt(o[0] / o[1]);
+    const double rp = (kind == PFEM_RES_U_P) ? ((o[3] == 0) ? big : std::sqrt(o[2] / o[3])) : 0.0;
+    return std::max(rv, rp);
+}
+
 // Chunk-size hint from the residual decay between two readbacks.
 int estimate_remaining(Req* r, const KScal& k) {
     int est = 8;
@@ -930,6 +930,12 @@ void Engine::tick(std::vector<Req*>& done) {
                 r->ops = (r->wc_step ? OP_WC : 0u) | (r->wc_cfl ? OP_CFL : 0u);
                 r->final_ = true;
                 break;
+            case RK_CALLBACK:
+                r->ops = r->cb_ops;
+                r->copy = r->cb_copy;
+                if (r->ops & OP_RES) r->wait = W_RESID;
+                else r->final_ = true;
+                break;
         }
         inflight_.push_back(r);
     }
@@ -969,7 +975,7 @@ void Engine::tick(std::vector<Req*>& done) {
             append_rows(tab[T_PRE], sl, c->n, c->pat.ell_nsl);
         }
         if (op & OP_RES) {
-            if (c->d.residual == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n, c->pat.ell_nsl);
+            if (res_kind_of(r) == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n, c->pat.ell_nsl);
             else append_rows(tab[T_RESU], sl, c->n, c->pat.ell_nsl);
         }
         if (op & OP_IRU) append_rows(tab[T_IRU], sl, c->n, c->pat.ell_nsl);
@@ -1269,18 +1275,13 @@ void Engine::tick(std::vector<Req*>& done) {
                     }
                     break;
                 }
+                case W_RESID: {  // pfem_residual: the value, then x_prev = x (PicardAlgo.cpp:72-73)
+                    r->res = picard_residual(res_kind_of(r), h.out);
+                    finish(r, PFEM_OK, COPY_SAVEIT);
+                    break;
+                }
                 case W_POST: {  // residual callback (MomContEquationPSPG.inl:314-372)
-                    const double* o = h.out;
-                    double res;
-                    if (c->d.residual == PFEM_RES_AX_F) {
-                        res = std::sqrt(o[0]);
-                    } else {
-                        const double big = std::numeric_limits<double>::max();
-                        const double rv = (o[1] == 0) ? big : std::sqrt(o[0] / o[1]);
-                        const double rp =
-                            (c->d.residual == PFEM_RES_U_P) ? ((o[3] == 0) ? big : std::sqrt(o[2] / o[3])) : 0.0;
-                        res = std::max(rv, rp);
-                    }
+                    const double res = picard_residual(res_kind_of(r), h.out);
                     r->res = res;
                     r->st.residual = res;
                     if (std::isnan(res)) {

@@ -45,6 +45,7 @@ __global__ __launch_bounds__(kBlock) void k_pic_copy(const PicardView* __restric
                 V.x_prev[q] = v;
             }
             if (op & COPY_ZERO_X) V.x[q] = 0.0;
+            if (op & COPY_RESET_X) V.x[q] = V.q_prev[q];
             if (op & COPY_SAVEIT) V.x_prev[q] = V.x[q];
             if (op & COPY_COMMIT) V.q_prev[q] = V.x[q];
         }

@@ -103,7 +103,25 @@ private:
     std::map<size_t, size_t> free_;  // offset -> bytes
 };
 
-enum ReqKind : int { RK_ASSEMBLE = 0, RK_SOLVE = 1, RK_STEP = 2, RK_WC = 3 };
+// Operations a request asks of the next engine tick (Req::ops).
+enum : unsigned {
+    OP_COPY = 1u << 0,
+    OP_UPD = 1u << 1,
+    OP_ASM = 1u << 2,
+    OP_RES = 1u << 3,
+    OP_IRU = 1u << 4,
+    OP_IRR = 1u << 5,
+    OP_K64_INIT = 1u << 6,
+    OP_K64_RUN = 1u << 7,
+    OP_K32_INIT = 1u << 8,
+    OP_K32_RUN = 1u << 9,
+    OP_WC = 1u << 10,
+    OP_CFL = 1u << 11,
+};
+
+// RK_CALLBACK: one of the Picard loop's device operations on its own (node save / restore,
+// position update, residual, commit), as the reference's callbacks call them one by one.
+enum ReqKind : int { RK_ASSEMBLE = 0, RK_SOLVE = 1, RK_STEP = 2, RK_WC = 3, RK_CALLBACK = 4 };
 
 struct Req {
     pfem_case* c = nullptr;
@@ -112,6 +130,8 @@ struct Req {
     int64_t tag = 0;
     bool wc_step = true, wc_cfl = false;
     double sc = 0.5, max_dt = 0;
+    int res_kind = -1;                    // residual kind of this request (-1: the case's)
+    unsigned cb_ops = 0, cb_copy = 0;     // RK_CALLBACK: the operations and copies to run
     // results
     int status = PFEM_OK;
     pfem_step_stats st{};
@@ -203,6 +223,9 @@ struct pfem_case {
     pfem::Workspace ws;
     pfem::AmgHierarchy amg;  // levels == 0: block-Jacobi preconditioning
     int lanes = 8;
+    // a node flagged boundary + Dirichlet but not fixed: its position, hence its Dirichlet
+    // value, changes at every Picard re-assembly (MomContEquationPSPG.inl:206-214)
+    bool moving_dirichlet = false;
     std::atomic<int> busy{0};  // a serving request holds the case
     std::unique_ptr<pfem::Engine> engine;  // direct calls
     ~pfem_case();

@@ -250,6 +250,7 @@ enum : int {
     COPY_COMMIT = 4,   // q_prev = x                                   (converged step)
     COPY_RESTORE = 8,  // pos = pos_save                               (Mesh::restoreNodesList)
     COPY_ZERO_X = 16,  // x = 0                                        (restart from zero)
+    COPY_RESET_X = 32, // x = q_prev                                   (node states restored)
 };
 
 // Scalars of one case in an engine slot, read back by the host once per tick.

@@ -402,3 +402,42 @@ def test_minmax_matches_brute_force_and_keeps_the_zero_start():
     shifted = m.pos - 10.0
     assert O.minmax(m, shifted, 1, "max") == 0.0
     assert O.minmax(m, shifted, 1, "min") == shifted[1][keep].min()
+
+
+def moving_wall_mesh():
+    """The 2D dam break with its left wall nodes un-fixed: their Dirichlet value depends on
+    where they are, so it has to be re-evaluated at every Picard re-assembly."""
+    m = mg.dambreak_2d(h=0.1)
+    fl = m.flags.copy()
+    left = ((fl & mg.BOUND) != 0) & (m.pos[0] < 0.5 * m.h)
+    fl[left] &= ~np.uint8(mg.FIXED)
+    m.flags = fl
+    return m, left
+
+
+def test_picard_position_dependent_dirichlet_fixed_point():
+    """v_bc(x) = A x + c re-evaluated before every assembly: at convergence each moving
+    Dirichlet node's velocity is the BC at the position that velocity moves it to, i.e.
+    v = A (x0 + dt v) + c  ->  v = (I - dt A)^-1 (A x0 + c), while evaluating at the saved
+    positions would give A x0 + c."""
+    m, left = moving_wall_mesh()
+    N = m.n_nodes
+    A = np.array([[0.5, 0.4], [-0.3, 0.2]])
+    c = np.array([0.01, -0.02])
+    aff = np.hstack([A, c[:, None]])
+    q = mg.pspg_hydrostatic(m)
+    pos = m.pos.reshape(-1).copy()
+    dt = 1e-3
+    st, info = O.pspg_step(m, pos, q, dt=dt, residual="U", min_res=1e-12, max_iter=30, solver_kind=2, rtol=1e-13,
+                           dir_affine=aff)
+    assert st == 0, info
+    x0 = m.pos[:, left]
+    v = q.reshape(3, N)[:2, left]
+    expect = np.linalg.solve(np.eye(2) - dt * A, A @ x0 + c[:, None])
+    np.testing.assert_allclose(v, expect, rtol=0, atol=1e-9 * np.abs(expect).max())
+    assert np.abs(expect - (A @ x0 + c[:, None])).max() > 1e-4 * np.abs(expect).max()
+    # the moving wall nodes moved by dt v; the fixed ones stayed
+    P = pos.reshape(2, N)
+    np.testing.assert_allclose(P[:, left], x0 + dt * v, rtol=0, atol=1e-15)
+    fixed = (m.flags & mg.FIXED) != 0
+    np.testing.assert_array_equal(P[:, fixed], m.pos[:, fixed])
