@@ -230,6 +230,17 @@ int pfem_case_mass(pfem_case* c, double* mass);
  * or largest coordinate `coordinate` (0 .. dim-1) over the nodes that are neither bound nor
  * free; the largest starts from 0, as the reference's does. */
 int pfem_case_minmax(pfem_case* c, int32_t coordinate, int32_t want_max, double* value);
+/* No counterpart in the reference: pfem_case_mass for callers that cannot allocate or
+ * synchronise (HIP graph capture, tight serving loops).  The same kernels, with the
+ * per-workgroup partial sums in a fixed 16 KiB device array of the library (2,048 doubles per
+ * GPU, held until the process ends; the element loop is grid-strided over at most 2,048
+ * workgroups, so every mesh fits) instead of the case's buffer, and the sum written to the
+ * device double d_mass (added in the same order as pfem_case_mass, so the same bits).
+ * Enqueued on the case's stream without waiting: read d_mass after pfem_case_sync or a
+ * later operation on that stream.  Allocates nothing.  Every call on one GPU shares the
+ * array: calls that overlap on one GPU (cases on different streams at the same time)
+ * race. */
+int pfem_case_mass_noalloc(pfem_case* c, double* d_mass);
 
 /* ---- batched serving (no counterpart in the reference: many independent cases) -------
  * Runs one time step of every listed case as one batch: each operation of the cases'

@@ -41,7 +41,7 @@ EXPORTS = [
     "pfem_profile_spmv", "pfem_bench_kernels", "pfem_probe_xcd", "pfem_now_ns", "pfem_case_workspace_bytes",
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
     "pfem_case_reserve_workspace", "pfem_case_release_workspace", "pfem_save_positions", "pfem_restore_positions",
-    "pfem_update_positions", "pfem_residual", "pfem_pspg_commit", "pfem_get_iterate",
+    "pfem_update_positions", "pfem_residual", "pfem_pspg_commit", "pfem_get_iterate", "pfem_case_mass_noalloc",
 ]
 
 
@@ -150,6 +150,7 @@ def lib():
         "pfem_residual": [_vp, C.c_int32, _dp, _dp, _dp],
         "pfem_pspg_commit": [_vp],
         "pfem_get_iterate": [_vp, _dp],
+        "pfem_case_mass_noalloc": [_vp, _vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

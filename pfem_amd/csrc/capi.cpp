@@ -253,6 +253,7 @@ int pfem_ctx_create(int32_t device, pfem_ctx** out) {
         PFEM_HIP(hipGetDeviceCount(&count));
         PFEM_REQUIRE(device >= 0 && device < count, PFEM_E_INVALID, "device index out of range");
         PFEM_HIP(hipSetDevice(device));
+        pfem::init_scratch(device);
         auto ctx = std::make_unique<pfem_ctx>();
         ctx->device = device;
         PFEM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -756,6 +757,27 @@ int pfem_case_mass(pfem_case* c, double* mass) {
         double m = 0.0;
         for (double v : h) m += v;
         *mass = m;
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_mass_noalloc(pfem_case* c, double* d_mass) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(d_mass != nullptr, PFEM_E_INVALID, "null output");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        const int nb = (int)std::min<long long>(((long long)c->E + pfem::kBlock - 1) / pfem::kBlock, pfem::kMaxCaseBlocks);
+        PFEM_REQUIRE(nb <= pfem::kScratchPartials, PFEM_E_INVALID,
+                     "more workgroups than the scratch holds partials (" + std::to_string(pfem::kScratchPartials) + ")");
+        double* part = pfem::scratch_partials(c->ctx->device);
+        const void* rho = nullptr;
+        const bool fp32 = c->d.wc_state == PFEM_STATE_FP32;
+        if (c->d.kind == PFEM_KIND_WC)
+            rho = static_cast<const char*>(c->wstate.p) + wc_state_size(c) * (size_t)(c->dim + 1) * c->N;
+        pfem::launch_mass(c->dim, c->N, c->E, c->pat.conn.as<int>(), c->pos.as<double>(), rho,
+                          c->d.kind == PFEM_KIND_WC && fp32 ? 1 : 0, part, nb, c->stream);
+        pfem::launch_sum_partials(part, nb, d_mass, c->stream);
+        PFEM_HIP(hipGetLastError());
         return PFEM_OK;
     });
 }

@@ -2,13 +2,26 @@
 // largest coordinate of the fluid nodes, reduced on the device.  Each workgroup writes one
 // partial; the host adds them in workgroup order, so a result does not vary between calls.
 #include <cfloat>
+#include <mutex>
 
 #include "common.hpp"
+#include "hostutil.hpp"
 #include "p1geom.hpp"
+#include "launch.hpp"
 
 namespace pfem {
 
+// Per-workgroup partial sums of the allocation-free mass entry point
+// (pfem_case_mass_noalloc): a fixed device array, one per GPU for the life of the process.
+// Host code never passes `scratch` itself to a kernel (its host-side name is not a device
+// address); it passes the per-device address that hipGetSymbolAddress returned
+// (scratch_partials).  Every call on one GPU shares it.
+static __device__ __align__(256) double scratch[kScratchPartials];
+
 namespace {
+
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
 
 // Sum over elements of |e| (rho null) or of |e| times the mean density of the element's
 // nodes (IncompNewton/Problem.cpp:173-190, WCompNewton/Problem.cpp:216-240).
@@ -61,7 +74,40 @@ __global__ __launch_bounds__(kBlock) void k_minmax(int N, const uint8_t* __restr
     }
 }
 
+// The partials of one k_mass launch added in workgroup order by one thread: the same
+// sequence of additions as pfem_case_mass makes on the host, so the same bits.
+__global__ void k_sum_partials(const double* __restrict__ partials, int n, double* __restrict__ out) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    double s = 0.0;
+    for (int i = 0; i < n; ++i) s += partials[i];
+    *out = s;
+}
+
 }  // namespace
+
+void init_scratch(int device) {
+    PFEM_REQUIRE(device >= 0 && device < kMaxDevices, E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return;
+    int cur = -1;
+    PFEM_HIP(hipGetDevice(&cur));
+    PFEM_REQUIRE(cur == device, E_STATE, "init_scratch needs its device to be current");
+    void* p = nullptr;
+    PFEM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    PFEM_REQUIRE(p != nullptr, E_HIP, "no device address for the scratch array");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+}
+
+double* scratch_partials(int device) {
+    PFEM_REQUIRE(device >= 0 && device < kMaxDevices, E_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    PFEM_REQUIRE(g_scratch_ptr[device] != nullptr, E_STATE, "scratch of this device not set up (pfem_ctx_create)");
+    return g_scratch_ptr[device];
+}
+
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s) {
+    k_sum_partials<<<1, 64, 0, s>>>(partials, n, out);
+}
 
 void launch_mass(int dim, int N, int E, const int* conn, const double* pos, const void* rho, int rho_fp32,
                  double* partials, int nblocks, hipStream_t s) {

@@ -66,6 +66,16 @@ void launch_mass(int dim, int N, int E, const int* conn, const double* pos, cons
                  double* partials, int nblocks, hipStream_t s);
 void launch_minmax(int N, const uint8_t* flags, const double* x, int want_max, double* partials, int nblocks,
                    hipStream_t s);
+// out[0] = partials[0] + ... + partials[n-1], added in that order (one thread).
+void launch_sum_partials(const double* partials, int n, double* out, hipStream_t s);
+
+// The fixed device scratch of pfem_case_mass_noalloc: kScratchPartials doubles per GPU.
+// init_scratch resolves its address on `device` (which must be current), once per device;
+// scratch_partials returns that address.
+constexpr int kScratchPartials = 2048;
+constexpr int kMaxDevices = 64;
+void init_scratch(int device);
+double* scratch_partials(int device);
 
 void launch_stream_read(const double* a, long long n, double* out, int nblocks, hipStream_t s);
 void launch_copy(const double* a, double* b, long long n, int nblocks, hipStream_t s);
