@@ -40,30 +40,6 @@ __device__ __forceinline__ T in_at(const AmgView<T>& V, int which, long long q) 
     return V.level == 0 ? (T)V.r0[which][q] : V.r[q];
 }
 
-// Invert the nf x nf block m (nf <= 4; rows/cols >= nf padded with identity) by
-// Gauss-Jordan with partial pivoting; the inverse is left in m[.][4..7].
-__device__ __forceinline__ void invert4(double (&m)[4][8]) {
-    for (int c = 0; c < 4; ++c) {
-        int piv = c;
-        for (int r = c + 1; r < 4; ++r)
-            if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
-        if (piv != c)
-            for (int k = 0; k < 8; ++k) {
-                const double tmp = m[c][k];
-                m[c][k] = m[piv][k];
-                m[piv][k] = tmp;
-            }
-        const double inv = m[c][c] != 0.0 ? 1.0 / m[c][c] : 0.0;
-        for (int k = 0; k < 8; ++k) m[c][k] *= inv;
-        for (int r = 0; r < 4; ++r) {
-            if (r == c) continue;
-            const double fct = m[r][c];
-            if (fct != 0.0)
-                for (int k = 0; k < 8; ++k) m[r][k] -= fct * m[c][k];
-        }
-    }
-}
-
 }  // namespace
 
 // ---- setup --------------------------------------------------------------------------------
@@ -150,14 +126,23 @@ __global__ __launch_bounds__(kBlock) void k_amg_bjac(const AmgView<T>* __restric
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int t = 0; t < 8; ++t) m[s][t] = (t - 4 == s) ? 1.0 : 0.0;
-        for (int s = 0; s < nf; ++s) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s >= nf) {
+                m[s][s] = 1.0;  // padding rows: identity
+                continue;
+            }
             const long long rs = row_start_of(inf, s, D, V.A.mom_nnz) + self;
-            for (int t = 0; t < nf; ++t) m[s][t] = (double)V.A.vals[rs + (long long)t * inf.z];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (t < nf) m[s][t] = (double)V.A.vals[rs + (long long)t * inf.z];
         }
-        for (int s = nf; s < 4; ++s) m[s][s] = 1.0;
-        invert4(m);
-        for (int s = 0; s < nf; ++s)
-            for (int t = 0; t < nf; ++t) V.binv_out[(long long)(s * nf + t) * N + I] = (T)m[s][4 + t];
+        gj_invert4<true>(m);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (s < nf && t < nf) V.binv_out[(long long)(s * nf + t) * N + I] = (T)m[s][4 + t];
     }
 }
 

@@ -336,6 +336,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         PFEM_HIP(hipSetDevice(c->ctx->device));
         c->has_mesh = false;
         c->ws.clear();
+        c->cfl_valid = false;
         c->moving_dirichlet = false;
         for (int32_t i = 0; i < N; ++i)
             if ((flags[i] & PFEM_NODE_BOUND) && (flags[i] & PFEM_NODE_DIRICHLET) && !(flags[i] & PFEM_NODE_FIXED))
@@ -369,7 +370,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             const size_t ss = wc_state_size(c);
             c->wstate.reserve(ss * (size_t)(2 * dim + 2) * N);
             PFEM_HIP(hipMemsetAsync(c->wstate.p, 0, ss * (size_t)(2 * dim + 2) * N, c->stream));
-            c->wacc.reserve(sizeof(double) * (size_t)(3 + dim) * N);
+            c->wacc.reserve(sizeof(double) * (size_t)(4 + dim) * N);
             c->wrec.reserve(sizeof(double) * 8 * (size_t)N);
             c->werec.reserve(sizeof(double) * 4 * (size_t)E);
             PFEM_HIP(hipMemsetAsync(c->wrec.p, 0, sizeof(double) * 8 * (size_t)N, c->stream));
@@ -384,6 +385,7 @@ int pfem_case_set_state(pfem_case* c, const double* q) {
     return guarded([&] {
         require_mesh(c);
         PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null state");
+        c->cfl_valid = false;
         PFEM_HIP(hipSetDevice(c->ctx->device));
         if (c->d.kind == PFEM_KIND_PSPG) {
             const size_t nb = sizeof(double) * (size_t)c->n;
@@ -431,6 +433,7 @@ int pfem_case_set_positions(pfem_case* c, const double* pos) {
     return guarded([&] {
         require_mesh(c);
         PFEM_REQUIRE(pos != nullptr, PFEM_E_INVALID, "null positions");
+        c->cfl_valid = false;
         PFEM_HIP(hipMemcpyAsync(c->pos.p, pos, sizeof(double) * (size_t)c->dim * c->N, hipMemcpyHostToDevice,
                                 c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
@@ -709,6 +712,7 @@ int pfem_wc_step(pfem_case* c, double dt, pfem_step_stats* stats) {
         r.dt = dt;
         r.wc_step = true;
         r.wc_cfl = false;
+        r.sc = c->cfl_sc;  // the CFL minimum after the step is kept for this securityCoeff
         const int rc = run_direct(c, r);
         if (stats) {
             *stats = pfem_step_stats{};
@@ -723,6 +727,14 @@ int pfem_wc_next_dt(pfem_case* c, double security_coeff, double max_dt, double* 
     return guarded([&] {
         require_mesh(c);
         PFEM_REQUIRE(c->d.kind == PFEM_KIND_WC && dt_out, PFEM_E_INVALID, "not a WC case or null output");
+        PFEM_REQUIRE(security_coeff > 0, PFEM_E_INVALID, "securityCoeff must be > 0");
+        if (c->cfl_valid && c->cfl_sc == security_coeff) {
+            // the last step already reduced the CFL minimum of this state
+            const double dt = std::min(std::sqrt(c->cfl_dt2), max_dt);
+            PFEM_REQUIRE(!std::isnan(dt), PFEM_NAN, "NaN time step");
+            *dt_out = dt;
+            return PFEM_OK;
+        }
         PFEM_HIP(hipSetDevice(c->ctx->device));
         Req r;
         r.kind = pfem::RK_WC;
@@ -930,6 +942,7 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
             r.kind = c->d.kind == PFEM_KIND_PSPG ? pfem::RK_STEP : pfem::RK_WC;
             r.wc_step = true;
             r.wc_cfl = false;
+            r.sc = c->cfl_sc;
         }
         if (!ctx->engine) ctx->engine = std::make_unique<pfem::Engine>(ctx->device, ctx->stream, std::max(1, ncases),
                                                                        nullptr);

@@ -28,6 +28,38 @@ struct BlockTask {
     int pad;
 };
 
+// XCD-aware work split: workgroups are dispatched round-robin over the kXcd XCDs; the
+// workgroups of a case that land on one XCD own one contiguous range of its `nodes` items,
+// so neighbouring items (and the data they share) stay in that XCD's L2.
+struct Seg {
+    int n0;   // first node of the range
+    int ns;   // nodes in the range
+    int lb;   // index of this workgroup among the case's workgroups on the range
+    int cnt;  // workgroups of the case on the range
+};
+
+__device__ __forceinline__ Seg xcd_segment(const BlockTask& t, int nodes) {
+    int seg, nseg, lb, cnt;
+    if (t.nblocks < kXcd) {
+        seg = t.block;
+        nseg = t.nblocks;
+        lb = 0;
+        cnt = 1;
+    } else {
+        const int g0 = (int)blockIdx.x - t.block;  // global id of the case's first workgroup
+        seg = ((int)blockIdx.x % kXcd - g0 % kXcd + kXcd) % kXcd;  // == t.block % kXcd
+        nseg = kXcd;
+        lb = (t.block - seg) / kXcd;
+        cnt = (t.nblocks - seg + kXcd - 1) / kXcd;
+    }
+    Seg s;
+    s.n0 = (int)((long long)nodes * seg / nseg);
+    s.ns = (int)((long long)nodes * (seg + 1) / nseg) - s.n0;
+    s.lb = lb;
+    s.cnt = cnt;
+    return s;
+}
+
 // Node flag bits (PFEM_NODE_* in include/pfem_gpu.h).
 enum : uint8_t { NODE_BOUND = 1, NODE_FREE = 2, NODE_ONFS = 4, NODE_FIXED = 8, NODE_DIRICHLET = 16 };
 
@@ -59,6 +91,18 @@ __device__ __forceinline__ T group_sum(T v) {
     return v;
 }
 
+// Smallest value over each aligned group of L lanes, in every lane of the group (same lane
+// rules as group_sum).
+template <int L>
+__device__ __forceinline__ double group_min(double v) {
+    static_assert(L == 4 || L == 8 || L == 16, "group width");
+    v = fmin(v, dpp_mov<0xB1>(v));
+    v = fmin(v, dpp_mov<0x4E>(v));
+    if (L >= 8) v = fmin(v, dpp_mov<0x141>(v));
+    if (L >= 16) v = fmin(v, dpp_mov<0x140>(v));
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -75,6 +119,65 @@ __device__ __forceinline__ double wave_min(double v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) v = fmin(v, __shfl_xor(v, o, 64));
     return v;
+}
+
+// Gauss-Jordan inversion of a 4 x 4 block with partial pivoting on [M | I] (m[r][0..3] = M,
+// m[r][4..7] = I on entry; the inverse in m[r][4..7] on exit).  Fully unrolled, the pivot
+// row swap done with selects, so the block stays in registers (a swap indexed by the
+// run-time pivot row put it in scratch memory).  ZERO_SAFE: a zero pivot gives a zero row
+// scale instead of an infinity.
+template <bool ZERO_SAFE>
+__device__ __forceinline__ void gj_invert4(double (&m)[4][8]) {
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        int piv = c;
+        double best = fabs(m[c][c]);
+#pragma unroll
+        for (int r = c + 1; r < 4; ++r) {
+            const double a = fabs(m[r][c]);
+            if (a > best) {
+                best = a;
+                piv = r;
+            }
+        }
+#pragma unroll
+        for (int r = c + 1; r < 4; ++r) {
+            const bool sw = (r == piv);
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                const double a = m[c][k], b = m[r][k];
+                m[c][k] = sw ? b : a;
+                m[r][k] = sw ? a : b;
+            }
+        }
+        const double inv = ZERO_SAFE ? (m[c][c] != 0.0 ? 1.0 / m[c][c] : 0.0) : 1.0 / m[c][c];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) m[c][k] *= inv;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            if (r == c) continue;
+            const double fct = m[r][c];
+            if (fct != 0.0)
+#pragma unroll
+                for (int k = 0; k < 8; ++k) m[r][k] -= fct * m[c][k];
+        }
+    }
+}
+
+// Smallest positive value over the workgroup folded into *addr (the bits of a positive
+// double, compared as integers): one atomic per workgroup, since every atomic on one address
+// is serialised (one per wavefront cost ~10 ns each, 0.3 ms over 2M nodes).
+__device__ __forceinline__ void block_min_to(unsigned long long* addr, double v) {
+    __shared__ double red[kWaves];
+    v = wave_min(v);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = red[0];
+#pragma unroll
+        for (int w = 1; w < kWaves; ++w) m = fmin(m, red[w]);
+        atomicMin(addr, (unsigned long long)__double_as_longlong(m));
+    }
 }
 
 // Sum NV values over the workgroup; the result is valid in thread 0.

@@ -154,7 +154,7 @@ int pick_lanes(double avg_deg) {
 // ---------------------------------------------------------------------------------------
 namespace {
 struct WsLayout {
-    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32_ell, vals32, binv32, kv32, total;
+    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32_ell, vals32, binv32, kv32, erec, total;
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
     size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
@@ -194,6 +194,7 @@ WsLayout ws_layout(const pfem_case* c) {
     L.pos_save = take(8 * (size_t)c->dim * N);
     L.kv64 = take(8 * 8 * n);
     L.vals_ell = take(8 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
+    L.erec = take(8 * (size_t)(c->dim == 2 ? 8 : 16) * (size_t)c->E);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     if (ir || amg_fp32(c)) {  // FP32 copies of the values and of the block-Jacobi inverse
         L.vals32 = take(4 * nnz);
@@ -242,6 +243,7 @@ void carve_workspace(pfem_case* c, char* base) {
     w.pos_save = reinterpret_cast<double*>(base + L.pos_save);
     w.kv64 = reinterpret_cast<double*>(base + L.kv64);
     w.vals_ell = reinterpret_cast<double*>(base + L.vals_ell);
+    w.erec = reinterpret_cast<double*>(base + L.erec);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     const bool v32 = ir || amg_fp32(c);
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
@@ -473,6 +475,7 @@ void Engine::Impl::load_views(Req* r) {
         pv.vals32_ell = w.vals32_ell;
         pv.binv = w.binv;
         pv.binv32 = w.binv32;
+        pv.erec = w.erec;
         pv.nnz = P.nnz;
         pv.rho = c->d.rho;
         pv.mu = c->d.mu;
@@ -597,6 +600,7 @@ void Engine::Impl::load_views(Req* r) {
             V.f_cont = acc + N;
             V.m_mom = acc + 2 * (size_t)N;
             V.f_mom = acc + 3 * (size_t)N;
+            V.hmin = acc + (3 + (size_t)dim) * N;
             V.rec = c->wrec.as<double>();
             V.erec = c->werec.as<double>();
             V.dt2 = &ds->dt2;
@@ -1015,13 +1019,17 @@ void Engine::tick(std::vector<Req*>& done) {
         }
         const int wi = c->d.wc_state == PFEM_STATE_FP32 ? 1 : 0;
         if (op & OP_WC) {
-            append(tab[T_WCN + wi], sl, c->N, kElemCap);
+            // node passes: at most kMaxCaseBlocks workgroups, each folding its CFL minimum
+            // into the slot with one atomic
+            append(tab[T_WCN + wi], sl, c->N, kMaxCaseBlocks);
             // node gathers over the incident elements: kWcLanes lanes per node
             append(tab[(c->dim == 2 ? T_WCE2 : T_WCE3) + wi], sl, (long long)c->N * kWcLanes[c->dim - 2], kElemCap);
             append(tab[(c->dim == 2 ? T_WEL2 : T_WEL3) + wi], sl, c->E, kElemCap);
         }
-        if (op & OP_CFL) {
-            append(tab[(c->dim == 2 ? T_CFE2 : T_CFE3) + wi], sl, c->E, kElemCap);
+        // the step's own kernels give the CFL minimum after it; the CFL pass runs on its own
+        // only without a step
+        if ((op & OP_CFL) && !(op & OP_WC)) append(tab[(c->dim == 2 ? T_CFE2 : T_CFE3) + wi], sl, c->E, kElemCap);
+        if (op & (OP_WC | OP_CFL)) {
             SlotScal& h = I.h_reset[sl];
             h.dt2 = 0x7fefffffffffffffULL;  // DBL_MAX
             PFEM_HIP(hipMemcpyAsync(&I.dslot(sl)->dt2, &h.dt2, sizeof(h.dt2), hipMemcpyHostToDevice, s));
@@ -1303,10 +1311,15 @@ void Engine::tick(std::vector<Req*>& done) {
             }
         }
         if (r->final_) {
-            if (r->kind == RK_WC && r->wc_cfl) {
+            if (r->kind == RK_WC) {
                 double dt2;
                 std::memcpy(&dt2, &I.h_scal[r->slot].dt2, sizeof(dt2));
-                r->next_dt = std::min(std::sqrt(dt2), r->max_dt);
+                if (r->wc_cfl) r->next_dt = std::min(std::sqrt(dt2), r->max_dt);
+                // the CFL minimum of the case's current state for securityCoeff r->sc, kept
+                // for a computeNextDT that follows before the state changes
+                c->cfl_dt2 = dt2;
+                c->cfl_sc = r->sc;
+                c->cfl_valid = true;
             }
             r->t_done = now_ns();
             r->completed = true;

@@ -31,36 +31,6 @@
 
 namespace pfem {
 
-// The node range and the stride over it owned by this workgroup.
-struct Seg {
-    int n0;   // first node of the range
-    int ns;   // nodes in the range
-    int lb;   // index of this workgroup among the case's workgroups on the range
-    int cnt;  // workgroups of the case on the range
-};
-
-__device__ __forceinline__ Seg xcd_segment(const BlockTask& t, int nodes) {
-    int seg, nseg, lb, cnt;
-    if (t.nblocks < kXcd) {
-        seg = t.block;
-        nseg = t.nblocks;
-        lb = 0;
-        cnt = 1;
-    } else {
-        const int g0 = (int)blockIdx.x - t.block;  // global id of the case's first workgroup
-        seg = ((int)blockIdx.x % kXcd - g0 % kXcd + kXcd) % kXcd;  // == t.block % kXcd
-        nseg = kXcd;
-        lb = (t.block - seg) / kXcd;
-        cnt = (t.nblocks - seg + kXcd - 1) / kXcd;
-    }
-    Seg s;
-    s.n0 = (int)((long long)nodes * seg / nseg);
-    s.ns = (int)((long long)nodes * (seg + 1) / nseg) - s.n0;
-    s.lb = lb;
-    s.cnt = cnt;
-    return s;
-}
-
 // Local row index l in [0, nf*ns) -> global row (field-blocked layout).
 __device__ __forceinline__ long long seg_row(const Seg& s, int nodes, long long l) {
     const long long f = l / s.ns;

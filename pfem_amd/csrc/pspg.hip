@@ -7,8 +7,9 @@
 // by element index, so the element contributions of an entry are summed in the same
 // order as Eigen's duplicate sum of the reference's triplet list).  Each element's
 // geometry (J, detJ, inverse J; Element.cpp:15-135), its P1 shape-function gradients
-// (MatricesBuilder.inl:93-128) and tau (MomContEquationPSPG.inl:238-259) are recomputed
-// in registers for every entry it touches; the entries of Ae (MomContEquationPSPG.inl:42)
+// (MatricesBuilder.inl:93-128) and tau (MomContEquationPSPG.inl:238-259) are evaluated
+// once per assembly by an element pass into a record the gathers read; the entries of Ae
+// (MomContEquationPSPG.inl:42)
 // follow from closed forms of the P1 element matrices, which the reference's Gauss rules
 // integrate exactly:
 //   M_ij  = rho |e| (1 + d_ij) / ((d+1)(d+2))                       getM   (MatricesBuilder.inl:217)
@@ -106,30 +107,141 @@ __device__ __forceinline__ void element_eval(const PspgView& V, const int (&nd)[
     G.tau = 1 / sqrt((2 / dt) * (2 / dt) + (2 * U / h) * (2 * U / h) + 9 * visc * visc);
 }
 
+// Element records: one pass evaluates every element once per assembly and leaves, for the
+// pair and diagonal gathers, kRecW<D> doubles per element (128 B in 3D, 64 B in 2D): the
+// shape-function gradients of local nodes 1..D (the columns of inv J), |e|, tau and the sums
+// over the element's nodes of each previous-step velocity component.  The gathers rebuild the
+// gradient of local node 0 with the same operations as element_eval, so the values equal
+// those of an evaluation in place.
+template <int D>
+constexpr int kRecW = D == 2 ? 8 : 16;
+
+template <int D>
+__device__ __forceinline__ void load_rec(const double* __restrict__ rec, Geom<D>& G, double (&sv)[D]) {
+    if constexpr (D == 2) {
+        const double4 r0 = *reinterpret_cast<const double4*>(rec);
+        const double4 r1 = *reinterpret_cast<const double4*>(rec + 4);
+        G.gN[0][1] = r0.x;
+        G.gN[1][1] = r0.y;
+        G.gN[0][2] = r0.z;
+        G.gN[1][2] = r0.w;
+        G.vol = r1.x;
+        G.tau = r1.y;
+        sv[0] = r1.z;
+        sv[1] = r1.w;
+    } else {
+        const double4 r0 = *reinterpret_cast<const double4*>(rec);
+        const double4 r1 = *reinterpret_cast<const double4*>(rec + 4);
+        const double4 r2 = *reinterpret_cast<const double4*>(rec + 8);
+        const double2 r3 = *reinterpret_cast<const double2*>(rec + 12);
+        G.gN[0][1] = r0.x;
+        G.gN[1][1] = r0.y;
+        G.gN[2][1] = r0.z;
+        G.gN[0][2] = r0.w;
+        G.gN[1][2] = r1.x;
+        G.gN[2][2] = r1.y;
+        G.gN[0][3] = r1.z;
+        G.gN[1][3] = r1.w;
+        G.gN[2][3] = r2.x;
+        G.vol = r2.y;
+        G.tau = r2.z;
+        sv[0] = r2.w;
+        sv[1] = r3.x;
+        sv[2] = r3.y;
+    }
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double g0 = -G.gN[k][1];
+#pragma unroll
+        for (int a = 2; a <= D; ++a) g0 -= G.gN[k][a];
+        G.gN[k][0] = g0;
+    }
+}
+
+// One lane per element: the element record (see kRecW).
+template <int D>
+__global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict__ views,
+                                                      const BlockTask* __restrict__ tasks) {
+    constexpr int NE = D + 1;
+    const BlockTask task = tasks[blockIdx.x];
+    if (task.case_id < 0) return;
+    const PspgView& V = views[task.case_id];
+    const int N = V.N;
+    const Seg sg = xcd_segment(task, V.E);
+    const long long eend = (long long)sg.n0 + sg.ns;
+    for (long long e = sg.n0 + (long long)sg.lb * kBlock + threadIdx.x; e < eend; e += (long long)sg.cnt * kBlock) {
+        int nd[NE];
+#pragma unroll
+        for (int c = 0; c < NE; ++c) nd[c] = V.conn[(long long)c * V.E + e];
+        Geom<D> G;
+        element_eval<D>(V, nd, G);
+        double sv[D];
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            double vs = 0;
+#pragma unroll
+            for (int c = 0; c < NE; ++c) vs += V.q_prev[(long long)s * N + nd[c]];
+            sv[s] = vs;
+        }
+        double* rec = V.erec + e * kRecW<D>;
+        if constexpr (D == 2) {
+            *reinterpret_cast<double4*>(rec) = make_double4(G.gN[0][1], G.gN[1][1], G.gN[0][2], G.gN[1][2]);
+            *reinterpret_cast<double4*>(rec + 4) = make_double4(G.vol, G.tau, sv[0], sv[1]);
+        } else {
+            *reinterpret_cast<double4*>(rec) = make_double4(G.gN[0][1], G.gN[1][1], G.gN[2][1], G.gN[0][2]);
+            *reinterpret_cast<double4*>(rec + 4) = make_double4(G.gN[1][2], G.gN[2][2], G.gN[0][3], G.gN[1][3]);
+            *reinterpret_cast<double4*>(rec + 8) = make_double4(G.gN[2][3], G.vol, G.tau, sv[0]);
+            *reinterpret_cast<double4*>(rec + 12) = make_double4(sv[1], sv[2], 0.0, 0.0);
+        }
+    }
+}
+
 // Adds the (a, b) block of Ae (row node a, column node b) to blk[s][t].
+// Column c of the gradients for a local node index c known only at run time, as a one-hot
+// weighted sum (exact: one term is 1 x g, the others add zeros).  Indexing the register
+// array with c, or a select chain the compiler folds back into such an index, places the
+// 3D array in scratch memory, which held the assembly gathers at ~1 ms on the 262k-node dam
+// break.
+template <int D>
+__device__ __forceinline__ void grad_col(const Geom<D>& G, int c, double (&g)[D]) {
+    double w[D + 1];
+#pragma unroll
+    for (int a = 0; a <= D; ++a) w[a] = (a == c) ? 1.0 : 0.0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        double r = 0.0;
+#pragma unroll
+        for (int a = 0; a <= D; ++a) r = fma(w[a], G.gN[k][a], r);
+        g[k] = r;
+    }
+}
+
 template <int D>
 __device__ __forceinline__ void add_block(const PspgView& V, const Geom<D>& G, int a, int b, bool mom,
                                           double (&blk)[D + 1][D + 1]) {
     constexpr int NE = D + 1;
     const double vol = G.vol, vol_ne = vol / NE, mu_vol = V.mu * vol, tau_dt = G.tau / V.dt;
+    double ga[D], gb[D];
+    grad_col<D>(G, a, ga);
+    grad_col<D>(G, b, gb);
     double gdot = 0;
 #pragma unroll
-    for (int k = 0; k < D; ++k) gdot += G.gN[k][a] * G.gN[k][b];
+    for (int k = 0; k < D; ++k) gdot += ga[k] * gb[k];
     if (mom) {
         const double mab = V.rho * vol * ((a == b) ? 2.0 : 1.0) / (double)((D + 1) * (D + 2)) / V.dt;
 #pragma unroll
         for (int s = 0; s < D; ++s) {
 #pragma unroll
             for (int t = 0; t < D; ++t) {
-                double val = mu_vol * G.gN[t][a] * G.gN[s][b];
+                double val = mu_vol * ga[t] * gb[s];
                 if (s == t) val += mab + mu_vol * gdot;
                 blk[s][t] += val;
             }
-            blk[s][D] += -vol_ne * G.gN[s][a];
+            blk[s][D] += -vol_ne * ga[s];
         }
     }
 #pragma unroll
-    for (int t = 0; t < D; ++t) blk[D][t] += vol_ne * (tau_dt * G.gN[t][a] + G.gN[t][b]);
+    for (int t = 0; t < D; ++t) blk[D][t] += vol_ne * (tau_dt * ga[t] + gb[t]);
     blk[D][D] += G.tau / V.rho * vol * gdot;
 }
 
@@ -148,7 +260,9 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
     if (task.case_id < 0) return;
     const PspgView& V = views[task.case_id];
     const int lane = threadIdx.x % L;
-    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < V.N; i += (long long)task.nblocks * NPB) {
+    const Seg sg = xcd_segment(task, V.N);
+    const long long iend = (long long)sg.n0 + sg.ns;
+    for (long long i = sg.n0 + (long long)sg.lb * NPB + threadIdx.x / L; i < iend; i += (long long)sg.cnt * NPB) {
         const int4 inf = V.node_info[i];
         const int deg = inf.z;
         const uint8_t f = (uint8_t)inf.w;
@@ -165,11 +279,9 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
             for (int q = V.pei_ptr[m]; q < V.pei_ptr[m + 1]; ++q) {
                 int e, a, b;
                 decode(V.pei[q], e, a, b);
-                int nd[NE];
-#pragma unroll
-                for (int c = 0; c < NE; ++c) nd[c] = V.conn[(long long)c * V.E + e];
                 Geom<D> G;
-                element_eval<D>(V, nd, G);
+                double sv[D];
+                load_rec<D>(V.erec + (long long)e * kRecW<D>, G, sv);
                 add_block<D>(V, G, a, b, mom, blk);
             }
 #pragma unroll
@@ -194,7 +306,9 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
     const PspgView& V = views[task.case_id];
     const int lane = threadIdx.x % L;
     const int N = V.N;
-    for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
+    const Seg sg = xcd_segment(task, N);
+    const long long iend = (long long)sg.n0 + sg.ns;
+    for (long long i = sg.n0 + (long long)sg.lb * NPB + threadIdx.x / L; i < iend; i += (long long)sg.cnt * NPB) {
         const int4 inf = V.node_info[i];
         const int deg = inf.z;
         const uint8_t f = (uint8_t)inf.w;
@@ -212,31 +326,29 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
 #pragma unroll
             for (int t = 0; t < NE; ++t) blk[s][t] = 0.0;
         }
+        double vown[D];  // node i's previous-step velocity (the a == i term of Me/dt vPrev)
+#pragma unroll
+        for (int s = 0; s < D; ++s) vown[s] = V.q_prev[(long long)s * N + i];
         const int q0 = V.pei_ptr[m], q1 = V.pei_ptr[m + 1];
         for (int q = q0 + lane; q < q1; q += L) {
             int e, a, b;
             decode(V.pei[q], e, a, b);
-            int nd[NE];
-#pragma unroll
-            for (int c = 0; c < NE; ++c) nd[c] = V.conn[(long long)c * V.E + e];
             Geom<D> G;
-            element_eval<D>(V, nd, G);
+            double sv[D];
+            load_rec<D>(V.erec + (long long)e * kRecW<D>, G, sv);
             add_block<D>(V, G, a, a, mom, blk);
-            // be (MomContEquationPSPG.inl:53): F + Me/dt vPrev ; H + C tau/dt vPrev
+            // be (MomContEquationPSPG.inl:53): F + Me/dt vPrev ; H + C tau/dt vPrev, with
+            // sum_c M_ac v_c = |e| (v_a + sum_c v_c) / ((d+1)(d+2))
             const double vol = G.vol, vol_ne = vol / NE;
             double hg = 0, cv = 0;
+            double ga[D];
+            grad_col<D>(G, a, ga);
 #pragma unroll
             for (int s = 0; s < D; ++s) {
-                double mv = 0, vs = 0;
-#pragma unroll
-                for (int c = 0; c < NE; ++c) {
-                    const double vp = V.q_prev[(long long)s * N + nd[c]];
-                    mv += ((c == a) ? 2.0 : 1.0) * vp;
-                    vs += vp;
-                }
+                const double mv = vown[s] + sv[s];
                 bv[s] += V.rho * vol_ne * V.g[s] + V.rho * vol / (double)((D + 1) * (D + 2)) / V.dt * mv;
-                hg += G.gN[s][a] * V.g[s];
-                cv += G.gN[s][a] * vs;
+                hg += ga[s] * V.g[s];
+                cv += ga[s] * sv[s];
             }
             bv[D] += G.tau * vol * hg + G.tau / V.dt * vol_ne * cv;
         }
@@ -326,37 +438,27 @@ __global__ __launch_bounds__(kBlock) void k_pspg_precond(const PspgView* __restr
         for (int s = 0; s < 4; ++s)
 #pragma unroll
             for (int t = 0; t < 8; ++t) m[s][t] = (t - 4 == s) ? 1.0 : 0.0;
-        for (int s = 0; s < nf; ++s) {
+#pragma unroll
+        for (int s = 0; s < 4; ++s) {
+            if (s >= nf) {
+                m[s][s] = 1.0;  // padding rows (2D): identity
+                continue;
+            }
             const bool full = (s < D) ? mom_full : cont_full;
             const long long rs = (s < D) ? (long long)s * V.mom_nnz + inf.x : cont_start;
-            for (int t = 0; t < nf; ++t) {
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (t >= nf) continue;
                 if (full) m[s][t] = V.vals[rs + (long long)t * inf.z + self];
                 else m[s][t] = (s == t) ? V.vals[rs] : 0.0;
             }
         }
-        for (int s = nf; s < 4; ++s) m[s][s] = 1.0;
-        // Gauss-Jordan on [M | I]
-        for (int c = 0; c < 4; ++c) {
-            int piv = c;
-            for (int r = c + 1; r < 4; ++r)
-                if (fabs(m[r][c]) > fabs(m[piv][c])) piv = r;
-            if (piv != c)
-                for (int k = 0; k < 8; ++k) {
-                    const double tmp = m[c][k];
-                    m[c][k] = m[piv][k];
-                    m[piv][k] = tmp;
-                }
-            const double inv = 1.0 / m[c][c];
-            for (int k = 0; k < 8; ++k) m[c][k] *= inv;
-            for (int r = 0; r < 4; ++r) {
-                if (r == c) continue;
-                const double fct = m[r][c];
-                if (fct != 0.0)
-                    for (int k = 0; k < 8; ++k) m[r][k] -= fct * m[c][k];
-            }
-        }
-        for (int s = 0; s < nf; ++s)
-            for (int t = 0; t < nf; ++t) {
+        gj_invert4<false>(m);
+#pragma unroll
+        for (int s = 0; s < 4; ++s)
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                if (s >= nf || t >= nf) continue;
                 const double v = m[s][4 + t];
                 V.binv[(long long)(s * nf + t) * N + i] = v;
                 if (V.binv32) V.binv32[(long long)(s * nf + t) * N + i] = (float)v;
@@ -370,9 +472,11 @@ __global__ __launch_bounds__(kBlock) void k_pspg_precond(const PspgView* __restr
 // Launchers ---------------------------------------------------------------------------
 void launch_pspg_assemble(int dim, const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s) {
     if (dim == 2) {
+        k_pspg_elem<2><<<ntasks, kBlock, 0, s>>>(views, tasks);
         k_pspg_offdiag<2><<<ntasks, kBlock, 0, s>>>(views, tasks);
         k_pspg_diag<2><<<ntasks, kBlock, 0, s>>>(views, tasks);
     } else {
+        k_pspg_elem<3><<<ntasks, kBlock, 0, s>>>(views, tasks);
         k_pspg_offdiag<3><<<ntasks, kBlock, 0, s>>>(views, tasks);
         k_pspg_diag<3><<<ntasks, kBlock, 0, s>>>(views, tasks);
     }

@@ -52,6 +52,7 @@ struct Workspace {
     double* pos_save = nullptr;
     double* kv64 = nullptr;  // 8 n: r, rh, p, v, s, t, ph, sh
     double* vals_ell = nullptr;  // sliced ELL copy of the values (EllMat)
+    double* erec = nullptr;      // element records of the assembly (E x 8 / 16 doubles)
     float* vals32_ell = nullptr;  // its FP32 copy (FP32 multigrid hierarchies)
     float* vals32 = nullptr;
     float* binv32 = nullptr;
@@ -70,7 +71,7 @@ struct Workspace {
         bytes = 0;
         arena = nullptr;
         off = 0;
-        vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = nullptr;
+        vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = erec = nullptr;
         vals32 = binv32 = kv32 = vals32_ell = nullptr;
         amg_w0 = amg_t0 = amg_dense = nullptr;
         for (auto& l : amg_lv)
@@ -226,6 +227,10 @@ struct pfem_case {
     // a node flagged boundary + Dirichlet but not fixed: its position, hence its Dirichlet
     // value, changes at every Picard re-assembly (MomContEquationPSPG.inl:206-214)
     bool moving_dirichlet = false;
+    // WC: the CFL minimum dt^2 that the last step (or CFL pass) left for securityCoeff cfl_sc,
+    // valid until the state or the positions change (pfem_wc_next_dt then needs no launch)
+    bool cfl_valid = false;
+    double cfl_dt2 = 0.0, cfl_sc = 0.5;
     std::atomic<int> busy{0};  // a serving request holds the case
     std::unique_ptr<pfem::Engine> engine;  // direct calls
     ~pfem_case();

@@ -28,6 +28,7 @@ struct PspgView {
     const int* ell_soff;      // slice offsets,
     double* vals_ell;         // values written after assembly and BC (null: none)
     float* vals32_ell;        // FP32 ELL copy for an FP32 multigrid hierarchy (null: none)
+    double* erec;             // [E][8 (2D) / 16 (3D)] element records of the assembly (pspg.hip)
     double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
     float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
@@ -215,6 +216,7 @@ struct WcView {
     double* f_cont;           // [N] continuity rhs
     double* m_mom;            // [N] lumped momentum mass (same for every component)
     double* f_mom;            // [dim][N] momentum force
+    double* hmin;             // [N] smallest he of each node's elements (CFL after the step)
     double* rec;              // [N][8] node records {x, y, z, p, vx, vy, vz, rho} (FP64; 2D: z = vz = 0),
                               // written by the node passes, read by the element gathers
     double* erec;             // [E][4] continuity element records {|e|, div v, sum P, 0} (FP64)

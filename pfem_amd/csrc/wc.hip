@@ -45,6 +45,45 @@ __device__ __forceinline__ void rec_load(const double* __restrict__ rec, int j, 
     rho = r1.w;
 }
 
+// he = 2 r_in of an element (Element::getRin, Element.cpp:226-294): 2D |e| / semi-perimeter,
+// 3D 3 |e| / total face area.
+template <int D>
+__device__ __forceinline__ double elem_he(const double (&x)[D][D + 1], double vol) {
+    double rin;
+    if constexpr (D == 2) {
+        auto dist = [&](int a, int b) {
+            const double dx = x[0][a] - x[0][b], dy = x[1][a] - x[1][b];
+            return sqrt(dx * dx + dy * dy);
+        };
+        const double s = (dist(0, 1) + dist(1, 2) + dist(0, 2)) / 2;
+        rin = vol / s;
+    } else {
+        auto area2 = [&](int o, int a, int b) {
+            const double ax = x[0][a] - x[0][o], ay = x[1][a] - x[1][o], az = x[2][a] - x[2][o];
+            const double bx = x[0][b] - x[0][o], by = x[1][b] - x[1][o], bz = x[2][b] - x[2][o];
+            const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
+            return sqrt(cx * cx + cy * cy + cz * cz);
+        };
+        rin = 6 * vol / (area2(0, 1, 2) + area2(0, 3, 2) + area2(0, 1, 3) + area2(3, 1, 2));
+    }
+    return 2 * rin;
+}
+
+// One node's share of computeNextDT (WCompNewton/Solver.cpp:192-234) through an element of
+// size he: sc^2 he^2 / max(u^2, c^2, 4 alpha^2 / he^2) with c^2 = (K0 + K0' p) / rho
+// (ContEquation.inl:117-121), alpha = mu / rho (MomEquation.inl:184-198).  The reference's
+// element value is the smallest of its nodes' shares (x / max(a, b) = min(x / a, x / b) in
+// correctly rounded arithmetic), and a share only grows with he, so a node's smallest share
+// over its elements is the one through its smallest he: the minimum over nodes of that
+// equals the reference's minimum over elements, bit for bit.
+__device__ __forceinline__ double cfl_share(double sc2, double he, double u2, double p, double rho, double K0,
+                                            double K0p, double mu) {
+    const double c2 = (K0 + K0p * p) / rho;
+    const double alpha = mu / rho;
+    const double maxS = fmax(fmax(u2, c2), 4 * alpha * alpha / (he * he));
+    return sc2 * he * he / maxS;
+}
+
 // v1/2 = v + dt/2 a (all nodes); x += dt v1/2 (non-fixed).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_wc_half(const WcView<S>* __restrict__ views,
@@ -119,7 +158,8 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_elem(const WcView<S>* __rest
 #pragma unroll
             for (int k = 0; k < D; ++k) divv += G.gN[k][a] * vel[k][a];
         }
-        reinterpret_cast<double4*>(V.erec)[e] = make_double4(G.vol, divv, psum, 0.0);
+        // he for the CFL time step after this step (positions do not move again in it)
+        reinterpret_cast<double4*>(V.erec)[e] = make_double4(G.vol, divv, psum, elem_he<D>(x, G.vol));
     }
 }
 
@@ -140,10 +180,11 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
     for (long long i = (long long)task.block * NPB + threadIdx.x / L; i < N; i += (long long)task.nblocks * NPB) {
         const int q1 = V.elem_ptr[i + 1];
         const double Pm = (double)V.p[i];
-        double ml = 0.0, fc = 0.0;
+        double ml = 0.0, fc = 0.0, hmin = 1.7976931348623157e308;
         for (int q = V.elem_ptr[i] + lane; q < q1; q += L) {
             const double4 er = erec[V.elem_list[q] >> 2];
             const double vol = er.x, divv = er.y, psum = er.z;
+            hmin = fmin(hmin, er.w);
             const double ci = V.K0 / NE + V.K0p * (Pm + psum) * c2;
             const double mlump = vol / NE;
             const double mp = V.meduri ? vol * (Pm + psum) * c2 : mlump * Pm;
@@ -152,9 +193,11 @@ __global__ __launch_bounds__(kBlock) void k_wc_cont_gather(const WcView<S>* __re
         }
         ml = group_sum<L>(ml);
         fc = group_sum<L>(fc);
+        hmin = group_min<L>(hmin);
         if (lane == 0) {
             V.m_cont[i] = ml;
             V.f_cont[i] = fc;
+            V.hmin[i] = hmin;
         }
     }
 }
@@ -278,7 +321,9 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_gather(const WcView<S>* __res
     }
 }
 
-// a = M^-1 F with the BC of MomEquation.inl:341-373; v = v1/2 + dt/2 a (:216-222).
+// a = M^-1 F with the BC of MomEquation.inl:341-373; v = v1/2 + dt/2 a (:216-222); then
+// the node's share of the next step's CFL time step from the final v, p, rho and its
+// smallest incident he (free nodes belong to no element and take no part).
 template <typename S>
 __global__ __launch_bounds__(kBlock) void k_wc_mom_node(const WcView<S>* __restrict__ views,
                                                         const BlockTask* __restrict__ tasks) {
@@ -287,9 +332,11 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_node(const WcView<S>* __restr
     const WcView<S>& V = views[task.case_id];
     const int N = V.N;
     const double dt = V.dt;
+    double best = 1.7976931348623157e308;
     for (int i = task.block * kBlock + threadIdx.x; i < N; i += task.nblocks * kBlock) {
         const uint8_t f = V.flags[i];
         const double invm = 1.0 / V.m_mom[i];
+        double u2 = 0.0;
         for (int k = 0; k < V.dim; ++k) {
             const long long q = (long long)k * N + i;
             double acc;
@@ -297,12 +344,18 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_node(const WcView<S>* __restr
             else if ((f & NODE_BOUND) && (f & NODE_DIRICHLET)) acc = V.dir_vals[q];
             else acc = invm * V.f_mom[q];
             V.a[q] = (S)acc;
-            V.v[q] = (S)((double)V.v[q] + 0.5 * dt * acc);
+            const S vs = (S)((double)V.v[q] + 0.5 * dt * acc);
+            V.v[q] = vs;
+            u2 += (double)vs * (double)vs;
         }
+        if (!(f & NODE_FREE))
+            best = fmin(best, cfl_share(V.sc2, V.hmin[i], u2, (double)V.p[i], (double)V.rho[i], V.K0, V.K0p, V.mu));
     }
+    block_min_to(V.dt2, best);
 }
 
-// CFL: dt^2 = min_e sc^2 he^2 / max_n(u^2, c^2, 4 alpha^2/he^2), he = 2 r_in (Element.cpp:226-294).
+// CFL on its own (no step before it): dt^2 = min_e sc^2 he^2 / max_n(u^2, c^2, 4 alpha^2/he^2),
+// he = 2 r_in (Element.cpp:226-294), as the minimum of every element node's share.
 template <int D, typename S>
 __global__ __launch_bounds__(kBlock) void k_wc_cfl(const WcView<S>* __restrict__ views,
                                                    const BlockTask* __restrict__ tasks) {
@@ -320,44 +373,20 @@ __global__ __launch_bounds__(kBlock) void k_wc_cfl(const WcView<S>* __restrict__
         double x[D][NE];
         WGeom<D> G;
         wc_geometry<D>(V.pos, N, nd, x, G);
-        double rin;
-        if constexpr (D == 2) {
-            auto dist = [&](int a, int b) {
-                const double dx = x[0][a] - x[0][b], dy = x[1][a] - x[1][b];
-                return sqrt(dx * dx + dy * dy);
-            };
-            const double s = (dist(0, 1) + dist(1, 2) + dist(0, 2)) / 2;
-            rin = G.vol / s;
-        } else {
-            auto area2 = [&](int o, int a, int b) {
-                const double ax = x[0][a] - x[0][o], ay = x[1][a] - x[1][o], az = x[2][a] - x[2][o];
-                const double bx = x[0][b] - x[0][o], by = x[1][b] - x[1][o], bz = x[2][b] - x[2][o];
-                const double cx = ay * bz - az * by, cy = az * bx - ax * bz, cz = ax * by - ay * bx;
-                return sqrt(cx * cx + cy * cy + cz * cz);
-            };
-            rin = 6 * G.vol / (area2(0, 1, 2) + area2(0, 3, 2) + area2(0, 1, 3) + area2(3, 1, 2));
-        }
-        const double he = 2 * rin;
-        double maxS = 0;
+        const double he = elem_he<D>(x, G.vol);
 #pragma unroll
         for (int a = 0; a < NE; ++a) {
             const int i = nd[a];
-            const double p = (double)V.p[i], rho = (double)V.rho[i];
-            const double c2 = (V.K0 + V.K0p * p) / rho;
             double u2 = 0;
 #pragma unroll
             for (int k = 0; k < D; ++k) {
                 const double vk = (double)V.v[(long long)k * N + i];
                 u2 += vk * vk;
             }
-            const double alpha = V.mu / rho;
-            maxS = fmax(fmax(u2, c2), maxS);
-            maxS = fmax(maxS, 4 * alpha * alpha / (he * he));
+            best = fmin(best, cfl_share(sc2, he, u2, (double)V.p[i], (double)V.rho[i], V.K0, V.K0p, V.mu));
         }
-        best = fmin(best, sc2 * he * he / maxS);
     }
-    best = wave_min(best);
-    if ((threadIdx.x & 63) == 0) atomic_min_pos_double(V.dt2, best);
+    block_min_to(V.dt2, best);
 }
 
 // Launchers ---------------------------------------------------------------------------
