@@ -169,6 +169,10 @@ int pfem_get_csr(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, double* vals)
 int pfem_get_rhs(pfem_case* c, double* b);
 /* y = A x with the assembled FP64 matrix (host arrays of n). */
 int pfem_spmv(pfem_case* c, const double* x, double* y);
+/* Instrumentation (no counterpart in the reference): y = A x through the sliced ELL copy of
+ * the values that the BiCGSTAB kernels read (the assembly and the BCs write it beside the
+ * CSR-order values); for checking the copy against pfem_spmv. */
+int pfem_spmv_ell(pfem_case* c, const double* x, double* y);
 
 /* One PSPG time step: PicardAlgo::solve (PicardAlgo.cpp:31-95) with the PSPG callbacks
  * (MomContEquationPSPG.inl:262-372), positions updated as updateNodesPositionFromSave

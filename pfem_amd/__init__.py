@@ -210,6 +210,13 @@ class Case:
         check(lib().pfem_spmv(self._h, _dp(x), _dp(y)))
         return y
 
+    def spmv_ell(self, x):
+        """A x through the sliced ELL copy the BiCGSTAB kernels read (instrumentation)."""
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        y = np.empty_like(x)
+        check(lib().pfem_spmv_ell(self._h, _dp(x), _dp(y)))
+        return y
+
     # ---- the Picard loop's callbacks one by one (PicardAlgo.cpp:31-95) -----------------
     def save_positions(self):
         """Mesh::saveNodesList at the start of a step (node states = q_prev)."""

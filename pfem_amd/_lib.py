@@ -42,6 +42,7 @@ EXPORTS = [
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
     "pfem_case_reserve_workspace", "pfem_case_release_workspace", "pfem_save_positions", "pfem_restore_positions",
     "pfem_update_positions", "pfem_residual", "pfem_pspg_commit", "pfem_get_iterate", "pfem_case_mass_noalloc",
+    "pfem_spmv_ell",
 ]
 
 
@@ -151,6 +152,7 @@ def lib():
         "pfem_pspg_commit": [_vp],
         "pfem_get_iterate": [_vp, _dp],
         "pfem_case_mass_noalloc": [_vp, _vp],
+        "pfem_spmv_ell": [_vp, _dp, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -601,6 +601,31 @@ int pfem_spmv(pfem_case* c, const double* x, double* y) {
     });
 }
 
+int pfem_spmv_ell(pfem_case* c, const double* x, double* y) {
+    return guarded([&] {
+        require_system(c);
+        PFEM_REQUIRE(x && y, PFEM_E_INVALID, "null vector");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        const size_t nb = sizeof(double) * (size_t)c->n;
+        double* dx = c->ws.kv64;
+        double* dy = dx + c->n;
+        PFEM_HIP(hipMemcpyAsync(dx, x, nb, hipMemcpyHostToDevice, c->stream));
+        pfem::EllMatT<double> E{};
+        E.N = c->N;
+        E.nf = c->dim + 1;
+        E.nsl = c->pat.ell_nsl;
+        E.soff = c->pat.ell_soff.as<int>();
+        E.idx = c->pat.ell_idx.as<int>();
+        E.vals = c->ws.vals_ell;
+        pfem::launch_spmv_ell(E, dx, dy, std::max(1, std::min(8192, (c->pat.ell_nsl + pfem::kWaves - 1) / pfem::kWaves)),
+                              c->stream);
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipMemcpyAsync(y, dy, nb, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
 int pfem_pspg_step(pfem_case* c, double dt, pfem_step_stats* stats) {
     return guarded([&] {
         require_pspg(c);

@@ -1,6 +1,6 @@
 // Node-major sliced ELL copy of the PSPG system (EllMat in views.hpp): the structure is
-// built once per mesh from the node graph, the values are refilled from the node-block
-// (CSR-order) values after every assembly.
+// built once per mesh from the node graph; the assembly gathers and the boundary conditions
+// (pspg.hip) write its values beside the node-block (CSR-order) ones.
 //
 // Measured on gfx950 with the C2 matrix (516 MB): the node-block SpMV, L lanes per node
 // with a cross-lane sum, runs 115 us per launch; this layout, one lane per node, 98 us
@@ -46,77 +46,6 @@ __global__ void k_ell_idx(int N, int nsl, const int* __restrict__ nbr_ptr, const
     }
 }
 
-__device__ __forceinline__ int rank_of(const int* __restrict__ nbr_idx, int beg, int deg, int target) {
-    int lo = 0, hi = deg;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (nbr_idx[beg + mid] < target) lo = mid + 1;
-        else hi = mid;
-    }
-    return lo;
-}
-
-// Value (s, t, rank k) of node i, with its row start rs and degree deg, from `row` (the
-// node-block values of row (i, s), in LDS or global memory).
-__device__ __forceinline__ double ell_value(const double* row, bool full, int deg, int self, int s, int t, int k) {
-    if (full) return k < deg ? row[(long long)t * deg + k] : 0.0;
-    return (k == self && t == s) ? row[0] : 0.0;
-}
-
-constexpr int kFillLds = 8192;  // doubles of one field block of a slice staged in LDS (64 KB)
-
-// One workgroup per slice.  For each field s the slice's rows (i, s) are consecutive in
-// the node-block values: they are staged in LDS with coalesced loads, then each wavefront
-// writes whole 64-value rank vectors of the copy (lane r = node 64 c + r).  A slice whose
-// field block exceeds the LDS stage reads its rows from global memory instead.
-__global__ __launch_bounds__(kBlock) void k_ell_fill(const PspgView* __restrict__ views,
-                                                     const BlockTask* __restrict__ tasks) {
-    __shared__ double stage[kFillLds];
-    const BlockTask task = tasks[blockIdx.x];
-    if (task.case_id < 0) return;
-    const PspgView& V = views[task.case_id];
-    if (!V.vals_ell) return;
-    const int N = V.N, D = V.dim, nf = D + 1;
-    const int lane = threadIdx.x & (kSlice - 1), wave = threadIdx.x >> 6;
-    for (long long c = task.block; c < V.ell_nsl; c += task.nblocks) {
-        const long long i = c * kSlice + lane;
-        const long long b = V.ell_soff[c];
-        const int W = V.ell_soff[c + 1] - (int)b;
-        const int i0 = (int)(c * kSlice), i1 = min(N, i0 + kSlice) - 1;  // first and last node
-        int4 inf = make_int4(0, 0, 0, 0);
-        int self = 0;
-        if (i < N) {
-            inf = V.node_info[i];
-            self = rank_of(V.nbr_idx, V.nbr_ptr[i], inf.z, (int)i);
-        }
-        const uint8_t f = (uint8_t)inf.w;
-        const bool mom_full = (f & (NODE_BOUND | NODE_FREE)) == 0, cont_full = (f & NODE_FREE) == 0;
-        const int4 inf0 = V.node_info[i0], inf1 = V.node_info[i1];
-        for (int s = 0; s < nf; ++s) {
-            const bool full = (s < D) ? mom_full : cont_full;
-            auto rstart = [&](const int4& q) {
-                return (s < D) ? (long long)s * V.mom_nnz + q.x : (long long)D * V.mom_nnz + q.y;
-            };
-            const uint8_t f1 = (uint8_t)inf1.w;
-            const bool full1 = (s < D) ? (f1 & (NODE_BOUND | NODE_FREE)) == 0 : (f1 & NODE_FREE) == 0;
-            const long long beg = rstart(inf0), end = rstart(inf1) + (full1 ? (long long)nf * inf1.z : 1);
-            const bool staged = end - beg <= kFillLds;
-            __syncthreads();  // the previous field's stage is no longer read
-            if (staged)
-                for (long long q = beg + threadIdx.x; q < end; q += kBlock) stage[q - beg] = V.vals[q];
-            __syncthreads();
-            const double* row = staged ? stage + (rstart(inf) - beg) : V.vals + rstart(inf);
-            for (int p = wave; p < W * nf; p += kWaves) {
-                const int k = p / nf, t = p - k * nf;
-                const double v = (i < N) ? ell_value(row, full, inf.z, self, s, t, k) : 0.0;
-                const long long q = ((b + k) * nf * nf + s * nf + t) * kSlice + lane;
-                V.vals_ell[q] = v;
-                if (V.vals32_ell) V.vals32_ell[q] = (float)v;
-            }
-        }
-    }
-}
-
 }  // namespace
 
 void build_ell(Pattern& pat, hipStream_t stream) {
@@ -141,8 +70,5 @@ void build_ell(Pattern& pat, hipStream_t stream) {
     PFEM_HIP(hipStreamSynchronize(stream));
 }
 
-void launch_ell_fill(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s) {
-    k_ell_fill<<<ntasks, kBlock, 0, s>>>(views, tasks);
-}
 
 }  // namespace pfem

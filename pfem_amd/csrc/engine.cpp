@@ -154,7 +154,7 @@ int pick_lanes(double avg_deg) {
 // ---------------------------------------------------------------------------------------
 namespace {
 struct WsLayout {
-    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32_ell, vals32, binv32, kv32, erec, total;
+    size_t vals, b, binv, x, x_prev, pos_save, kv64, vals_ell, vals32_ell, vals32, binv32, kv32, erec, esv, total;
     size_t amg_w0, amg_t0, amg_dense;
     size_t amg_lv[kAmgMaxLevels][6];
     size_t amg_ap[kAmgMaxLevels];  // A P values of level l (rows) against level l+1
@@ -194,7 +194,8 @@ WsLayout ws_layout(const pfem_case* c) {
     L.pos_save = take(8 * (size_t)c->dim * N);
     L.kv64 = take(8 * 8 * n);
     L.vals_ell = take(8 * nf * nf * kEllSlice * (size_t)c->pat.ell_ranks);
-    L.erec = take(8 * (size_t)(c->dim == 2 ? 8 : 16) * (size_t)c->E);
+    L.erec = take(8 * 16 * (size_t)c->E);  // assembly element records (pspg.hip)
+    L.esv = take(8 * 4 * (size_t)c->E);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     if (ir || amg_fp32(c)) {  // FP32 copies of the values and of the block-Jacobi inverse
         L.vals32 = take(4 * nnz);
@@ -244,6 +245,7 @@ void carve_workspace(pfem_case* c, char* base) {
     w.kv64 = reinterpret_cast<double*>(base + L.kv64);
     w.vals_ell = reinterpret_cast<double*>(base + L.vals_ell);
     w.erec = reinterpret_cast<double*>(base + L.erec);
+    w.esv = reinterpret_cast<double*>(base + L.esv);
     const bool ir = c->d.precision == PFEM_PREC_MIXED_IR;
     const bool v32 = ir || amg_fp32(c);
     w.vals32 = v32 ? reinterpret_cast<float*>(base + L.vals32) : nullptr;
@@ -476,6 +478,7 @@ void Engine::Impl::load_views(Req* r) {
         pv.binv = w.binv;
         pv.binv32 = w.binv32;
         pv.erec = w.erec;
+        pv.esv = w.esv;
         pv.nnz = P.nnz;
         pv.rho = c->d.rho;
         pv.mu = c->d.mu;
@@ -1065,10 +1068,8 @@ void Engine::tick(std::vector<Req*>& done) {
     if (NT(T_ASM2)) launch_pspg_assemble(2, vpv, T(T_ASM2), NT(T_ASM2), s);
     if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
     if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
-    if (NT(T_PRE)) {
-        launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
-        launch_ell_fill(vpv, T(T_PRE), NT(T_PRE), s);
-    }
+    // (the assembly and the BC write the sliced ELL copy themselves)
+    if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
     const AmgView<double>* vamg64 = I.d_amg64.as<AmgView<double>>();
     const AmgView<float>* vamg32 = I.d_amg32.as<AmgView<float>>();
     // multigrid setup of both hierarchy types (TA: first table of the type)

@@ -499,6 +499,27 @@ void launch_kry_phase(int phase, int lanes, const KrylovView<T>* v, const BlockT
     }
 }
 
+// y = A x over the sliced ELL copy (one wavefront per 64-node slice), the SpMV the BiCGSTAB
+// kernels run; for checking the copy against the node-block values.
+template <int NF>
+__global__ __launch_bounds__(kBlock) void k_spmv_ell(const EllMatT<double> E, const double* __restrict__ x,
+                                                     double* __restrict__ y) {
+    const int lane = threadIdx.x & (kEllSlice - 1);
+    for (long long c = (long long)blockIdx.x * kWaves + (threadIdx.x >> 6); c < E.nsl; c += (long long)gridDim.x * kWaves) {
+        double r[kMaxNf];
+        ell_rows<NF, double, double>(E, x, c, lane, r);
+        const long long i = c * kEllSlice + lane;
+        if (i < E.N)
+#pragma unroll
+            for (int s = 0; s < NF; ++s) y[i + (long long)s * E.N] = r[s];
+    }
+}
+
+void launch_spmv_ell(const EllMatT<double>& E, const double* x, double* y, int nblocks, hipStream_t s) {
+    if (E.nf == 3) k_spmv_ell<3><<<nblocks, kBlock, 0, s>>>(E, x, y);
+    else k_spmv_ell<4><<<nblocks, kBlock, 0, s>>>(E, x, y);
+}
+
 template <typename T>
 void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s) {
     PFEM_DISPATCH_L(lanes, (k_spmv<T, LL><<<nblocks, kBlock, 0, s>>>(A, x, y)));

@@ -10,8 +10,6 @@ namespace pfem {
 void launch_pspg_assemble(int dim, const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
-// refills the sliced ELL copy of the values (after assembly and BC)
-void launch_ell_fill(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 
 // ids: the cases (case_id of each entry) whose KScal::req_init is set, nids of them
 template <typename T>
@@ -22,6 +20,8 @@ void launch_kry_phase(int phase, int lanes, const KrylovView<T>* v, const BlockT
                       hipStream_t s, hipEvent_t* ev = nullptr);
 template <typename T>
 void launch_spmv(int lanes, const NodeMat<T>& A, const T* x, T* y, int nblocks, hipStream_t s);
+// y = A x over the sliced ELL copy
+void launch_spmv_ell(const EllMatT<double>& E, const double* x, double* y, int nblocks, hipStream_t s);
 
 // aggregation multigrid (T: FP64 or FP32 hierarchy)
 template <typename T>

@@ -108,57 +108,61 @@ __device__ __forceinline__ void element_eval(const PspgView& V, const int (&nd)[
 }
 
 // Element records: one pass evaluates every element once per assembly and leaves, for the
-// pair and diagonal gathers, kRecW<D> doubles per element (128 B in 3D, 64 B in 2D): the
-// shape-function gradients of local nodes 1..D (the columns of inv J), |e|, tau and the sums
-// over the element's nodes of each previous-step velocity component.  The gathers rebuild the
-// gradient of local node 0 with the same operations as element_eval, so the values equal
-// those of an evaluation in place.
-template <int D>
-constexpr int kRecW = D == 2 ? 8 : 16;
+// pair and diagonal gathers, 16 doubles per element (128 B, one cache line): the
+// shape-function gradients of local nodes 1..D (the columns of inv J; the gathers rebuild
+// node 0's as -(sum of the others), as element_eval does) and the element factors the
+// closed forms below multiply them by, so that the gathers divide nothing:
+//   vol_ne = |e|/(d+1), mu_vol = mu |e|, mab = rho |e| / ((d+1)(d+2)) / dt,
+//   tvn = tau/dt |e|/(d+1), trv = tau/rho |e|, tv = tau |e|, rvn = rho |e|/(d+1);
+// and, in a second array of 4 doubles per element, the sums over the element's nodes of
+// each previous-step velocity component (the diagonal gather's rhs).
+constexpr int kRecW = 16;
+enum : int { R_VOLNE = 9, R_MUVOL, R_MAB, R_TVN, R_TRV, R_TV, R_RVN };
 
 template <int D>
-__device__ __forceinline__ void load_rec(const double* __restrict__ rec, Geom<D>& G, double (&sv)[D]) {
+struct ERec {
+    double gN[D][D + 1];
+    double vol_ne, mu_vol, mab, tvn, trv, tv, rvn;
+};
+
+template <int D>
+__device__ __forceinline__ void load_rec(const double* __restrict__ rec, ERec<D>& R) {
+    const double4* r4 = reinterpret_cast<const double4*>(rec);
+    const double4 r0 = r4[0], r2 = r4[2], r3 = r4[3];
     if constexpr (D == 2) {
-        const double4 r0 = *reinterpret_cast<const double4*>(rec);
-        const double4 r1 = *reinterpret_cast<const double4*>(rec + 4);
-        G.gN[0][1] = r0.x;
-        G.gN[1][1] = r0.y;
-        G.gN[0][2] = r0.z;
-        G.gN[1][2] = r0.w;
-        G.vol = r1.x;
-        G.tau = r1.y;
-        sv[0] = r1.z;
-        sv[1] = r1.w;
+        R.gN[0][1] = r0.x;
+        R.gN[1][1] = r0.y;
+        R.gN[0][2] = r0.z;
+        R.gN[1][2] = r0.w;
     } else {
-        const double4 r0 = *reinterpret_cast<const double4*>(rec);
-        const double4 r1 = *reinterpret_cast<const double4*>(rec + 4);
-        const double4 r2 = *reinterpret_cast<const double4*>(rec + 8);
-        const double2 r3 = *reinterpret_cast<const double2*>(rec + 12);
-        G.gN[0][1] = r0.x;
-        G.gN[1][1] = r0.y;
-        G.gN[2][1] = r0.z;
-        G.gN[0][2] = r0.w;
-        G.gN[1][2] = r1.x;
-        G.gN[2][2] = r1.y;
-        G.gN[0][3] = r1.z;
-        G.gN[1][3] = r1.w;
-        G.gN[2][3] = r2.x;
-        G.vol = r2.y;
-        G.tau = r2.z;
-        sv[0] = r2.w;
-        sv[1] = r3.x;
-        sv[2] = r3.y;
+        const double4 r1 = r4[1];
+        R.gN[0][1] = r0.x;
+        R.gN[1][1] = r0.y;
+        R.gN[2][1] = r0.z;
+        R.gN[0][2] = r0.w;
+        R.gN[1][2] = r1.x;
+        R.gN[2][2] = r1.y;
+        R.gN[0][3] = r1.z;
+        R.gN[1][3] = r1.w;
+        R.gN[2][3] = r2.x;
     }
+    R.vol_ne = r2.y;
+    R.mu_vol = r2.z;
+    R.mab = r2.w;
+    R.tvn = r3.x;
+    R.trv = r3.y;
+    R.tv = r3.z;
+    R.rvn = r3.w;
 #pragma unroll
     for (int k = 0; k < D; ++k) {
-        double g0 = -G.gN[k][1];
+        double g0 = -R.gN[k][1];
 #pragma unroll
-        for (int a = 2; a <= D; ++a) g0 -= G.gN[k][a];
-        G.gN[k][0] = g0;
+        for (int a = 2; a <= D; ++a) g0 -= R.gN[k][a];
+        R.gN[k][0] = g0;
     }
 }
 
-// One lane per element: the element record (see kRecW).
+// One lane per element: the element record and the velocity sums.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict__ views,
                                                       const BlockTask* __restrict__ tasks) {
@@ -175,7 +179,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict
         for (int c = 0; c < NE; ++c) nd[c] = V.conn[(long long)c * V.E + e];
         Geom<D> G;
         element_eval<D>(V, nd, G);
-        double sv[D];
+        double sv[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             double vs = 0;
@@ -183,27 +187,35 @@ __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict
             for (int c = 0; c < NE; ++c) vs += V.q_prev[(long long)s * N + nd[c]];
             sv[s] = vs;
         }
-        double* rec = V.erec + e * kRecW<D>;
-        if constexpr (D == 2) {
-            *reinterpret_cast<double4*>(rec) = make_double4(G.gN[0][1], G.gN[1][1], G.gN[0][2], G.gN[1][2]);
-            *reinterpret_cast<double4*>(rec + 4) = make_double4(G.vol, G.tau, sv[0], sv[1]);
-        } else {
-            *reinterpret_cast<double4*>(rec) = make_double4(G.gN[0][1], G.gN[1][1], G.gN[2][1], G.gN[0][2]);
-            *reinterpret_cast<double4*>(rec + 4) = make_double4(G.gN[1][2], G.gN[2][2], G.gN[0][3], G.gN[1][3]);
-            *reinterpret_cast<double4*>(rec + 8) = make_double4(G.gN[2][3], G.vol, G.tau, sv[0]);
-            *reinterpret_cast<double4*>(rec + 12) = make_double4(sv[1], sv[2], 0.0, 0.0);
-        }
+        const double vol = G.vol, vol_ne = vol / NE;
+        double rec[kRecW];
+#pragma unroll
+        for (int q = 0; q < kRecW; ++q) rec[q] = 0.0;
+#pragma unroll
+        for (int a = 1; a <= D; ++a)
+#pragma unroll
+            for (int k = 0; k < D; ++k) rec[(a - 1) * D + k] = G.gN[k][a];
+        rec[R_VOLNE] = vol_ne;
+        rec[R_MUVOL] = V.mu * vol;
+        rec[R_MAB] = V.rho * vol / (double)((D + 1) * (D + 2)) / V.dt;
+        rec[R_TVN] = G.tau / V.dt * vol_ne;
+        rec[R_TRV] = G.tau / V.rho * vol;
+        rec[R_TV] = G.tau * vol;
+        rec[R_RVN] = V.rho * vol_ne;
+        double4* out = reinterpret_cast<double4*>(V.erec + e * kRecW);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) out[q] = make_double4(rec[4 * q], rec[4 * q + 1], rec[4 * q + 2], rec[4 * q + 3]);
+        reinterpret_cast<double4*>(V.esv)[e] = make_double4(sv[0], sv[1], sv[2], sv[3]);
     }
 }
 
-// Adds the (a, b) block of Ae (row node a, column node b) to blk[s][t].
 // Column c of the gradients for a local node index c known only at run time, as a one-hot
 // weighted sum (exact: one term is 1 x g, the others add zeros).  Indexing the register
 // array with c, or a select chain the compiler folds back into such an index, places the
 // 3D array in scratch memory, which held the assembly gathers at ~1 ms on the 262k-node dam
 // break.
 template <int D>
-__device__ __forceinline__ void grad_col(const Geom<D>& G, int c, double (&g)[D]) {
+__device__ __forceinline__ void grad_col(const ERec<D>& R, int c, double (&g)[D]) {
     double w[D + 1];
 #pragma unroll
     for (int a = 0; a <= D; ++a) w[a] = (a == c) ? 1.0 : 0.0;
@@ -211,38 +223,61 @@ __device__ __forceinline__ void grad_col(const Geom<D>& G, int c, double (&g)[D]
     for (int k = 0; k < D; ++k) {
         double r = 0.0;
 #pragma unroll
-        for (int a = 0; a <= D; ++a) r = fma(w[a], G.gN[k][a], r);
+        for (int a = 0; a <= D; ++a) r = fma(w[a], R.gN[k][a], r);
         g[k] = r;
     }
 }
 
+// Adds the (a, b) block of Ae (row node a, column node b) to blk[s][t].
 template <int D>
-__device__ __forceinline__ void add_block(const PspgView& V, const Geom<D>& G, int a, int b, bool mom,
-                                          double (&blk)[D + 1][D + 1]) {
-    constexpr int NE = D + 1;
-    const double vol = G.vol, vol_ne = vol / NE, mu_vol = V.mu * vol, tau_dt = G.tau / V.dt;
+__device__ __forceinline__ void add_block(const ERec<D>& R, int a, int b, bool mom, double (&blk)[D + 1][D + 1]) {
     double ga[D], gb[D];
-    grad_col<D>(G, a, ga);
-    grad_col<D>(G, b, gb);
+    grad_col<D>(R, a, ga);
+    grad_col<D>(R, b, gb);
     double gdot = 0;
 #pragma unroll
     for (int k = 0; k < D; ++k) gdot += ga[k] * gb[k];
     if (mom) {
-        const double mab = V.rho * vol * ((a == b) ? 2.0 : 1.0) / (double)((D + 1) * (D + 2)) / V.dt;
+        const double mdiag = (a == b) ? 2.0 * R.mab : R.mab;
+        const double dterm = mdiag + R.mu_vol * gdot;
 #pragma unroll
         for (int s = 0; s < D; ++s) {
 #pragma unroll
             for (int t = 0; t < D; ++t) {
-                double val = mu_vol * ga[t] * gb[s];
-                if (s == t) val += mab + mu_vol * gdot;
+                double val = R.mu_vol * ga[t] * gb[s];
+                if (s == t) val += dterm;
                 blk[s][t] += val;
             }
-            blk[s][D] += -vol_ne * ga[s];
+            blk[s][D] += -R.vol_ne * ga[s];
         }
     }
 #pragma unroll
-    for (int t = 0; t < D; ++t) blk[D][t] += vol_ne * (tau_dt * ga[t] + gb[t]);
-    blk[D][D] += G.tau / V.rho * vol * gdot;
+    for (int t = 0; t < D; ++t) blk[D][t] += R.tvn * ga[t] + R.vol_ne * gb[t];
+    blk[D][D] += R.trv * gdot;
+}
+
+// Writes node i's block at rank k into the sliced ELL copy (EllMat layout: value (s, t) of
+// lane r = i % 64 of slice c = i / 64 at ((soff[c] + k) nf^2 + s nf + t) 64 + r); rows that
+// are not full hold zeros there (their unit diagonal comes from K2).  The 16 nodes of a
+// workgroup write the same 128-byte line of each (k, s, t), so the L2 merges the lanes'
+// 8-byte stores into whole lines.
+template <int NE>
+__device__ __forceinline__ void ell_put(const PspgView& V, long long i, int k, const double (&blk)[NE][NE],
+                                        bool mom, bool cont) {
+    if (!V.vals_ell) return;
+    const long long c = i / kEllSlice, r = i % kEllSlice;
+    const long long base = ((long long)V.ell_soff[c] + k) * (NE * NE) * kEllSlice + r;
+#pragma unroll
+    for (int s = 0; s < NE; ++s) {
+        const bool full = (s < NE - 1) ? mom : cont;
+#pragma unroll
+        for (int t = 0; t < NE; ++t) {
+            const double v = full ? blk[s][t] : 0.0;
+            const long long q = base + (long long)(s * NE + t) * kEllSlice;
+            V.vals_ell[q] = v;
+            if (V.vals32_ell) V.vals32_ell[q] = (float)v;
+        }
+    }
 }
 
 __device__ __forceinline__ void decode(uint32_t w, int& e, int& a, int& b) {
@@ -266,8 +301,20 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
         const int4 inf = V.node_info[i];
         const int deg = inf.z;
         const uint8_t f = (uint8_t)inf.w;
-        const bool mom = (f & (NODE_BOUND | NODE_FREE)) == 0;
+        const bool mom = (f & (NODE_BOUND | NODE_FREE)) == 0, cont = (f & NODE_FREE) == 0;
         const int nb = V.nbr_ptr[i];
+        // ranks of the node's ELL slice past its degree: zeros (a free node's rank 0 too;
+        // K2 sets its unit diagonal)
+        const long long c = i / kEllSlice;
+        const int W = V.vals_ell ? V.ell_soff[c + 1] - V.ell_soff[c] : 0;
+        for (int k = deg + lane; k < W; k += L) {
+            double zero[NE][NE];
+#pragma unroll
+            for (int s = 0; s < NE; ++s)
+#pragma unroll
+                for (int t = 0; t < NE; ++t) zero[s][t] = 0.0;
+            ell_put<NE>(V, i, k, zero, false, false);
+        }
         for (int k = lane; k < deg; k += L) {
             if (V.nbr_idx[nb + k] == (int)i) continue;
             double blk[NE][NE];
@@ -279,10 +326,9 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
             for (int q = V.pei_ptr[m]; q < V.pei_ptr[m + 1]; ++q) {
                 int e, a, b;
                 decode(V.pei[q], e, a, b);
-                Geom<D> G;
-                double sv[D];
-                load_rec<D>(V.erec + (long long)e * kRecW<D>, G, sv);
-                add_block<D>(V, G, a, b, mom, blk);
+                ERec<D> R;
+                load_rec<D>(V.erec + (long long)e * kRecW, R);
+                add_block<D>(R, a, b, mom, blk);
             }
 #pragma unroll
             for (int s = 0; s < NE; ++s) {
@@ -291,6 +337,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
 #pragma unroll
                 for (int t = 0; t < NE; ++t) V.vals[rs + (long long)t * deg + k] = blk[s][t];
             }
+            ell_put<NE>(V, i, k, blk, mom, cont);
         }
     }
 }
@@ -333,24 +380,23 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
         for (int q = q0 + lane; q < q1; q += L) {
             int e, a, b;
             decode(V.pei[q], e, a, b);
-            Geom<D> G;
-            double sv[D];
-            load_rec<D>(V.erec + (long long)e * kRecW<D>, G, sv);
-            add_block<D>(V, G, a, a, mom, blk);
+            ERec<D> R;
+            load_rec<D>(V.erec + (long long)e * kRecW, R);
+            add_block<D>(R, a, a, mom, blk);
             // be (MomContEquationPSPG.inl:53): F + Me/dt vPrev ; H + C tau/dt vPrev, with
             // sum_c M_ac v_c = |e| (v_a + sum_c v_c) / ((d+1)(d+2))
-            const double vol = G.vol, vol_ne = vol / NE;
+            const double4 sv4 = reinterpret_cast<const double4*>(V.esv)[e];
+            const double sv[3] = {sv4.x, sv4.y, sv4.z};
             double hg = 0, cv = 0;
             double ga[D];
-            grad_col<D>(G, a, ga);
+            grad_col<D>(R, a, ga);
 #pragma unroll
             for (int s = 0; s < D; ++s) {
-                const double mv = vown[s] + sv[s];
-                bv[s] += V.rho * vol_ne * V.g[s] + V.rho * vol / (double)((D + 1) * (D + 2)) / V.dt * mv;
+                bv[s] += R.rvn * V.g[s] + R.mab * (vown[s] + sv[s]);
                 hg += ga[s] * V.g[s];
                 cv += ga[s] * sv[s];
             }
-            bv[D] += G.tau * vol * hg + G.tau / V.dt * vol_ne * cv;
+            bv[D] += R.tv * hg + R.tvn * cv;
         }
 #pragma unroll
         for (int s = 0; s < NE; ++s) {
@@ -367,6 +413,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
 #pragma unroll
                 for (int t = 0; t < NE; ++t) V.vals[rs + (long long)t * deg + self] = blk[s][t];
             }
+            ell_put<NE>(V, i, self, blk, mom, true);
         }
     }
 }
@@ -378,16 +425,32 @@ __global__ __launch_bounds__(kBlock) void k_pspg_bc(const PspgView* __restrict__
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const PspgView& V = views[task.case_id];
-    const int N = V.N, D = V.dim;
+    const int N = V.N, D = V.dim, nf = D + 1;
     for (int j = task.block * kBlock + threadIdx.x; j < N; j += task.nblocks * kBlock) {
         const int4 inf = V.node_info[j];
         const uint8_t f = (uint8_t)inf.w;
         const bool isfree = f & NODE_FREE, bound = f & NODE_BOUND;
         const bool mom = !(bound || isfree), cont = !isfree;
+        // the same entries in the ELL copy: value (s, t) of node j at rank k
+        const long long ebase = V.vals_ell ? (long long)V.ell_soff[j / kEllSlice] * nf * nf * kEllSlice + j % kEllSlice : 0;
+        auto ell_set = [&](int k, int s, int t, double v) {
+            if (!V.vals_ell) return;
+            const long long q = ebase + ((long long)k * nf * nf + s * nf + t) * kEllSlice;
+            V.vals_ell[q] = v;
+            if (V.vals32_ell) V.vals32_ell[q] = (float)v;
+        };
+        // rank of j among its own neighbours (a free node keeps its diagonal at rank 0)
+        const int self = isfree ? 0
+                                : (int)(V.diag_pos[(long long)D * N + j] - ((long long)D * V.mom_nnz + inf.y) -
+                                        (long long)D * inf.z);
         if (bound || isfree)
-            for (int s = 0; s < D; ++s) V.vals[(long long)s * V.mom_nnz + inf.x] = 1.0;
+            for (int s = 0; s < D; ++s) {
+                V.vals[(long long)s * V.mom_nnz + inf.x] = 1.0;
+                ell_set(self, s, s, 1.0);
+            }
         if (isfree) {
             V.vals[(long long)D * V.mom_nnz + inf.y] = 1.0;
+            ell_set(self, D, D, 1.0);
             V.b[(long long)D * N + j] = 0.0;
             if (!bound)
                 for (int s = 0; s < D; ++s) V.b[(long long)s * N + j] = V.q_prev[(long long)s * N + j] + V.dt * V.g[s];
@@ -410,6 +473,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_bc(const PspgView* __restrict__
                     const long long pos = rs + (long long)d * deg + k;
                     V.b[(long long)s * N + j] -= V.vals[pos] * vbc;
                     V.vals[pos] = 0.0;
+                    ell_set(k, s, d, 0.0);
                 }
             }
         }

@@ -52,7 +52,8 @@ struct Workspace {
     double* pos_save = nullptr;
     double* kv64 = nullptr;  // 8 n: r, rh, p, v, s, t, ph, sh
     double* vals_ell = nullptr;  // sliced ELL copy of the values (EllMat)
-    double* erec = nullptr;      // element records of the assembly (E x 8 / 16 doubles)
+    double* erec = nullptr;      // element records of the assembly (E x 16 doubles)
+    double* esv = nullptr;       // per-element previous-velocity sums (E x 4 doubles)
     float* vals32_ell = nullptr;  // its FP32 copy (FP32 multigrid hierarchies)
     float* vals32 = nullptr;
     float* binv32 = nullptr;
@@ -71,7 +72,7 @@ struct Workspace {
         bytes = 0;
         arena = nullptr;
         off = 0;
-        vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = erec = nullptr;
+        vals = b = binv = x = x_prev = pos_save = kv64 = vals_ell = erec = esv = nullptr;
         vals32 = binv32 = kv32 = vals32_ell = nullptr;
         amg_w0 = amg_t0 = amg_dense = nullptr;
         for (auto& l : amg_lv)

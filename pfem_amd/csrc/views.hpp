@@ -28,7 +28,8 @@ struct PspgView {
     const int* ell_soff;      // slice offsets,
     double* vals_ell;         // values written after assembly and BC (null: none)
     float* vals32_ell;        // FP32 ELL copy for an FP32 multigrid hierarchy (null: none)
-    double* erec;             // [E][8 (2D) / 16 (3D)] element records of the assembly (pspg.hip)
+    double* erec;             // [E][16] element records of the assembly (pspg.hip)
+    double* esv;              // [E][4] sums of the previous-step velocities over each element
     double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
     float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
