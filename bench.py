@@ -446,17 +446,16 @@ def run_dambreak2d(ctx, steps=10, threads=1, with_cpu=True):
 # the 3D weakly compressible sloshing tank and the mixed-precision 3D column field
 # ------------------------------------------------------------------------------------------
 def wc_step_bytes(dim, nodes, elems, s_state):
-    """Algorithmic bytes of one WC step plus computeNextDT: connectivity read by the element
-    passes (continuity, momentum, CFL), the node state v, a (dim each), p, rho read and
-    written once, positions read and written once, the FP64 accumulators (continuity mass and
-    rhs, momentum mass, dim force components) written and read once, and the CFL pass's read of
-    v, p, rho."""
-    conn = 3 * elems * (dim + 1) * 4
+    """Algorithmic bytes of one WC step with its computeNextDT (folded into the step's kernels):
+    connectivity read by the two element passes (continuity, momentum), the node state v, a
+    (dim each), p, rho read and written once, positions read and written once, and the FP64
+    accumulators (continuity mass and rhs, each node's smallest element size for the CFL,
+    momentum mass, dim force components) written and read once."""
+    conn = 2 * elems * (dim + 1) * 4
     state = 2 * nodes * (2 * dim + 2) * s_state
     pos = 2 * nodes * dim * 8
-    acc = 2 * nodes * (3 + dim) * 8
-    cfl = nodes * (dim + 2) * s_state
-    return conn + state + pos + acc + cfl
+    acc = 2 * nodes * (4 + dim) * 8
+    return conn + state + pos + acc
 
 
 def run_wc3d(ctx, steps, warmup=3):

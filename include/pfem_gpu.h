@@ -96,7 +96,15 @@ typedef struct pfem_case_desc {
     double inner_rtol;  /* PFEM_PREC_MIXED_IR: relative tolerance of each FP32 inner solve */
     int32_t inner_max_iter; /* PFEM_PREC_MIXED_IR: iteration cap of each FP32 inner solve */
     int32_t precond;        /* PFEM_PRECOND_* */
+    /* material law of the PSPG momentum equation (MomContEquation.inl:33-66, 100-123) */
+    int32_t problem;        /* PFEM_PROBLEM_* */
+    double tau0;            /* Bingham yield stress tau0 */
+    double m_reg;           /* Bingham regularisation mReg */
 } pfem_case_desc;
+
+#define PFEM_PROBLEM_NEWTONIAN 0 /* "IncompNewtonNoT": K factor mu */
+#define PFEM_PROBLEM_BINGHAM 1   /* "Bingham": K factor mu + tau0 (1 - exp(-mReg g)) / g, g the element's
+                                    strain rate sqrt(V^T B^T ddev B V) in the current node states */
 
 /* BiCGSTAB preconditioner of a PSPG case (chosen at pfem_case_set_mesh). */
 #define PFEM_PRECOND_AUTO 0   /* multigrid for 2D cases of >= 2,000 nodes, 3D of >= 50,000 (MIXED: 5,000) */

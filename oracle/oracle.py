@@ -72,6 +72,7 @@ def lib():
         L.orc_minmax.argtypes = [C.c_int, C.c_int64, _u8p, _dp, C.c_int, C.c_int]
         L.orc_picard_residual.restype = C.c_double
         L.orc_picard_residual.argtypes = [C.c_int, C.c_int64, _u8p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
+        L.orc_set_material.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_wc_parts.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, _dp,
                                    C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         _lib = L
@@ -302,3 +303,19 @@ def wc_parts(mesh, pos, v, p, rho, dt=1e-4, mu=1e-3, g=None, K0=2.2e9, K0p=7.0, 
     L.orc_wc_parts(d, N, mesh.n_elems, _i(conn), _d(pos), _d(v), _d(p), _d(rho), mu, _d(g), K0, K0p,
                    1 if meduri else 0, dt, _d(ML), _d(F0), _d(Mv), _d(F), nthreads)
     return ML, F0, Mv, F
+
+
+class material:
+    """Context manager selecting the momentum equation's material law for the PSPG calls inside
+    it (MomContEquation.inl:33-66, 100-123): "Newtonian" or "Bingham" (tau0, mReg)."""
+
+    def __init__(self, problem="Newtonian", tau0=0.0, mReg=0.0):
+        self.args = ({"Newtonian": 0, "Bingham": 1}[problem], float(tau0), float(mReg))
+
+    def __enter__(self):
+        lib().orc_set_material(*self.args)
+        return self
+
+    def __exit__(self, *exc):
+        lib().orc_set_material(0, 0.0, 0.0)
+        return False

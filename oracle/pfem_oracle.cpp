@@ -303,13 +303,57 @@ struct Triplet {
     double val = 0;
 };
 
+// Material law of the momentum equation (MomContEquation.inl:33-66, 100-123): problem 0 is a
+// Newtonian fluid (K factor mu), 1 "Bingham" (K factor mu + mu_eq of the element's strain
+// rate, Papanastasiou-regularised with tau0 and mReg).  Set by orc_set_material; read by the
+// PSPG entry points.
+struct Material {
+    int problem = 0;
+    double tau0 = 0, mReg = 0;
+};
+static Material g_mat;
+
 struct Params {
     int dim;
     int64_t N, E;
     const int* conn;
     const uint8_t* flags;  // FREE recomputed from connectivity (Node.inl: isFree = no element)
     double rho, mu, g[3], dt;
+    Material mat = g_mat;
 };
+
+// K factor of an element (MomContEquation.inl:100-123): mu, plus for a Bingham fluid
+// mu_eq = tau0 (1 - exp(-mReg gammaDot)) / gammaDot (tau0 mReg when gammaDot < 1e-15) with
+// gammaDot = sqrt(V^T B^T ddev B V), V the element's node velocities in the current node
+// states (getElementVecState, StatesFromToQ.hpp:137-170) and ddev the incompressible one.
+template <int D>
+static double kFactor(const Params& P, const double Bm[MB<D>::nv][D * (D + 1)], const double dd[MB<D>::nv][MB<D>::nv],
+                      const int* nodes, const double* states) {
+    if (P.mat.problem != 1) return P.mu;
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    double V[nd];
+    for (int d = 0; d < D; ++d)
+        for (int i = 0; i < ne; ++i) V[i + d * ne] = states[nodes[i] + d * P.N];
+    double BV[nv];
+    for (int r = 0; r < nv; ++r) {
+        double t = 0;
+        for (int c = 0; c < nd; ++c) t += Bm[r][c] * V[c];
+        BV[r] = t;
+    }
+    double q = 0;
+    for (int r = 0; r < nv; ++r) {
+        double t = 0;
+        for (int c = 0; c < nv; ++c) t += dd[r][c] * BV[c];
+        q += BV[r] * t;
+    }
+    const double gammaDot = std::sqrt(q);
+    double muEq = P.mat.tau0;
+    if (gammaDot < 1e-15)
+        muEq *= P.mat.mReg;
+    else
+        muEq *= (1 - std::exp(-P.mat.mReg * gammaDot)) / gammaDot;
+    return P.mu + muEq;
+}
 
 // m_computeTauPSPG: PSPG.inl:238-259 (node states are the current Picard iterate).
 template <int D>
@@ -350,7 +394,7 @@ static void pspgElement(const Params& P, const MB<D>& mb, const double* pos, con
     double dd[nv][nv];
     ddevIncomp<D>(dd);
     double Ke[nd][nd];
-    mb.K(geo.detJ, P.mu, dd, Bm, Ke);
+    mb.K(geo.detJ, kFactor<D>(P, Bm, dd, nodes, qIter), dd, Bm, Ke);
     double De[ne][nd];
     mb.Dm(geo.detJ, [](const double*) { return 1.0; }, Bm, De);
     double Ce[ne][nd];
@@ -1490,6 +1534,13 @@ void orc_wc_parts(int dim, int64_t N, int64_t E, const int* conn, const double* 
         wcContAssemble<3>(W, dt, pos, v, p, ML, F0);
         wcMomAssemble<3>(W, pos, v, p, rho, Mv, F);
     }
+}
+
+// Material law of the PSPG entry points that follow (problem 0 Newtonian, 1 Bingham).
+void orc_set_material(int problem, double tau0, double m_reg) {
+    g_mat.problem = problem;
+    g_mat.tau0 = tau0;
+    g_mat.mReg = m_reg;
 }
 
 }  // extern "C"

@@ -65,7 +65,7 @@ class Case:
     def __init__(self, ctx: Context, dim: int, kind: str = "PSPG", rho=1000.0, mu=1e-3, gamma=0.0, body_force=None,
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
-                 inner_rtol=1e-3, inner_max_iter=300, precond="auto"):
+                 inner_rtol=1e-3, inner_max_iter=300, precond="auto", problem="Newtonian", tau0=0.0, mReg=0.0):
         d = _lib.CaseDesc()
         d.dim = dim
         d.kind = _lib.KIND_PSPG if kind == "PSPG" else _lib.KIND_WC
@@ -90,6 +90,9 @@ class Case:
         if precond not in _lib.PRECOND:
             raise PfemError(_lib.PFEM_E_INVALID, "unknown preconditioner: " + str(precond))
         d.precond = _lib.PRECOND[precond]
+        if problem not in _lib.PROBLEM:
+            raise PfemError(_lib.PFEM_E_INVALID, "unknown problem: " + str(problem))
+        d.problem, d.tau0, d.m_reg = _lib.PROBLEM[problem], tau0, mReg
         self.ctx = ctx
         self.dim = dim
         self.kind = kind

@@ -27,6 +27,7 @@ PREC_FP64 = 0
 PREC_MIXED = 1
 PREC_MIXED_IR = 2
 PRECISION = {"fp64": PREC_FP64, "mixed": PREC_MIXED, "mixed_ir": PREC_MIXED_IR}
+PROBLEM = {"Newtonian": 0, "Bingham": 1}
 PRECOND = {"auto": 0, "jacobi": 1, "amg": 2}
 STATE_FP64 = 0
 STATE_FP32 = 1
@@ -53,6 +54,7 @@ class CaseDesc(C.Structure):
         ("meduri", C.c_int32), ("wc_state", C.c_int32), ("residual", C.c_int32), ("max_iter", C.c_int32),
         ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
         ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("precond", C.c_int32),
+        ("problem", C.c_int32), ("tau0", C.c_double), ("m_reg", C.c_double),
     ]
 
 

@@ -297,6 +297,10 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
             PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED_IR || (d.inner_rtol > 0 && d.inner_max_iter > 0),
                          PFEM_E_INVALID,
                          "inner_rtol and inner_max_iter must be > 0");
+            PFEM_REQUIRE(d.problem == PFEM_PROBLEM_NEWTONIAN || d.problem == PFEM_PROBLEM_BINGHAM, PFEM_E_INVALID,
+                         "unknown problem");
+            PFEM_REQUIRE(d.problem != PFEM_PROBLEM_BINGHAM || (d.tau0 >= 0 && d.m_reg > 0), PFEM_E_INVALID,
+                         "Bingham needs tau0 >= 0 and mReg > 0");
         } else {
             PFEM_REQUIRE(d.K0 > 0 && d.K0p > 0 && d.rho_star > 0, PFEM_E_INVALID, "invalid Tait parameters");
             PFEM_REQUIRE(d.wc_state == PFEM_STATE_FP64 || d.wc_state == PFEM_STATE_FP32, PFEM_E_INVALID,

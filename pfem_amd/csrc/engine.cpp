@@ -484,6 +484,9 @@ void Engine::Impl::load_views(Req* r) {
         pv.mu = c->d.mu;
         pv.dt = r->dt;
         for (int k = 0; k < 3; ++k) pv.g[k] = c->d.body_force[k];
+        pv.problem = c->d.problem;
+        pv.tau0 = c->d.tau0;
+        pv.mreg = c->d.m_reg;
 
         PicardView& pc = h_pic[slot];
         pc = PicardView{};

@@ -162,6 +162,38 @@ __device__ __forceinline__ void load_rec(const double* __restrict__ rec, ERec<D>
     }
 }
 
+// K factor of an element (MomContEquation.inl:100-123): mu, or for a Bingham fluid
+// mu + tau0 (1 - exp(-mReg g)) / g (tau0 mReg when g < 1e-15), with g = sqrt(V^T B^T ddev B V)
+// = sqrt(2 sum_k L_kk^2 + sum_{k<m} (L_km + L_mk)^2) for the velocity gradient L of the
+// element's node states (the current Picard iterate).
+template <int D>
+__device__ __forceinline__ double k_factor(const PspgView& V, const int (&nd)[D + 1], const Geom<D>& G) {
+    if (V.problem != 1) return V.mu;  // 1: PFEM_PROBLEM_BINGHAM
+    double Lg[D][D];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int m = 0; m < D; ++m) {
+            double t = 0;
+#pragma unroll
+            for (int a = 0; a <= D; ++a) t += G.gN[m][a] * V.q_iter[(long long)k * V.N + nd[a]];
+            Lg[k][m] = t;
+        }
+    double q = 0;
+#pragma unroll
+    for (int k = 0; k < D; ++k) q += 2.0 * Lg[k][k] * Lg[k][k];
+#pragma unroll
+    for (int k = 0; k < D; ++k)
+#pragma unroll
+        for (int m = k + 1; m < D; ++m) {
+            const double sh = Lg[k][m] + Lg[m][k];
+            q += sh * sh;
+        }
+    const double gd = sqrt(q);
+    const double mu_eq = gd < 1e-15 ? V.tau0 * V.mreg : V.tau0 * ((1.0 - exp(-V.mreg * gd)) / gd);
+    return V.mu + mu_eq;
+}
+
 // One lane per element: the element record and the velocity sums.
 template <int D>
 __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict__ views,
@@ -196,7 +228,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict
 #pragma unroll
             for (int k = 0; k < D; ++k) rec[(a - 1) * D + k] = G.gN[k][a];
         rec[R_VOLNE] = vol_ne;
-        rec[R_MUVOL] = V.mu * vol;
+        rec[R_MUVOL] = k_factor<D>(V, nd, G) * vol;
         rec[R_MAB] = V.rho * vol / (double)((D + 1) * (D + 2)) / V.dt;
         rec[R_TVN] = G.tau / V.dt * vol_ne;
         rec[R_TRV] = G.tau / V.rho * vol;

@@ -35,6 +35,8 @@ struct PspgView {
     long long nnz;
     double rho, mu, dt;
     double g[3];
+    int problem, pad_problem;  // PFEM_PROBLEM_*
+    double tau0, mreg;         // Bingham parameters
 };
 
 // The assembled PSPG matrix seen through its node structure.  Values are the CSR values

@@ -357,3 +357,47 @@ def test_ell_copy_matches_node_block_values(ctx, name, precision):
         scale = np.abs(y_csr).max()
         assert np.abs(y_ell - y_csr).max() <= 1e-13 * scale
     c.close()
+
+
+@pytest.mark.parametrize("name", ["2d_dambreak", "3d_column"])
+def test_bingham_assembly_and_step_parity(ctx, name):
+    """Bingham fluid (MomContEquation.inl:100-123): the K factor of every element is
+    mu + mu_eq of its strain rate in the current iterate; assembly and a Picard step against
+    the oracle."""
+    mesh = small_meshes()[name]
+    d, N = mesh.dim, mesh.n_nodes
+    dt = 1e-3 if d == 2 else 5e-4
+    tau0, mreg = 5.0, 200.0
+    qprev, qiter = rand_state(mesh, 21), rand_state(mesh, 22)
+    say(f"[bingham] {name} N={N}")
+    with O.material("Bingham", tau0=tau0, mReg=mreg):
+        ref = O.pspg_assemble(mesh, qprev, qiter, dt=dt)
+    c = pfem_amd.Case(ctx, d, kind="PSPG", problem="Bingham", tau0=tau0, mReg=mreg, krylov_rtol=1e-11)
+    c.set_mesh(mesh)
+    c.assemble(dt, q_prev=qprev, q_iter=qiter)
+    rp, ci, va = c.csr()
+    np.testing.assert_array_equal(rp, ref.row_ptr)
+    np.testing.assert_array_equal(ci, ref.col_idx)
+    assert row_rel_err(rp, ref.vals, va) <= 1e-12
+    assert block_rel_err(d, N, ref.b, c.rhs()) <= 1e-12
+    # the Newtonian system differs (the yield stress matters on this field)
+    newt = O.pspg_assemble(mesh, qprev, qiter, dt=dt)
+    assert row_rel_err(rp, newt.vals, va) > 1e-6
+    c.close()
+    # one Picard step from the hydrostatic state
+    q0 = mg.pspg_hydrostatic(mesh)
+    # the viscosity follows the iterate, so the loop needs more passes than a Newtonian one
+    # (about 21 on the 2D case at minRes 1e-6)
+    eq = pfem_amd.MomContEqIncompNewton(ctx, d, problem="Bingham", tau0=tau0, mReg=mreg, krylov_rtol=1e-11,
+                                        max_iter=30)
+    eq.set_mesh(mesh)
+    eq.set_state(q0)
+    assert eq.solve(dt), eq.last_stats
+    pos = mesh.pos.reshape(-1).copy()
+    q = q0.copy()
+    with O.material("Bingham", tau0=tau0, mReg=mreg):
+        st, info = O.pspg_step(mesh, pos, q, dt=dt, solver_kind=2, rtol=1e-13, max_iter=30)
+    assert st == 0
+    assert abs(eq.last_stats["picard_iters"] - info["picard_iters"]) <= 1
+    assert block_rel_err(d, N, q, eq.get_state()) <= 1e-6
+    eq.close()

@@ -441,3 +441,37 @@ def test_picard_position_dependent_dirichlet_fixed_point():
     np.testing.assert_allclose(P[:, left], x0 + dt * v, rtol=0, atol=1e-15)
     fixed = (m.flags & mg.FIXED) != 0
     np.testing.assert_array_equal(P[:, fixed], m.pos[:, fixed])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_bingham_k_factor_closed_form(dim):
+    """Bingham (MomContEquation.inl:100-123): only K changes, by mu_eq / mu; with an affine
+    velocity the strain rate is uniform, gammaDot^2 = 2 sum eps_ii^2 + sum gamma_ij^2 (ddev of
+    the incompressible solver), and mu_eq = tau0 (1 - exp(-m gammaDot)) / gammaDot; at rest
+    mu_eq = tau0 m."""
+    ne = dim + 1
+    nd = dim * ne
+    P = rand_simplex(dim, scale=0.05)
+    A = np.array([[0.3, -1.2, 0.5], [0.8, -0.4, 0.2], [-0.6, 0.9, 0.1]])[:dim, :dim]
+    vel = A @ P  # dim x ne
+    qiter = np.concatenate([vel.reshape(-1), np.zeros(ne)])
+    qprev = RNG.normal(size=(dim + 1) * ne)
+    mu, tau0, m = 1e-3, 7.0, 300.0
+    eps = 0.5 * (A + A.T)
+    gd = math.sqrt(2 * np.sum(np.diag(eps) ** 2) + sum((A[i, j] + A[j, i]) ** 2 for i in range(dim)
+                                                        for j in range(i + 1, dim)))
+    for vel_scale, mu_eq in ((1.0, tau0 * (1 - math.exp(-m * gd)) / gd), (0.0, tau0 * m)):
+        qi = qiter * vel_scale
+        An, bn, taun = O.pspg_element(dim, P, qprev, qi, mu=mu)
+        with O.material("Bingham", tau0=tau0, mReg=m):
+            Ab, bb, taub = O.pspg_element(dim, P, qprev, qi, mu=mu)
+        assert taub == taun  # tau keeps the material mu (MomContEquationPSPG.inl:257-258)
+        np.testing.assert_array_equal(bb, bn)
+        # non-momentum blocks unchanged; momentum block = M/dt + K(mu + mu_eq)
+        np.testing.assert_array_equal(Ab[nd:], An[nd:])
+        np.testing.assert_array_equal(Ab[:nd, nd:], An[:nd, nd:])
+        vol = volume(P)
+        Mdt = 1000.0 * vol * (np.ones((ne, ne)) + np.eye(ne)) / ((dim + 1) * (dim + 2)) / 1e-3
+        Kn = An[:nd, :nd] - np.kron(np.eye(dim), Mdt)
+        Kb = Ab[:nd, :nd] - np.kron(np.eye(dim), Mdt)
+        np.testing.assert_allclose(Kb, Kn * (mu + mu_eq) / mu, rtol=1e-9, atol=1e-9 * np.abs(Kb).max())
