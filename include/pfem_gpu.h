@@ -334,6 +334,66 @@ int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
 int pfem_case_reserve_workspace(pfem_case* c);
 int pfem_case_release_workspace(pfem_case* c);
 
+/* ---- Host remesher (no GPU work) ------------------------------------------------------------
+ * The reference's Mesh class after loadFromFile: triangulateAlphaShape (Mesh2D.cpp:32-251,
+ * Mesh3D.cpp:36-286) and remesh (Mesh.cpp:919-926 = laplacianSmoothingBoundaries, addNodes,
+ * removeNodes, checkBoundingBox, triangulateAlphaShape), without CGAL: an exact-predicate
+ * Delaunay triangulation with the lexicographic symbolic perturbation, whose element set is the
+ * one CGAL's builds.  Elements come out in a canonical order (sorted node tuples, smallest node
+ * first, positive orientation), not CGAL's iteration order; the nodes addNodes appends are
+ * numbered in that order.  One mesher per case; distinct meshers may run on distinct host
+ * threads concurrently. */
+typedef struct pfem_mesher pfem_mesher;
+
+typedef struct pfem_mesher_desc {
+    int32_t dim;
+    double hchar, alpha, omega, gamma;   /* MeshCreateInfo (Mesh.hpp:19-32): Mesh.hchar, alpha, omega, gamma */
+    int32_t add_on_fs;                   /* Mesh.addOnFS */
+    int32_t delete_flying_nodes;         /* Mesh.deleteFlyingNodes */
+    int32_t laplacian_smoothing_boundaries; /* Mesh.laplacianSmoothingBoundaries */
+    int32_t has_bounding_box;            /* 0: no node is removed for leaving a box */
+    double bounding_box[6];              /* Mesh.boundingBox: [xmin, ymin, (zmin,) xmax, ymax, (zmax)] */
+    int32_t n_exclusion_zones;           /* Mesh.exclusionZones: elements whose centre lies in one are dropped */
+    const double* exclusion_zones;       /* n_exclusion_zones x 2 dim, bounding-box layout (copied) */
+} pfem_mesher_desc;
+
+/* Mesh::Mesh (Mesh.cpp:12-25) without the file read (gmsh_reader.py builds the node cloud). */
+int pfem_mesher_create(const pfem_mesher_desc* desc, pfem_mesher** out);
+int pfem_mesher_destroy(pfem_mesher* m);
+/* The node cloud: pos dim x n, states nstates x n (field-blocked, nullable), flags n bytes of
+ * PFEM_NODE_* (BOUND, FIXED, DIRICHLET are kept; FREE and ONFS are computed), tags n (nullable;
+ * Node::m_tag, -1 for nodes remeshing adds).  Clears the triangulation. */
+int pfem_mesher_set_nodes(pfem_mesher* m, int32_t n, int32_t nstates, const double* pos, const double* states,
+                          const uint8_t* flags, const int32_t* tags);
+/* Positions and node states after time steps, same node count (either pointer nullable). */
+int pfem_mesher_set_positions_states(pfem_mesher* m, const double* pos, const double* states);
+/* Mesh::triangulateAlphaShape (Mesh.cpp:1055-1061). */
+int pfem_mesher_triangulate(pfem_mesher* m);
+/* Mesh::remesh (Mesh.cpp:919-926); needs a triangulation. */
+int pfem_mesher_remesh(pfem_mesher* m);
+int pfem_mesher_sizes(pfem_mesher* m, int32_t* n_nodes, int32_t* n_elems, int32_t* n_facets);
+/* pos dim x N, states nstates x N, flags N, tags N: any may be null. */
+int pfem_mesher_get_nodes(pfem_mesher* m, double* pos, double* states, uint8_t* flags, int32_t* tags);
+/* E x (dim+1) node indices, detJ > 0. */
+int pfem_mesher_get_elements(pfem_mesher* m, int32_t* conn);
+/* Free-surface facets (Mesh2D.cpp:192-242, Mesh3D.cpp:215-282): nodes F x dim (oriented so that
+ * the out node, the element's vertex opposite the facet, is on the positive side), out_node F,
+ * elem F; any may be null. */
+int pfem_mesher_get_facets(pfem_mesher* m, int32_t* nodes, int32_t* out_node, int32_t* elem);
+/* Last triangulate / remesh: {nodes added, removed, out of the box, flying nodes deleted,
+ * duplicate positions not inserted, seconds in node operations, in the Delaunay triangulation,
+ * in the alpha shape and adjacency}. */
+int pfem_mesher_stats(pfem_mesher* m, double* out8);
+/* Remeshes a case between steps: reads its positions and node states (PSPG (dim+1) states per
+ * node, WC 2 dim + 2), hands them to the mesher (which holds the case's last triangulation),
+ * remeshes, and sets the new mesh and states on the case (pfem_case_set_mesh,
+ * pfem_case_set_state).  Dirichlet values are per node: set them again afterwards.  Replaces
+ * the m_pMesh->remesh call of SolverIncompNewton::m_solveIncompNewtonNoT
+ * (IncompNewton/Solver.cpp:236-246) and SolverWCompNewton (WCompNewton/Solver.cpp:262-270). */
+int pfem_case_remesh(pfem_case* c, pfem_mesher* m);
+/* Sets the mesher's current mesh and node states on a case (after pfem_mesher_triangulate). */
+int pfem_case_set_mesh_from(pfem_case* c, pfem_mesher* m);
+
 #ifdef __cplusplus
 }
 #endif

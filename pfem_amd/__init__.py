@@ -125,6 +125,17 @@ class Case:
         self.N = mesh.n_nodes
         self.E = mesh.n_elems
 
+    def set_mesh_from(self, mesher: "Mesher"):
+        """The mesher's current mesh and node states (after Mesher.triangulate)."""
+        check(lib().pfem_case_set_mesh_from(self._h, mesher._h))
+        self.N, self.E, _ = mesher.sizes()
+
+    def remesh(self, mesher: "Mesher"):
+        """Mesh::remesh between steps (pfem_case_remesh): positions and node states go to the
+        mesher, the new mesh and states come back; Dirichlet values must be set again."""
+        check(lib().pfem_case_remesh(self._h, mesher._h))
+        self.N, self.E, _ = mesher.sizes()
+
     @property
     def n_state(self):
         return (self.dim + 1) * self.N if self.kind == "PSPG" else (2 * self.dim + 2) * self.N
@@ -452,3 +463,110 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+class Mesher:
+    """The reference's Mesh after loadFromFile (srcs/mesh/Mesh.hpp): alpha-shape triangulation
+    and remeshing of a node cloud on the host (pfem_mesher_*).  Keyword names follow the Lua
+    Mesh table (Problem.cpp:56-70): hchar, alpha, omega, gamma, boundingBox, exclusionZones,
+    addOnFS, deleteFlyingNodes, laplacianSmoothingBoundaries."""
+
+    def __init__(self, dim: int, hchar: float, alpha: float = 1.0, omega: float = 1e16, gamma: float = 0.0,
+                 boundingBox=None, exclusionZones=(), addOnFS: bool = True, deleteFlyingNodes: bool = False,
+                 laplacianSmoothingBoundaries: bool = False):
+        d = _lib.MesherDesc()
+        d.dim, d.hchar, d.alpha, d.omega, d.gamma = dim, hchar, alpha, omega, gamma
+        d.add_on_fs, d.delete_flying_nodes = int(addOnFS), int(deleteFlyingNodes)
+        d.laplacian_smoothing_boundaries = int(laplacianSmoothingBoundaries)
+        if boundingBox is not None:
+            if len(boundingBox) != 2 * dim:
+                raise PfemError(_lib.PFEM_E_INVALID, f"Invalid bounding box size: {len(boundingBox)}")
+            d.has_bounding_box = 1
+            for k, v in enumerate(boundingBox):
+                d.bounding_box[k] = float(v)
+        zones = np.ascontiguousarray(np.asarray(exclusionZones, dtype=np.float64).reshape(-1, 2 * dim))
+        self._zones = zones
+        d.n_exclusion_zones = zones.shape[0]
+        d.exclusion_zones = _dp(zones) if zones.shape[0] else None
+        self.dim = dim
+        self._h = C.c_void_p()
+        check(lib().pfem_mesher_create(C.byref(d), C.byref(self._h)))
+        self.n_states = 0
+
+    def close(self):
+        if self._h:
+            lib().pfem_mesher_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_nodes(self, pos, flags, states=None, tags=None):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        n = pos.shape[1]
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        ns = 0
+        st = None
+        if states is not None:
+            st = np.ascontiguousarray(states, dtype=np.float64).reshape(-1)
+            ns = st.size // n
+        tg = None if tags is None else np.ascontiguousarray(tags, dtype=np.int32)
+        check(lib().pfem_mesher_set_nodes(self._h, n, ns, _dp(pos), _dp(st) if st is not None else None, _u8(flags),
+                                          _ip(tg) if tg is not None else None))
+        self.n_states = ns
+
+    def set_positions_states(self, pos=None, states=None):
+        p = None if pos is None else np.ascontiguousarray(pos, dtype=np.float64)
+        s = None if states is None else np.ascontiguousarray(states, dtype=np.float64)
+        check(lib().pfem_mesher_set_positions_states(self._h, _dp(p) if p is not None else None,
+                                                     _dp(s) if s is not None else None))
+
+    def triangulate(self):
+        check(lib().pfem_mesher_triangulate(self._h))
+
+    def remesh(self):
+        check(lib().pfem_mesher_remesh(self._h))
+
+    def sizes(self):
+        n, e, f = C.c_int32(), C.c_int32(), C.c_int32()
+        check(lib().pfem_mesher_sizes(self._h, C.byref(n), C.byref(e), C.byref(f)))
+        return n.value, e.value, f.value
+
+    def nodes(self):
+        """(pos dim x N, states nstates x N, flags N, tags N)."""
+        n, _, _ = self.sizes()
+        pos = np.empty((self.dim, n))
+        st = np.empty(max(self.n_states, 1) * n)
+        fl = np.empty(n, dtype=np.uint8)
+        tg = np.empty(n, dtype=np.int32)
+        check(lib().pfem_mesher_get_nodes(self._h, _dp(pos), _dp(st), _u8(fl), _ip(tg)))
+        return pos, st[: self.n_states * n].reshape(self.n_states, n), fl, tg
+
+    def elements(self) -> np.ndarray:
+        _, e, _ = self.sizes()
+        conn = np.empty((e, self.dim + 1), dtype=np.int32)
+        check(lib().pfem_mesher_get_elements(self._h, _ip(conn)))
+        return conn
+
+    def facets(self):
+        """(nodes F x dim, out node F, element F) of the free-surface facets."""
+        _, _, f = self.sizes()
+        nodes = np.empty((f, self.dim), dtype=np.int32)
+        out = np.empty(f, dtype=np.int32)
+        el = np.empty(f, dtype=np.int32)
+        check(lib().pfem_mesher_get_facets(self._h, _ip(nodes), _ip(out), _ip(el)))
+        return nodes, out, el
+
+    def stats(self) -> dict:
+        o = np.zeros(8)
+        check(lib().pfem_mesher_stats(self._h, _dp(o)))
+        keys = ["added", "removed", "out_of_box", "flying_deleted", "duplicates", "t_nodes", "t_delaunay", "t_alpha"]
+        return dict(zip(keys, o.tolist()))
+
+    def mesh(self, h: float | None = None, name: str = "") -> PfemMesh:
+        """The current triangulation as a PfemMesh (flags include ONFS and FREE)."""
+        pos, _, fl, _ = self.nodes()
+        return PfemMesh(self.dim, pos, self.elements(), fl, h if h is not None else 0.0, name=name)

@@ -43,7 +43,10 @@ EXPORTS = [
     "pfem_server_create", "pfem_server_destroy", "pfem_server_submit", "pfem_server_poll", "pfem_server_stats",
     "pfem_case_reserve_workspace", "pfem_case_release_workspace", "pfem_save_positions", "pfem_restore_positions",
     "pfem_update_positions", "pfem_residual", "pfem_pspg_commit", "pfem_get_iterate", "pfem_case_mass_noalloc",
-    "pfem_spmv_ell",
+    "pfem_spmv_ell", "pfem_mesher_create", "pfem_mesher_destroy", "pfem_mesher_set_nodes",
+    "pfem_mesher_set_positions_states", "pfem_mesher_triangulate", "pfem_mesher_remesh", "pfem_mesher_sizes",
+    "pfem_mesher_get_nodes", "pfem_mesher_get_elements", "pfem_mesher_get_facets", "pfem_mesher_stats",
+    "pfem_case_remesh", "pfem_case_set_mesh_from",
 ]
 
 
@@ -55,6 +58,15 @@ class CaseDesc(C.Structure):
         ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
         ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("precond", C.c_int32),
         ("problem", C.c_int32), ("tau0", C.c_double), ("m_reg", C.c_double),
+    ]
+
+
+class MesherDesc(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32), ("hchar", C.c_double), ("alpha", C.c_double), ("omega", C.c_double),
+        ("gamma", C.c_double), ("add_on_fs", C.c_int32), ("delete_flying_nodes", C.c_int32),
+        ("laplacian_smoothing_boundaries", C.c_int32), ("has_bounding_box", C.c_int32),
+        ("bounding_box", C.c_double * 6), ("n_exclusion_zones", C.c_int32), ("exclusion_zones", C.POINTER(C.c_double)),
     ]
 
 
@@ -155,6 +167,19 @@ def lib():
         "pfem_get_iterate": [_vp, _dp],
         "pfem_case_mass_noalloc": [_vp, _vp],
         "pfem_spmv_ell": [_vp, _dp, _dp],
+        "pfem_mesher_create": [C.POINTER(MesherDesc), C.POINTER(_vp)],
+        "pfem_mesher_destroy": [_vp],
+        "pfem_mesher_set_nodes": [_vp, C.c_int32, C.c_int32, _dp, _dp, _u8p, _ip],
+        "pfem_mesher_set_positions_states": [_vp, _dp, _dp],
+        "pfem_mesher_triangulate": [_vp],
+        "pfem_mesher_remesh": [_vp],
+        "pfem_mesher_sizes": [_vp, _ip, _ip, _ip],
+        "pfem_mesher_get_nodes": [_vp, _dp, _dp, _u8p, _ip],
+        "pfem_mesher_get_elements": [_vp, _ip],
+        "pfem_mesher_get_facets": [_vp, _ip, _ip, _ip],
+        "pfem_mesher_stats": [_vp, _dp],
+        "pfem_case_remesh": [_vp, _vp],
+        "pfem_case_set_mesh_from": [_vp, _vp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -12,7 +12,10 @@ BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfem_gpu.so")
 SOURCES = ["pattern.hip", "ell.hip", "pspg.hip", "krylov.hip", "picard.hip", "wc.hip", "extract.hip", "amg.hip", "microbench.hip",
            "amg_hierarchy.cpp",
-           "engine.cpp", "capi.cpp"]
+           "engine.cpp", "capi.cpp", "mesher.cpp"]
+# host-only translation units (no device code): plain C++, no floating-point contraction (the
+# mesher's predicates bound the rounding of every product and sum separately)
+HOST_ONLY = {"mesher.cpp"}
 ARCH = os.environ.get("PFEM_OFFLOAD_ARCH", "gfx950")
 
 CXXFLAGS = [
@@ -31,7 +34,11 @@ def _hipcc() -> str:
 
 def _compile(src: str) -> str:
     obj = os.path.join(BUILD, os.path.splitext(src)[0] + ".o")
-    cmd = [_hipcc(), *CXXFLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
+    if src in HOST_ONLY:
+        flags = [f for f in CXXFLAGS if not f.startswith("--offload-arch") and f != "-munsafe-fp-atomics"]
+        cmd = [_hipcc(), *flags, "-ffp-contract=off", "-x", "c++", "-c", os.path.join(CSRC, src), "-o", obj]
+    else:
+        cmd = [_hipcc(), *CXXFLAGS, "-x", "hip", "-c", os.path.join(CSRC, src), "-o", obj]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stdout}\n{r.stderr}")

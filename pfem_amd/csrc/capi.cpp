@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "mesher.hpp"
 #include "runtime.hpp"
 
 using pfem::DevBuf;
@@ -1138,6 +1139,179 @@ int pfem_server_stats(pfem_server* S, int64_t* out8) {
         out8[6] = S->arena_used;
         out8[7] = (int64_t)S->incoming.size();
         return PFEM_OK;
+    });
+}
+
+// ---- host remesher ----------------------------------------------------------------------------
+struct pfem_mesher {
+    pfem::mesh::Mesher m;
+    explicit pfem_mesher(const pfem::mesh::MesherParams& p) : m(p) {}
+};
+
+int pfem_mesher_create(const pfem_mesher_desc* d, pfem_mesher** out) {
+    return guarded([&] {
+        PFEM_REQUIRE(d && out, PFEM_E_INVALID, "null argument");
+        PFEM_REQUIRE(d->dim == 2 || d->dim == 3, PFEM_E_INVALID, "dim must be 2 or 3");
+        PFEM_REQUIRE(d->hchar > 0 && d->alpha > 0 && d->omega > 0 && d->gamma >= 0, PFEM_E_INVALID,
+                     "hchar, alpha, omega must be > 0 and gamma >= 0");
+        PFEM_REQUIRE(d->n_exclusion_zones >= 0 && (d->n_exclusion_zones == 0 || d->exclusion_zones), PFEM_E_INVALID,
+                     "invalid exclusion zones");
+        pfem::mesh::MesherParams p;
+        p.dim = d->dim;
+        p.hchar = d->hchar;
+        p.alpha = d->alpha;
+        p.omega = d->omega;
+        p.gamma = d->gamma;
+        p.add_on_fs = d->add_on_fs != 0;
+        p.delete_flying_nodes = d->delete_flying_nodes != 0;
+        p.laplacian_smoothing_boundaries = d->laplacian_smoothing_boundaries != 0;
+        p.has_bounding_box = d->has_bounding_box != 0;
+        for (int k = 0; k < 2 * d->dim; ++k) p.bounding_box[k] = d->bounding_box[k];
+        for (int z = 0; z < d->n_exclusion_zones; ++z) {
+            std::array<double, 6> zone{};
+            for (int k = 0; k < 2 * d->dim; ++k) zone[k] = d->exclusion_zones[(size_t)z * 2 * d->dim + k];
+            p.exclusion_zones.push_back(zone);
+        }
+        *out = new pfem_mesher(p);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_destroy(pfem_mesher* m) {
+    delete m;
+    return PFEM_OK;
+}
+
+int pfem_mesher_set_nodes(pfem_mesher* m, int32_t n, int32_t nstates, const double* pos, const double* states,
+                          const uint8_t* flags, const int32_t* tags) {
+    return guarded([&] {
+        PFEM_REQUIRE(m && pos && flags, PFEM_E_INVALID, "null argument");
+        PFEM_REQUIRE(n > m->m.params().dim && nstates >= 0, PFEM_E_INVALID, "invalid node count");
+        m->m.set_nodes(n, nstates, pos, states, flags, tags);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_set_positions_states(pfem_mesher* m, const double* pos, const double* states) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        m->m.set_positions_states(pos, states);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_triangulate(pfem_mesher* m) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        PFEM_REQUIRE(m->m.n_nodes() > 0, PFEM_E_STATE, "no nodes set");
+        m->m.triangulate();
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_remesh(pfem_mesher* m) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        PFEM_REQUIRE(m->m.n_elems() > 0, PFEM_E_STATE, "triangulate the mesh before remeshing it");
+        m->m.remesh();
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_sizes(pfem_mesher* m, int32_t* n_nodes, int32_t* n_elems, int32_t* n_facets) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        if (n_nodes) *n_nodes = m->m.n_nodes();
+        if (n_elems) *n_elems = m->m.n_elems();
+        if (n_facets) *n_facets = m->m.n_facets();
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_get_nodes(pfem_mesher* m, double* pos, double* states, uint8_t* flags, int32_t* tags) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        m->m.get_nodes(pos, states, flags, tags);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_get_elements(pfem_mesher* m, int32_t* conn) {
+    return guarded([&] {
+        PFEM_REQUIRE(m && conn, PFEM_E_INVALID, "null argument");
+        std::copy(m->m.elems().begin(), m->m.elems().end(), conn);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_get_facets(pfem_mesher* m, int32_t* nodes, int32_t* out_node, int32_t* elem) {
+    return guarded([&] {
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        if (nodes) std::copy(m->m.facets().begin(), m->m.facets().end(), nodes);
+        if (out_node) std::copy(m->m.facet_out().begin(), m->m.facet_out().end(), out_node);
+        if (elem) std::copy(m->m.facet_elem().begin(), m->m.facet_elem().end(), elem);
+        return PFEM_OK;
+    });
+}
+
+int pfem_mesher_stats(pfem_mesher* m, double* out8) {
+    return guarded([&] {
+        PFEM_REQUIRE(m && out8, PFEM_E_INVALID, "null argument");
+        const auto& M = m->m;
+        out8[0] = M.added_;
+        out8[1] = M.removed_;
+        out8[2] = M.out_of_box_;
+        out8[3] = M.flying_deleted_;
+        out8[4] = M.skipped_;
+        out8[5] = M.t_nodes_;
+        out8[6] = M.t_delaunay_;
+        out8[7] = M.t_alpha_;
+        return PFEM_OK;
+    });
+}
+
+static int mesher_states_per_node(pfem_case* c) {
+    return c->d.kind == PFEM_KIND_PSPG ? c->dim + 1 : 2 * c->dim + 2;
+}
+
+int pfem_case_set_mesh_from(pfem_case* c, pfem_mesher* m) {
+    return guarded([&] {
+        require_case(c);
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        const auto& M = m->m;
+        PFEM_REQUIRE(M.params().dim == c->dim, PFEM_E_INVALID, "mesher and case dimensions differ");
+        PFEM_REQUIRE(M.n_states() == mesher_states_per_node(c), PFEM_E_INVALID,
+                     "the mesher's node states do not match the case kind");
+        PFEM_REQUIRE(M.n_elems() > 0, PFEM_E_STATE, "the mesher holds no triangulation");
+        const int N = M.n_nodes(), E = M.n_elems(), ns = M.n_states();
+        std::vector<double> pos((size_t)c->dim * N), st((size_t)ns * N);
+        std::vector<uint8_t> fl(N);
+        M.get_nodes(pos.data(), st.data(), fl.data(), nullptr);
+        int rc = pfem_case_set_mesh(c, N, E, M.elems().data(), fl.data(), pos.data());
+        if (rc != PFEM_OK) return rc;
+        return pfem_case_set_state(c, st.data());
+    });
+}
+
+int pfem_case_remesh(pfem_case* c, pfem_mesher* m) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(m != nullptr, PFEM_E_INVALID, "null mesher");
+        auto& M = m->m;
+        PFEM_REQUIRE(M.params().dim == c->dim, PFEM_E_INVALID, "mesher and case dimensions differ");
+        PFEM_REQUIRE(M.n_nodes() == c->N && M.n_states() == mesher_states_per_node(c), PFEM_E_STATE,
+                     "the mesher does not hold this case's nodes (set them and triangulate first)");
+        PFEM_REQUIRE(M.n_elems() > 0, PFEM_E_STATE, "the mesher holds no triangulation");
+        const int N = c->N, ns = M.n_states();
+        std::vector<double> pos((size_t)c->dim * N), st((size_t)ns * N);
+        int rc = pfem_case_get_positions(c, pos.data());
+        if (rc != PFEM_OK) return rc;
+        rc = pfem_case_get_state(c, st.data());
+        if (rc != PFEM_OK) return rc;
+        M.set_positions_states(pos.data(), st.data());
+        M.remesh();
+        rc = pfem_case_set_mesh_from(c, m);
+        return rc;
     });
 }
 
