@@ -393,6 +393,54 @@ class PicardAlgo:
         return True
 
 
+class SolverIncompNewton:
+    """SolverIncompNewton with Solver.id = "PSPG" (srcs/simulation/physics/IncompNewton/Solver.cpp):
+    solveOneTimeStep = one PSPG time step, then, on success, the time advances and the mesh is
+    remeshed (m_solveIncompNewtonNoT, Solver.cpp:236-246); computeNextDT (Solver.cpp:217-232)
+    divides dt by coeffDTDecrease after a failure and multiplies it by coeffDTincrease (capped
+    at maxDT) after a success when adaptDT, and raises on a failure otherwise.
+
+    The step is the fused pfem_pspg_step, or PicardAlgo when ``dirichlet(pos)`` is given (it is
+    then re-evaluated before every assembly and after every remesh, as the reference's Lua
+    BC function is)."""
+
+    def __init__(self, case: Case, mesher: Mesher, initial_dt: float, max_dt: float, adapt_dt: bool = True,
+                 coeff_dt_increase: float = 1.5, coeff_dt_decrease: float = 2.0, dirichlet=None, remesh: bool = True):
+        self.case, self.mesher = case, mesher
+        self.dt = initial_dt
+        self.max_dt, self.adapt_dt = max_dt, adapt_dt
+        self.coeff_inc, self.coeff_dec = coeff_dt_increase, coeff_dt_decrease
+        self.dirichlet = dirichlet
+        self.remesh_enabled = remesh
+        self.time = 0.0
+        self.succeeded = False
+        self.picard = PicardAlgo(case, dirichlet=dirichlet) if dirichlet is not None else None
+
+    def solveOneTimeStep(self) -> bool:  # noqa: N802  (reference name)
+        if self.picard is not None:
+            ok = self.picard.solve(self.dt)
+        else:
+            rc, _ = self.case.pspg_step(self.dt)
+            ok = rc == _lib.PFEM_OK
+        self.succeeded = ok
+        if ok:
+            self.time += self.dt
+            if self.remesh_enabled:
+                self.case.remesh(self.mesher)
+                if self.dirichlet is not None:
+                    self.case.set_dirichlet(self.dirichlet(self.case.get_positions()))
+        return ok
+
+    def computeNextDT(self):  # noqa: N802
+        if not self.succeeded:
+            if not self.adapt_dt:
+                raise PfemError(_lib.PFEM_NOT_CONVERGED, "solving the precedent time step was not successful!")
+            self.dt /= self.coeff_dec
+        elif self.adapt_dt:
+            self.dt = min(self.max_dt, self.dt * self.coeff_inc)
+        return self.dt
+
+
 class WCompNewtonStep(Case):
     """Weakly compressible CDS_dpdt solver step (WCompNewton/Solver.cpp)."""
 

@@ -179,9 +179,15 @@ private:
     int init_[4] = {-1, -1, -1, -1};
     std::mt19937 rng_;
     std::vector<int> cavity_, bfaces_, created_;
-    // ridge hash (linking new simplices): open addressing, cleared through the used list
-    std::vector<uint64_t> hkey_;
-    std::vector<int> hval_, hused_;
+    // ridge hash (linking new simplices): open addressing; an entry is live when its generation
+    // is the current one, so a new insertion starts from an empty table without clearing it
+    struct HEntry {
+        uint64_t key;
+        int val;
+        uint32_t gen;
+    };
+    std::vector<HEntry> ht_;
+    uint32_t gen_ = 0;
 
     const double* pt(int v) const { return P_ + (size_t)D * v; }
 
@@ -299,65 +305,53 @@ private:
             S_[s0].n[i] = s;
             inf_s[i] = s;
         }
-        link_by_ridges(inf_s, NV, /*skip_slot=*/D);
+        link_by_ridges(inf_s, NV);
         last_ = s0;
         return true;
     }
 
-    // ridge key of the face of simplex s opposite slot j, minus the vertex at slot skip
-    uint64_t ridge_key(int s, int j, int skip) const
+    // link the faces of the given simplices that hold their last vertex (slot D): every such
+    // face is shared by two of them.  The face opposite slot j < D holds the other D-1 vertices
+    // of the first D slots (its ridge): 2D the vertex at slot 1-j, 3D the edge of the other two.
+    void link_by_ridges(const int* simp, int count)
     {
-        int r[D - 1];
-        int c = 0;
-        for (int k = 0; k < NV; ++k)
-            if (k != j && k != skip) r[c++] = S_[s].v[k];
-        if constexpr (D == 2) {
-            return (uint64_t)(uint32_t)r[0];
-        } else {
-            const uint32_t a = (uint32_t)std::min(r[0], r[1]), b = (uint32_t)std::max(r[0], r[1]);
-            return ((uint64_t)a << 32) | b;
-        }
-    }
-
-    // link the faces of the given simplices that contain the vertex at slot `skip` (shared pairwise)
-    void link_by_ridges(const int* simp, int count, int skip)
-    {
-        size_t cap = 64;
+        size_t cap = 256;
         while (cap < (size_t)count * D * 2) cap <<= 1;
-        if (hkey_.size() < cap) {
-            hkey_.assign(cap, ~0ULL);
-            hval_.assign(cap, -1);
+        if (ht_.size() < cap) {
+            ht_.assign(cap, HEntry{0, 0, 0});
+            gen_ = 0;
         }
-        const size_t mask = hkey_.size() - 1;
+        if (++gen_ == 0) {
+            for (auto& e : ht_) e.gen = 0;
+            gen_ = 1;
+        }
+        const size_t mask = ht_.size() - 1;
         for (int c = 0; c < count; ++c) {
             const int s = simp[c];
-            for (int j = 0; j < NV; ++j) {
-                if (j == skip) continue;
-                const uint64_t key = ridge_key(s, j, skip);
-                size_t h = (key * 0x9E3779B97F4A7C15ULL) >> 20;
-                for (;; ++h) {
-                    const size_t b = h & mask;
-                    if (hkey_[b] == ~0ULL) {
-                        hkey_[b] = key;
-                        hval_[b] = s * NV + j;
-                        hused_.push_back((int)b);
-                        break;
-                    }
-                    if (hkey_[b] == key) {
-                        const int t = hval_[b] / NV, tj = hval_[b] % NV;
-                        S_[s].n[j] = t;
-                        S_[t].n[tj] = s;
-                        hkey_[b] = ~0ULL - 1;  // matched: a tombstone that no key equals
-                        break;
-                    }
+            const int* v = S_[s].v;
+            for (int j = 0; j < D; ++j) {
+                uint64_t key;
+                if constexpr (D == 2) {
+                    key = (uint32_t)v[1 - j];
+                } else {
+                    const int a = v[(j + 1) % 3], b = v[(j + 2) % 3];
+                    key = a < b ? ((uint64_t)(uint32_t)a << 32) | (uint32_t)b : ((uint64_t)(uint32_t)b << 32) | (uint32_t)a;
+                }
+                size_t h = (size_t)((key * 0x9E3779B97F4A7C15ULL) >> 32) & mask;
+                while (ht_[h].gen == gen_ && ht_[h].key != key) h = (h + 1) & mask;
+                HEntry& e = ht_[h];
+                if (e.gen == gen_) {
+                    const int t = e.val / NV, tj = e.val % NV;
+                    S_[s].n[j] = t;
+                    S_[t].n[tj] = s;
+                    e.key = ~0ULL;  // matched: stays occupied (probe chains intact), equals no key
+                } else {
+                    e.key = key;
+                    e.val = s * NV + j;
+                    e.gen = gen_;
                 }
             }
         }
-        for (int b : hused_) {
-            hkey_[b] = ~0ULL;
-            hval_[b] = -1;
-        }
-        hused_.clear();
     }
 
     int locate(const double* p)
@@ -445,7 +439,7 @@ private:
                 if (S_[out].n[k] == s) S_[out].n[k] = ns;
             created_.push_back(ns);
         }
-        link_by_ridges(created_.data(), (int)created_.size(), /*skip_slot=*/D);
+        link_by_ridges(created_.data(), (int)created_.size());
         for (int s : cavity_) {
             S_[s].v[0] = -1;
             free_.push_back(s);
@@ -528,12 +522,21 @@ static void canonical(int dim, int* v)
         std::copy(best, best + 3, v);
         return;
     }
-    static const int EVEN[12][4] = {{0, 1, 2, 3}, {0, 2, 3, 1}, {0, 3, 1, 2}, {1, 0, 3, 2}, {1, 2, 0, 3}, {1, 3, 2, 0},
-                                    {2, 0, 1, 3}, {2, 1, 3, 0}, {2, 3, 0, 1}, {3, 0, 2, 1}, {3, 1, 0, 2}, {3, 2, 1, 0}};
-    int best[4] = {v[0], v[1], v[2], v[3]};
-    for (int r = 1; r < 12; ++r) {
-        const int c[4] = {v[EVEN[r][0]], v[EVEN[r][1]], v[EVEN[r][2]], v[EVEN[r][3]]};
-        if (std::lexicographical_compare(c, c + 4, best, best + 4)) std::copy(c, c + 4, best);
+    // the even permutation that brings the smallest node to slot 0, then the cyclic rotation of
+    // the other three that brings their smallest to slot 1
+    static const int FIRST[4][4] = {{0, 1, 2, 3}, {1, 0, 3, 2}, {2, 0, 1, 3}, {3, 0, 2, 1}};
+    int k = 0;
+    for (int j = 1; j < 4; ++j)
+        if (v[j] < v[k]) k = j;
+    const int a = v[FIRST[k][1]], b = v[FIRST[k][2]], c = v[FIRST[k][3]];
+    int best[4];
+    best[0] = v[k];
+    if (a < b && a < c) {
+        best[1] = a; best[2] = b; best[3] = c;
+    } else if (b < c) {
+        best[1] = b; best[2] = c; best[3] = a;
+    } else {
+        best[1] = c; best[2] = a; best[3] = b;
     }
     std::copy(best, best + 4, v);
 }
@@ -633,23 +636,30 @@ void Mesher::triangulate()
     // vertex -> incident finite simplices (for the all-bound deletion rule)
     std::vector<int> vptr(n + 1, 0), vs;
     for (int s = 0; s < ns; ++s)
-        for (int k = 0; k < nv; ++k) vptr[simp[(size_t)s * nv + k] + 1]++;
+        for (int k = 0; k < nv; ++k) {
+            const int v = simp[(size_t)s * nv + k];
+            if (is_bound(v)) vptr[v + 1]++;
+        }
     for (int i = 0; i < n; ++i) vptr[i + 1] += vptr[i];
     vs.resize(vptr[n]);
     {
         std::vector<int> fill(vptr.begin(), vptr.end() - 1);
         for (int s = 0; s < ns; ++s)
-            for (int k = 0; k < nv; ++k) vs[fill[simp[(size_t)s * nv + k]]++] = s;
+            for (int k = 0; k < nv; ++k) {
+                const int v = simp[(size_t)s * nv + k];
+                if (is_bound(v)) vs[fill[v]++] = s;
+            }
     }
     const double h3 = prm_.hchar * prm_.hchar * prm_.hchar;
     // checkFaceDeletion / checkCellDeletion: true = delete
     auto deleted = [&](int s) -> bool {
         const int* v = &simp[(size_t)s * nv];
         double mean[3] = {0, 0, 0};
-        for (int k = 0; k < d; ++k) {
-            for (int j = 0; j < nv; ++j) mean[k] += pos_[(size_t)d * v[j] + k];
-            mean[k] /= nv;
-        }
+        if (!prm_.exclusion_zones.empty())
+            for (int k = 0; k < d; ++k) {
+                for (int j = 0; j < nv; ++j) mean[k] += pos_[(size_t)d * v[j] + k];
+                mean[k] /= nv;
+            }
         for (const auto& z : prm_.exclusion_zones) {
             bool in = true;
             for (int k = 0; k < d; ++k) in = in && mean[k] > z[k] && mean[k] < z[d + k];
@@ -677,17 +687,32 @@ void Mesher::triangulate()
     for (int s = 0; s < ns; ++s) keep[s] = interior[s] && !deleted(s);
     // 3. kept elements in canonical form and order
     std::vector<std::pair<std::array<int, 4>, int>> kept;
-    for (int s = 0; s < ns; ++s) {
-        if (!keep[s]) continue;
-        std::array<int, 4> v = {-1, -1, -1, -1};
-        for (int k = 0; k < nv; ++k) v[k] = simp[(size_t)s * nv + k];
-        canonical(d, v.data());
-        kept.push_back({v, s});
+    {
+        // bucket by the first (smallest) node, then sort each bucket: sorted tuples overall
+        std::vector<int> cnt(n + 1, 0);
+        std::vector<std::array<int, 4>> canon(ns);
+        int nk = 0;
+        for (int s = 0; s < ns; ++s) {
+            if (!keep[s]) continue;
+            std::array<int, 4> v = {-1, -1, -1, -1};
+            for (int k = 0; k < nv; ++k) v[k] = simp[(size_t)s * nv + k];
+            canonical(d, v.data());
+            canon[s] = v;
+            cnt[v[0] + 1]++;
+            ++nk;
+        }
+        if (nk == 0)
+            throw std::runtime_error("Something went wrong while remeshing. You might have not chosen a good \"hchar\" "
+                                     "with regard to your .msh file");
+        for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+        kept.resize(nk);
+        for (int s = 0; s < ns; ++s)
+            if (keep[s]) kept[cnt[canon[s][0]]++] = {canon[s], s};
+        for (int i = n; i > 0; --i) cnt[i] = cnt[i - 1];
+        cnt[0] = 0;
+        for (int i = 0; i < n; ++i)
+            if (cnt[i + 1] - cnt[i] > 1) std::sort(kept.begin() + cnt[i], kept.begin() + cnt[i + 1]);
     }
-    if (kept.empty())
-        throw std::runtime_error("Something went wrong while remeshing. You might have not chosen a good \"hchar\" with "
-                                 "regard to your .msh file");
-    std::sort(kept.begin(), kept.end());
     std::vector<int> elem_of(ns, -1);
     elems_.resize(kept.size() * nv);
     for (size_t e = 0; e < kept.size(); ++e) {
@@ -745,24 +770,27 @@ void Mesher::build_adjacency()
     const int E = (int)(elems_.size() / nv);
     nelem_.assign(n, 0);
     for (int v : elems_) nelem_[v]++;
-    // every incidence lists the element's other nodes; sort and dedupe each node's run in place
-    std::vector<int> ptr(n + 1, 0);
-    for (int i = 0; i < n; ++i) ptr[i + 1] = ptr[i] + nelem_[i] * d;
-    std::vector<int> all(ptr[n]), fill(ptr.begin(), ptr.end() - 1);
-    for (int e = 0; e < E; ++e) {
-        const int* v = &elems_[(size_t)e * nv];
-        for (int a = 0; a < nv; ++a)
-            for (int b = 0; b < nv; ++b)
-                if (a != b) all[fill[v[a]]++] = v[b];
-    }
+    // node -> elements, then each node's neighbours deduplicated with a stamp and sorted
+    std::vector<int> eptr(n + 1, 0);
+    for (int i = 0; i < n; ++i) eptr[i + 1] = eptr[i] + nelem_[i];
+    std::vector<int> eidx(eptr[n]), fill(eptr.begin(), eptr.end() - 1);
+    for (int e = 0; e < E; ++e)
+        for (int a = 0; a < nv; ++a) eidx[fill[elems_[(size_t)e * nv + a]]++] = e;
+    std::vector<int> seen(n, -1);
     nbr_ptr_.assign(n + 1, 0);
     nbr_.clear();
-    nbr_.reserve(all.size() / 2);
+    nbr_.reserve((size_t)eptr[n] * 2);
     for (int i = 0; i < n; ++i) {
-        auto b = all.begin() + ptr[i], e = all.begin() + ptr[i + 1];
-        std::sort(b, e);
-        auto u = std::unique(b, e);
-        nbr_.insert(nbr_.end(), b, u);
+        const size_t b = nbr_.size();
+        for (int q = eptr[i]; q < eptr[i + 1]; ++q) {
+            const int* v = &elems_[(size_t)eidx[q] * nv];
+            for (int a = 0; a < nv; ++a)
+                if (v[a] != i && seen[v[a]] != i) {
+                    seen[v[a]] = i;
+                    nbr_.push_back(v[a]);
+                }
+        }
+        std::sort(nbr_.begin() + b, nbr_.end());
         nbr_ptr_[i + 1] = (int)nbr_.size();
     }
     for (int i = 0; i < n; ++i)
