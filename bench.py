@@ -672,8 +672,54 @@ def run_dambreak3d(ctx, steps, warmup):
         "ceilings_GBps": {"stream_read": nnz * 8 / kb["stream_read_us"] / 1e3,
                           "copy": kb["copy_bytes"] / kb["copy_us"] / 1e3},
     }
+    out["remesh"] = run_remesh_cycle(case, mesh, q0, dt)
     case.close()
     return out, roof
+
+
+def run_remesh_cycle(case, mesh, q0, dt):
+    """What the reference does after every successful step (IncompNewton/Solver.cpp:236-246,
+    Mesh.cpp:919-926), on the C2 nodes: the host mesher's alpha-shape triangulation of the node
+    cloud, the new mesh's set-up on the GPU (pattern, ELL, multigrid), a step on it, then one
+    full remesh (pfem_case_remesh: nodes to the host, smoothing / add / remove / Delaunay /
+    alpha shape, set-up) and the next step.  Mesh parameters of the 3D Koshizuka dam break
+    (examples/3D/damBreakKoshizuka/damBreakKoshizuka3DComp.lua)."""
+    import pfem_amd
+
+    m = pfem_amd.Mesher(3, hchar=mesh.h, alpha=1.2, omega=0.7, gamma=0.35, addOnFS=True, deleteFlyingNodes=False)
+    m.set_nodes(mesh.pos, mesh.flags, q0.reshape(4, -1))
+    t0 = time.perf_counter()
+    m.triangulate()
+    t_tri = time.perf_counter() - t0
+    st_tri = m.stats()
+    t0 = time.perf_counter()
+    case.set_mesh_from(m)
+    t_set0 = time.perf_counter() - t0
+    setup0 = case.setup_times()
+    t0 = time.perf_counter()
+    ok1 = case.solve(dt)
+    t_step1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    case.remesh(m)
+    t_rem = time.perf_counter() - t0
+    st = m.stats()
+    setup1 = case.setup_times()
+    t0 = time.perf_counter()
+    ok2 = case.solve(dt)
+    t_step2 = time.perf_counter() - t0
+    n, e, _ = m.sizes()
+    log(f"[bench] remesh: triangulate {t_tri:.3f}s set-up {t_set0:.3f}s step {t_step1 * 1e3:.1f}ms "
+        f"remesh {t_rem:.3f}s ({st}) step {t_step2 * 1e3:.1f}ms ok={ok1},{ok2}")
+    return dict(
+        triangulate_s=t_tri, triangulate_delaunay_s=st_tri["t_delaunay"], triangulate_alpha_s=st_tri["t_alpha"],
+        set_mesh_s=t_set0, set_mesh_phases_s=setup0, first_step_ms=t_step1 * 1e3, first_step_ok=bool(ok1),
+        remesh_call_s=t_rem, remesh_host_s={"nodes": st["t_nodes"], "delaunay": st["t_delaunay"],
+                                            "alpha_shape": st["t_alpha"]},
+        remesh_set_mesh_phases_s=setup1, nodes_added=st["added"], nodes_removed=st["removed"],
+        nodes_after=n, elements_after=e, next_step_ms=t_step2 * 1e3, next_step_ok=bool(ok2),
+        note="host mesher single-threaded; pfem_case_remesh = positions/states download + host remesh + set_mesh "
+             "+ state upload",
+    )
 
 
 # ------------------------------------------------------------------------------------------
@@ -706,6 +752,9 @@ def main():
     import pfem_amd
 
     ctx = pfem_amd.Context(0)
+    xcd = ctx.xcd_map(2048)
+    xcd_round_robin = float(np.mean(xcd == (np.arange(xcd.size) % 8)))
+    log(f"[bench] workgroup g on XCD g mod 8 for {xcd_round_robin:.3f} of 2048 workgroups")
     if args.only == "dambreak":
         db, roof = run_dambreak3d(ctx, args.dambreak_steps, 2)
         print(json.dumps({"dambreak3d_fp64": db, "roofline": roof}), flush=True)
@@ -852,6 +901,7 @@ def main():
         "wc3d": wc3d,
         "precision3d": prec3d,
         "roofline": roof,
+        "xcd_round_robin": xcd_round_robin,
     }
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or cpu_threads()

@@ -325,6 +325,10 @@ int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
 /* Device footprint of a case in bytes: the resident mesh, pattern and state plus the
  * step workspace. */
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);
+/* Instrumentation: host seconds of the last pfem_case_set_mesh (the work a remesh triggers):
+ * {PSPG pattern build (node graph, pair-element incidence, CSR rows), sliced ELL structure,
+ * multigrid hierarchy (host aggregation + device set-up), total including uploads}. */
+int pfem_case_setup_times(pfem_case* c, double* out4);
 /* Bytes of the step workspace alone (what a serving request borrows from the arena). */
 int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
 /* Allocates the case's own step workspace and direct-call resources now (the direct calls

@@ -47,6 +47,13 @@ class Context:
         check(lib().pfem_ctx_create(device, C.byref(self._h)))
         self.device = device
 
+    def xcd_map(self, nblocks: int = 2048) -> np.ndarray:
+        """XCD id each of nblocks one-wavefront workgroups ran on (pfem_probe_xcd): the
+        block-to-XCD mapping the XCD-aware kernel segments assume (workgroup g on XCD g mod 8)."""
+        out = np.empty(nblocks, dtype=np.int32)
+        check(lib().pfem_probe_xcd(self._h, int(nblocks), _ip(out)))
+        return out
+
     def close(self):
         if self._h:
             lib().pfem_ctx_destroy(self._h)
@@ -129,6 +136,12 @@ class Case:
         """The mesher's current mesh and node states (after Mesher.triangulate)."""
         check(lib().pfem_case_set_mesh_from(self._h, mesher._h))
         self.N, self.E, _ = mesher.sizes()
+
+    def setup_times(self) -> dict:
+        """Seconds of the last set_mesh: pattern, ELL structure, multigrid hierarchy, total."""
+        o = np.zeros(4)
+        check(lib().pfem_case_setup_times(self._h, _dp(o)))
+        return dict(zip(["pattern", "ell", "multigrid", "total"], o.tolist()))
 
     def remesh(self, mesher: "Mesher"):
         """Mesh::remesh between steps (pfem_case_remesh): positions and node states go to the

@@ -46,7 +46,7 @@ EXPORTS = [
     "pfem_spmv_ell", "pfem_mesher_create", "pfem_mesher_destroy", "pfem_mesher_set_nodes",
     "pfem_mesher_set_positions_states", "pfem_mesher_triangulate", "pfem_mesher_remesh", "pfem_mesher_sizes",
     "pfem_mesher_get_nodes", "pfem_mesher_get_elements", "pfem_mesher_get_facets", "pfem_mesher_stats",
-    "pfem_case_remesh", "pfem_case_set_mesh_from",
+    "pfem_case_remesh", "pfem_case_set_mesh_from", "pfem_case_setup_times",
 ]
 
 
@@ -180,6 +180,7 @@ def lib():
         "pfem_mesher_stats": [_vp, _dp],
         "pfem_case_remesh": [_vp, _vp],
         "pfem_case_set_mesh_from": [_vp, _vp],
+        "pfem_case_setup_times": [_vp, _dp],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

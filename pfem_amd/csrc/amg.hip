@@ -241,9 +241,34 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(const AmgView<T>* __restri
         }
     }
     __syncthreads();
-    // in-place Gauss-Jordan inversion: step k scales row k by 1/a_kk and eliminates column k
-    // from every other row, storing the inverse's column k in place of the eliminated one
+    // in-place Gauss-Jordan inversion with partial pivoting: step k brings the largest entry of
+    // column k (rows >= k) to the diagonal, scales row k by its inverse and eliminates column k
+    // from every other row, storing the inverse's column k in place of the eliminated one; the
+    // row exchanges are undone on the inverse's columns at the end, last one first.  A column
+    // that is zero on and below the diagonal (an exactly singular operator) keeps a unit pivot.
+    __shared__ int perm[kAmgMaxDense];
     for (int k = 0; k < n; ++k) {
+        if (threadIdx.x == 0) {
+            int r = k;
+            double best = fabs(a[k * n + k]);
+            for (int i = k + 1; i < n; ++i) {
+                const double v = fabs(a[i * n + k]);
+                if (v > best) {
+                    best = v;
+                    r = i;
+                }
+            }
+            perm[k] = r;
+        }
+        __syncthreads();
+        const int r = perm[k];
+        if (r != k)
+            for (int j = threadIdx.x; j < n; j += kBlock) {
+                const double t = a[k * n + j];
+                a[k * n + j] = a[r * n + j];
+                a[r * n + j] = t;
+            }
+        __syncthreads();
         double p = a[k * n + k];
         p = (p != 0.0) ? 1.0 / p : 1.0;
         for (int j = threadIdx.x; j < n; j += kBlock) {
@@ -257,6 +282,16 @@ __global__ __launch_bounds__(kBlock) void k_amg_dense(const AmgView<T>* __restri
             else if (j == k) a[e] = -colk[i] * p;
             else a[e] -= colk[i] * pivrow[j];
         }
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int r = perm[k];
+        if (r != k)
+            for (int i = threadIdx.x; i < n; i += kBlock) {
+                const double t = a[i * n + k];
+                a[i * n + k] = a[i * n + r];
+                a[i * n + r] = t;
+            }
         __syncthreads();
     }
     for (int e = threadIdx.x; e < n * n; e += kBlock) V.dense[e] = a[e];

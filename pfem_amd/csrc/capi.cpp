@@ -339,6 +339,14 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         PFEM_REQUIRE((long long)ne * N <= INT32_MAX && (long long)ne * ne * E <= INT32_MAX, PFEM_E_INVALID,
                      "mesh too large: (d+1) N and (d+1)^2 E must fit 32-bit indices");
         PFEM_HIP(hipSetDevice(c->ctx->device));
+        const auto t_start = std::chrono::steady_clock::now();
+        auto lap = [&](int k, std::chrono::steady_clock::time_point& t0) {
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            const auto t1 = std::chrono::steady_clock::now();
+            c->setup_s[k] = std::chrono::duration<double>(t1 - t0).count();
+            t0 = t1;
+        };
+        for (double& t : c->setup_s) t = 0;
         c->has_mesh = false;
         c->ws.clear();
         c->cfl_valid = false;
@@ -362,14 +370,18 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         c->counters.reserve(sizeof(unsigned) * 8);
         PFEM_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned) * 8, c->stream));
         if (c->d.kind == PFEM_KIND_PSPG) {
+            auto t0 = std::chrono::steady_clock::now();
             pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
+            lap(0, t0);
             pfem::build_ell(c->pat, c->stream);
+            lap(1, t0);
             const size_t n = (size_t)c->n;
             c->q_prev.reserve(sizeof(double) * n);
             PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * n, c->stream));
             c->lanes = pfem::pick_lanes((double)c->pat.graph_nnz / (double)N);
             c->amg = pfem::AmgHierarchy{};
             if (wants_amg(c)) pfem::build_amg(c->pat, dim, pfem::kAmgMaxDense / (dim + 1), c->stream, c->amg);
+            lap(2, t0);
         } else {
             pfem::build_mesh_basic(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             const size_t ss = wc_state_size(c);
@@ -381,6 +393,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             PFEM_HIP(hipMemsetAsync(c->wrec.p, 0, sizeof(double) * 8 * (size_t)N, c->stream));
         }
         PFEM_HIP(hipStreamSynchronize(c->stream));
+        c->setup_s[3] = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
         c->has_mesh = true;
         return PFEM_OK;
     });
@@ -480,6 +493,14 @@ int pfem_case_footprint(pfem_case* c, int64_t* bytes) {
     return guarded([&] {
         PFEM_REQUIRE(c != nullptr && bytes != nullptr, PFEM_E_INVALID, "null argument");
         *bytes = (int64_t)(persistent_bytes(c) + pfem::workspace_bytes(c));
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_setup_times(pfem_case* c, double* out4) {
+    return guarded([&] {
+        PFEM_REQUIRE(c != nullptr && out4 != nullptr, PFEM_E_INVALID, "null argument");
+        for (int k = 0; k < 4; ++k) out4[k] = c->setup_s[k];
         return PFEM_OK;
     });
 }

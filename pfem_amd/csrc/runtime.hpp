@@ -210,6 +210,7 @@ struct pfem_ctx {
 };
 
 struct pfem_case {
+    double setup_s[4] = {0, 0, 0, 0};  // last pfem_case_set_mesh: pattern, ELL, multigrid, total (seconds)
     pfem_ctx* ctx = nullptr;
     pfem_case_desc d{};
     hipStream_t stream = nullptr;
