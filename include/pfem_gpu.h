@@ -325,10 +325,12 @@ int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
 /* Device footprint of a case in bytes: the resident mesh, pattern and state plus the
  * step workspace. */
 int pfem_case_footprint(pfem_case* c, int64_t* bytes);
-/* Instrumentation: host seconds of the last pfem_case_set_mesh (the work a remesh triggers):
- * {PSPG pattern build (node graph, pair-element incidence, CSR rows), sliced ELL structure,
- * multigrid hierarchy (host aggregation + device set-up), total including uploads}. */
-int pfem_case_setup_times(pfem_case* c, double* out4);
+/* Instrumentation: host seconds of the last pfem_case_set_mesh (the work a remesh triggers)
+ * and what it found: {PSPG pattern build (node graph, pair-element incidence, CSR rows),
+ * sliced ELL structure, multigrid hierarchy (host aggregation + device set-up), total
+ * including uploads, smallest element shape quality 6 sqrt(2) |e| / l_rms^3 (3D; 1 in 2D),
+ * the multigrid smoother damping it implies (0.9, or 0.5 below quality 0.1)}. */
+int pfem_case_setup_times(pfem_case* c, double* out6);
 /* Bytes of the step workspace alone (what a serving request borrows from the arena). */
 int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
 /* Allocates the case's own step workspace and direct-call resources now (the direct calls

@@ -138,10 +138,11 @@ class Case:
         self.N, self.E, _ = mesher.sizes()
 
     def setup_times(self) -> dict:
-        """Seconds of the last set_mesh: pattern, ELL structure, multigrid hierarchy, total."""
-        o = np.zeros(4)
+        """The last set_mesh: seconds in the pattern, ELL structure, multigrid hierarchy and in
+        total, the smallest element shape quality (3D) and the smoother damping it implies."""
+        o = np.zeros(6)
         check(lib().pfem_case_setup_times(self._h, _dp(o)))
-        return dict(zip(["pattern", "ell", "multigrid", "total"], o.tolist()))
+        return dict(zip(["pattern", "ell", "multigrid", "total", "min_quality", "amg_omega"], o.tolist()))
 
     def remesh(self, mesher: "Mesher"):
         """Mesh::remesh between steps (pfem_case_remesh): positions and node states go to the

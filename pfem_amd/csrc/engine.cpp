@@ -663,7 +663,7 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
         AmgView<T>& V = av[l];
         V = AmgView<T>{};
         V.level = l;
-        V.omega = amg_omega;
+        V.omega = std::min(amg_omega, c->amg_omega_cap);
         V.alpha = amg_alpha;
         V.done_flag = &ds->k[0].done;
         if (l == 0) {
