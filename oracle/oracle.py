@@ -75,6 +75,16 @@ def lib():
         L.orc_set_material.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_wc_parts.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, _dp,
                                    C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
+        L.orc_heat_assemble.restype = C.c_void_p
+        L.orc_heat_assemble.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_int64, _ip, _ip, _u8p, _dp]
+        L.orc_heat_step.restype = C.c_int
+        L.orc_heat_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_double, C.c_double, C.c_int64, _ip, _ip,
+                                    _u8p, _dp, C.c_double, C.c_int, _ip, _dp]
+        L.orc_heat_facet_terms.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _ip, _ip, _dp, C.c_double,
+                                           C.c_double, C.c_double, _dp, _dp, _dp]
         _lib = L
     return _lib
 
@@ -319,3 +329,88 @@ class material:
     def __exit__(self, *exc):
         lib().orc_set_material(0, 0.0, 0.0)
         return False
+
+
+# ---- heat equation (HeatEquation.inl) ------------------------------------------------------
+HEAT_Q, HEAT_QH, HEAT_QR = 1, 2, 4
+
+
+@dataclass
+class HeatCase:
+    """Inputs of HeatEqIncompNewton without phase change: material (Material table: k, rho, cv,
+    h, Tinf, epsRad) and the facets with their BC kinds (HEAT_* bits) and, for HEAT_Q facets,
+    the "<Tag>Q" vectors at the 3 facet Gauss points (F x 3 x dim)."""
+    rho: float = 1000.0
+    cv: float = 4180.0
+    k: float = 0.6
+    h: float = 0.0
+    Tinf: float = 0.0
+    epsRad: float = 0.0
+    facets: np.ndarray | None = None
+    facet_out: np.ndarray | None = None
+    facet_kind: np.ndarray | None = None
+    qgp: np.ndarray | None = None
+
+
+def _heat_args(mesh, hc):
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
+    pos = np.ascontiguousarray(mesh.pos, dtype=np.float64)
+    d = mesh.dim
+    F = 0 if hc.facets is None else len(hc.facets)
+    fac = np.ascontiguousarray(hc.facets if F else np.zeros((1, d)), dtype=np.int32)
+    out = np.ascontiguousarray(hc.facet_out if F else np.zeros(1), dtype=np.int32)
+    kind = np.ascontiguousarray(hc.facet_kind if F else np.zeros(1), dtype=np.uint8)
+    qgp = None if hc.qgp is None else np.ascontiguousarray(hc.qgp, dtype=np.float64)
+    return conn, flags, pos, F, fac, out, kind, qgp
+
+
+def heat_assemble(mesh, T, Tbc=None, hc: HeatCase = HeatCase(), dt=1e-2) -> CsrSystem:
+    """m_buildAb + m_applyBC (HeatEquation.inl:209-364) from the node temperatures T."""
+    L = lib()
+    conn, flags, pos, F, fac, out, kind, qgp = _heat_args(mesh, hc)
+    N = mesh.n_nodes
+    T = np.ascontiguousarray(T, dtype=np.float64)
+    Tbc = np.zeros(N) if Tbc is None else np.ascontiguousarray(Tbc, dtype=np.float64)
+    h = L.orc_heat_assemble(mesh.dim, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(T), _d(Tbc), hc.rho, hc.cv,
+                            hc.k, hc.h, hc.Tinf, hc.epsRad, dt, F, _i(fac), _i(out), _u8(kind),
+                            _d(qgp) if qgp is not None else C.cast(None, _dp))
+    try:
+        n = L.orc_sys_n(h)
+        nnz = L.orc_sys_nnz(h)
+        rp = np.empty(n + 1, np.int32)
+        ci = np.empty(nnz, np.int32)
+        v = np.empty(nnz)
+        b = np.empty(n)
+        L.orc_sys_copy(h, _i(rp), _i(ci), _d(v), _d(b))
+    finally:
+        L.orc_sys_free(h)
+    return CsrSystem(n, rp, ci, v, b, np.zeros(5))
+
+
+def heat_step(mesh, T, Tbc=None, hc: HeatCase = HeatCase(), dt=1e-2, rtol=1e-13, maxit=100000):
+    """HeatEqIncompNewton::solve without phase change (one Picard pass): T updated in place.
+    Returns (status, iterations, relres)."""
+    L = lib()
+    conn, flags, pos, F, fac, out, kind, qgp = _heat_args(mesh, hc)
+    N = mesh.n_nodes
+    assert T.dtype == np.float64 and T.flags.c_contiguous
+    Tbc = np.zeros(N) if Tbc is None else np.ascontiguousarray(Tbc, dtype=np.float64)
+    it = C.c_int(0)
+    rr = C.c_double(0)
+    st = L.orc_heat_step(mesh.dim, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(T), _d(Tbc), hc.rho, hc.cv,
+                         hc.k, hc.h, hc.Tinf, hc.epsRad, dt, F, _i(fac), _i(out), _u8(kind),
+                         _d(qgp) if qgp is not None else C.cast(None, _dp), rtol, maxit, C.byref(it), C.byref(rr))
+    return st, it.value, rr.value
+
+
+def heat_facet_terms(mesh, T, hc: HeatCase):
+    """getQN, getSGamma (convection) and getSGamma (radiation) of every facet (F x dim each)."""
+    L = lib()
+    conn, flags, pos, F, fac, out, kind, qgp = _heat_args(mesh, hc)
+    d = mesh.dim
+    qn, sgh, sgr = np.zeros((F, d)), np.zeros((F, d)), np.zeros((F, d))
+    L.orc_heat_facet_terms(d, mesh.n_nodes, _d(pos), _d(np.ascontiguousarray(T, dtype=np.float64)), F, _i(fac),
+                           _i(out), _d(qgp) if qgp is not None else C.cast(None, _dp), hc.h, hc.Tinf, hc.epsRad,
+                           _d(qn), _d(sgh), _d(sgr))
+    return qn, sgh, sgr

@@ -1219,6 +1219,321 @@ static double wcNextDT(const WCParams& W, const double* pos, const double* v, co
 // ======================================================================================
 // C interface (ctypes) — used by tests/, __graft_entry__.smoke() and bench.py's CPU leg.
 // ======================================================================================
+// --------------------------------------------------------------------------------------
+// Heat equation (conduction, no phase change): srcs/simulation/physics/IncompNewton/
+// HeatEquation.inl:6-425 — m_buildAb (:209-280), m_applyBC (:283-364), the Picard callbacks
+// with runOnlyOnce (:111-194) and Eigen::ConjugateGradient<Lower|Upper> (HeatEquation.hpp:50)
+// with its default diagonal preconditioner, solveWithGuess(b, qPrev) (:131).
+// --------------------------------------------------------------------------------------
+enum HeatFacetKind : uint8_t { HEAT_Q = 1, HEAT_QH = 2, HEAT_QR = 4 };
+
+struct HeatParams {
+    int dim = 2;
+    int64_t N = 0, E = 0, F = 0;
+    const int* conn = nullptr;
+    const uint8_t* flags = nullptr;  // DIRICHLET: the node's tag has a "<Tag>T" function
+    double rho = 0, cv = 0, k = 0, h = 0, Tinf = 0, epsRad = 0, dt = 0;
+    const int* facets = nullptr;     // F x dim
+    const int* fout = nullptr;       // F: the facet's out node (Facet::m_outNodeIndex)
+    const uint8_t* fkind = nullptr;  // F: HEAT_* bits (the facet's nodes all carry that BC)
+    const double* qgp = nullptr;     // F x 3 x dim: "<Tag>Q"(gauss point, t + dt), nullable
+};
+
+// facet quadrature, nGPLD = 3 (HeatEquation.inl:12-23): Mesh.cpp:342-530 for dimension dim-1
+template <int D> struct QuadLD;
+template <> struct QuadLD<2> {  // segments on [-1, 1], refSize 2 (Mesh.cpp:479)
+    static constexpr double refSize = 2.0;
+    double N[3][2], w[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};  // Mesh.cpp:429
+    QuadLD() {
+        const double xi[3] = {-std::sqrt(3.0 / 5.0), 0, std::sqrt(3.0 / 5.0)};  // Mesh.cpp:356-358
+        for (int g = 0; g < 3; ++g) {
+            N[g][0] = (1 - xi[g]) / 2;  // Mesh.cpp:506-507
+            N[g][1] = (1 + xi[g]) / 2;
+        }
+    }
+};
+template <> struct QuadLD<3> {  // triangles, refSize 0.5 (Mesh.cpp:482)
+    static constexpr double refSize = 0.5;
+    double N[3][3], w[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};  // Mesh.cpp:451
+    QuadLD() {
+        const double gp[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}, {2.0 / 3.0, 1.0 / 6.0}};
+        for (int g = 0; g < 3; ++g) {
+            N[g][0] = 1 - gp[g][0] - gp[g][1];
+            N[g][1] = gp[g][0];
+            N[g][2] = gp[g][1];
+        }
+    }
+};
+
+// Facet::computeJ / computeDetJ (Facet.cpp:15-77) and computeNormal (Facet.cpp:130-210)
+template <int D>
+static void facetGeo(const double* pos, int64_t N, const int* fn, int out, double& detJ, double nrm[3]) {
+    auto P = [&](int node, int k) { return pos[node + k * N]; };
+    nrm[0] = nrm[1] = nrm[2] = 0;
+    if (D == 2) {
+        const double J00 = (P(fn[1], 0) - P(fn[0], 0)) / 2, J10 = (P(fn[1], 1) - P(fn[0], 1)) / 2;
+        detJ = std::sqrt(J00 * J00 + J10 * J10);
+        double nx = P(fn[1], 1) - P(fn[0], 1), ny = P(fn[0], 0) - P(fn[1], 0);
+        const double vx = P(out, 0) - P(fn[0], 0), vy = P(out, 1) - P(fn[0], 1);
+        if (nx * vx + ny * vy > 0) {
+            nx = -nx;
+            ny = -ny;
+        }
+        const double nn = std::sqrt(nx * nx + ny * ny);
+        nrm[0] = nx / nn;
+        nrm[1] = ny / nn;
+    } else {
+        double J[3][2];
+        for (int k = 0; k < 3; ++k) {
+            J[k][0] = P(fn[1], k) - P(fn[0], k);
+            J[k][1] = P(fn[2], k) - P(fn[0], k);
+        }
+        const double g0 = J[1][0] * J[2][1] - J[1][1] * J[2][0], g1 = J[2][0] * J[0][1] - J[2][1] * J[0][0],
+                     g2 = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+        detJ = std::sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        double a[3], b[3], n[3], v[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = P(fn[1], k) - P(fn[0], k);
+            b[k] = P(fn[2], k) - P(fn[0], k);
+            v[k] = P(out, k) - P(fn[0], k);
+        }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        if (v[0] * n[0] + v[1] * n[1] + v[2] * n[2] > 0)
+            for (double& c : n) c *= -1.0;
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) nrm[k] = n[k] / nn;
+    }
+}
+
+// m_buildAb: HeatEquation.inl:209-280 (triplets M then dt L per node pair of rows that are
+// neither Dirichlet nor free; unit diagonals pushed for Dirichlet or free nodes; b = sum of
+// M theta_prev per element; setFromTriplets as in buildAbPSPG, the default-triplet padding
+// keeping (0, 0) structural)
+template <int D>
+static void buildAbHeat(const HeatParams& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
+                        CsrSys& S) {
+    constexpr int ne = D + 1;
+    const int64_t N = P.N, E = P.E;
+    MB<D> mb;
+    std::vector<Triplet> indexA((size_t)E * 2 * ne * ne + 1, Triplet{0, 0, 0.0});
+    std::vector<std::pair<int64_t, double>> indexb((size_t)E * ne);
+#pragma omp parallel for schedule(static)
+    for (int64_t elm = 0; elm < E; ++elm) {
+        const int* nodes = P.conn + elm * ne;
+        Geo geo;
+        computeGeo<D>(pos, N, nodes, geo);
+        double G[D][ne], Me[ne][ne], Le[ne][ne];
+        mb.gradN(geo, G);
+        mb.M(geo.detJ, [&](const double*) { return P.cv * P.rho; }, Me);
+        mb.L(geo.detJ, P.k, G, Le);
+        for (int i = 0; i < ne; ++i)
+            for (int j = 0; j < ne; ++j) Le[i][j] = P.dt * Le[i][j];
+        double th[ne], Mth[ne];
+        for (int a = 0; a < ne; ++a) th[a] = qPrev[nodes[a]];
+        for (int i = 0; i < ne; ++i) {
+            double t = 0;
+            for (int j = 0; j < ne; ++j) t += Me[i][j] * th[j];
+            Mth[i] = t;
+        }
+        int64_t countA = 0;
+        for (int i = 0; i < ne; ++i) {
+            const uint8_t fi = fl[nodes[i]];
+            for (int j = 0; j < ne; ++j) {
+                if (!(fi & DIRICHLET)) {
+                    if (!(fi & FREE))
+                        indexA[(size_t)2 * ne * ne * elm + countA] = Triplet{nodes[i], nodes[j], Me[i][j]};
+                    countA++;
+                    if (!(fi & FREE))
+                        indexA[(size_t)2 * ne * ne * elm + countA] = Triplet{nodes[i], nodes[j], Le[i][j]};
+                    countA++;
+                }
+            }
+            indexb[(size_t)ne * elm + i] = {nodes[i], Mth[i]};
+        }
+    }
+    for (int64_t nn = 0; nn < N; ++nn)
+        if ((fl[nn] & DIRICHLET) || (fl[nn] & FREE)) indexA.push_back(Triplet{nn, nn, 1});
+    std::vector<int64_t> cnt(N + 1, 0);
+    for (const auto& t : indexA) cnt[t.row + 1]++;
+    for (int64_t r = 0; r < N; ++r) cnt[r + 1] += cnt[r];
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1);
+    std::vector<int64_t> ecol(indexA.size());
+    std::vector<double> eval(indexA.size());
+    for (const auto& t : indexA) {
+        const int64_t q = cur[t.row]++;
+        ecol[q] = t.col;
+        eval[q] = t.val;
+    }
+    S.n = N;
+    S.rowPtr.assign(N + 1, 0);
+    std::vector<std::vector<std::pair<int64_t, double>>> rows(N);
+    for (int64_t r = 0; r < N; ++r) {
+        auto& out = rows[r];
+        for (int64_t q = cnt[r]; q < cnt[r + 1]; ++q) {
+            bool found = false;
+            for (auto& pr : out)
+                if (pr.first == ecol[q]) {
+                    pr.second += eval[q];
+                    found = true;
+                    break;
+                }
+            if (!found) out.push_back({ecol[q], eval[q]});
+        }
+        std::sort(out.begin(), out.end());
+        S.rowPtr[r + 1] = S.rowPtr[r] + (int)out.size();
+    }
+    S.colIdx.resize(S.rowPtr[N]);
+    S.vals.resize(S.rowPtr[N]);
+    for (int64_t r = 0; r < N; ++r) {
+        int64_t q = S.rowPtr[r];
+        for (const auto& pr : rows[r]) {
+            S.colIdx[q] = (int)pr.first;
+            S.vals[q] = pr.second;
+            ++q;
+        }
+    }
+    S.b.assign(N, 0.0);
+    for (const auto& db : indexb) S.b[db.first] += db.second;
+}
+
+// getQN (MatricesBuilder.inl:247-274) and getSGamma (:358-371) of one facet with the two
+// SGamma factors of HeatEquation.inl:87-99 (convection h (T - Tinf), radiation
+// sigma epsRad (T^4 - Tinf^4), T = N theta at each Gauss point)
+template <int D>
+static void heatFacetTerms(const HeatParams& P, const double* pos, const double* T, int64_t f, double qn[D],
+                           double sgh[D], double sgr[D]) {
+    QuadLD<D> q;
+    const int* fn = P.facets + f * D;
+    double detJ, nrm[3];
+    facetGeo<D>(pos, P.N, fn, P.fout[f], detJ, nrm);
+    const double s = q.refSize * detJ;
+    for (int i = 0; i < D; ++i) qn[i] = sgh[i] = sgr[i] = 0;
+    for (int g = 0; g < 3; ++g) {
+        double qv = 0;
+        if (P.qgp)
+            for (int k = 0; k < D; ++k) qv += P.qgp[(f * 3 + g) * D + k] * nrm[k];
+        const double fact = qv * q.w[g];
+        double Tg = 0;
+        for (int j = 0; j < D; ++j) Tg += q.N[g][j] * T[fn[j]];
+        const double fh = P.h * (Tg - P.Tinf);
+        const double sigma = 5.670374419e-8;
+        const double fr = sigma * P.epsRad * (Tg * Tg * Tg * Tg - P.Tinf * P.Tinf * P.Tinf * P.Tinf);
+        for (int i = 0; i < D; ++i) {
+            qn[i] += fact * q.N[g][i];
+            sgh[i] += fh * q.N[g][i] * q.w[g];
+            sgr[i] += fr * q.N[g][i] * q.w[g];
+        }
+    }
+    for (int i = 0; i < D; ++i) {
+        qn[i] = qn[i] * s;
+        sgh[i] *= s;
+        sgr[i] *= s;
+    }
+}
+
+// m_applyBC: HeatEquation.inl:283-364 (facet fluxes in facet order, then free-node rhs and the
+// Dirichlet column elimination in node order)
+template <int D>
+static void applyBCHeat(const HeatParams& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
+                        const double* Tbc, CsrSys& S) {
+    for (int64_t f = 0; f < P.F; ++f) {
+        const uint8_t kind = P.fkind[f];
+        if (!kind) continue;
+        double qn[D], sgh[D], sgr[D];
+        heatFacetTerms<D>(P, pos, qPrev, f, qn, sgh, sgr);
+        const int* fn = P.facets + f * D;
+        if (kind & HEAT_Q)
+            for (int i = 0; i < D; ++i) S.b[fn[i]] -= P.dt * qn[i];
+        if (kind & HEAT_QH)
+            for (int i = 0; i < D; ++i) S.b[fn[i]] -= P.dt * sgh[i];
+        if (kind & HEAT_QR)
+            for (int i = 0; i < D; ++i) S.b[fn[i]] -= P.dt * sgr[i];
+    }
+    const int64_t N = P.N;
+    std::vector<int64_t> ccnt(N + 1, 0);
+    for (int c : S.colIdx) ccnt[c + 1]++;
+    for (int64_t c = 0; c < N; ++c) ccnt[c + 1] += ccnt[c];
+    std::vector<int64_t> cpos(ccnt.begin(), ccnt.end() - 1), cRow(S.colIdx.size()), cK(S.colIdx.size());
+    for (int64_t r = 0; r < N; ++r)
+        for (int64_t q = S.rowPtr[r]; q < S.rowPtr[r + 1]; ++q) {
+            const int64_t c = S.colIdx[q];
+            cRow[cpos[c]] = r;
+            cK[cpos[c]] = q;
+            cpos[c]++;
+        }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        if ((fl[nn] & FREE) && !(fl[nn] & DIRICHLET)) {
+            S.b[nn] = qPrev[nn];
+        } else if (fl[nn] & DIRICHLET) {
+            const double res = Tbc[nn];
+            S.b[nn] = res;
+            for (int64_t q = ccnt[nn]; q < ccnt[nn + 1]; ++q) {
+                const int64_t row = cRow[q];
+                if (row == nn) continue;
+                S.b[row] -= S.vals[cK[q]] * res;
+                S.vals[cK[q]] = 0;
+            }
+        }
+    }
+}
+
+// Conjugate gradient with the diagonal preconditioner (Eigen's ConjugateGradient default), from
+// the initial guess x; stops on ||b - A x|| <= rtol ||b|| or after maxit iterations.
+static int pcg(int64_t n, const int* rp, const int* ci, const double* v, const double* b, double* x, double rtol,
+               int maxit, int* itersOut, double* relresOut) {
+    std::vector<double> dinv(n, 1.0), r(n), z(n), p(n), q(n);
+    for (int64_t i = 0; i < n; ++i)
+        for (int64_t k = rp[i]; k < rp[i + 1]; ++k)
+            if (ci[k] == i && v[k] != 0) dinv[i] = 1.0 / v[k];
+    spmv(n, rp, ci, v, x, q.data());
+    for (int64_t i = 0; i < n; ++i) r[i] = b[i] - q[i];
+    const double bb = dot(n, b, b);
+    if (bb == 0) {
+        std::fill(x, x + n, 0.0);
+        *itersOut = 0;
+        *relresOut = 0;
+        return 0;
+    }
+    const double thr = rtol * rtol * bb;
+    double rr = dot(n, r.data(), r.data());
+    int it = 0;
+    if (rr >= thr) {
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] = dinv[i] * r[i];
+        double rz = dot(n, r.data(), z.data());
+        for (; it < maxit;) {
+            spmv(n, rp, ci, v, p.data(), q.data());
+            const double alpha = rz / dot(n, p.data(), q.data());
+            for (int64_t i = 0; i < n; ++i) {
+                x[i] += alpha * p[i];
+                r[i] -= alpha * q[i];
+            }
+            ++it;
+            rr = dot(n, r.data(), r.data());
+            if (rr < thr) break;
+            for (int64_t i = 0; i < n; ++i) z[i] = dinv[i] * r[i];
+            const double rzNew = dot(n, r.data(), z.data());
+            const double beta = rzNew / rz;
+            rz = rzNew;
+            for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+        }
+    }
+    *itersOut = it;
+    *relresOut = std::sqrt(rr / bb);
+    return std::isfinite(rr) ? (rr < thr ? 0 : 1) : 2;
+}
+
+template <int D>
+static void heatSystem(const HeatParams& P, const double* pos, const double* qPrev, const double* Tbc, CsrSys& S) {
+    std::vector<uint8_t> fl(P.flags, P.flags + P.N);
+    std::vector<char> used(P.N, 0);
+    for (int64_t q = 0; q < P.E * (D + 1); ++q) used[P.conn[q]] = 1;
+    for (int64_t n = 0; n < P.N; ++n) fl[n] = (fl[n] & ~FREE) | (used[n] ? 0 : FREE);
+    buildAbHeat<D>(P, fl, pos, qPrev, S);
+    applyBCHeat<D>(P, fl, pos, qPrev, Tbc, S);
+}
+
 extern "C" {
 
 struct orc_sys {
@@ -1545,3 +1860,74 @@ void orc_set_material(int problem, double tau0, double m_reg) {
 
 }  // extern "C"
 
+extern "C" {
+// The heat system (m_buildAb + m_applyBC) of one step from T (the previous step's node states):
+// CSR of N rows and the rhs.
+orc_sys* orc_heat_assemble(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, const double* pos,
+                           const double* T, const double* Tbc, double rho, double cv, double k, double h,
+                           double Tinf, double epsRad, double dt, int64_t F, const int* facets, const int* fout,
+                           const uint8_t* fkind, const double* qgp) {
+    HeatParams P;
+    P.dim = dim;
+    P.N = N;
+    P.E = E;
+    P.F = F;
+    P.conn = conn;
+    P.flags = flags;
+    P.rho = rho;
+    P.cv = cv;
+    P.k = k;
+    P.h = h;
+    P.Tinf = Tinf;
+    P.epsRad = epsRad;
+    P.dt = dt;
+    P.facets = facets;
+    P.fout = fout;
+    P.fkind = fkind;
+    P.qgp = qgp;
+    auto* o = new orc_sys;
+    if (dim == 2)
+        heatSystem<2>(P, pos, T, Tbc, o->S);
+    else
+        heatSystem<3>(P, pos, T, Tbc, o->S);
+    return o;
+}
+
+// One HeatEqIncompNewton::solve without phase change: the system, the CG solve from T, the new
+// node states in T.  Returns 0, 1 (not converged) or 2 (NaN).
+int orc_heat_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, const double* pos, double* T,
+                  const double* Tbc, double rho, double cv, double k, double h, double Tinf, double epsRad, double dt,
+                  int64_t F, const int* facets, const int* fout, const uint8_t* fkind, const double* qgp, double rtol,
+                  int maxit, int* iters, double* relres) {
+    orc_sys* o = orc_heat_assemble(dim, N, E, conn, flags, pos, T, Tbc, rho, cv, k, h, Tinf, epsRad, dt, F, facets,
+                                   fout, fkind, qgp);
+    std::vector<double> x(T, T + N);
+    const int st = pcg(N, o->S.rowPtr.data(), o->S.colIdx.data(), o->S.vals.data(), o->S.b.data(), x.data(), rtol,
+                       maxit, iters, relres);
+    delete o;
+    if (st == 0) std::copy(x.begin(), x.end(), T);
+    return st;
+}
+
+// getQN / getSGamma of every facet from the node temperatures T (F x dim each).
+void orc_heat_facet_terms(int dim, int64_t N, const double* pos, const double* T, int64_t F, const int* facets,
+                          const int* fout, const double* qgp, double h, double Tinf, double epsRad, double* qn,
+                          double* sgh, double* sgr) {
+    HeatParams P;
+    P.dim = dim;
+    P.N = N;
+    P.F = F;
+    P.facets = facets;
+    P.fout = fout;
+    P.qgp = qgp;
+    P.h = h;
+    P.Tinf = Tinf;
+    P.epsRad = epsRad;
+    for (int64_t f = 0; f < F; ++f) {
+        if (dim == 2)
+            heatFacetTerms<2>(P, pos, T, f, qn + f * 2, sgh + f * 2, sgr + f * 2);
+        else
+            heatFacetTerms<3>(P, pos, T, f, qn + f * 3, sgh + f * 3, sgr + f * 3);
+    }
+}
+}  // extern "C"
