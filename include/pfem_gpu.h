@@ -49,6 +49,7 @@ extern "C" {
 /* ---- enums ------------------------------------------------------------------------- */
 #define PFEM_KIND_PSPG 0 /* MomContEqIncompNewton with Solver.id = "PSPG" */
 #define PFEM_KIND_WC 1   /* WCompNewtonNoT with Solver.id = "CDS_dpdt" */
+#define PFEM_KIND_HEAT 2 /* "Conduction": HeatEqIncompNewton alone, no phase change */
 
 #define PFEM_RES_U 0    /* MomContEquation.inl:229-236 residual = "U" */
 #define PFEM_RES_U_P 1  /* "U_P" */
@@ -100,6 +101,12 @@ typedef struct pfem_case_desc {
     int32_t problem;        /* PFEM_PROBLEM_* */
     double tau0;            /* Bingham yield stress tau0 */
     double m_reg;           /* Bingham regularisation mReg */
+    /* heat equation (Material table of HeatEqIncompNewton, HeatEquation.inl:27-32); rho above */
+    double cv;              /* cv: specific heat */
+    double k_cond;          /* k: conductivity */
+    double h_conv;          /* h: convection coefficient of the "<Tag>Qh" facets */
+    double t_inf;           /* Tinf: ambient temperature of convection and radiation */
+    double eps_rad;         /* epsRad: emissivity of the "<Tag>Qr" facets */
 } pfem_case_desc;
 
 #define PFEM_PROBLEM_NEWTONIAN 0 /* "IncompNewtonNoT": K factor mu */
@@ -339,6 +346,38 @@ int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
  * iterate are lost on release. */
 int pfem_case_reserve_workspace(pfem_case* c);
 int pfem_case_release_workspace(pfem_case* c);
+
+/* ---- Heat (conduction) -----------------------------------------------------------------------
+ * HeatEqIncompNewton without phase change (srcs/simulation/physics/IncompNewton/HeatEquation.inl),
+ * as SolverIncompNewton::m_solveConduction runs it (IncompNewton/Solver.cpp:286-296): one node
+ * state T, nodes fixed.  pfem_case_set_state / get_state take N temperatures; the node flag
+ * PFEM_NODE_DIRICHLET marks the nodes whose tag has a "<Tag>T" function. */
+#define PFEM_HEAT_Q 1  /* facet with a prescribed flux "<Tag>Q" (every node's tag has it) */
+#define PFEM_HEAT_QH 2 /* facet with convection h (T - Tinf) ("<Tag>Qh") */
+#define PFEM_HEAT_QR 4 /* facet with radiation sigma epsRad (T^4 - Tinf^4) ("<Tag>Qr") */
+/* The boundary facets (pfem_mesher_get_facets: nodes F x dim, out node F) and each one's BC
+ * kinds, PFEM_HEAT_* bits: m_applyBC (HeatEquation.inl:289-348) applies a kind to a facet whose
+ * nodes all carry it, with the tag "FreeSurface" for a facet on the free surface. */
+int pfem_heat_set_facets(pfem_case* c, int32_t F, const int32_t* nodes, const int32_t* out_node,
+                         const uint8_t* kinds);
+/* Gauss points of every facet (F x 3 x dim; Facet::getPosFromGP), where the caller evaluates
+ * the flux "<Tag>Q"(pos, t + dt) of PFEM_HEAT_Q facets. */
+int pfem_heat_facet_points(pfem_case* c, double* xgp);
+/* The flux vectors at those points (F x 3 x dim), or NULL for none. */
+int pfem_heat_set_flux(pfem_case* c, const double* q);
+/* Dirichlet temperatures, N values ("<Tag>T"(pos, t + dt), HeatEquation.inl:352-356). */
+int pfem_heat_set_dirichlet(pfem_case* c, const double* t_bc);
+/* m_buildAb + m_applyBC (HeatEquation.inl:209-364) from the node temperatures, for checks:
+ * afterwards pfem_heat_get_system exports the system. */
+int pfem_heat_assemble(pfem_case* c, double dt);
+int pfem_heat_system_nnz(pfem_case* c, int64_t* nnz);
+/* CSR rows N + 1, columns and values nnz, rhs N. */
+int pfem_heat_get_system(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, double* vals, double* b);
+/* One HeatEqIncompNewton::solve: the system, the conjugate-gradient solve with the diagonal
+ * preconditioner from the current temperatures (the reference's Eigen::ConjugateGradient,
+ * HeatEquation.hpp:50) to krylov_rtol, the new temperatures.  PFEM_NOT_CONVERGED / PFEM_NAN
+ * leave the temperatures unchanged. */
+int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats);
 
 /* ---- Host remesher (no GPU work) ------------------------------------------------------------
  * The reference's Mesh class after loadFromFile: triangulateAlphaShape (Mesh2D.cpp:32-251,

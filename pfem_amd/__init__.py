@@ -72,10 +72,15 @@ class Case:
     def __init__(self, ctx: Context, dim: int, kind: str = "PSPG", rho=1000.0, mu=1e-3, gamma=0.0, body_force=None,
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
-                 inner_rtol=1e-3, inner_max_iter=300, precond="auto", problem="Newtonian", tau0=0.0, mReg=0.0):
+                 inner_rtol=1e-3, inner_max_iter=300, precond="auto", problem="Newtonian", tau0=0.0, mReg=0.0,
+                 cv=0.0, k=0.0, h=0.0, Tinf=0.0, epsRad=0.0):
         d = _lib.CaseDesc()
         d.dim = dim
-        d.kind = _lib.KIND_PSPG if kind == "PSPG" else _lib.KIND_WC
+        kinds = {"PSPG": _lib.KIND_PSPG, "WC": _lib.KIND_WC, "HEAT": _lib.KIND_HEAT}
+        if kind not in kinds:
+            raise PfemError(_lib.PFEM_E_INVALID, "unknown case kind: " + str(kind))
+        d.kind = kinds[kind]
+        d.cv, d.k_cond, d.h_conv, d.t_inf, d.eps_rad = cv, k, h, Tinf, epsRad
         d.rho, d.mu, d.gamma = rho, mu, gamma
         bf = body_force if body_force is not None else ([0.0, -9.81] if dim == 2 else [0.0, 0.0, -9.81])
         for k in range(3):
@@ -152,6 +157,8 @@ class Case:
 
     @property
     def n_state(self):
+        if self.kind == "HEAT":
+            return self.N
         return (self.dim + 1) * self.N if self.kind == "PSPG" else (2 * self.dim + 2) * self.N
 
     def set_state(self, q):
@@ -405,6 +412,71 @@ class PicardAlgo:
         c.commit()
         self.iterations = it
         return True
+
+
+class HeatEqIncompNewton(Case):
+    """HeatEqIncompNewton without phase change (IncompNewton/HeatEquation.inl) as the Conduction
+    problem runs it: one node state T, fixed nodes.  Material keywords follow the Lua Material
+    table (k, rho, cv, h, Tinf, epsRad); the facets and their BC kinds come from the mesher
+    (set_facets), the Dirichlet temperatures and the flux at the facet Gauss points from the
+    caller (set_dirichlet_T, set_flux).  ``solve(dt)`` returns True on success, False on a
+    solver failure, like Equation::solve."""
+
+    def __init__(self, ctx, dim, rho=1000.0, cv=4180.0, k=0.6, h=0.0, Tinf=0.0, epsRad=0.0, krylov_rtol=1e-12,
+                 krylov_max_iter=100000, **kw):
+        super().__init__(ctx, dim, kind="HEAT", rho=rho, cv=cv, k=k, h=h, Tinf=Tinf, epsRad=epsRad,
+                         krylov_rtol=krylov_rtol, krylov_max_iter=krylov_max_iter, **kw)
+        self.last_stats = None
+        self.F = 0
+
+    def set_facets(self, nodes, out_node, kinds):
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        out_node = np.ascontiguousarray(out_node, dtype=np.int32)
+        kinds = np.ascontiguousarray(kinds, dtype=np.uint8)
+        F = len(out_node)
+        check(lib().pfem_heat_set_facets(self._h, F, _ip(nodes) if F else None, _ip(out_node) if F else None,
+                                         _u8(kinds) if F else None))
+        self.F = F
+
+    def facet_points(self) -> np.ndarray:
+        out = np.empty((max(self.F, 1), 3, self.dim))
+        check(lib().pfem_heat_facet_points(self._h, _dp(out)))
+        return out[: self.F]
+
+    def set_flux(self, q=None):
+        if q is None:
+            check(lib().pfem_heat_set_flux(self._h, None))
+            return
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
+        if q.size != self.F * 3 * self.dim:
+            raise PfemError(_lib.PFEM_E_INVALID, "flux must hold F x 3 x dim values")
+        check(lib().pfem_heat_set_flux(self._h, _dp(q)))
+
+    def set_dirichlet_T(self, t_bc):
+        t = np.ascontiguousarray(t_bc, dtype=np.float64).reshape(-1)
+        if t.size != self.N:
+            raise PfemError(_lib.PFEM_E_INVALID, "Dirichlet temperatures must hold N values")
+        check(lib().pfem_heat_set_dirichlet(self._h, _dp(t)))
+
+    def assemble(self, dt):  # noqa: D401
+        check(lib().pfem_heat_assemble(self._h, float(dt)))
+
+    def system(self):
+        """(row_ptr, col_idx, vals, b) of the last assembled system."""
+        nnz = C.c_int64()
+        check(lib().pfem_heat_system_nnz(self._h, C.byref(nnz)))
+        rp = np.empty(self.N + 1, np.int32)
+        ci = np.empty(nnz.value, np.int32)
+        v = np.empty(nnz.value)
+        b = np.empty(self.N)
+        check(lib().pfem_heat_get_system(self._h, _ip(rp), _ip(ci), _dp(v), _dp(b)))
+        return rp, ci, v, b
+
+    def solve(self, dt) -> bool:  # noqa: D401  (mirrors Equation::solve)
+        st = _lib.StepStats()
+        rc = check(lib().pfem_heat_step(self._h, float(dt), C.byref(st)), allow_positive=True)
+        self.last_stats = st.as_dict()
+        return rc == _lib.PFEM_OK
 
 
 class SolverIncompNewton:

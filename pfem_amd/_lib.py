@@ -22,6 +22,8 @@ PFEM_NAN = 2
 
 KIND_PSPG = 0
 KIND_WC = 1
+KIND_HEAT = 2
+HEAT_Q, HEAT_QH, HEAT_QR = 1, 2, 4
 RES = {"U": 0, "U_P": 1, "Ax_f": 2}
 PREC_FP64 = 0
 PREC_MIXED = 1
@@ -46,7 +48,9 @@ EXPORTS = [
     "pfem_spmv_ell", "pfem_mesher_create", "pfem_mesher_destroy", "pfem_mesher_set_nodes",
     "pfem_mesher_set_positions_states", "pfem_mesher_triangulate", "pfem_mesher_remesh", "pfem_mesher_sizes",
     "pfem_mesher_get_nodes", "pfem_mesher_get_elements", "pfem_mesher_get_facets", "pfem_mesher_stats",
-    "pfem_case_remesh", "pfem_case_set_mesh_from", "pfem_case_setup_times",
+    "pfem_case_remesh", "pfem_case_set_mesh_from", "pfem_case_setup_times", "pfem_heat_set_facets",
+    "pfem_heat_facet_points", "pfem_heat_set_flux", "pfem_heat_set_dirichlet", "pfem_heat_assemble",
+    "pfem_heat_system_nnz", "pfem_heat_get_system", "pfem_heat_step",
 ]
 
 
@@ -58,6 +62,8 @@ class CaseDesc(C.Structure):
         ("min_res", C.c_double), ("precision", C.c_int32), ("krylov_max_iter", C.c_int32),
         ("krylov_rtol", C.c_double), ("inner_rtol", C.c_double), ("inner_max_iter", C.c_int32), ("precond", C.c_int32),
         ("problem", C.c_int32), ("tau0", C.c_double), ("m_reg", C.c_double),
+        ("cv", C.c_double), ("k_cond", C.c_double), ("h_conv", C.c_double), ("t_inf", C.c_double),
+        ("eps_rad", C.c_double),
     ]
 
 
@@ -181,6 +187,14 @@ def lib():
         "pfem_case_remesh": [_vp, _vp],
         "pfem_case_set_mesh_from": [_vp, _vp],
         "pfem_case_setup_times": [_vp, _dp],
+        "pfem_heat_set_facets": [_vp, C.c_int32, _ip, _ip, _u8p],
+        "pfem_heat_facet_points": [_vp, _dp],
+        "pfem_heat_set_flux": [_vp, _dp],
+        "pfem_heat_set_dirichlet": [_vp, _dp],
+        "pfem_heat_assemble": [_vp, C.c_double],
+        "pfem_heat_system_nnz": [_vp, C.POINTER(C.c_int64)],
+        "pfem_heat_get_system": [_vp, _ip, _ip, _dp, _dp],
+        "pfem_heat_step": [_vp, C.c_double, C.POINTER(StepStats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

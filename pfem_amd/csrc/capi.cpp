@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "heat.hpp"
 #include "mesher.hpp"
 #include "runtime.hpp"
 
@@ -37,6 +38,107 @@ int guarded(F&& f) {
 
 int grid_for_rows(int n) { return pfem::row_blocks(n); }
 
+// Heat case: the scalar CSR over the node graph (full rows hold the node's sorted graph
+// entries, Dirichlet and free rows their diagonal only) and the step's device arrays.
+void setup_heat(pfem_case* c) {
+    const int N = c->N, E = c->E;
+    const pfem::Pattern& P = c->pat;
+    std::vector<int> ptr((size_t)N + 1), idx((size_t)P.graph_nnz);
+    std::vector<uint8_t> fl((size_t)N);
+    PFEM_HIP(hipMemcpyAsync(ptr.data(), P.nbr_ptr.p, sizeof(int) * ptr.size(), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipMemcpyAsync(idx.data(), P.nbr_idx.p, sizeof(int) * idx.size(), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipMemcpyAsync(fl.data(), P.flags.p, fl.size(), hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipStreamSynchronize(c->stream));
+    std::vector<int> rp((size_t)N + 1, 0), col, dpos((size_t)N);
+    col.reserve(idx.size());
+    for (int i = 0; i < N; ++i) {
+        const bool full = !(fl[i] & (pfem::NODE_DIRICHLET | pfem::NODE_FREE));
+        if (full) {
+            for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
+                if (idx[q] == i) dpos[i] = (int)col.size();
+                col.push_back(idx[q]);
+            }
+        } else {
+            dpos[i] = (int)col.size();
+            col.push_back(i);
+        }
+        rp[i + 1] = (int)col.size();
+    }
+    auto& H = c->heat;
+    H.nnz = (long long)col.size();
+    auto up = [&](pfem::DevBuf& d, const void* src, size_t bytes) {
+        d.reserve(bytes);
+        PFEM_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+    };
+    up(H.row_ptr, rp.data(), sizeof(int) * rp.size());
+    up(H.col, col.data(), sizeof(int) * col.size());
+    up(H.diag_pos, dpos.data(), sizeof(int) * dpos.size());
+    const int nb = pfem::heat_blocks(N);
+    H.vals.reserve(sizeof(double) * col.size());
+    for (pfem::DevBuf* b : {&H.b, &H.x, &H.r, &H.z, &H.p, &H.q, &H.dinv}) b->reserve(sizeof(double) * (size_t)N);
+    H.erec.reserve(sizeof(double) * 16 * (size_t)E);
+    H.part.reserve(sizeof(double) * 5 * (size_t)nb);
+    H.scal.reserve(sizeof(double) * 2);
+    c->q_prev.reserve(sizeof(double) * (size_t)N);
+    PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * (size_t)N, c->stream));
+    c->dir_vals.reserve(sizeof(double) * (size_t)N);
+    PFEM_HIP(hipMemsetAsync(c->dir_vals.p, 0, sizeof(double) * (size_t)N, c->stream));
+    // the facets of a previous mesh no longer apply
+    H.F = 0;
+    H.has_q = false;
+    std::vector<int> zeros((size_t)N + 1, 0);
+    up(H.nf_ptr, zeros.data(), sizeof(int) * zeros.size());
+}
+
+pfem::HeatView heat_view(pfem_case* c, double dt) {
+    auto& H = c->heat;
+    pfem::HeatView V;
+    V.dim = c->dim;
+    V.N = c->N;
+    V.E = c->E;
+    V.F = H.F;
+    V.nb = pfem::heat_blocks(c->N);
+    V.conn = c->pat.conn.as<int>();
+    V.flags = c->pat.flags.as<uint8_t>();
+    V.pos = c->pos.as<double>();
+    V.nbr_ptr = c->pat.nbr_ptr.as<int>();
+    V.nbr_idx = c->pat.nbr_idx.as<int>();
+    V.pei_ptr = c->pat.pei_ptr.as<int>();
+    V.pei = c->pat.pei.as<unsigned>();
+    V.row_ptr = H.row_ptr.as<int>();
+    V.col = H.col.as<int>();
+    V.diag_pos = H.diag_pos.as<int>();
+    V.vals = H.vals.as<double>();
+    V.b = H.b.as<double>();
+    V.erec = H.erec.as<double>();
+    V.Tprev = c->q_prev.as<double>();
+    V.Tbc = c->dir_vals.as<double>();
+    V.facets = H.facets.as<int>();
+    V.fout = H.fout.as<int>();
+    V.fkind = H.fkind.as<uint8_t>();
+    V.qgp = H.has_q ? H.qgp.as<double>() : nullptr;
+    V.fterm = H.fterm.as<double>();
+    V.nf_ptr = H.nf_ptr.as<int>();
+    V.nf_list = H.nf_list.as<int>();
+    V.rho = c->d.rho;
+    V.cv = c->d.cv;
+    V.k = c->d.k_cond;
+    V.h = c->d.h_conv;
+    V.Tinf = c->d.t_inf;
+    V.epsRad = c->d.eps_rad;
+    V.dt = dt;
+    V.x = H.x.as<double>();
+    V.r = H.r.as<double>();
+    V.z = H.z.as<double>();
+    V.p = H.p.as<double>();
+    V.q = H.q.as<double>();
+    V.dinv = H.dinv.as<double>();
+    V.part = H.part.as<double>();
+    V.scal = H.scal.as<double>();
+    return V;
+}
+
+
 template <class T>
 pfem::NodeMat<T> node_mat(const pfem::Pattern& P, int dim, const T* vals) {
     pfem::NodeMat<T> A{};
@@ -61,6 +163,10 @@ void require_mesh(pfem_case* c) {
 void require_pspg(pfem_case* c) {
     require_mesh(c);
     PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG, PFEM_E_STATE, "not a PSPG case");
+}
+void require_heat(pfem_case* c) {
+    require_mesh(c);
+    PFEM_REQUIRE(c->d.kind == PFEM_KIND_HEAT, PFEM_E_STATE, "not a heat case");
 }
 // The fused Picard step takes the Dirichlet values once per step; a Dirichlet node that
 // moves needs them re-evaluated at its new position before every re-assembly
@@ -280,7 +386,8 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
         PFEM_REQUIRE(ctx && desc && out, PFEM_E_INVALID, "null argument");
         const auto& d = *desc;
         PFEM_REQUIRE(d.dim == 2 || d.dim == 3, PFEM_E_INVALID, "dim must be 2 or 3");
-        PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC, PFEM_E_INVALID, "unknown case kind");
+        PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC || d.kind == PFEM_KIND_HEAT, PFEM_E_INVALID,
+                     "unknown case kind");
         PFEM_REQUIRE(std::fabs(d.gamma) < 1e-15, PFEM_E_UNSUPPORTED,
                      "surface tension (gamma > 0) is not supported: the FST facet term is not implemented");
         PFEM_REQUIRE(d.mu >= 0, PFEM_E_INVALID, "mu must be >= 0");
@@ -302,6 +409,10 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
                          "unknown problem");
             PFEM_REQUIRE(d.problem != PFEM_PROBLEM_BINGHAM || (d.tau0 >= 0 && d.m_reg > 0), PFEM_E_INVALID,
                          "Bingham needs tau0 >= 0 and mReg > 0");
+        } else if (d.kind == PFEM_KIND_HEAT) {
+            PFEM_REQUIRE(d.rho > 0 && d.cv > 0 && d.k_cond >= 0 && d.h_conv >= 0 && d.eps_rad >= 0, PFEM_E_INVALID,
+                         "heat needs rho > 0, cv > 0, k >= 0, h >= 0, epsRad >= 0");
+            PFEM_REQUIRE(d.krylov_rtol > 0 && d.krylov_max_iter > 0, PFEM_E_INVALID, "invalid Krylov settings");
         } else {
             PFEM_REQUIRE(d.K0 > 0 && d.K0p > 0 && d.rho_star > 0, PFEM_E_INVALID, "invalid Tait parameters");
             PFEM_REQUIRE(d.wc_state == PFEM_STATE_FP64 || d.wc_state == PFEM_STATE_FP32, PFEM_E_INVALID,
@@ -412,6 +523,9 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             c->amg = pfem::AmgHierarchy{};
             if (wants_amg(c)) pfem::build_amg(c->pat, dim, pfem::kAmgMaxDense / (dim + 1), c->stream, c->amg);
             lap(2, t0);
+        } else if (c->d.kind == PFEM_KIND_HEAT) {
+            pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
+            setup_heat(c);
         } else {
             pfem::build_mesh_basic(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             const size_t ss = wc_state_size(c);
@@ -439,6 +553,8 @@ int pfem_case_set_state(pfem_case* c, const double* q) {
             const size_t nb = sizeof(double) * (size_t)c->n;
             PFEM_HIP(hipMemcpyAsync(c->q_prev.p, q, nb, hipMemcpyHostToDevice, c->stream));
             if (c->ws.base) PFEM_HIP(hipMemcpyAsync(c->ws.x, q, nb, hipMemcpyHostToDevice, c->stream));
+        } else if (c->d.kind == PFEM_KIND_HEAT) {
+            PFEM_HIP(hipMemcpyAsync(c->q_prev.p, q, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
         } else {
             const size_t cnt = (size_t)(2 * c->dim + 2) * c->N;
             if (c->d.wc_state == PFEM_STATE_FP32) {
@@ -461,6 +577,8 @@ int pfem_case_get_state(pfem_case* c, double* q) {
         PFEM_HIP(hipSetDevice(c->ctx->device));
         if (c->d.kind == PFEM_KIND_PSPG) {
             PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+        } else if (c->d.kind == PFEM_KIND_HEAT) {
+            PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
         } else {
             const size_t cnt = (size_t)(2 * c->dim + 2) * c->N;
             if (c->d.wc_state == PFEM_STATE_FP32) {
@@ -1014,6 +1132,7 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
         for (int i = 0; i < ncases; ++i) {
             pfem_case* c = cases[i];
             require_mesh(c);
+            PFEM_REQUIRE(c->d.kind != PFEM_KIND_HEAT, PFEM_E_UNSUPPORTED, "heat cases are stepped with pfem_heat_step");
             if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
             PFEM_REQUIRE(c->ctx == ctx, PFEM_E_INVALID, "case belongs to another context");
             PFEM_REQUIRE(dts[i] > 0, PFEM_E_INVALID, "dt must be > 0");
@@ -1108,6 +1227,7 @@ int pfem_server_submit(pfem_server* S, pfem_case* c, double dt, int64_t tag, int
         PFEM_REQUIRE(c->has_mesh, PFEM_E_STATE, "no mesh set on this case");
         PFEM_REQUIRE(c->ctx == S->ctx, PFEM_E_INVALID, "case belongs to another context");
         PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        PFEM_REQUIRE(c->d.kind != PFEM_KIND_HEAT, PFEM_E_UNSUPPORTED, "heat cases are stepped with pfem_heat_step");
         if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
         const size_t wb = c->ws.base ? 0 : pfem::workspace_bytes(c);
         PFEM_REQUIRE(wb <= S->arena.capacity(), PFEM_E_OOM, "the case's step workspace exceeds the server budget");
@@ -1192,6 +1312,188 @@ int pfem_server_stats(pfem_server* S, int64_t* out8) {
         out8[6] = S->arena_used;
         out8[7] = (int64_t)S->incoming.size();
         return PFEM_OK;
+    });
+}
+
+// ---- heat (conduction) -----------------------------------------------------------------------
+int pfem_heat_set_facets(pfem_case* c, int32_t F, const int32_t* nodes, const int32_t* out_node,
+                         const uint8_t* kinds) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(F >= 0 && (F == 0 || (nodes && out_node && kinds)), PFEM_E_INVALID, "invalid facets");
+        const int d = c->dim, N = c->N;
+        for (int64_t q = 0; q < (int64_t)F * d; ++q)
+            PFEM_REQUIRE(nodes[q] >= 0 && nodes[q] < N, PFEM_E_INVALID, "facet node out of range");
+        for (int32_t f = 0; f < F; ++f)
+            PFEM_REQUIRE(out_node[f] >= 0 && out_node[f] < N, PFEM_E_INVALID, "facet out node out of range");
+        auto& H = c->heat;
+        // node -> (facet, local) entries in facet order
+        std::vector<int> cnt((size_t)N + 1, 0), list((size_t)F * d);
+        for (int64_t q = 0; q < (int64_t)F * d; ++q) cnt[nodes[q] + 1]++;
+        for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (int64_t q = 0; q < (int64_t)F * d; ++q) list[fill[nodes[q]]++] = (int)q;
+        auto up = [&](pfem::DevBuf& b, const void* src, size_t bytes) {
+            b.reserve(bytes);
+            if (bytes) PFEM_HIP(hipMemcpyAsync(b.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+        };
+        up(H.facets, nodes, sizeof(int) * (size_t)F * d);
+        up(H.fout, out_node, sizeof(int) * (size_t)F);
+        up(H.fkind, kinds, (size_t)F);
+        up(H.nf_ptr, cnt.data(), sizeof(int) * cnt.size());
+        up(H.nf_list, list.data(), sizeof(int) * list.size());
+        H.fterm.reserve(sizeof(double) * 3 * (size_t)F * d);
+        H.F = F;
+        H.has_q = false;
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_facet_points(pfem_case* c, double* xgp) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(xgp != nullptr || c->heat.F == 0, PFEM_E_INVALID, "null output");
+        const int d = c->dim, N = c->N, F = c->heat.F;
+        std::vector<double> pos((size_t)d * N);
+        std::vector<int> fn((size_t)F * d);
+        PFEM_HIP(hipMemcpyAsync(pos.data(), c->pos.p, sizeof(double) * pos.size(), hipMemcpyDeviceToHost, c->stream));
+        if (F) PFEM_HIP(hipMemcpyAsync(fn.data(), c->heat.facets.p, sizeof(int) * fn.size(), hipMemcpyDeviceToHost,
+                                       c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        // Facet::getPosFromGP: x = sum_i N_i(gp) x_i with the facet rule of the heat equation
+        const double s35 = std::sqrt(3.0 / 5.0);
+        const double N2[3][2] = {{(1 + s35) / 2, (1 - s35) / 2}, {0.5, 0.5}, {(1 - s35) / 2, (1 + s35) / 2}};
+        const double gp3[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}, {2.0 / 3.0, 1.0 / 6.0}};
+        for (int f = 0; f < F; ++f)
+            for (int g = 0; g < 3; ++g)
+                for (int k = 0; k < d; ++k) {
+                    double x = 0;
+                    for (int i = 0; i < d; ++i) {
+                        const double Ni = d == 2 ? N2[g][i]
+                                                 : (i == 0 ? 1 - gp3[g][0] - gp3[g][1] : gp3[g][i - 1]);
+                        x += Ni * pos[(size_t)k * N + fn[(size_t)f * d + i]];
+                    }
+                    xgp[((size_t)f * 3 + g) * d + k] = x;
+                }
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_set_flux(pfem_case* c, const double* q) {
+    return guarded([&] {
+        require_heat(c);
+        auto& H = c->heat;
+        if (!q || H.F == 0) {
+            H.has_q = false;
+            return PFEM_OK;
+        }
+        const size_t bytes = sizeof(double) * (size_t)H.F * 3 * c->dim;
+        H.qgp.reserve(bytes);
+        PFEM_HIP(hipMemcpyAsync(H.qgp.p, q, bytes, hipMemcpyHostToDevice, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        H.has_q = true;
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_set_dirichlet(pfem_case* c, const double* t_bc) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(t_bc != nullptr, PFEM_E_INVALID, "null Dirichlet values");
+        PFEM_HIP(hipMemcpyAsync(c->dir_vals.p, t_bc, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_assemble(pfem_case* c, double dt) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        pfem::launch_heat_assemble(heat_view(c, dt), c->stream);
+        PFEM_HIP(hipGetLastError());
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_system_nnz(pfem_case* c, int64_t* nnz) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(nnz != nullptr, PFEM_E_INVALID, "null output");
+        *nnz = c->heat.nnz;
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_get_system(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, double* vals, double* b) {
+    return guarded([&] {
+        require_heat(c);
+        auto& H = c->heat;
+        const size_t N = (size_t)c->N, nnz = (size_t)H.nnz;
+        if (row_ptr) PFEM_HIP(hipMemcpyAsync(row_ptr, H.row_ptr.p, sizeof(int) * (N + 1), hipMemcpyDeviceToHost, c->stream));
+        if (col_idx) PFEM_HIP(hipMemcpyAsync(col_idx, H.col.p, sizeof(int) * nnz, hipMemcpyDeviceToHost, c->stream));
+        if (vals) PFEM_HIP(hipMemcpyAsync(vals, H.vals.p, sizeof(double) * nnz, hipMemcpyDeviceToHost, c->stream));
+        if (b) PFEM_HIP(hipMemcpyAsync(b, H.b.p, sizeof(double) * N, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        return PFEM_OK;
+    });
+}
+
+int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats) {
+    return guarded([&] {
+        require_heat(c);
+        PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        using clk = std::chrono::steady_clock;
+        const auto t0 = clk::now();
+        const pfem::HeatView V = heat_view(c, dt);
+        pfem::launch_heat_assemble(V, c->stream);
+        const auto t1 = clk::now();
+        // solveWithGuess(b, qPrev) (HeatEquation.inl:131)
+        PFEM_HIP(hipMemcpyAsync(V.x, V.Tprev, sizeof(double) * (size_t)V.N, hipMemcpyDeviceToDevice, c->stream));
+        pfem::launch_cg_init(V, c->stream);
+        double sc[2] = {0, 0};
+        PFEM_HIP(hipMemcpyAsync(sc, V.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        const double bb = sc[1], thr = c->d.krylov_rtol * c->d.krylov_rtol * bb;
+        double rr = sc[0];
+        int it = 0;
+        // iterations in chunks of 8 between host checks of r.r (a chunk may run a few
+        // iterations past convergence; CG keeps reducing the residual meanwhile)
+        while (bb > 0 && std::isfinite(rr) && rr > thr && it < c->d.krylov_max_iter) {
+            const int chunk = std::min(8, c->d.krylov_max_iter - it);
+            for (int k = 0; k < chunk; ++k, ++it) pfem::launch_cg_iteration(V, it & 1, c->stream);
+            PFEM_HIP(hipMemcpyAsync(sc, V.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+            PFEM_HIP(hipStreamSynchronize(c->stream));
+            rr = sc[0];
+        }
+        PFEM_HIP(hipGetLastError());
+        int status = PFEM_OK;
+        if (!std::isfinite(rr)) status = PFEM_NAN;
+        else if (bb > 0 && rr > thr) status = PFEM_NOT_CONVERGED;
+        if (status == PFEM_OK) {
+            if (bb > 0)
+                PFEM_HIP(hipMemcpyAsync(c->q_prev.p, V.x, sizeof(double) * (size_t)V.N, hipMemcpyDeviceToDevice, c->stream));
+            else
+                PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * (size_t)V.N, c->stream));
+        }
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        const auto t2 = clk::now();
+        if (stats) {
+            *stats = pfem_step_stats{};
+            stats->status = status;
+            stats->picard_iters = 1;
+            stats->krylov_iters = it;
+            stats->assemblies = 1;
+            stats->max_relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+            stats->residual = stats->max_relres;
+            stats->t_assemble = std::chrono::duration<double>(t1 - t0).count();
+            stats->t_solve = std::chrono::duration<double>(t2 - t1).count();
+        }
+        return status;
     });
 }
 
@@ -1324,7 +1626,7 @@ int pfem_mesher_stats(pfem_mesher* m, double* out8) {
 }
 
 static int mesher_states_per_node(pfem_case* c) {
-    return c->d.kind == PFEM_KIND_PSPG ? c->dim + 1 : 2 * c->dim + 2;
+    return c->d.kind == PFEM_KIND_PSPG ? c->dim + 1 : (c->d.kind == PFEM_KIND_HEAT ? 1 : 2 * c->dim + 2);
 }
 
 int pfem_case_set_mesh_from(pfem_case* c, pfem_mesher* m) {

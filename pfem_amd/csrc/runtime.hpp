@@ -236,6 +236,14 @@ struct pfem_case {
     // valid until the state or the positions change (pfem_wc_next_dt then needs no launch)
     bool cfl_valid = false;
     double cfl_dt2 = 0.0, cfl_sc = 0.5;
+    // heat (conduction) cases: scalar CSR over the node graph, element records, facets, CG vectors
+    struct HeatBufs {
+        pfem::DevBuf row_ptr, col, diag_pos, vals, b, erec, facets, fout, fkind, qgp, fterm, nf_ptr, nf_list;
+        pfem::DevBuf x, r, z, p, q, dinv, part, scal;
+        long long nnz = 0;
+        int F = 0;
+        bool has_q = false;
+    } heat;
     std::atomic<int> busy{0};  // a serving request holds the case
     std::unique_ptr<pfem::Engine> engine;  // direct calls
     ~pfem_case();
