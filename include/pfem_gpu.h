@@ -50,6 +50,7 @@ extern "C" {
 #define PFEM_KIND_PSPG 0 /* MomContEqIncompNewton with Solver.id = "PSPG" */
 #define PFEM_KIND_WC 1   /* WCompNewtonNoT with Solver.id = "CDS_dpdt" */
 #define PFEM_KIND_HEAT 2 /* "Conduction": HeatEqIncompNewton alone, no phase change */
+#define PFEM_KIND_FRACSTEP 3 /* MomContEqIncompNewton with Solver.id = "FracStep" */
 
 #define PFEM_RES_U 0    /* MomContEquation.inl:229-236 residual = "U" */
 #define PFEM_RES_U_P 1  /* "U_P" */
@@ -107,6 +108,8 @@ typedef struct pfem_case_desc {
     double h_conv;          /* h: convection coefficient of the "<Tag>Qh" facets */
     double t_inf;           /* Tinf: ambient temperature of convection and radiation */
     double eps_rad;         /* epsRad: emissivity of the "<Tag>Qr" facets */
+    /* fractional step (MomContEquation.inl:151-161) */
+    double gamma_fs;        /* gammaFS in [0, 1]: weight of the previous pressure */
 } pfem_case_desc;
 
 #define PFEM_PROBLEM_NEWTONIAN 0 /* "IncompNewtonNoT": K factor mu */
@@ -378,6 +381,22 @@ int pfem_heat_get_system(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, doubl
  * HeatEquation.hpp:50) to krylov_rtol, the new temperatures.  PFEM_NOT_CONVERGED / PFEM_NAN
  * leave the temperatures unchanged. */
 int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats);
+
+/* ---- Fractional step (PFEM_KIND_FRACSTEP) --------------------------------------------------
+ * One MomContEqIncompNewton::solve with Solver.id = "FracStep" (MomContEquation.inl:296-303;
+ * the Picard callbacks of MomContEquationFracStep.inl:462-630): per Picard iteration the
+ * velocity prediction (M/dt + K) vt = F + M/dt v_prev + gammaFS D^T p_prev (free rows
+ * v_prev + dt g, Dirichlet rows and columns from pfem_case_set_dirichlet), the pressure
+ * correction L p = -(rho/dt) D vt + gammaFS L p_prev (zero right-hand side on free-surface and
+ * free nodes), the velocity correction M dv = dt D^T (p - gammaFS p_prev), v = vt + dv,
+ * positions = saved + dt v; until the U / U_P residual (residual, min_res, max_iter of the
+ * descriptor; "Ax_f" is refused at pfem_case_create, as the reference throws) drops below
+ * min_res.  Each system is solved by diagonal-preconditioned conjugate gradient from a zero
+ * guess (the reference's Eigen::ConjugateGradient<Lower|Upper>) to krylov_rtol within
+ * krylov_max_iter iterations.  The state is q = (v, p), (dim + 1) N values, as for PSPG.
+ * PFEM_NOT_CONVERGED (Picard iterations or a CG solve) / PFEM_NAN restore the positions and
+ * leave the state unchanged.  Dirichlet nodes must be fixed (as for pfem_pspg_step). */
+int pfem_frac_step(pfem_case* c, double dt, pfem_step_stats* stats);
 
 /* ---- Host remesher (no GPU work) ------------------------------------------------------------
  * The reference's Mesh class after loadFromFile: triangulateAlphaShape (Mesh2D.cpp:32-251,

@@ -85,6 +85,10 @@ def lib():
                                     _u8p, _dp, C.c_double, C.c_int, _ip, _dp]
         L.orc_heat_facet_terms.argtypes = [C.c_int, C.c_int64, _dp, _dp, C.c_int64, _ip, _ip, _dp, C.c_double,
                                            C.c_double, C.c_double, _dp, _dp, _dp]
+        L.orc_frac_step.restype = C.c_int
+        L.orc_frac_step.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _u8p, _dp, _dp, _dp, C.c_double, C.c_double,
+                                    _dp, C.c_double, C.c_double, C.c_int, C.c_double, C.c_int, C.c_double, C.c_int,
+                                    _dp]
         _lib = L
     return _lib
 
@@ -414,3 +418,26 @@ def heat_facet_terms(mesh, T, hc: HeatCase):
                            _i(out), _d(qgp) if qgp is not None else C.cast(None, _dp), hc.h, hc.Tinf, hc.epsRad,
                            _d(qn), _d(sgh), _d(sgr))
     return qn, sgh, sgr
+
+
+def frac_step(mesh, pos, q, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3, gamma_fs=1.0, residual="U",
+              min_res=1e-6, max_iter=10, rtol=1e-13, cg_maxit=0):
+    """One fractional-step time step (MomContEquationFracStep.inl:7-630): pos and q = (v, p)
+    updated in place on success.  Returns (status, stats) with status 0 converged, 1 Picard not
+    converged, 2 NaN.  Each CG solve stops at rtol or cg_maxit iterations (2 n when 0) and is
+    used as it stands, converged or not, as in the reference."""
+    L = lib()
+    d = mesh.dim
+    N = mesh.n_nodes
+    g = _g3(g if g is not None else ([0.0, -9.81] if d == 2 else [0.0, 0.0, -9.81]))
+    dir_vals = np.zeros(d * N) if dir_vals is None else np.ascontiguousarray(dir_vals, dtype=np.float64)
+    conn = np.ascontiguousarray(mesh.conn, dtype=np.int32)
+    flags = np.ascontiguousarray(mesh.flags, dtype=np.uint8)
+    assert pos.dtype == np.float64 and pos.flags.c_contiguous and q.dtype == np.float64 and q.flags.c_contiguous
+    assert q.size == (d + 1) * N and pos.size == d * N
+    stats = np.zeros(5)
+    st = L.orc_frac_step(d, N, mesh.n_elems, _i(conn), _u8(flags), _d(pos), _d(q), _d(dir_vals), rho, mu, _d(g), dt,
+                         gamma_fs, RES_KIND[residual], min_res, max_iter, rtol, cg_maxit, _d(stats))
+    keys = ["picard_iters", "cg_iters_vapp", "cg_iters_pcorr", "cg_iters_v", "final_res"]
+    return st, dict(zip(keys, stats.tolist()))
+

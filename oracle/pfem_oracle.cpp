@@ -1534,6 +1534,361 @@ static void heatSystem(const HeatParams& P, const double* pos, const double* qPr
     applyBCHeat<D>(P, fl, pos, qPrev, Tbc, S);
 }
 
+// --------------------------------------------------------------------------------------
+// Fractional step (Solver.id = "FracStep"): srcs/simulation/physics/IncompNewton/
+// MomContEquationFracStep.inl — m_buildMatFracStep (:7-195), m_applyBCVAppStep (:197-266, the
+// FST loop skipped at gamma = 0), m_buildMatPcorrStep (:268-314), m_applyBCPCorrStep (:316-341),
+// m_buildMatVStep (:343-429), m_applyBCVStep (:431-460), the Picard callbacks (:462-630): three
+// ConjugateGradient<Lower|Upper> solves per iteration (MomContEquation.hpp:49) from a zero guess.
+// --------------------------------------------------------------------------------------
+// setFromTriplets as in buildAbPSPG (duplicates summed in triplet order, columns sorted)
+static void csrFromTriplets(int64_t n, const std::vector<Triplet>& T, CsrSys& S) {
+    std::vector<int64_t> cnt(n + 1, 0);
+    for (const auto& t : T) cnt[t.row + 1]++;
+    for (int64_t r = 0; r < n; ++r) cnt[r + 1] += cnt[r];
+    std::vector<int64_t> cur(cnt.begin(), cnt.end() - 1), ecol(T.size());
+    std::vector<double> eval(T.size());
+    for (const auto& t : T) {
+        const int64_t q = cur[t.row]++;
+        ecol[q] = t.col;
+        eval[q] = t.val;
+    }
+    S.n = n;
+    S.rowPtr.assign(n + 1, 0);
+    std::vector<std::vector<std::pair<int64_t, double>>> rows(n);
+#pragma omp parallel for schedule(dynamic, 1024)
+    for (int64_t r = 0; r < n; ++r) {
+        auto& out = rows[r];
+        for (int64_t q = cnt[r]; q < cnt[r + 1]; ++q) {
+            bool found = false;
+            for (auto& pr : out)
+                if (pr.first == ecol[q]) {
+                    pr.second += eval[q];
+                    found = true;
+                    break;
+                }
+            if (!found) out.push_back({ecol[q], eval[q]});
+        }
+        std::sort(out.begin(), out.end());
+    }
+    for (int64_t r = 0; r < n; ++r) S.rowPtr[r + 1] = S.rowPtr[r] + (int)rows[r].size();
+    S.colIdx.resize(S.rowPtr[n]);
+    S.vals.resize(S.rowPtr[n]);
+    for (int64_t r = 0; r < n; ++r) {
+        int64_t q = S.rowPtr[r];
+        for (const auto& pr : rows[r]) {
+            S.colIdx[q] = (int)pr.first;
+            S.vals[q] = pr.second;
+            ++q;
+        }
+    }
+}
+
+struct FracSys {
+    CsrSys M, MK, L;                   // m_M, m_MK_dt (with bVAppStep in MK.b), m_L
+    std::vector<std::array<double, 12 * 4>> DT;  // m_DTelm: D^T (d ne x ne) per element, row-major
+    std::vector<std::array<double, 16>> Le;      // m_Lelm (ne x ne)
+};
+
+// m_buildMatFracStep + m_applyBCVAppStep
+template <int D>
+static void buildFracVApp(const Params& P, const std::vector<uint8_t>& fl, const double* pos, const double* vPrevQ,
+                          const double* pPrevQ, const double* statesV, const double* dirVals, double gammaFS,
+                          FracSys& F) {
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    const int64_t N = P.N, E = P.E;
+    const double dt = P.dt;
+    MB<D> mb;
+    double dd[nv][nv];
+    ddevIncomp<D>(dd);
+    const int64_t tpM = (int64_t)D * ne * ne, tpMK = (int64_t)D * ne * ne + (int64_t)D * ne * D * ne,
+                  tpL = (int64_t)ne * ne, dpb = 3 * (int64_t)D * ne;
+    std::vector<Triplet> iM(tpM * E), iMK(tpMK * E), iL(tpL * E);
+    std::vector<std::pair<int64_t, double>> ib(dpb * E, {0, 0.0});
+    F.DT.assign(E, {});
+    F.Le.assign(E, {});
+#pragma omp parallel for schedule(static)
+    for (int64_t elm = 0; elm < E; ++elm) {
+        const int* nodes = P.conn + elm * ne;
+        Geo geo;
+        computeGeo<D>(pos, N, nodes, geo);
+        double G[D][ne], Bm[nv][nd];
+        mb.gradN(geo, G);
+        mb.B(G, Bm);
+        double Ms[ne][ne], Ke[nd][nd], Dm[ne][nd], Le[ne][ne], Fe[nd];
+        mb.M(geo.detJ, [&](const double*) { return P.rho; }, Ms);
+        mb.K(geo.detJ, kFactor<D>(P, Bm, dd, nodes, statesV), dd, Bm, Ke);
+        mb.Dm(geo.detJ, [&](const double*) { return 1.0; }, Bm, Dm);
+        mb.L(geo.detJ, 1.0, G, Le);
+        mb.Fv(geo.detJ, [&](const double*) { return P.rho; }, P.g, Fe);
+        const double idt = 1 / dt;
+        double Ms_dt[ne][ne];
+        for (int a = 0; a < ne; ++a)
+            for (int b = 0; b < ne; ++b) Ms_dt[a][b] = idt * Ms[a][b];
+        for (int r = 0; r < nd; ++r)
+            for (int c = 0; c < ne; ++c) F.DT[elm][r * ne + c] = Dm[c][r];
+        for (int a = 0; a < ne; ++a)
+            for (int b = 0; b < ne; ++b) F.Le[elm][a * ne + b] = Le[a][b];
+        double vP[nd], pP[ne];
+        for (int d = 0; d < D; ++d)
+            for (int a = 0; a < ne; ++a) vP[a + d * ne] = vPrevQ[nodes[a] + d * N];
+        for (int a = 0; a < ne; ++a) pP[a] = pPrevQ[nodes[a]];
+        double Mv[nd], DTp[nd];
+        for (int d = 0; d < D; ++d)
+            for (int a = 0; a < ne; ++a) {
+                double t = 0;
+                for (int b = 0; b < ne; ++b) t += Ms_dt[a][b] * vP[b + d * ne];
+                Mv[a + d * ne] = t;
+            }
+        for (int r = 0; r < nd; ++r) {
+            double t = 0;
+            for (int c = 0; c < ne; ++c) t += F.DT[elm][r * ne + c] * pP[c];
+            DTp[r] = gammaFS * t;
+        }
+        int64_t cM = 0, cMK = 0, cL = 0, cb = 0;
+        for (int i = 0; i < ne; ++i) {
+            const uint8_t fi = fl[nodes[i]];
+            const bool bf = (fi & (BOUND | FREE)) != 0;
+            for (int j = 0; j < ne; ++j) {
+                for (int d = 0; d < D; ++d) {
+                    if (!bf) {
+                        iM[tpM * elm + cM] = Triplet{nodes[i] + d * N, nodes[j] + d * N, Ms[i][j]};
+                        iMK[tpMK * elm + cMK] = Triplet{nodes[i] + d * N, nodes[j] + d * N, Ms_dt[i][j]};
+                    }
+                    cM++;
+                    cMK++;
+                    for (int d2 = 0; d2 < D; ++d2) {
+                        if (!bf)
+                            iMK[tpMK * elm + cMK] =
+                                Triplet{nodes[i] + d * N, nodes[j] + d2 * N, Ke[i + d * ne][j + d2 * ne]};
+                        cMK++;
+                    }
+                }
+                if (!(fi & FREE)) iL[tpL * elm + cL] = Triplet{nodes[i], nodes[j], Le[i][j]};
+                cL++;
+            }
+            for (int d = 0; d < D; ++d)
+                if (!bf) {
+                    ib[dpb * elm + cb++] = {nodes[i] + d * N, Fe[i + d * ne]};
+                    ib[dpb * elm + cb++] = {nodes[i] + d * N, Mv[i + d * ne]};
+                    ib[dpb * elm + cb++] = {nodes[i] + d * N, DTp[i + d * ne]};
+                }
+        }
+    }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        if (fl[nn] & FREE) iL.push_back(Triplet{nn, nn, 1});
+        if (fl[nn] & (BOUND | FREE))
+            for (int d = 0; d < D; ++d) {
+                iMK.push_back(Triplet{nn + d * N, nn + d * N, 1});
+                iM.push_back(Triplet{nn + d * N, nn + d * N, 1});
+            }
+    }
+    csrFromTriplets(D * N, iM, F.M);
+    csrFromTriplets(D * N, iMK, F.MK);
+    csrFromTriplets(N, iL, F.L);
+    F.MK.b.assign(D * N, 0.0);
+    for (const auto& db : ib) F.MK.b[db.first] += db.second;
+    // m_applyBCVAppStep (gamma = 0: no FST)
+    CsrSys& S = F.MK;
+    std::vector<int64_t> ccnt(S.n + 1, 0);
+    for (int c : S.colIdx) ccnt[c + 1]++;
+    for (int64_t c = 0; c < S.n; ++c) ccnt[c + 1] += ccnt[c];
+    std::vector<int64_t> cpos(ccnt.begin(), ccnt.end() - 1), cRow(S.colIdx.size()), cK(S.colIdx.size());
+    for (int64_t r = 0; r < S.n; ++r)
+        for (int64_t q = S.rowPtr[r]; q < S.rowPtr[r + 1]; ++q) {
+            const int64_t c = S.colIdx[q];
+            cRow[cpos[c]] = r;
+            cK[cpos[c]] = q;
+            cpos[c]++;
+        }
+    for (int64_t nn = 0; nn < N; ++nn) {
+        if ((fl[nn] & FREE) && !(fl[nn] & BOUND))
+            for (int d = 0; d < D; ++d) S.b[nn + d * N] = vPrevQ[nn + d * N] + dt * P.g[d];
+        if ((fl[nn] & BOUND) && (fl[nn] & DIRICHLET))
+            for (int d = 0; d < D; ++d) {
+                const double res = dirVals[nn + d * N];
+                const int64_t col = nn + d * N;
+                S.b[col] = res;
+                for (int64_t q = ccnt[col]; q < ccnt[col + 1]; ++q) {
+                    const int64_t row = cRow[q];
+                    if (row == col) continue;
+                    S.b[row] -= S.vals[cK[q]] * res;
+                    S.vals[cK[q]] = 0;
+                }
+            }
+    }
+}
+
+// column c of a CSR system set to the unit column (diagonal 1, other entries 0)
+static void unitColumns(CsrSys& S, const std::vector<char>& isUnit) {
+    for (int64_t r = 0; r < S.n; ++r)
+        for (int64_t q = S.rowPtr[r]; q < S.rowPtr[r + 1]; ++q) {
+            const int64_t c = S.colIdx[q];
+            if (isUnit[c]) S.vals[q] = (c == r) ? 1.0 : 0.0;
+        }
+}
+
+struct FracStats {
+    int picardIters = 0, cgIters[3] = {0, 0, 0};
+    double finalRes = 0;
+};
+
+// One FracStep time step (PicardAlgo::solve with the callbacks of :462-630); q = (v, p)
+// field-blocked (d+1) N, updated with pos on success.  Returns 0, 1 (Picard not converged) or 2
+// (NaN).  Each solve is CG to rtol, at most cgMaxit iterations (2 n when cgMaxit <= 0, Eigen's
+// default); like the reference, which checks only the preconditioner's set-up (:489-504), an
+// unconverged solve is used as it stands (the pressure Laplacian is singular: a right-hand side
+// off its range, as on a free surface, stalls CG).
+template <int D>
+static int fracStep(const Params& P, double* pos, double* q, const double* dirVals, double gammaFS, int resKind,
+                    double minRes, int maxIter, double rtol, int cgMaxit, FracStats& st) {
+    const int64_t N = P.N, nv = D * N;
+    auto fl = derivedFlags(P);
+    std::vector<double> posSave(pos, pos + D * N);
+    std::vector<double> vPrev(q, q + nv), pPrev(q + nv, q + nv + N);
+    std::vector<double> vIter(nv, 0.0), pIter(N, 0.0), vIterPrev(vPrev), pIterPrev(pPrev);
+    std::vector<double> statesV(vPrev);  // node velocity states read by the K factor
+    auto restore = [&]() { std::copy(posSave.begin(), posSave.end(), pos); };
+    unsigned it = 0;
+    double res = std::numeric_limits<double>::max();
+    while (res > minRes) {
+        if (it > (unsigned)maxIter) {
+            restore();
+            st.picardIters = it;
+            return 1;
+        }
+        FracSys F;
+        if (D == 2)
+            buildFracVApp<2>(P, fl, pos, vPrev.data(), pPrev.data(), statesV.data(), dirVals, gammaFS, F);
+        else
+            buildFracVApp<3>(P, fl, pos, vPrev.data(), pPrev.data(), statesV.data(), dirVals, gammaFS, F);
+        int its;
+        double rr;
+        std::vector<double> vTilde(nv, 0.0);
+        if (pcg(nv, F.MK.rowPtr.data(), F.MK.colIdx.data(), F.MK.vals.data(), F.MK.b.data(), vTilde.data(), rtol,
+                cgMaxit > 0 ? cgMaxit : (int)(2 * nv), &its, &rr) == 2) {
+            restore();
+            st.picardIters = it;
+            return 2;
+        }
+        st.cgIters[0] += its;
+        // m_buildMatPcorrStep + m_applyBCPCorrStep
+        constexpr int ne = D + 1;
+        std::vector<std::pair<int64_t, double>> ibp((size_t)2 * ne * P.E, {0, 0.0});
+        for (int64_t elm = 0; elm < P.E; ++elm) {
+            const int* nodes = P.conn + elm * ne;
+            double vT[D * ne], pP[ne];
+            for (int d = 0; d < D; ++d)
+                for (int a = 0; a < ne; ++a) vT[a + d * ne] = vTilde[nodes[a] + d * N];
+            for (int a = 0; a < ne; ++a) pP[a] = pPrev[nodes[a]];
+            int64_t c = 0;
+            for (int i = 0; i < ne; ++i) {
+                double Dv = 0, Lp = 0;
+                for (int r = 0; r < D * ne; ++r) Dv += F.DT[elm][r * ne + i] * vT[r];
+                for (int b = 0; b < ne; ++b) Lp += F.Le[elm][i * ne + b] * pP[b];
+                const uint8_t fi = fl[nodes[i]];
+                if (!(fi & FREE) && !(fi & ONFS)) {
+                    ibp[(size_t)2 * ne * elm + c++] = {nodes[i], -((P.rho / P.dt) * Dv)};
+                    ibp[(size_t)2 * ne * elm + c++] = {nodes[i], gammaFS * Lp};
+                }
+            }
+        }
+        F.L.b.assign(N, 0.0);
+        for (const auto& db : ibp) F.L.b[db.first] += db.second;
+        std::vector<char> freeNode(N, 0);
+        for (int64_t nn = 0; nn < N; ++nn)
+            if (fl[nn] & FREE) {
+                F.L.b[nn] = 0;
+                freeNode[nn] = 1;
+            }
+        unitColumns(F.L, freeNode);
+        std::vector<double> pNew(N, 0.0);
+        if (pcg(N, F.L.rowPtr.data(), F.L.colIdx.data(), F.L.vals.data(), F.L.b.data(), pNew.data(), rtol,
+                cgMaxit > 0 ? cgMaxit : (int)(2 * N), &its, &rr) == 2) {
+            restore();
+            st.picardIters = it;
+            return 2;
+        }
+        st.cgIters[1] += its;
+        // m_buildMatVStep + m_applyBCVStep
+        std::vector<double> dP(N);
+        for (int64_t nn = 0; nn < N; ++nn) dP[nn] = pNew[nn] - gammaFS * pPrev[nn];
+        std::vector<std::pair<int64_t, double>> ibv((size_t)D * ne * P.E, {0, 0.0});
+        for (int64_t elm = 0; elm < P.E; ++elm) {
+            const int* nodes = P.conn + elm * ne;
+            double dPe[ne];
+            for (int a = 0; a < ne; ++a) dPe[a] = dP[nodes[a]];
+            int64_t c = 0;
+            for (int i = 0; i < ne; ++i) {
+                const uint8_t fi = fl[nodes[i]];
+                if (!(fi & FREE) && !(fi & BOUND))
+                    for (int d = 0; d < D; ++d) {
+                        double t = 0;
+                        for (int b = 0; b < ne; ++b) t += F.DT[elm][(i + d * ne) * ne + b] * dPe[b];
+                        ibv[(size_t)D * ne * elm + c++] = {nodes[i] + d * N, P.dt * t};
+                    }
+            }
+        }
+        F.M.b.assign(nv, 0.0);
+        for (const auto& db : ibv) F.M.b[db.first] += db.second;
+        std::vector<char> unitV(nv, 0);
+        for (int64_t nn = 0; nn < N; ++nn)
+            if (fl[nn] & (FREE | BOUND))
+                for (int d = 0; d < D; ++d) {
+                    F.M.b[nn + d * N] = 0;
+                    unitV[nn + d * N] = 1;
+                }
+        unitColumns(F.M, unitV);
+        std::vector<double> dV(nv, 0.0);
+        if (pcg(nv, F.M.rowPtr.data(), F.M.colIdx.data(), F.M.vals.data(), F.M.b.data(), dV.data(), rtol,
+                cgMaxit > 0 ? cgMaxit : (int)(2 * nv), &its, &rr) == 2) {
+            restore();
+            st.picardIters = it;
+            return 2;
+        }
+        st.cgIters[2] += its;
+        for (int64_t k = 0; k < nv; ++k) vIter[k] = vTilde[k] + dV[k];
+        pIter = pNew;
+        statesV = vIter;  // setNodesStatesfromQ (:558-559)
+        for (int64_t nn = 0; nn < N; ++nn)  // updateNodesPositionFromSave (Mesh.cpp:1238-1277)
+            if (!(P.flags[nn] & FIXED))
+                for (int d = 0; d < D; ++d) pos[nn + d * N] = posSave[nn + d * N] + vIter[nn + d * N] * P.dt;
+        // residual (:568-628): U or U_P over the nodes that are not free
+        double num = 0, den = 0, resV, resP = 0;
+        for (int64_t nn = 0; nn < N; ++nn)
+            if (!(fl[nn] & FREE))
+                for (int d = 0; d < D; ++d) {
+                    const double a = vIter[nn + d * N], b = vIterPrev[nn + d * N];
+                    num += (a - b) * (a - b);
+                    den += b * b;
+                }
+        resV = den == 0 ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+        if (resKind == 1) {
+            num = den = 0;
+            for (int64_t nn = 0; nn < N; ++nn)
+                if (!(fl[nn] & FREE)) {
+                    num += (pIter[nn] - pIterPrev[nn]) * (pIter[nn] - pIterPrev[nn]);
+                    den += pIterPrev[nn] * pIterPrev[nn];
+                }
+            resP = den == 0 ? std::numeric_limits<double>::max() : std::sqrt(num / den);
+        }
+        res = std::max(resV, resP);
+        vIterPrev = vIter;
+        pIterPrev = pIter;
+        if (std::isnan(res)) {
+            restore();
+            st.picardIters = it + 1;
+            return 2;
+        }
+        it++;
+    }
+    std::copy(vIter.begin(), vIter.end(), q);
+    std::copy(pIter.begin(), pIter.end(), q + nv);
+    st.picardIters = it;
+    st.finalRes = res;
+    return 0;
+}
+
 extern "C" {
 
 struct orc_sys {
@@ -1929,5 +2284,33 @@ void orc_heat_facet_terms(int dim, int64_t N, const double* pos, const double* T
         else
             heatFacetTerms<3>(P, pos, T, f, qn + f * 3, sgh + f * 3, sgr + f * 3);
     }
+}
+}  // extern "C"
+
+extern "C" {
+// One FracStep time step; q = (v, p) field-blocked, (dim+1) N; resKind 0 "U", 1 "U_P".
+// stats5: {Picard iterations, CG iterations of the three solves, final residual}.
+int orc_frac_step(int dim, int64_t N, int64_t E, const int* conn, const uint8_t* flags, double* pos, double* q,
+                  const double* dirVals, double rho, double mu, const double* g, double dt, double gammaFS,
+                  int resKind, double minRes, int maxIter, double rtol, int cgMaxit, double* stats5) {
+    Params P{};
+    P.dim = dim;
+    P.N = N;
+    P.E = E;
+    P.conn = conn;
+    P.flags = flags;
+    P.rho = rho;
+    P.mu = mu;
+    for (int k = 0; k < 3; ++k) P.g[k] = g[k];
+    P.dt = dt;
+    FracStats st;
+    const int r = dim == 2 ? fracStep<2>(P, pos, q, dirVals, gammaFS, resKind, minRes, maxIter, rtol, cgMaxit, st)
+                           : fracStep<3>(P, pos, q, dirVals, gammaFS, resKind, minRes, maxIter, rtol, cgMaxit, st);
+    if (stats5) {
+        stats5[0] = st.picardIters;
+        for (int k = 0; k < 3; ++k) stats5[1 + k] = st.cgIters[k];
+        stats5[4] = st.finalRes;
+    }
+    return r;
 }
 }  // extern "C"

@@ -22,8 +22,8 @@ from . import _lib
 from ._lib import PfemError, check, lib
 from .meshgen import BOUND, DIRICHLET, FIXED, FREE, ONFS, PfemMesh
 
-__all__ = ["Context", "Case", "MomContEqIncompNewton", "PicardAlgo", "WCompNewtonStep", "batch_step", "Server",
-           "now_ns",
+__all__ = ["Context", "Case", "MomContEqIncompNewton", "MomContEqFracStep", "PicardAlgo", "WCompNewtonStep",
+           "HeatEqIncompNewton", "SolverIncompNewton", "Mesher", "batch_step", "Server", "now_ns",
            "PfemError", "PfemMesh", "BOUND", "FREE", "ONFS", "FIXED", "DIRICHLET"]
 
 
@@ -73,15 +73,16 @@ class Case:
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
                  inner_rtol=1e-3, inner_max_iter=300, precond="auto", problem="Newtonian", tau0=0.0, mReg=0.0,
-                 cv=0.0, k=0.0, h=0.0, Tinf=0.0, epsRad=0.0):
+                 cv=0.0, k=0.0, h=0.0, Tinf=0.0, epsRad=0.0, gammaFS=0.0):
         d = _lib.CaseDesc()
         d.dim = dim
-        kinds = {"PSPG": _lib.KIND_PSPG, "WC": _lib.KIND_WC, "HEAT": _lib.KIND_HEAT}
+        kinds = {"PSPG": _lib.KIND_PSPG, "WC": _lib.KIND_WC, "HEAT": _lib.KIND_HEAT, "FracStep": _lib.KIND_FRACSTEP}
         if kind not in kinds:
             raise PfemError(_lib.PFEM_E_INVALID, "unknown case kind: " + str(kind))
         d.kind = kinds[kind]
         d.cv, d.k_cond, d.h_conv, d.t_inf, d.eps_rad = cv, k, h, Tinf, epsRad
         d.rho, d.mu, d.gamma = rho, mu, gamma
+        d.gamma_fs = gammaFS
         bf = body_force if body_force is not None else ([0.0, -9.81] if dim == 2 else [0.0, 0.0, -9.81])
         for k in range(3):
             d.body_force[k] = bf[k] if k < len(bf) else 0.0
@@ -159,7 +160,7 @@ class Case:
     def n_state(self):
         if self.kind == "HEAT":
             return self.N
-        return (self.dim + 1) * self.N if self.kind == "PSPG" else (2 * self.dim + 2) * self.N
+        return (self.dim + 1) * self.N if self.kind in ("PSPG", "FracStep") else (2 * self.dim + 2) * self.N
 
     def set_state(self, q):
         q = np.ascontiguousarray(q, dtype=np.float64).reshape(-1)
@@ -348,6 +349,30 @@ class MomContEqIncompNewton(Case):
     def solve(self, dt):  # noqa: D401  (mirrors Equation::solve)
         rc, st = self.pspg_step(dt)
         self.last_stats = st
+        return rc == _lib.PFEM_OK
+
+
+class MomContEqFracStep(Case):
+    """The momentum-continuity equation with Solver.id = "FracStep" (MomContEquationFracStep.inl)
+    on the GPU: keywords of the MomContEq table (residual "U" or "U_P", maxIter, minRes, gammaFS)
+    and of the Material table (rho, mu; problem "Bingham" with tau0, mReg).  ``solve(dt)`` runs
+    PicardAlgo::solve with the three-solve callbacks for one time step and returns True on
+    success, False on a numerical failure (positions and states restored), like the reference."""
+
+    def __init__(self, ctx, dim, gammaFS=1.0, residual="U", krylov_rtol=1e-12, krylov_max_iter=100000, **kw):
+        super().__init__(ctx, dim, kind="FracStep", gammaFS=gammaFS, residual=residual, krylov_rtol=krylov_rtol,
+                         krylov_max_iter=krylov_max_iter, **kw)
+        self.last_stats = None
+
+    def step(self, dt):
+        """(status, stats dict) of one time step."""
+        st = _lib.StepStats()
+        rc = check(lib().pfem_frac_step(self._h, float(dt), C.byref(st)), allow_positive=True)
+        self.last_stats = st.as_dict()
+        return rc, self.last_stats
+
+    def solve(self, dt) -> bool:  # noqa: D401  (mirrors Equation::solve)
+        rc, _ = self.step(dt)
         return rc == _lib.PFEM_OK
 
 

@@ -23,6 +23,7 @@ PFEM_NAN = 2
 KIND_PSPG = 0
 KIND_WC = 1
 KIND_HEAT = 2
+KIND_FRACSTEP = 3
 HEAT_Q, HEAT_QH, HEAT_QR = 1, 2, 4
 RES = {"U": 0, "U_P": 1, "Ax_f": 2}
 PREC_FP64 = 0
@@ -50,7 +51,7 @@ EXPORTS = [
     "pfem_mesher_get_nodes", "pfem_mesher_get_elements", "pfem_mesher_get_facets", "pfem_mesher_stats",
     "pfem_case_remesh", "pfem_case_set_mesh_from", "pfem_case_setup_times", "pfem_heat_set_facets",
     "pfem_heat_facet_points", "pfem_heat_set_flux", "pfem_heat_set_dirichlet", "pfem_heat_assemble",
-    "pfem_heat_system_nnz", "pfem_heat_get_system", "pfem_heat_step",
+    "pfem_heat_system_nnz", "pfem_heat_get_system", "pfem_heat_step", "pfem_frac_step",
 ]
 
 
@@ -64,6 +65,7 @@ class CaseDesc(C.Structure):
         ("problem", C.c_int32), ("tau0", C.c_double), ("m_reg", C.c_double),
         ("cv", C.c_double), ("k_cond", C.c_double), ("h_conv", C.c_double), ("t_inf", C.c_double),
         ("eps_rad", C.c_double),
+        ("gamma_fs", C.c_double),
     ]
 
 
@@ -195,6 +197,7 @@ def lib():
         "pfem_heat_system_nnz": [_vp, C.POINTER(C.c_int64)],
         "pfem_heat_get_system": [_vp, _ip, _ip, _dp, _dp],
         "pfem_heat_step": [_vp, C.c_double, C.POINTER(StepStats)],
+        "pfem_frac_step": [_vp, C.c_double, C.POINTER(StepStats)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

@@ -10,7 +10,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libpfem_gpu.so")
-SOURCES = ["pattern.hip", "ell.hip", "pspg.hip", "heat.hip", "krylov.hip", "picard.hip", "wc.hip", "extract.hip", "amg.hip", "microbench.hip",
+SOURCES = ["pattern.hip", "ell.hip", "pspg.hip", "heat.hip", "fracstep.hip", "krylov.hip", "picard.hip", "wc.hip", "extract.hip", "amg.hip", "microbench.hip",
            "amg_hierarchy.cpp",
            "engine.cpp", "capi.cpp", "mesher.cpp"]
 # host-only translation units (no device code): plain C++, no floating-point contraction (the

@@ -10,6 +10,7 @@
 #include <string>
 #include <vector>
 
+#include "fracstep.hpp"
 #include "heat.hpp"
 #include "mesher.hpp"
 #include "runtime.hpp"
@@ -139,6 +140,125 @@ pfem::HeatView heat_view(pfem_case* c, double dt) {
 }
 
 
+// Fractional-step case: the three graph matrices, element records, right-hand sides, the
+// Picard iterate and the CG vectors.
+void setup_frac(pfem_case* c) {
+    const int D = c->dim, N = c->N, E = c->E;
+    const size_t G = (size_t)c->pat.graph_nnz, vN = (size_t)D * N;
+    auto& F = c->frac;
+    F.erec.reserve(sizeof(double) * 16 * (size_t)E);
+    F.mk.reserve(sizeof(double) * G * D * D);
+    F.ms.reserve(sizeof(double) * G);
+    F.lv.reserve(sizeof(double) * G);
+    for (DevBuf* b : {&F.dinv_mk, &F.dinv_m, &F.bv, &F.vt, &F.dv, &F.v_it, &F.v_itp, &F.pos_save, &F.r, &F.z, &F.p,
+                      &F.q})
+        b->reserve(sizeof(double) * vN);
+    for (DevBuf* b : {&F.dinv_l, &F.bp, &F.pn, &F.p_itp}) b->reserve(sizeof(double) * (size_t)N);
+    F.res_part.reserve(sizeof(double) * 4 * (size_t)pfem::frac_blocks(N));
+    F.part.reserve(sizeof(double) * 5 * (size_t)pfem::frac_blocks(D * N));
+    F.scal.reserve(sizeof(double) * 2);
+    const size_t nq = sizeof(double) * (size_t)(D + 1) * N;
+    c->q_prev.reserve(nq);
+    PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, nq, c->stream));
+}
+
+pfem::FracView frac_view(pfem_case* c, double dt) {
+    auto& F = c->frac;
+    const int D = c->dim, N = c->N;
+    pfem::FracView V;
+    V.dim = D;
+    V.N = N;
+    V.E = c->E;
+    V.nb = pfem::frac_blocks(N);
+    V.conn = c->pat.conn.as<int>();
+    V.flags = c->pat.flags.as<uint8_t>();
+    V.pos = c->pos.as<double>();
+    V.pos_save = F.pos_save.as<double>();
+    V.nbr_ptr = c->pat.nbr_ptr.as<int>();
+    V.nbr_idx = c->pat.nbr_idx.as<int>();
+    V.pei_ptr = c->pat.pei_ptr.as<int>();
+    V.pei = c->pat.pei.as<unsigned>();
+    V.erec = F.erec.as<double>();
+    V.mk = F.mk.as<double>();
+    V.ms = F.ms.as<double>();
+    V.lv = F.lv.as<double>();
+    V.dinv_mk = F.dinv_mk.as<double>();
+    V.dinv_m = F.dinv_m.as<double>();
+    V.dinv_l = F.dinv_l.as<double>();
+    V.bv = F.bv.as<double>();
+    V.bp = F.bp.as<double>();
+    V.v_prev = c->q_prev.as<double>();
+    V.p_prev = c->q_prev.as<double>() + (size_t)D * N;
+    V.vbc = c->dir_vals.as<double>();
+    V.vt = F.vt.as<double>();
+    V.pn = F.pn.as<double>();
+    V.dv = F.dv.as<double>();
+    V.v_it = F.v_it.as<double>();
+    V.v_itp = F.v_itp.as<double>();
+    V.p_itp = F.p_itp.as<double>();
+    V.res_part = F.res_part.as<double>();
+    V.rho = c->d.rho;
+    V.mu = c->d.mu;
+    V.dt = dt;
+    V.gamma_fs = c->d.gamma_fs;
+    V.problem = c->d.problem;
+    V.tau0 = c->d.tau0;
+    V.mreg = c->d.m_reg;
+    for (int k = 0; k < 3; ++k) V.g[k] = c->d.body_force[k];
+    return V;
+}
+
+// op: 0 L (N rows), 1 M per velocity component, 2 M/dt + K (dim N rows)
+pfem::CgView frac_cg(pfem_case* c, int op, const double* vals, const double* b, const double* dinv, double* x) {
+    auto& F = c->frac;
+    pfem::CgView C;
+    C.op = op;
+    C.dim = c->dim;
+    C.N = c->N;
+    C.n = op == 0 ? c->N : c->dim * c->N;
+    C.nb = pfem::frac_blocks(C.n);
+    C.nbr_ptr = c->pat.nbr_ptr.as<int>();
+    C.nbr_idx = c->pat.nbr_idx.as<int>();
+    C.vals = vals;
+    C.b = b;
+    C.dinv = dinv;
+    C.x = x;
+    C.r = F.r.as<double>();
+    C.z = F.z.as<double>();
+    C.p = F.p.as<double>();
+    C.q = F.q.as<double>();
+    C.part = F.part.as<double>();
+    C.scal = F.scal.as<double>();
+    return C;
+}
+
+// Conjugate gradient from a zero guess until ||b - A x||^2 <= rtol^2 ||b||^2 (r.r checked by
+// the host every 8 iterations; a chunk may run a few iterations past convergence, which CG
+// keeps reducing).  Returns PFEM_OK, PFEM_NOT_CONVERGED or PFEM_NAN.
+int frac_cg_solve(pfem_case* c, const pfem::CgView& C, int* iters, double* relres) {
+    PFEM_HIP(hipMemsetAsync(C.x, 0, sizeof(double) * (size_t)C.n, c->stream));
+    pfem::launch_cg2_init(C, c->stream);
+    double sc[2] = {0, 0};
+    PFEM_HIP(hipMemcpyAsync(sc, C.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+    PFEM_HIP(hipStreamSynchronize(c->stream));
+    const double bb = sc[1], thr = c->d.krylov_rtol * c->d.krylov_rtol * bb;
+    double rr = sc[0];
+    int it = 0;
+    while (std::isfinite(rr) && rr > thr && it < c->d.krylov_max_iter) {
+        const int chunk = std::min(8, c->d.krylov_max_iter - it);
+        for (int k = 0; k < chunk; ++k, ++it) pfem::launch_cg2_iteration(C, it & 1, c->stream);
+        PFEM_HIP(hipMemcpyAsync(sc, C.scal, sizeof sc, hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        rr = sc[0];
+    }
+    PFEM_HIP(hipGetLastError());
+    *iters = it;
+    *relres = bb > 0 ? std::sqrt(rr / bb) : 0.0;
+    if (!std::isfinite(rr)) return PFEM_NAN;
+    return rr > thr ? PFEM_NOT_CONVERGED : PFEM_OK;
+}
+
+
 template <class T>
 pfem::NodeMat<T> node_mat(const pfem::Pattern& P, int dim, const T* vals) {
     pfem::NodeMat<T> A{};
@@ -167,6 +287,10 @@ void require_pspg(pfem_case* c) {
 void require_heat(pfem_case* c) {
     require_mesh(c);
     PFEM_REQUIRE(c->d.kind == PFEM_KIND_HEAT, PFEM_E_STATE, "not a heat case");
+}
+void require_frac(pfem_case* c) {
+    require_mesh(c);
+    PFEM_REQUIRE(c->d.kind == PFEM_KIND_FRACSTEP, PFEM_E_STATE, "not a fractional-step case");
 }
 // The fused Picard step takes the Dirichlet values once per step; a Dirichlet node that
 // moves needs them re-evaluated at its new position before every re-assembly
@@ -213,6 +337,15 @@ size_t persistent_bytes(pfem_case* c) {
                       &c->pat.conn, &c->pat.flags, &c->pat.nbr_ptr, &c->pat.nbr_idx, &c->pat.node_info,
                       &c->pat.row_ptr, &c->pat.diag_pos, &c->pat.pei_ptr, &c->pat.pei, &c->pat.elem_ptr,
                       &c->pat.elem_list, &c->pat.elem_nodes, &c->pat.ell_soff, &c->pat.ell_idx})
+        b += p->bytes;
+    auto& H = c->heat;
+    for (DevBuf* p : {&H.row_ptr, &H.col, &H.diag_pos, &H.vals, &H.b, &H.erec, &H.facets, &H.fout, &H.fkind, &H.qgp,
+                      &H.fterm, &H.nf_ptr, &H.nf_list, &H.x, &H.r, &H.z, &H.p, &H.q, &H.dinv, &H.part, &H.scal})
+        b += p->bytes;
+    auto& F = c->frac;
+    for (DevBuf* p : {&F.erec, &F.mk, &F.ms, &F.lv, &F.dinv_mk, &F.dinv_m, &F.dinv_l, &F.bv, &F.bp, &F.vt, &F.pn,
+                      &F.dv, &F.v_it, &F.v_itp, &F.p_itp, &F.pos_save, &F.res_part, &F.r, &F.z, &F.p, &F.q, &F.part,
+                      &F.scal})
         b += p->bytes;
     for (auto& L : c->amg.lev)
         for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,
@@ -386,8 +519,9 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
         PFEM_REQUIRE(ctx && desc && out, PFEM_E_INVALID, "null argument");
         const auto& d = *desc;
         PFEM_REQUIRE(d.dim == 2 || d.dim == 3, PFEM_E_INVALID, "dim must be 2 or 3");
-        PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC || d.kind == PFEM_KIND_HEAT, PFEM_E_INVALID,
-                     "unknown case kind");
+        PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC || d.kind == PFEM_KIND_HEAT ||
+                         d.kind == PFEM_KIND_FRACSTEP,
+                     PFEM_E_INVALID, "unknown case kind");
         PFEM_REQUIRE(std::fabs(d.gamma) < 1e-15, PFEM_E_UNSUPPORTED,
                      "surface tension (gamma > 0) is not supported: the FST facet term is not implemented");
         PFEM_REQUIRE(d.mu >= 0, PFEM_E_INVALID, "mu must be >= 0");
@@ -405,6 +539,20 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
             PFEM_REQUIRE(d.precision != PFEM_PREC_MIXED_IR || (d.inner_rtol > 0 && d.inner_max_iter > 0),
                          PFEM_E_INVALID,
                          "inner_rtol and inner_max_iter must be > 0");
+            PFEM_REQUIRE(d.problem == PFEM_PROBLEM_NEWTONIAN || d.problem == PFEM_PROBLEM_BINGHAM, PFEM_E_INVALID,
+                         "unknown problem");
+            PFEM_REQUIRE(d.problem != PFEM_PROBLEM_BINGHAM || (d.tau0 >= 0 && d.m_reg > 0), PFEM_E_INVALID,
+                         "Bingham needs tau0 >= 0 and mReg > 0");
+        } else if (d.kind == PFEM_KIND_FRACSTEP) {
+            PFEM_REQUIRE(d.rho > 0, PFEM_E_INVALID, "rho must be > 0");
+            PFEM_REQUIRE(d.residual == PFEM_RES_U || d.residual == PFEM_RES_U_P || d.residual == PFEM_RES_AX_F,
+                         PFEM_E_INVALID, "unknown residual kind");
+            // MomContEquationFracStep.inl:626: "Ax-f residual currently unsupported"
+            PFEM_REQUIRE(d.residual != PFEM_RES_AX_F, PFEM_E_UNSUPPORTED,
+                         "the Ax_f residual is not supported by the fractional-step solver");
+            PFEM_REQUIRE(d.max_iter >= 0, PFEM_E_INVALID, "maxIter must be >= 0");
+            PFEM_REQUIRE(d.krylov_rtol > 0 && d.krylov_max_iter > 0, PFEM_E_INVALID, "invalid Krylov settings");
+            PFEM_REQUIRE(d.gamma_fs >= 0 && d.gamma_fs <= 1, PFEM_E_INVALID, "gammaFS should be between 0 and 1");
             PFEM_REQUIRE(d.problem == PFEM_PROBLEM_NEWTONIAN || d.problem == PFEM_PROBLEM_BINGHAM, PFEM_E_INVALID,
                          "unknown problem");
             PFEM_REQUIRE(d.problem != PFEM_PROBLEM_BINGHAM || (d.tau0 >= 0 && d.m_reg > 0), PFEM_E_INVALID,
@@ -526,6 +674,9 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         } else if (c->d.kind == PFEM_KIND_HEAT) {
             pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             setup_heat(c);
+        } else if (c->d.kind == PFEM_KIND_FRACSTEP) {
+            pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
+            setup_frac(c);
         } else {
             pfem::build_mesh_basic(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             const size_t ss = wc_state_size(c);
@@ -549,7 +700,7 @@ int pfem_case_set_state(pfem_case* c, const double* q) {
         PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null state");
         c->cfl_valid = false;
         PFEM_HIP(hipSetDevice(c->ctx->device));
-        if (c->d.kind == PFEM_KIND_PSPG) {
+        if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) {
             const size_t nb = sizeof(double) * (size_t)c->n;
             PFEM_HIP(hipMemcpyAsync(c->q_prev.p, q, nb, hipMemcpyHostToDevice, c->stream));
             if (c->ws.base) PFEM_HIP(hipMemcpyAsync(c->ws.x, q, nb, hipMemcpyHostToDevice, c->stream));
@@ -575,7 +726,7 @@ int pfem_case_get_state(pfem_case* c, double* q) {
         require_mesh(c);
         PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null state");
         PFEM_HIP(hipSetDevice(c->ctx->device));
-        if (c->d.kind == PFEM_KIND_PSPG) {
+        if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) {
             PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
         } else if (c->d.kind == PFEM_KIND_HEAT) {
             PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
@@ -1132,7 +1283,8 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
         for (int i = 0; i < ncases; ++i) {
             pfem_case* c = cases[i];
             require_mesh(c);
-            PFEM_REQUIRE(c->d.kind != PFEM_KIND_HEAT, PFEM_E_UNSUPPORTED, "heat cases are stepped with pfem_heat_step");
+            PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_WC, PFEM_E_UNSUPPORTED,
+                         "heat and fractional-step cases are stepped with pfem_heat_step / pfem_frac_step");
             if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
             PFEM_REQUIRE(c->ctx == ctx, PFEM_E_INVALID, "case belongs to another context");
             PFEM_REQUIRE(dts[i] > 0, PFEM_E_INVALID, "dt must be > 0");
@@ -1227,7 +1379,8 @@ int pfem_server_submit(pfem_server* S, pfem_case* c, double dt, int64_t tag, int
         PFEM_REQUIRE(c->has_mesh, PFEM_E_STATE, "no mesh set on this case");
         PFEM_REQUIRE(c->ctx == S->ctx, PFEM_E_INVALID, "case belongs to another context");
         PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
-        PFEM_REQUIRE(c->d.kind != PFEM_KIND_HEAT, PFEM_E_UNSUPPORTED, "heat cases are stepped with pfem_heat_step");
+        PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_WC, PFEM_E_UNSUPPORTED,
+                     "heat and fractional-step cases are stepped with pfem_heat_step / pfem_frac_step");
         if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
         const size_t wb = c->ws.base ? 0 : pfem::workspace_bytes(c);
         PFEM_REQUIRE(wb <= S->arena.capacity(), PFEM_E_OOM, "the case's step workspace exceeds the server budget");
@@ -1497,6 +1650,118 @@ int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats) {
     });
 }
 
+// ---- fractional step ---------------------------------------------------------------------------
+int pfem_frac_step(pfem_case* c, double dt, pfem_step_stats* stats) {
+    return guarded([&] {
+        require_frac(c);
+        require_static_dirichlet(c);
+        PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        using clk = std::chrono::steady_clock;
+        auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+        const int D = c->dim, N = c->N;
+        const size_t vN = sizeof(double) * (size_t)D * N, pN = sizeof(double) * (size_t)N;
+        auto& F = c->frac;
+        const pfem::FracView V = frac_view(c, dt);
+        hipStream_t s = c->stream;
+        // saveNodesList (:478) and the first iterate's "previous" = the step's start state
+        PFEM_HIP(hipMemcpyAsync(F.pos_save.p, c->pos.p, vN, hipMemcpyDeviceToDevice, s));
+        PFEM_HIP(hipMemcpyAsync(F.v_it.p, c->q_prev.p, vN, hipMemcpyDeviceToDevice, s));
+        PFEM_HIP(hipMemcpyAsync(F.v_itp.p, c->q_prev.p, vN, hipMemcpyDeviceToDevice, s));
+        PFEM_HIP(hipMemcpyAsync(F.p_itp.p, V.p_prev, pN, hipMemcpyDeviceToDevice, s));
+        const pfem::CgView cg_mk = frac_cg(c, 2, V.mk, V.bv, V.dinv_mk, V.vt);
+        const pfem::CgView cg_l = frac_cg(c, 0, V.lv, V.bp, V.dinv_l, V.pn);
+        const pfem::CgView cg_m = frac_cg(c, 1, V.ms, V.bv, V.dinv_m, V.dv);
+        pfem_step_stats st{};
+        int status = PFEM_OK, it = 0;
+        double res = std::numeric_limits<double>::max();
+        std::vector<double> part((size_t)4 * V.nb);
+        auto restore = [&]() {
+            PFEM_HIP(hipMemcpyAsync(c->pos.p, F.pos_save.p, vN, hipMemcpyDeviceToDevice, s));
+        };
+        while (res > c->d.min_res) {
+            if (it > c->d.max_iter) {
+                restore();
+                status = PFEM_NOT_CONVERGED;
+                break;
+            }
+            auto t0 = clk::now();
+            pfem::launch_frac_assemble(V, s);
+            PFEM_HIP(hipGetLastError());
+            st.assemblies++;
+            auto t1 = clk::now();
+            st.t_assemble += secs(t0, t1);
+            int ci = 0;
+            double rel = 0;
+            int rc = frac_cg_solve(c, cg_mk, &ci, &rel);
+            st.krylov_iters += ci;
+            st.max_relres = std::max(st.max_relres, rel);
+            if (rc == PFEM_OK) {
+                auto t2 = clk::now();
+                st.t_solve += secs(t1, t2);
+                pfem::launch_frac_pcorr(V, s);
+                auto t3 = clk::now();
+                st.t_assemble += secs(t2, t3);
+                rc = frac_cg_solve(c, cg_l, &ci, &rel);
+                st.krylov_iters += ci;
+                st.max_relres = std::max(st.max_relres, rel);
+                t1 = clk::now();
+                st.t_solve += secs(t3, t1);
+            }
+            if (rc == PFEM_OK) {
+                pfem::launch_frac_vstep(V, s);
+                auto t2 = clk::now();
+                st.t_assemble += secs(t1, t2);
+                rc = frac_cg_solve(c, cg_m, &ci, &rel);
+                st.krylov_iters += ci;
+                st.max_relres = std::max(st.max_relres, rel);
+                st.t_solve += secs(t2, clk::now());
+            }
+            if (rc != PFEM_OK) {
+                restore();
+                status = rc;
+                break;
+            }
+            auto tu = clk::now();
+            pfem::launch_frac_update(V, s);
+            PFEM_HIP(hipGetLastError());
+            PFEM_HIP(hipMemcpyAsync(part.data(), V.res_part, sizeof(double) * part.size(), hipMemcpyDeviceToHost, s));
+            PFEM_HIP(hipStreamSynchronize(s));
+            auto tr = clk::now();
+            st.t_update += secs(tu, tr);
+            double acc[4] = {0, 0, 0, 0};
+            for (int k = 0; k < 4; ++k)
+                for (int b = 0; b < V.nb; ++b) acc[k] += part[(size_t)k * V.nb + b];
+            const double big = std::numeric_limits<double>::max();
+            const double resV = acc[1] == 0 ? big : std::sqrt(acc[0] / acc[1]);
+            double resP = 0;
+            if (c->d.residual == PFEM_RES_U_P) resP = acc[3] == 0 ? big : std::sqrt(acc[2] / acc[3]);
+            res = std::max(resV, resP);
+            if (std::isnan(acc[0]) || std::isnan(acc[1]) || std::isnan(acc[2]) || std::isnan(acc[3])) res = NAN;
+            st.t_residual += secs(tr, clk::now());
+            if (std::isnan(res)) {
+                restore();
+                status = PFEM_NAN;
+                it++;
+                break;
+            }
+            it++;
+        }
+        if (status == PFEM_OK) {
+            PFEM_HIP(hipMemcpyAsync(c->q_prev.p, F.v_it.p, vN, hipMemcpyDeviceToDevice, s));
+            PFEM_HIP(hipMemcpyAsync(c->q_prev.as<double>() + (size_t)D * N, F.pn.p, pN, hipMemcpyDeviceToDevice, s));
+        }
+        PFEM_HIP(hipStreamSynchronize(s));
+        if (stats) {
+            st.status = status;
+            st.picard_iters = it;
+            st.residual = res;
+            *stats = st;
+        }
+        return status;
+    });
+}
+
 // ---- host remesher ----------------------------------------------------------------------------
 struct pfem_mesher {
     pfem::mesh::Mesher m;
@@ -1626,7 +1891,8 @@ int pfem_mesher_stats(pfem_mesher* m, double* out8) {
 }
 
 static int mesher_states_per_node(pfem_case* c) {
-    return c->d.kind == PFEM_KIND_PSPG ? c->dim + 1 : (c->d.kind == PFEM_KIND_HEAT ? 1 : 2 * c->dim + 2);
+    if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) return c->dim + 1;
+    return c->d.kind == PFEM_KIND_HEAT ? 1 : 2 * c->dim + 2;
 }
 
 int pfem_case_set_mesh_from(pfem_case* c, pfem_mesher* m) {

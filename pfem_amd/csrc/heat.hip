@@ -17,6 +17,7 @@
 // record per element (gradients and measure) written by an element pass.
 #include "common.hpp"
 #include "heat.hpp"
+#include "p1geom.hpp"
 
 namespace pfem {
 
@@ -28,50 +29,18 @@ template <int D>
 __global__ __launch_bounds__(kBlock) void k_heat_elem(HeatView V) {
     const int e = blockIdx.x * kBlock + threadIdx.x;
     if (e >= V.E) return;
-    double x[D + 1][D];
+    int nd[D + 1];
 #pragma unroll
-    for (int a = 0; a <= D; ++a) {
-        const int n = V.conn[(size_t)a * V.E + e];
-#pragma unroll
-        for (int k = 0; k < D; ++k) x[a][k] = V.pos[(size_t)k * V.N + n];
-    }
-    double J[D][D];  // J[r][c] = x_{c+1,r} - x_{0,r}
-#pragma unroll
-    for (int r = 0; r < D; ++r)
-#pragma unroll
-        for (int c = 0; c < D; ++c) J[r][c] = x[c + 1][r] - x[0][r];
-    double inv[D][D], det;
-    if constexpr (D == 2) {
-        det = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-        inv[0][0] = J[1][1] / det;
-        inv[0][1] = -J[0][1] / det;
-        inv[1][0] = -J[1][0] / det;
-        inv[1][1] = J[0][0] / det;
-    } else {
-        det = J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-              J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-        inv[0][0] = (J[1][1] * J[2][2] - J[1][2] * J[2][1]) / det;
-        inv[0][1] = (J[2][1] * J[0][2] - J[2][2] * J[0][1]) / det;
-        inv[0][2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) / det;
-        inv[1][0] = (J[2][0] * J[1][2] - J[1][0] * J[2][2]) / det;
-        inv[1][1] = (J[0][0] * J[2][2] - J[2][0] * J[0][2]) / det;
-        inv[1][2] = (J[1][0] * J[0][2] - J[0][0] * J[1][2]) / det;
-        inv[2][0] = (J[1][0] * J[2][1] - J[2][0] * J[1][1]) / det;
-        inv[2][1] = (J[2][0] * J[0][1] - J[0][0] * J[2][1]) / det;
-        inv[2][2] = (J[0][0] * J[1][1] - J[1][0] * J[0][1]) / det;
-    }
+    for (int a = 0; a <= D; ++a) nd[a] = V.conn[(size_t)a * V.E + e];
+    double x[D][D + 1];
+    WGeom<D> G;
+    wc_geometry<D>(V.pos, V.N, nd, x, G);
     double* rec = V.erec + (size_t)kHeatRec * e;
-    // dN_0/dx_k = -sum_r invJ[r][k], dN_{i+1}/dx_k = invJ[i][k] (MatricesBuilder.inl:93-128)
 #pragma unroll
-    for (int k = 0; k < D; ++k) {
-        double g0 = 0;
+    for (int k = 0; k < D; ++k)
 #pragma unroll
-        for (int r = 0; r < D; ++r) g0 -= inv[r][k];
-        rec[k * (D + 1)] = g0;
-#pragma unroll
-        for (int i = 0; i < D; ++i) rec[k * (D + 1) + i + 1] = inv[i][k];
-    }
-    rec[15] = det * (D == 2 ? 0.5 : 1.0 / 6.0);
+        for (int a = 0; a <= D; ++a) rec[k * (D + 1) + a] = G.gN[k][a];
+    rec[15] = G.vol;
 }
 
 // one thread per node: its row of A = M + dt L (or the unit row) and, for a full row, its

@@ -244,6 +244,13 @@ struct pfem_case {
         int F = 0;
         bool has_q = false;
     } heat;
+    // fractional-step cases: element records, the three graph matrices, right-hand sides, the
+    // Picard iterate and the CG vectors
+    struct FracBufs {
+        pfem::DevBuf erec, mk, ms, lv, dinv_mk, dinv_m, dinv_l, bv, bp, vt, pn, dv, v_it, v_itp, p_itp, pos_save,
+            res_part;
+        pfem::DevBuf r, z, p, q, part, scal;
+    } frac;
     std::atomic<int> busy{0};  // a serving request holds the case
     std::unique_ptr<pfem::Engine> engine;  // direct calls
     ~pfem_case();

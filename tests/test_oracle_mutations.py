@@ -44,10 +44,42 @@ MUTATIONS = {
     "bc_free_rhs": ("S.b[nn + d * N] = qPrev[nn + d * N] + P.dt * P.g[d];", "S.b[nn + d * N] = qPrev[nn + d * N];"),
 }
 
+HEAT_PINS = ["tests/test_oracle_heat_pins.py"]
+FRAC_PINS = ["tests/test_oracle_frac_pins.py"]
+# (old, new, pin files) of the heat and fractional-step restatements
+MUTATIONS_PINNED = {
+    # heat: L without the time step (HeatEquation.inl:245-250)
+    "heat_dt_L": ("for (int j = 0; j < ne; ++j) Le[i][j] = P.dt * Le[i][j];",
+                  "for (int j = 0; j < ne; ++j) Le[i][j] = Le[i][j];", HEAT_PINS),
+    # heat: convection added instead of subtracted (HeatEquation.inl:331-341)
+    "heat_conv_sign": ("for (int i = 0; i < D; ++i) S.b[fn[i]] -= P.dt * sgh[i];",
+                       "for (int i = 0; i < D; ++i) S.b[fn[i]] += P.dt * sgh[i];", HEAT_PINS),
+    # heat: radiation with T^3 (HeatEquation.inl:95-99)
+    "heat_rad_power": ("(Tg * Tg * Tg * Tg - P.Tinf * P.Tinf * P.Tinf * P.Tinf)",
+                       "(Tg * Tg * Tg - P.Tinf * P.Tinf * P.Tinf)", HEAT_PINS),
+    # FracStep: M instead of M/dt in the prediction matrix and right-hand side (FracStep.inl:43)
+    "frac_m_dt": ("Ms_dt[a][b] = idt * Ms[a][b];", "Ms_dt[a][b] = Ms[a][b];", FRAC_PINS),
+    # FracStep: sign of gammaFS D^T p_prev (FracStep.inl:61)
+    "frac_dtp_sign": ("DTp[r] = gammaFS * t;", "DTp[r] = -gammaFS * t;", FRAC_PINS),
+    # FracStep: rho dt instead of rho / dt in the pressure right-hand side (FracStep.inl:292)
+    "frac_pcorr_scale": ("-((P.rho / P.dt) * Dv)", "-((P.rho * P.dt) * Dv)", FRAC_PINS),
+    # FracStep: gammaFS dropped from L p_prev (FracStep.inl:293)
+    "frac_pcorr_gamma": ("{nodes[i], gammaFS * Lp}", "{nodes[i], Lp}", FRAC_PINS),
+    # FracStep: dp = p - p_prev whatever gammaFS (FracStep.inl:533)
+    "frac_dp_gamma": ("dP[nn] = pNew[nn] - gammaFS * pPrev[nn];", "dP[nn] = pNew[nn] - pPrev[nn];", FRAC_PINS),
+    # FracStep: sign of the velocity correction (FracStep.inl:375-380)
+    "frac_vstep_sign": ("{nodes[i] + d * N, P.dt * t}", "{nodes[i] + d * N, -P.dt * t}", FRAC_PINS),
+    # FracStep: K coupling block (d, d2) read as (d, d) (FracStep.inl:108-112)
+    "frac_k_block": ("Ke[i + d * ne][j + d2 * ne]};", "Ke[i + d * ne][j + d * ne]};", FRAC_PINS),
+    # FracStep: free nodes keep v_prev without dt g (FracStep.inl:244-252)
+    "frac_free_rhs": ("S.b[nn + d * N] = vPrevQ[nn + d * N] + dt * P.g[d];", "S.b[nn + d * N] = vPrevQ[nn + d * N];",
+                      FRAC_PINS),
+}
 
-@pytest.mark.parametrize("name", sorted(MUTATIONS))
+
+@pytest.mark.parametrize("name", sorted(MUTATIONS) + sorted(MUTATIONS_PINNED))
 def test_planted_mistake_fails_a_pin(name, tmp_path):
-    old, new = MUTATIONS[name]
+    old, new, pins = MUTATIONS[name] + (PINS,) if name in MUTATIONS else MUTATIONS_PINNED[name]
     src = open(SRC).read()
     assert src.count(old) == 1, f"mutation site not found once: {old}"
     mut = tmp_path / "pfem_oracle.cpp"
@@ -56,6 +88,6 @@ def test_planted_mistake_fails_a_pin(name, tmp_path):
     subprocess.run(["g++", "-O1", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-o", str(lib), str(mut)], check=True,
                    capture_output=True)
     env = dict(os.environ, PFEM_ORACLE_LIB=str(lib), OMP_NUM_THREADS="2")
-    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", *PINS], cwd=ROOT, env=env,
+    r = subprocess.run([sys.executable, "-m", "pytest", "-x", "-q", "-p", "no:cacheprovider", *pins], cwd=ROOT, env=env,
                        capture_output=True, text=True, timeout=600)
     assert r.returncode == 1, f"no pin caught the mutation {name}:\n{r.stdout[-2000:]}"
