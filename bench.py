@@ -723,6 +723,141 @@ def run_remesh_cycle(case, mesh, q0, dt):
 
 
 # ------------------------------------------------------------------------------------------
+# ------------------------------------------------------------------------------------------
+# the other solvers of the path: heat (Conduction), fractional step, Boussinesq
+# ------------------------------------------------------------------------------------------
+def run_secondary(ctx, steps=10, with_cpu=True, threads=1):
+    """Steps of the heat equation (3D cavity, 40^3 cells), the fractional-step solver (3D
+    lid-driven cavity, 24^3 cells) and the Boussinesq problem (2D heated cavity, 100^2 cells) on
+    the GPU, each with the oracle's time for one step of the same case on the host."""
+    import pfem_amd
+    from pfem_amd import meshgen as mg
+
+    out = {}
+
+    def timed(fn, n):
+        ms = []
+        for _ in range(n):
+            t0 = time.perf_counter()
+            fn()
+            ms.append((time.perf_counter() - t0) * 1e3)
+        return ms
+
+    # heat: bottom 360 K / top 300 K, insulated sides, steel-like material, dt 0.5 s
+    m = mg.cavity(3, 40)
+    N = m.n_nodes
+    fl = np.where(m.meta["top"] | m.meta["bottom"], mg.DIRICHLET | mg.FIXED, mg.FIXED).astype(np.uint8)
+    hm = mg.PfemMesh(3, m.pos, m.conn, fl, m.h)
+    Tbc = np.where(m.meta["bottom"], 360.0, np.where(m.meta["top"], 300.0, 0.0))
+    T0 = np.where(m.meta["bottom"], 360.0, np.where(m.meta["top"], 300.0, 330.0))
+    mat = dict(rho=8000.0, cv=500.0, k=50.0)
+    dt = 0.5
+    hc = pfem_amd.HeatEqIncompNewton(ctx, 3, krylov_rtol=1e-10, **mat)
+    hc.set_mesh(hm)
+    hc.set_state(T0)
+    hc.set_dirichlet_T(Tbc)
+    hc.solve(dt)
+    iters = []
+
+    def heat_step():
+        assert hc.solve(dt)
+        iters.append(hc.last_stats["krylov_iters"])
+    ms = timed(heat_step, steps)
+    res = dict(workload=f"3D heat conduction, cavity 40^3 cells ({N} nodes, {m.n_elems} tets), Dirichlet bottom / "
+                        f"top, rho 8000 cv 500 k 50, dt {dt} s, CG to relres 1e-10",
+               steps_per_s=1e3 * len(ms) / sum(ms), **_stats(ms), cg_iters_per_step=float(np.mean(iters)))
+    if with_cpu:
+        from oracle import oracle as O
+        T = T0.copy()
+        t0 = time.perf_counter()
+        O.heat_step(hm, T, Tbc=Tbc, hc=O.HeatCase(**mat), dt=dt, rtol=1e-10)
+        res["oracle_ms_per_step"] = (time.perf_counter() - t0) * 1e3
+        res["oracle_threads"] = threads
+    hc.close()
+    out["heat3d"] = res
+    log(f"[bench] heat 3D: {res}")
+
+    # fractional step: lid (1, 0, 0) on the top wall, gammaFS 1, residual U
+    m = mg.cavity(3, 24)
+    N = m.n_nodes
+    vbc = np.zeros((3, N))
+    vbc[0, m.meta["lid"]] = 1.0
+    q0 = np.zeros(4 * N)
+    q0[:3 * N] = vbc.reshape(-1)
+    kw = dict(rho=1000.0, mu=1e-2, body_force=[0.0, 0.0, -9.81], gammaFS=1.0, residual="U", min_res=1e-6,
+              max_iter=10)
+    dt = 1e-3
+    fc = pfem_amd.MomContEqFracStep(ctx, 3, krylov_rtol=1e-10, **kw)
+    fc.set_mesh(m)
+    fc.set_state(q0)
+    fc.set_dirichlet(vbc)
+    stats = []
+
+    def frac_step():
+        rc, st = fc.step(dt)
+        assert rc == 0, st
+        stats.append(st)
+    frac_step()
+    stats.clear()
+    ms = timed(frac_step, steps)
+    res = dict(workload=f"3D lid-driven cavity, fractional step, 24^3 cells ({N} nodes, {m.n_elems} tets), "
+                        f"rho 1000 mu 1e-2, dt {dt}, Picard U 1e-6, CG to relres 1e-10",
+               steps_per_s=1e3 * len(ms) / sum(ms), **_stats(ms),
+               picard_iters_per_step=float(np.mean([x["picard_iters"] for x in stats])),
+               cg_iters_per_step=float(np.mean([x["krylov_iters"] for x in stats])))
+    if with_cpu:
+        from oracle import oracle as O
+        q, pos = q0.copy(), m.pos.copy()
+        t0 = time.perf_counter()
+        O.frac_step(m, pos, q, dir_vals=vbc.reshape(-1).copy(), rho=1000.0, mu=1e-2, g=[0.0, 0.0, -9.81], dt=dt,
+                    gamma_fs=1.0, residual="U", min_res=1e-6, max_iter=10, rtol=1e-10)
+        res["oracle_ms_per_step"] = (time.perf_counter() - t0) * 1e3
+        res["oracle_threads"] = threads
+    fc.close()
+    out["frac3d"] = res
+    log(f"[bench] FracStep 3D: {res}")
+
+    # Boussinesq: heated bottom (360 K), cold top (300 K), walls at rest
+    m = mg.cavity(2, 100)
+    N = m.n_nodes
+    fl = m.flags.copy()
+    fl[m.meta["top"] | m.meta["bottom"]] |= 32  # PFEM_NODE_DIRICHLET_T
+    bm = mg.PfemMesh(2, m.pos, m.conn, fl, m.h)
+    Tbc = np.where(m.meta["bottom"], 360.0, np.where(m.meta["top"], 300.0, 0.0))
+    rng = np.random.default_rng(3)
+    T0 = np.where(m.meta["bottom"], 360.0, np.where(m.meta["top"], 300.0, 330.0 + rng.normal(scale=2.0, size=N)))
+    g = [0.0, -9.81]
+    q0 = np.concatenate([np.zeros(2 * N), 1000.0 * (m.pos.T @ np.array(g)), T0])
+    bmat = dict(rho=1000.0, mu=1e-2, cv=4000.0, k=50.0, alpha=2e-3, Tr=330.0)
+    dt = 5e-3
+    bc = pfem_amd.BoussinesqIncompNewton(ctx, 2, body_force=g, residual="U", min_res=1e-6, max_iter=10,
+                                         krylov_rtol=1e-10, **bmat)
+    bc.set_mesh(bm)
+    bc.set_state(q0)
+    bc.set_dirichlet(np.zeros((2, N)))
+    bc.set_dirichlet_T(Tbc)
+
+    def bq_step():
+        assert bc.solve(dt), bc.failed
+    bq_step()
+    ms = timed(bq_step, steps)
+    res = dict(workload=f"2D heated cavity, Boussinesq (PSPG + heat, heat first), 100^2 cells ({N} nodes, "
+                        f"{m.n_elems} triangles), alpha 2e-3, dt {dt}, Picard U 1e-6, solves to relres 1e-10",
+               steps_per_s=1e3 * len(ms) / sum(ms), **_stats(ms))
+    if with_cpu:
+        from oracle import oracle as O
+        q, pos = q0.copy(), m.pos.copy()
+        t0 = time.perf_counter()
+        O.boussinesq_step(bm, pos, q, O.HeatCase(rho=1000.0, cv=4000.0, k=50.0), 2e-3, 330.0, Tbc=Tbc, rho=1000.0,
+                          mu=1e-2, g=g, dt=dt, residual="U", min_res=1e-6, max_iter=10, rtol=1e-10)
+        res["oracle_ms_per_step"] = (time.perf_counter() - t0) * 1e3
+        res["oracle_threads"] = threads
+    bc.close()
+    out["boussinesq2d"] = res
+    log(f"[bench] Boussinesq 2D: {res}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -741,7 +876,10 @@ def main():
                     help="PSPG solve precision of the batched path")
     ap.add_argument("--wc-steps", type=int, default=200, help="steps of the 3D WC sloshing tank")
     ap.add_argument("--precision-steps", type=int, default=3, help="steps of the 3D mixed-precision column field")
-    ap.add_argument("--only", choices=["batched", "plain", "dambreak", "dambreak2d", "wc3d", "precision"],
+    ap.add_argument("--secondary-steps", type=int, default=10,
+                    help="steps of the heat, fractional-step and Boussinesq cases")
+    ap.add_argument("--only", choices=["batched", "plain", "dambreak", "dambreak2d", "wc3d", "precision",
+                                       "secondary"],
                     default=None,
                     help="run one workload once and print its result only (for profilers)")
     args = ap.parse_args()
@@ -767,6 +905,10 @@ def main():
         return
     if args.only == "precision":
         print(json.dumps({"precision3d": run_precision3d(ctx, args.precision_steps)}), flush=True)
+        return
+    if args.only == "secondary":
+        print(json.dumps({"secondary": run_secondary(ctx, args.secondary_steps, with_cpu=not args.no_cpu_baseline,
+                                                     threads=args.cpu_threads or cpu_threads())}), flush=True)
         return
     n_req = max(1, args.stream_cases)
     stream = build_stream(n_req)
@@ -799,8 +941,13 @@ def main():
     if args.only:
         cases = batched if args.only == "batched" else plain
         reset_all(cases)
+        # the XCD probe's one-workgroup-per-slot kernel brackets the timed window in a kernel
+        # trace, so a profile can be cut to the dispatches of the timed run (the cases' mesh
+        # set-up comes before it)
+        ctx.xcd_map(8)
         res = (run_batched(ctx, stream, cases, False, budget, args.slots) if args.only == "batched" else
                run_plain(stream, cases, False))
+        ctx.xcd_map(8)
         print(json.dumps({"path": args.only, "requests": n_req, **res}), flush=True)
         return
 
@@ -846,6 +993,8 @@ def main():
     log(f"[bench] WC sloshing 3D: {wc3d}")
     prec3d = run_precision3d(ctx, args.precision_steps) if args.precision_steps > 0 else None
     log(f"[bench] precision 3D: {prec3d}")
+    secondary = (run_secondary(ctx, args.secondary_steps, with_cpu=not args.no_cpu_baseline,
+                               threads=args.cpu_threads or cpu_threads()) if args.secondary_steps > 0 else None)
     n2 = sum(r["dim"] == 2 for r in stream)
     npspg = sum(r["kind"] == "PSPG" for r in stream)
     bs, ps = timed, plain_sat
@@ -900,6 +1049,7 @@ def main():
         "dambreak3d_fp64": db,
         "wc3d": wc3d,
         "precision3d": prec3d,
+        "secondary_solvers": secondary,
         "roofline": roof,
         "xcd_round_robin": xcd_round_robin,
     }

@@ -44,13 +44,17 @@ extern "C" {
 #define PFEM_NODE_FREE 2 /* recomputed by the library: node in no element (Node.inl isFree) */
 #define PFEM_NODE_ONFS 4
 #define PFEM_NODE_FIXED 8     /* IC.BoundaryFixed: position never updated */
-#define PFEM_NODE_DIRICHLET 16 /* boundary node whose tag has a "<Tag>V" BC function */
+#define PFEM_NODE_DIRICHLET 16 /* boundary node whose tag has a "<Tag>V" BC function (a Conduction case:
+                                  "<Tag>T") */
+#define PFEM_NODE_DIRICHLET_T 32 /* Boussinesq cases: node whose tag has a "<Tag>T" BC function */
 
 /* ---- enums ------------------------------------------------------------------------- */
 #define PFEM_KIND_PSPG 0 /* MomContEqIncompNewton with Solver.id = "PSPG" */
 #define PFEM_KIND_WC 1   /* WCompNewtonNoT with Solver.id = "CDS_dpdt" */
 #define PFEM_KIND_HEAT 2 /* "Conduction": HeatEqIncompNewton alone, no phase change */
 #define PFEM_KIND_FRACSTEP 3 /* MomContEqIncompNewton with Solver.id = "FracStep" */
+#define PFEM_KIND_BOUSSINESQ 4 /* "Boussinesq": the PSPG MomContEqIncompNewton with the temperature-dependent
+                                  body force, and HeatEqIncompNewton (no phase change), on one mesh */
 
 #define PFEM_RES_U 0    /* MomContEquation.inl:229-236 residual = "U" */
 #define PFEM_RES_U_P 1  /* "U_P" */
@@ -110,6 +114,11 @@ typedef struct pfem_case_desc {
     double eps_rad;         /* epsRad: emissivity of the "<Tag>Qr" facets */
     /* fractional step (MomContEquation.inl:151-161) */
     double gamma_fs;        /* gammaFS in [0, 1]: weight of the previous pressure */
+    /* Boussinesq (Material table, MomContEquation.inl:164-197): F factor rho (1 - alpha (T - Tr)),
+       PSPG H factor 1 - alpha (T - Tr), T = N T_e at each Gauss point */
+    double alpha_exp;       /* alpha: thermal expansion coefficient */
+    double t_ref;           /* Tr: reference temperature */
+    double dgamma_dt;       /* DgammaDT: must be 0 (the surface-tension facet term is not implemented) */
 } pfem_case_desc;
 
 #define PFEM_PROBLEM_NEWTONIAN 0 /* "IncompNewtonNoT": K factor mu */

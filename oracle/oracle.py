@@ -73,6 +73,7 @@ def lib():
         L.orc_picard_residual.restype = C.c_double
         L.orc_picard_residual.argtypes = [C.c_int, C.c_int64, _u8p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
         L.orc_set_material.argtypes = [C.c_int, C.c_double, C.c_double]
+        L.orc_set_boussinesq.argtypes = [C.c_double, C.c_double, _dp, C.c_int64]
         L.orc_wc_parts.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, _dp,
                                    C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         L.orc_heat_assemble.restype = C.c_void_p
@@ -321,17 +322,22 @@ def wc_parts(mesh, pos, v, p, rho, dt=1e-4, mu=1e-3, g=None, K0=2.2e9, K0p=7.0, 
 
 class material:
     """Context manager selecting the momentum equation's material law for the PSPG calls inside
-    it (MomContEquation.inl:33-66, 100-123): "Newtonian" or "Bingham" (tau0, mReg)."""
+    it (MomContEquation.inl:33-66, 100-123, 164-197): "Newtonian", "Bingham" (tau0, mReg) or
+    "Boussinesq" (alpha, Tr and the node temperatures T of the F and H factors)."""
 
-    def __init__(self, problem="Newtonian", tau0=0.0, mReg=0.0):
-        self.args = ({"Newtonian": 0, "Bingham": 1}[problem], float(tau0), float(mReg))
+    def __init__(self, problem="Newtonian", tau0=0.0, mReg=0.0, alpha=0.0, Tr=0.0, T=None):
+        self.args = ({"Newtonian": 0, "Bingham": 1, "Boussinesq": 2}[problem], float(tau0), float(mReg))
+        self.bq = (float(alpha), float(Tr), None if T is None else np.ascontiguousarray(T, dtype=np.float64))
 
     def __enter__(self):
         lib().orc_set_material(*self.args)
+        a, tr, T = self.bq
+        lib().orc_set_boussinesq(a, tr, _d(T) if T is not None else C.cast(None, _dp), 0 if T is None else T.size)
         return self
 
     def __exit__(self, *exc):
         lib().orc_set_material(0, 0.0, 0.0)
+        lib().orc_set_boussinesq(0.0, 0.0, C.cast(None, _dp), 0)
         return False
 
 
@@ -440,4 +446,41 @@ def frac_step(mesh, pos, q, dir_vals=None, rho=1000.0, mu=1e-3, g=None, dt=1e-3,
                          gamma_fs, RES_KIND[residual], min_res, max_iter, rtol, cg_maxit, _d(stats))
     keys = ["picard_iters", "cg_iters_vapp", "cg_iters_pcorr", "cg_iters_v", "final_res"]
     return st, dict(zip(keys, stats.tolist()))
+
+
+DIRICHLET_T = 32
+
+
+def boussinesq_step(mesh, pos, q, hc: "HeatCase", alpha, Tr, dir_vals=None, Tbc=None, rho=1000.0, mu=1e-3, g=None,
+                    dt=1e-3, solve_heat_first=True, residual="Ax_f", min_res=1e-6, max_iter=10, rtol=1e-10,
+                    krylov_maxit=20000, heat_rtol=1e-13):
+    """SolverIncompNewton::m_solveBoussinesq without the remesh (IncompNewton/Solver.cpp:249-283):
+    the heat equation (on the positions of its turn) and the PSPG step whose F and H factors read
+    the node temperatures.  q = (v, p, T), (dim + 2) N, and pos updated in place on success.  The
+    heat equation's Dirichlet nodes are those flagged DIRICHLET_T.  Returns (status, which) with
+    which "heat" or "momentum" naming the equation that failed."""
+    d = mesh.dim
+    N = mesh.n_nodes
+    nm = (d + 1) * N
+    qm = np.ascontiguousarray(q[:nm])
+    T = np.ascontiguousarray(q[nm:])
+    hflags = np.where(mesh.flags & DIRICHLET_T, mesh.flags | 16, mesh.flags & np.uint8(0xFF ^ 16)).astype(np.uint8)
+
+    def heat():
+        hm = type(mesh)(d, pos.copy(), mesh.conn, hflags, getattr(mesh, "h", 1.0))
+        st, _, _ = heat_step(hm, T, Tbc=Tbc, hc=hc, dt=dt, rtol=heat_rtol)
+        return st
+
+    if solve_heat_first and heat() != 0:
+        return 1, "heat"
+    with material("Boussinesq", alpha=alpha, Tr=Tr, T=T):
+        st, _ = pspg_step(mesh, pos, qm, dir_vals=dir_vals, rho=rho, mu=mu, g=g, dt=dt, residual=residual,
+                          min_res=min_res, max_iter=max_iter, rtol=rtol, krylov_maxit=krylov_maxit)
+    if st != 0:
+        return st, "momentum"
+    if not solve_heat_first and heat() != 0:
+        return 1, "heat"
+    q[:nm] = qm
+    q[nm:] = T
+    return 0, ""
 

@@ -264,14 +264,16 @@ template <int D> struct MB {
         for (int c = 0; c < nd; ++c) Fout[c] *= s;
     }
     // getH: MatricesBuilder.inl:373-386; H = detJ*refSize * sum_g f_g gradN^T vec w_g.
-    void H(double detJ, const double G[D][ne], const double* vec, double Hout[ne]) const {
+    template <class F> void H(double detJ, F fac, const double G[D][ne], const double* vec, double Hout[ne]) const {
         for (int i = 0; i < ne; ++i) Hout[i] = 0;
-        for (int g = 0; g < Quad<D>::ng; ++g)
+        for (int g = 0; g < Quad<D>::ng; ++g) {
+            const double f = fac(q.N[g]);
             for (int i = 0; i < ne; ++i) {
                 double t = 0;
                 for (int k = 0; k < D; ++k) t += G[k][i] * vec[k];
-                Hout[i] += 1.0 * t * q.w[g];
+                Hout[i] += f * t * q.w[g];
             }
+        }
         const double s = detJ * Quad<D>::refSize;
         for (int i = 0; i < ne; ++i) Hout[i] *= s;
     }
@@ -307,11 +309,17 @@ struct Triplet {
 // Newtonian fluid (K factor mu), 1 "Bingham" (K factor mu + mu_eq of the element's strain
 // rate, Papanastasiou-regularised with tau0 and mReg).  Set by orc_set_material; read by the
 // PSPG entry points.
+// Problem 2 "Boussinesq" (MomContEquation.inl:164-197): Newtonian K, the F factor
+// rho (1 - alpha (T - Tr)) and the PSPG H factor 1 - alpha (T - Tr) with T = N T_e at each
+// Gauss point, T the node temperatures set by orc_set_boussinesq.
 struct Material {
     int problem = 0;
     double tau0 = 0, mReg = 0;
+    double alpha = 0, Tr = 0;
+    const double* T = nullptr;
 };
 static Material g_mat;
+static std::vector<double> g_T;
 
 struct Params {
     int dim;
@@ -405,10 +413,18 @@ static void pspgElement(const Params& P, const MB<D>& mb, const double* pos, con
     mb.L(geo.detJ, 1 / P.rho, G, Le);
     for (int i = 0; i < ne; ++i)
         for (int j = 0; j < ne; ++j) Le[i][j] = tau * Le[i][j];
+    // T at a Gauss point (Boussinesq): N T_e
+    auto Tg = [&](const double* Ng) {
+        double t = 0;
+        for (int a = 0; a < ne; ++a) t += Ng[a] * P.mat.T[nodes[a]];
+        return t;
+    };
+    const bool bq = P.mat.problem == 2;
     double Fe[nd];
-    mb.Fv(geo.detJ, [&](const double*) { return P.rho; }, P.g, Fe);
+    mb.Fv(geo.detJ, [&](const double* Ng) { return bq ? P.rho * (1 - P.mat.alpha * (Tg(Ng) - P.mat.Tr)) : P.rho; },
+          P.g, Fe);
     double He[ne];
-    mb.H(geo.detJ, G, P.g, He);
+    mb.H(geo.detJ, [&](const double* Ng) { return bq ? 1 - P.mat.alpha * (Tg(Ng) - P.mat.Tr) : 1.0; }, G, P.g, He);
     for (int i = 0; i < ne; ++i) He[i] = tau * He[i];
 
     constexpr int na = (D + 1) * ne;
@@ -2211,6 +2227,20 @@ void orc_set_material(int problem, double tau0, double m_reg) {
     g_mat.problem = problem;
     g_mat.tau0 = tau0;
     g_mat.mReg = m_reg;
+}
+
+// Boussinesq factors: thermal expansion alpha, reference temperature Tr, node temperatures
+// (N values, copied; NULL clears)
+void orc_set_boussinesq(double alpha, double Tr, const double* T, int64_t N) {
+    g_mat.alpha = alpha;
+    g_mat.Tr = Tr;
+    if (T) {
+        g_T.assign(T, T + N);
+        g_mat.T = g_T.data();
+    } else {
+        g_T.clear();
+        g_mat.T = nullptr;
+    }
 }
 
 }  // extern "C"

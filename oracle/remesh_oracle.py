@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 from scipy.spatial import Delaunay
 
-BOUND, FREE, ONFS, FIXED, DIRICHLET = 1, 2, 4, 8, 16
+BOUND, FREE, ONFS, FIXED, DIRICHLET, DIRICHLET_T = 1, 2, 4, 8, 16, 32
 
 # faces opposite each vertex, oriented to see that vertex on their positive side
 FACE = {2: ((1, 2), (2, 0), (0, 1)), 3: ((1, 3, 2), (0, 2, 3), (0, 3, 1), (0, 1, 2))}
@@ -64,7 +64,7 @@ class MeshState:
     dim: int
     pos: np.ndarray            # N x dim
     states: np.ndarray         # N x nstates
-    flags: np.ndarray          # N uint8 (BOUND, FIXED, DIRICHLET kept; FREE, ONFS computed)
+    flags: np.ndarray          # N uint8 (BOUND, FIXED, DIRICHLET, DIRICHLET_T kept; FREE, ONFS computed)
     tags: np.ndarray           # N int32
     elems: np.ndarray = None   # E x (dim+1)
     facets: np.ndarray = None  # F x dim
@@ -361,6 +361,6 @@ def from_cloud(pos_fb, flags, states_fb=None, tags=None) -> MeshState:
     pos = np.ascontiguousarray(np.asarray(pos_fb, dtype=np.float64).T)
     n = len(pos)
     st = np.zeros((n, 0)) if states_fb is None else np.ascontiguousarray(np.asarray(states_fb).reshape(-1, n).T)
-    fl = np.asarray(flags, dtype=np.uint8) & np.uint8(BOUND | FIXED | DIRICHLET)
+    fl = np.asarray(flags, dtype=np.uint8) & np.uint8(BOUND | FIXED | DIRICHLET | DIRICHLET_T)
     tg = -np.ones(n, dtype=np.int32) if tags is None else np.asarray(tags, dtype=np.int32).copy()
     return MeshState(pos.shape[1], pos, st, fl, tg)

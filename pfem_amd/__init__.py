@@ -23,7 +23,8 @@ from ._lib import PfemError, check, lib
 from .meshgen import BOUND, DIRICHLET, FIXED, FREE, ONFS, PfemMesh
 
 __all__ = ["Context", "Case", "MomContEqIncompNewton", "MomContEqFracStep", "PicardAlgo", "WCompNewtonStep",
-           "HeatEqIncompNewton", "SolverIncompNewton", "Mesher", "batch_step", "Server", "now_ns",
+           "HeatEqIncompNewton", "BoussinesqIncompNewton", "SolverIncompNewton", "Mesher", "batch_step", "Server",
+           "now_ns",
            "PfemError", "PfemMesh", "BOUND", "FREE", "ONFS", "FIXED", "DIRICHLET"]
 
 
@@ -73,16 +74,18 @@ class Case:
                  K0=2.2e9, K0p=7.0, rho_star=1000.0, stabilization="Meduri", wc_state="fp64", residual="Ax_f",
                  max_iter=10, min_res=1e-6, precision="fp64", krylov_rtol=1e-9, krylov_max_iter=20000,
                  inner_rtol=1e-3, inner_max_iter=300, precond="auto", problem="Newtonian", tau0=0.0, mReg=0.0,
-                 cv=0.0, k=0.0, h=0.0, Tinf=0.0, epsRad=0.0, gammaFS=0.0):
+                 cv=0.0, k=0.0, h=0.0, Tinf=0.0, epsRad=0.0, gammaFS=0.0, alpha=0.0, Tr=0.0, DgammaDT=0.0):
         d = _lib.CaseDesc()
         d.dim = dim
-        kinds = {"PSPG": _lib.KIND_PSPG, "WC": _lib.KIND_WC, "HEAT": _lib.KIND_HEAT, "FracStep": _lib.KIND_FRACSTEP}
+        kinds = {"PSPG": _lib.KIND_PSPG, "WC": _lib.KIND_WC, "HEAT": _lib.KIND_HEAT, "FracStep": _lib.KIND_FRACSTEP,
+                 "Boussinesq": _lib.KIND_BOUSSINESQ}
         if kind not in kinds:
             raise PfemError(_lib.PFEM_E_INVALID, "unknown case kind: " + str(kind))
         d.kind = kinds[kind]
         d.cv, d.k_cond, d.h_conv, d.t_inf, d.eps_rad = cv, k, h, Tinf, epsRad
         d.rho, d.mu, d.gamma = rho, mu, gamma
         d.gamma_fs = gammaFS
+        d.alpha_exp, d.t_ref, d.dgamma_dt = alpha, Tr, DgammaDT
         bf = body_force if body_force is not None else ([0.0, -9.81] if dim == 2 else [0.0, 0.0, -9.81])
         for k in range(3):
             d.body_force[k] = bf[k] if k < len(bf) else 0.0
@@ -160,6 +163,8 @@ class Case:
     def n_state(self):
         if self.kind == "HEAT":
             return self.N
+        if self.kind == "Boussinesq":
+            return (self.dim + 2) * self.N
         return (self.dim + 1) * self.N if self.kind in ("PSPG", "FracStep") else (2 * self.dim + 2) * self.N
 
     def set_state(self, q):
@@ -439,20 +444,8 @@ class PicardAlgo:
         return True
 
 
-class HeatEqIncompNewton(Case):
-    """HeatEqIncompNewton without phase change (IncompNewton/HeatEquation.inl) as the Conduction
-    problem runs it: one node state T, fixed nodes.  Material keywords follow the Lua Material
-    table (k, rho, cv, h, Tinf, epsRad); the facets and their BC kinds come from the mesher
-    (set_facets), the Dirichlet temperatures and the flux at the facet Gauss points from the
-    caller (set_dirichlet_T, set_flux).  ``solve(dt)`` returns True on success, False on a
-    solver failure, like Equation::solve."""
-
-    def __init__(self, ctx, dim, rho=1000.0, cv=4180.0, k=0.6, h=0.0, Tinf=0.0, epsRad=0.0, krylov_rtol=1e-12,
-                 krylov_max_iter=100000, **kw):
-        super().__init__(ctx, dim, kind="HEAT", rho=rho, cv=cv, k=k, h=h, Tinf=Tinf, epsRad=epsRad,
-                         krylov_rtol=krylov_rtol, krylov_max_iter=krylov_max_iter, **kw)
-        self.last_stats = None
-        self.F = 0
+class _HeatBC:
+    """The heat equation's boundary data and solve on a case that holds it (HEAT, Boussinesq)."""
 
     def set_facets(self, nodes, out_node, kinds):
         nodes = np.ascontiguousarray(nodes, dtype=np.int32)
@@ -483,11 +476,11 @@ class HeatEqIncompNewton(Case):
             raise PfemError(_lib.PFEM_E_INVALID, "Dirichlet temperatures must hold N values")
         check(lib().pfem_heat_set_dirichlet(self._h, _dp(t)))
 
-    def assemble(self, dt):  # noqa: D401
+    def heat_assemble(self, dt):
         check(lib().pfem_heat_assemble(self._h, float(dt)))
 
     def system(self):
-        """(row_ptr, col_idx, vals, b) of the last assembled system."""
+        """(row_ptr, col_idx, vals, b) of the last assembled heat system."""
         nnz = C.c_int64()
         check(lib().pfem_heat_system_nnz(self._h, C.byref(nnz)))
         rp = np.empty(self.N + 1, np.int32)
@@ -497,11 +490,72 @@ class HeatEqIncompNewton(Case):
         check(lib().pfem_heat_get_system(self._h, _ip(rp), _ip(ci), _dp(v), _dp(b)))
         return rp, ci, v, b
 
-    def solve(self, dt) -> bool:  # noqa: D401  (mirrors Equation::solve)
+    def heat_step(self, dt):
+        """(status, stats dict) of one HeatEqIncompNewton::solve."""
         st = _lib.StepStats()
         rc = check(lib().pfem_heat_step(self._h, float(dt), C.byref(st)), allow_positive=True)
-        self.last_stats = st.as_dict()
+        return rc, st.as_dict()
+
+
+class HeatEqIncompNewton(_HeatBC, Case):
+    """HeatEqIncompNewton without phase change (IncompNewton/HeatEquation.inl) as the Conduction
+    problem runs it: one node state T, fixed nodes.  Material keywords follow the Lua Material
+    table (k, rho, cv, h, Tinf, epsRad); the facets and their BC kinds come from the mesher
+    (set_facets), the Dirichlet temperatures and the flux at the facet Gauss points from the
+    caller (set_dirichlet_T, set_flux).  ``solve(dt)`` returns True on success, False on a
+    solver failure, like Equation::solve."""
+
+    def __init__(self, ctx, dim, rho=1000.0, cv=4180.0, k=0.6, h=0.0, Tinf=0.0, epsRad=0.0, krylov_rtol=1e-12,
+                 krylov_max_iter=100000, **kw):
+        super().__init__(ctx, dim, kind="HEAT", rho=rho, cv=cv, k=k, h=h, Tinf=Tinf, epsRad=epsRad,
+                         krylov_rtol=krylov_rtol, krylov_max_iter=krylov_max_iter, **kw)
+        self.last_stats = None
+        self.F = 0
+
+    def assemble(self, dt):  # noqa: D401
+        self.heat_assemble(dt)
+
+    def solve(self, dt) -> bool:  # noqa: D401  (mirrors Equation::solve)
+        rc, self.last_stats = self.heat_step(dt)
         return rc == _lib.PFEM_OK
+
+
+class BoussinesqIncompNewton(_HeatBC, Case):
+    """The "Boussinesq" problem of SolverIncompNewton (IncompNewton/Solver.cpp:58-130, 249-283): the
+    PSPG momentum-continuity equation whose body force F and PSPG term H are scaled by
+    1 - alpha (T - Tr) at each Gauss point (MomContEquation.inl:164-197), and the heat equation
+    (no phase change), on one mesh.  Node states (v, p, T), (dim + 2) N.  Velocity Dirichlet
+    nodes carry PFEM_NODE_DIRICHLET (values: set_dirichlet), temperature Dirichlet nodes
+    PFEM_NODE_DIRICHLET_T (values: set_dirichlet_T); the heat facets as for
+    HeatEqIncompNewton.  Keywords of the Material table (rho, mu, k, cv, alpha, Tr, h, Tinf,
+    epsRad; gamma and DgammaDT must be 0) and of Solver (solveHeatFirst) and MomContEq."""
+
+    def __init__(self, ctx, dim, solve_heat_first=True, rho=1000.0, mu=1e-3, cv=4180.0, k=0.6, alpha=0.0, Tr=0.0,
+                 h=0.0, Tinf=0.0, epsRad=0.0, heat_rtol=1e-12, **kw):
+        super().__init__(ctx, dim, kind="Boussinesq", rho=rho, mu=mu, cv=cv, k=k, alpha=alpha, Tr=Tr, h=h, Tinf=Tinf,
+                         epsRad=epsRad, **kw)
+        self.solve_heat_first = solve_heat_first
+        self.heat_rtol = heat_rtol
+        self.F = 0
+        self.last_stats = None
+        self.failed = ""
+
+    def solve(self, dt) -> bool:  # noqa: D401  (m_solveBoussinesq without the remesh)
+        """Heat and momentum in the solveHeatFirst order; False on the first failure (``failed``
+        names the equation; a momentum failure after the heat step keeps the new temperatures,
+        as the reference's node list saved after the heat solve does)."""
+        self.failed = ""
+        order = ("heat", "momentum") if self.solve_heat_first else ("momentum", "heat")
+        stats = {}
+        for eq in order:
+            rc, st = self.heat_step(dt) if eq == "heat" else self.pspg_step(dt)
+            stats[eq] = st
+            if rc != _lib.PFEM_OK:
+                self.failed = eq
+                self.last_stats = stats
+                return False
+        self.last_stats = stats
+        return True
 
 
 class SolverIncompNewton:

@@ -24,6 +24,7 @@ KIND_PSPG = 0
 KIND_WC = 1
 KIND_HEAT = 2
 KIND_FRACSTEP = 3
+KIND_BOUSSINESQ = 4
 HEAT_Q, HEAT_QH, HEAT_QR = 1, 2, 4
 RES = {"U": 0, "U_P": 1, "Ax_f": 2}
 PREC_FP64 = 0
@@ -66,6 +67,9 @@ class CaseDesc(C.Structure):
         ("cv", C.c_double), ("k_cond", C.c_double), ("h_conv", C.c_double), ("t_inf", C.c_double),
         ("eps_rad", C.c_double),
         ("gamma_fs", C.c_double),
+        ("alpha_exp", C.c_double),
+        ("t_ref", C.c_double),
+        ("dgamma_dt", C.c_double),
     ]
 
 

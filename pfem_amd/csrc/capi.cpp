@@ -50,10 +50,12 @@ void setup_heat(pfem_case* c) {
     PFEM_HIP(hipMemcpyAsync(idx.data(), P.nbr_idx.p, sizeof(int) * idx.size(), hipMemcpyDeviceToHost, c->stream));
     PFEM_HIP(hipMemcpyAsync(fl.data(), P.flags.p, fl.size(), hipMemcpyDeviceToHost, c->stream));
     PFEM_HIP(hipStreamSynchronize(c->stream));
+    auto& H = c->heat;
+    H.dbit = c->d.kind == PFEM_KIND_BOUSSINESQ ? pfem::NODE_DIRICHLET_T : pfem::NODE_DIRICHLET;
     std::vector<int> rp((size_t)N + 1, 0), col, dpos((size_t)N);
     col.reserve(idx.size());
     for (int i = 0; i < N; ++i) {
-        const bool full = !(fl[i] & (pfem::NODE_DIRICHLET | pfem::NODE_FREE));
+        const bool full = !(fl[i] & (H.dbit | pfem::NODE_FREE));
         if (full) {
             for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
                 if (idx[q] == i) dpos[i] = (int)col.size();
@@ -65,7 +67,6 @@ void setup_heat(pfem_case* c) {
         }
         rp[i + 1] = (int)col.size();
     }
-    auto& H = c->heat;
     H.nnz = (long long)col.size();
     auto up = [&](pfem::DevBuf& d, const void* src, size_t bytes) {
         d.reserve(bytes);
@@ -80,10 +81,10 @@ void setup_heat(pfem_case* c) {
     H.erec.reserve(sizeof(double) * 16 * (size_t)E);
     H.part.reserve(sizeof(double) * 5 * (size_t)nb);
     H.scal.reserve(sizeof(double) * 2);
-    c->q_prev.reserve(sizeof(double) * (size_t)N);
-    PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * (size_t)N, c->stream));
-    c->dir_vals.reserve(sizeof(double) * (size_t)N);
-    PFEM_HIP(hipMemsetAsync(c->dir_vals.p, 0, sizeof(double) * (size_t)N, c->stream));
+    H.T.reserve(sizeof(double) * (size_t)N);
+    PFEM_HIP(hipMemsetAsync(H.T.p, 0, sizeof(double) * (size_t)N, c->stream));
+    H.Tbc.reserve(sizeof(double) * (size_t)N);
+    PFEM_HIP(hipMemsetAsync(H.Tbc.p, 0, sizeof(double) * (size_t)N, c->stream));
     // the facets of a previous mesh no longer apply
     H.F = 0;
     H.has_q = false;
@@ -112,8 +113,9 @@ pfem::HeatView heat_view(pfem_case* c, double dt) {
     V.vals = H.vals.as<double>();
     V.b = H.b.as<double>();
     V.erec = H.erec.as<double>();
-    V.Tprev = c->q_prev.as<double>();
-    V.Tbc = c->dir_vals.as<double>();
+    V.dbit = H.dbit;
+    V.Tprev = H.T.as<double>();
+    V.Tbc = H.Tbc.as<double>();
     V.facets = H.facets.as<int>();
     V.fout = H.fout.as<int>();
     V.fkind = H.fkind.as<uint8_t>();
@@ -282,11 +284,11 @@ void require_mesh(pfem_case* c) {
 }
 void require_pspg(pfem_case* c) {
     require_mesh(c);
-    PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG, PFEM_E_STATE, "not a PSPG case");
+    PFEM_REQUIRE(pfem::pspg_like(c), PFEM_E_STATE, "not a PSPG case");
 }
 void require_heat(pfem_case* c) {
     require_mesh(c);
-    PFEM_REQUIRE(c->d.kind == PFEM_KIND_HEAT, PFEM_E_STATE, "not a heat case");
+    PFEM_REQUIRE(pfem::heat_like(c), PFEM_E_STATE, "not a heat case");
 }
 void require_frac(pfem_case* c) {
     require_mesh(c);
@@ -340,7 +342,8 @@ size_t persistent_bytes(pfem_case* c) {
         b += p->bytes;
     auto& H = c->heat;
     for (DevBuf* p : {&H.row_ptr, &H.col, &H.diag_pos, &H.vals, &H.b, &H.erec, &H.facets, &H.fout, &H.fkind, &H.qgp,
-                      &H.fterm, &H.nf_ptr, &H.nf_list, &H.x, &H.r, &H.z, &H.p, &H.q, &H.dinv, &H.part, &H.scal})
+                      &H.fterm, &H.nf_ptr, &H.nf_list, &H.x, &H.r, &H.z, &H.p, &H.q, &H.dinv, &H.part, &H.scal, &H.T,
+                      &H.Tbc})
         b += p->bytes;
     auto& F = c->frac;
     for (DevBuf* p : {&F.erec, &F.mk, &F.ms, &F.lv, &F.dinv_mk, &F.dinv_m, &F.dinv_l, &F.bv, &F.bp, &F.vt, &F.pn,
@@ -520,12 +523,20 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
         const auto& d = *desc;
         PFEM_REQUIRE(d.dim == 2 || d.dim == 3, PFEM_E_INVALID, "dim must be 2 or 3");
         PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC || d.kind == PFEM_KIND_HEAT ||
-                         d.kind == PFEM_KIND_FRACSTEP,
+                         d.kind == PFEM_KIND_FRACSTEP || d.kind == PFEM_KIND_BOUSSINESQ,
                      PFEM_E_INVALID, "unknown case kind");
         PFEM_REQUIRE(std::fabs(d.gamma) < 1e-15, PFEM_E_UNSUPPORTED,
                      "surface tension (gamma > 0) is not supported: the FST facet term is not implemented");
         PFEM_REQUIRE(d.mu >= 0, PFEM_E_INVALID, "mu must be >= 0");
-        if (d.kind == PFEM_KIND_PSPG) {
+        if (d.kind == PFEM_KIND_BOUSSINESQ) {
+            PFEM_REQUIRE(d.cv > 0 && d.k_cond >= 0 && d.h_conv >= 0 && d.eps_rad >= 0, PFEM_E_INVALID,
+                         "heat needs cv > 0, k >= 0, h >= 0, epsRad >= 0");
+            PFEM_REQUIRE(d.problem == PFEM_PROBLEM_NEWTONIAN, PFEM_E_INVALID, "a Boussinesq fluid is Newtonian");
+            // MomContEquationPSPG.inl:160: the FST facet term also runs when DgammaDT > 0
+            PFEM_REQUIRE(std::fabs(d.dgamma_dt) < 1e-15, PFEM_E_UNSUPPORTED,
+                         "DgammaDT > 0 needs the surface-tension facet term, which is not implemented");
+        }
+        if (d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_BOUSSINESQ) {
             PFEM_REQUIRE(d.rho > 0, PFEM_E_INVALID, "rho must be > 0");
             PFEM_REQUIRE(d.residual >= 0 && d.residual <= 2, PFEM_E_INVALID, "unknown residual kind");
             PFEM_REQUIRE(d.max_iter >= 0, PFEM_E_INVALID, "maxIter must be >= 0");
@@ -658,7 +669,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         c->partials.reserve(sizeof(double) * pfem::kMaxCaseBlocks * pfem::kPartStride);
         c->counters.reserve(sizeof(unsigned) * 8);
         PFEM_HIP(hipMemsetAsync(c->counters.p, 0, sizeof(unsigned) * 8, c->stream));
-        if (c->d.kind == PFEM_KIND_PSPG) {
+        if (pfem::pspg_like(c)) {
             auto t0 = std::chrono::steady_clock::now();
             pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             lap(0, t0);
@@ -671,6 +682,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             c->amg = pfem::AmgHierarchy{};
             if (wants_amg(c)) pfem::build_amg(c->pat, dim, pfem::kAmgMaxDense / (dim + 1), c->stream, c->amg);
             lap(2, t0);
+            if (c->d.kind == PFEM_KIND_BOUSSINESQ) setup_heat(c);
         } else if (c->d.kind == PFEM_KIND_HEAT) {
             pfem::build_pattern(dim, N, E, d_conn.as<int>(), d_flags.as<uint8_t>(), c->stream, c->pat);
             setup_heat(c);
@@ -700,12 +712,15 @@ int pfem_case_set_state(pfem_case* c, const double* q) {
         PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null state");
         c->cfl_valid = false;
         PFEM_HIP(hipSetDevice(c->ctx->device));
-        if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) {
+        if (pfem::pspg_like(c) || c->d.kind == PFEM_KIND_FRACSTEP) {
             const size_t nb = sizeof(double) * (size_t)c->n;
             PFEM_HIP(hipMemcpyAsync(c->q_prev.p, q, nb, hipMemcpyHostToDevice, c->stream));
             if (c->ws.base) PFEM_HIP(hipMemcpyAsync(c->ws.x, q, nb, hipMemcpyHostToDevice, c->stream));
+            if (c->d.kind == PFEM_KIND_BOUSSINESQ)  // (v, p, T): the temperatures follow
+                PFEM_HIP(hipMemcpyAsync(c->heat.T.p, q + c->n, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice,
+                                        c->stream));
         } else if (c->d.kind == PFEM_KIND_HEAT) {
-            PFEM_HIP(hipMemcpyAsync(c->q_prev.p, q, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+            PFEM_HIP(hipMemcpyAsync(c->heat.T.p, q, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
         } else {
             const size_t cnt = (size_t)(2 * c->dim + 2) * c->N;
             if (c->d.wc_state == PFEM_STATE_FP32) {
@@ -726,10 +741,13 @@ int pfem_case_get_state(pfem_case* c, double* q) {
         require_mesh(c);
         PFEM_REQUIRE(q != nullptr, PFEM_E_INVALID, "null state");
         PFEM_HIP(hipSetDevice(c->ctx->device));
-        if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) {
+        if (pfem::pspg_like(c) || c->d.kind == PFEM_KIND_FRACSTEP) {
             PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->n, hipMemcpyDeviceToHost, c->stream));
+            if (c->d.kind == PFEM_KIND_BOUSSINESQ)
+                PFEM_HIP(hipMemcpyAsync(q + c->n, c->heat.T.p, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost,
+                                        c->stream));
         } else if (c->d.kind == PFEM_KIND_HEAT) {
-            PFEM_HIP(hipMemcpyAsync(q, c->q_prev.p, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
+            PFEM_HIP(hipMemcpyAsync(q, c->heat.T.p, sizeof(double) * (size_t)c->N, hipMemcpyDeviceToHost, c->stream));
         } else {
             const size_t cnt = (size_t)(2 * c->dim + 2) * c->N;
             if (c->d.wc_state == PFEM_STATE_FP32) {
@@ -1284,7 +1302,8 @@ int pfem_batch_step(pfem_ctx* ctx, int32_t ncases, pfem_case* const* cases, cons
             pfem_case* c = cases[i];
             require_mesh(c);
             PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_WC, PFEM_E_UNSUPPORTED,
-                         "heat and fractional-step cases are stepped with pfem_heat_step / pfem_frac_step");
+                         "heat, fractional-step and Boussinesq cases are stepped with pfem_heat_step / "
+                         "pfem_frac_step / pfem_heat_step + pfem_pspg_step");
             if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
             PFEM_REQUIRE(c->ctx == ctx, PFEM_E_INVALID, "case belongs to another context");
             PFEM_REQUIRE(dts[i] > 0, PFEM_E_INVALID, "dt must be > 0");
@@ -1380,7 +1399,8 @@ int pfem_server_submit(pfem_server* S, pfem_case* c, double dt, int64_t tag, int
         PFEM_REQUIRE(c->ctx == S->ctx, PFEM_E_INVALID, "case belongs to another context");
         PFEM_REQUIRE(dt > 0, PFEM_E_INVALID, "dt must be > 0");
         PFEM_REQUIRE(c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_WC, PFEM_E_UNSUPPORTED,
-                     "heat and fractional-step cases are stepped with pfem_heat_step / pfem_frac_step");
+                     "heat, fractional-step and Boussinesq cases are stepped with pfem_heat_step / "
+                     "pfem_frac_step / pfem_heat_step + pfem_pspg_step");
         if (c->d.kind == PFEM_KIND_PSPG) require_static_dirichlet(c);
         const size_t wb = c->ws.base ? 0 : pfem::workspace_bytes(c);
         PFEM_REQUIRE(wb <= S->arena.capacity(), PFEM_E_OOM, "the case's step workspace exceeds the server budget");
@@ -1554,7 +1574,7 @@ int pfem_heat_set_dirichlet(pfem_case* c, const double* t_bc) {
     return guarded([&] {
         require_heat(c);
         PFEM_REQUIRE(t_bc != nullptr, PFEM_E_INVALID, "null Dirichlet values");
-        PFEM_HIP(hipMemcpyAsync(c->dir_vals.p, t_bc, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
+        PFEM_HIP(hipMemcpyAsync(c->heat.Tbc.p, t_bc, sizeof(double) * (size_t)c->N, hipMemcpyHostToDevice, c->stream));
         PFEM_HIP(hipStreamSynchronize(c->stream));
         return PFEM_OK;
     });
@@ -1629,9 +1649,9 @@ int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats) {
         else if (bb > 0 && rr > thr) status = PFEM_NOT_CONVERGED;
         if (status == PFEM_OK) {
             if (bb > 0)
-                PFEM_HIP(hipMemcpyAsync(c->q_prev.p, V.x, sizeof(double) * (size_t)V.N, hipMemcpyDeviceToDevice, c->stream));
+                PFEM_HIP(hipMemcpyAsync(c->heat.T.p, V.x, sizeof(double) * (size_t)V.N, hipMemcpyDeviceToDevice, c->stream));
             else
-                PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * (size_t)V.N, c->stream));
+                PFEM_HIP(hipMemsetAsync(c->heat.T.p, 0, sizeof(double) * (size_t)V.N, c->stream));
         }
         PFEM_HIP(hipStreamSynchronize(c->stream));
         const auto t2 = clk::now();
@@ -1892,6 +1912,7 @@ int pfem_mesher_stats(pfem_mesher* m, double* out8) {
 
 static int mesher_states_per_node(pfem_case* c) {
     if (c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_FRACSTEP) return c->dim + 1;
+    if (c->d.kind == PFEM_KIND_BOUSSINESQ) return c->dim + 2;
     return c->d.kind == PFEM_KIND_HEAT ? 1 : 2 * c->dim + 2;
 }
 

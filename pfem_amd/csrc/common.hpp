@@ -61,7 +61,7 @@ __device__ __forceinline__ Seg xcd_segment(const BlockTask& t, int nodes) {
 }
 
 // Node flag bits (PFEM_NODE_* in include/pfem_gpu.h).
-enum : uint8_t { NODE_BOUND = 1, NODE_FREE = 2, NODE_ONFS = 4, NODE_FIXED = 8, NODE_DIRICHLET = 16 };
+enum : uint8_t { NODE_BOUND = 1, NODE_FREE = 2, NODE_ONFS = 4, NODE_FIXED = 8, NODE_DIRICHLET = 16, NODE_DIRICHLET_T = 32 };
 
 // Lane permutations inside 16-lane DPP rows (VALU moves; no LDS round trip as __shfl_xor's
 // ds_bpermute takes).  CTRL is a DPP control: 0xB1 / 0x4E quad_perm [1,0,3,2] / [2,3,0,1],

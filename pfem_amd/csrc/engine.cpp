@@ -227,7 +227,7 @@ WsLayout ws_layout(const pfem_case* c) {
 }  // namespace
 
 size_t workspace_bytes(const pfem_case* c) {
-    if (c->d.kind != PFEM_KIND_PSPG || !c->has_mesh) return 0;
+    if (!pspg_like(c) || !c->has_mesh) return 0;
     return ws_layout(c).total;
 }
 
@@ -268,7 +268,7 @@ void carve_workspace(pfem_case* c, char* base) {
 }
 
 void own_workspace(pfem_case* c, hipStream_t s) {
-    if (c->ws.base || c->d.kind != PFEM_KIND_PSPG || !c->has_mesh) return;
+    if (c->ws.base || !pspg_like(c) || !c->has_mesh) return;
     const size_t wb = workspace_bytes(c);
     c->ws.own.reserve(wb);
     carve_workspace(c, c->ws.own.as<char>());
@@ -447,7 +447,7 @@ void Engine::Impl::load_views(Req* r) {
     const int N = c->N, dim = c->dim;
     const long long n = c->n;
     SlotScal* ds = dslot(slot);
-    if (c->d.kind == PFEM_KIND_PSPG) {
+    if (pspg_like(c)) {
         const Pattern& P = c->pat;
         const Workspace& w = c->ws;
         const bool mixed = c->d.precision == PFEM_PREC_MIXED_IR;  // FP32 inner solves
@@ -487,6 +487,11 @@ void Engine::Impl::load_views(Req* r) {
         pv.problem = c->d.problem;
         pv.tau0 = c->d.tau0;
         pv.mreg = c->d.m_reg;
+        if (c->d.kind == PFEM_KIND_BOUSSINESQ) {
+            pv.T = c->heat.T.as<double>();
+            pv.alpha = c->d.alpha_exp;
+            pv.t_ref = c->d.t_ref;
+        }
 
         PicardView& pc = h_pic[slot];
         pc = PicardView{};
@@ -896,7 +901,7 @@ void Engine::tick(std::vector<Req*>& done) {
     while (!waitq_.empty() && !I.free_slots.empty()) {
         Req* r = waitq_.front();
         pfem_case* c = r->c;
-        if (c->d.kind == PFEM_KIND_PSPG && !c->ws.base) {
+        if (pspg_like(c) && !c->ws.base) {
             const size_t wb = workspace_bytes(c);
             if (I.arena) {
                 size_t off = 0;
@@ -995,7 +1000,7 @@ void Engine::tick(std::vector<Req*>& done) {
         if (op & OP_K64_INIT) tab[T_F64 + li].push_back(BlockTask{sl, 0, 1, 0});
         if (op & OP_K32_INIT) tab[T_F32 + li].push_back(BlockTask{sl, 0, 1, 0});
         const AmgHierarchy& H = c->amg;
-        if (c->d.kind == PFEM_KIND_PSPG && H.levels > 1) {
+        if (pspg_like(c) && H.levels > 1) {
             const int base = sl * kAmgMaxLevels;  // AmgView index of (slot, level 0)
             std::vector<BlockTask>* A = tab + T_AMG + (amg_fp32(c) ? A_N : 0);
             // the coarse operators are rebuilt at a step's first assembly; the Picard

@@ -52,7 +52,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_asm(HeatView V) {
     if (i >= V.N) return;
     const uint8_t f = V.flags[i];
     const int rp = V.row_ptr[i];
-    if ((f & NODE_DIRICHLET) || (f & NODE_FREE)) {
+    if ((f & V.dbit) || (f & NODE_FREE)) {
         V.vals[rp] = 1.0;
         return;
     }
@@ -192,7 +192,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_bc(HeatView V) {
     if (i >= V.N) return;
     const uint8_t f = V.flags[i];
     const int rp = V.row_ptr[i];
-    if (f & NODE_DIRICHLET) {
+    if (f & V.dbit) {
         V.b[i] = V.Tbc[i];
     } else if (f & NODE_FREE) {
         V.b[i] = V.Tprev[i];
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kBlock) void k_heat_bc(HeatView V) {
         const int nb = V.nbr_ptr[i], deg = V.nbr_ptr[i + 1] - nb;
         for (int k = 0; k < deg; ++k) {
             const int j = V.nbr_idx[nb + k];
-            if (j != i && (V.flags[j] & NODE_DIRICHLET)) {
+            if (j != i && (V.flags[j] & V.dbit)) {
                 bi -= V.vals[rp + k] * V.Tbc[j];
                 V.vals[rp + k] = 0.0;
             }

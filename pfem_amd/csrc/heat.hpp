@@ -9,7 +9,9 @@ namespace pfem {
 struct HeatView {
     int dim = 2, N = 0, E = 0, F = 0, nb = 0;  // nb: workgroups over the nodes (partial-sum slots)
     const int* conn = nullptr;      // [dim+1][E] structure of arrays
-    const uint8_t* flags = nullptr; // [N] NODE_DIRICHLET from the caller, NODE_FREE derived
+    const uint8_t* flags = nullptr; // [N] caller flags, NODE_FREE derived
+    uint8_t dbit = 16;              // flag of the Dirichlet temperatures (NODE_DIRICHLET, or NODE_DIRICHLET_T
+                                    // in a Boussinesq case, whose NODE_DIRICHLET marks the velocities)
     const double* pos = nullptr;    // [dim][N]
     const int* nbr_ptr = nullptr;   // node graph (self included, sorted)
     const int* nbr_idx = nullptr;

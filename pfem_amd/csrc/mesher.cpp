@@ -564,7 +564,7 @@ void Mesher::set_nodes(int n, int nstates, const double* pos_fb, const double* s
     for (int i = 0; i < n; ++i)
         for (int s = 0; s < nstates; ++s) st_[(size_t)nstates * i + s] = states_fb ? states_fb[(size_t)s * n + i] : 0.0;
     flags_.assign(n, 0);
-    for (int i = 0; i < n; ++i) flags_[i] = flags ? (uint8_t)(flags[i] & (kBound | kFixed | kDirichlet)) : 0;
+    for (int i = 0; i < n; ++i) flags_[i] = flags ? (uint8_t)(flags[i] & (kBound | kFixed | kDirichlet | kDirichletT)) : 0;
     tag_.assign(n, -1);
     if (tags)
         for (int i = 0; i < n; ++i) tag_[i] = tags[i];

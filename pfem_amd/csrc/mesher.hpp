@@ -21,7 +21,7 @@ struct MesherParams {
 class Mesher {
 public:
     // node flag bits (the PFEM_NODE_* values of the C-ABI)
-    static constexpr uint8_t kBound = 1, kFree = 2, kOnFS = 4, kFixed = 8, kDirichlet = 16;
+    static constexpr uint8_t kBound = 1, kFree = 2, kOnFS = 4, kFixed = 8, kDirichlet = 16, kDirichletT = 32;
 
     explicit Mesher(const MesherParams& prm);
 

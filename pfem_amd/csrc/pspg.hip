@@ -18,6 +18,7 @@
 //   (C tau/dt + D)_(i)(jt) = |e|/(d+1) (tau/dt d_tNi + d_tNj)       getC   (:325)
 //   tau L_ij = tau/rho |e| gradNi.gradNj                            getL   (:309)
 //   F_(is) = rho |e| g_s/(d+1),  H_i = tau |e| gradNi.g             getF, getH (:341, :373)
+//   (Boussinesq: both scaled by 1 - alpha (T - Tr) under the integral, T linear on the element)
 // Off-diagonal node pairs take one lane each; the diagonal block and the right-hand side
 // of a node (every element around it) are spread over the node's lane group and reduced
 // with cross-lane shuffles.  Momentum rows of boundary nodes are not assembled
@@ -219,6 +220,12 @@ __global__ __launch_bounds__(kBlock) void k_pspg_elem(const PspgView* __restrict
             for (int c = 0; c < NE; ++c) vs += V.q_prev[(long long)s * N + nd[c]];
             sv[s] = vs;
         }
+        if (V.T) {  // Boussinesq: sum of the element's node temperatures
+            double ts = 0;
+#pragma unroll
+            for (int c = 0; c < NE; ++c) ts += V.T[nd[c]];
+            sv[3] = ts;
+        }
         const double vol = G.vol, vol_ne = vol / NE;
         double rec[kRecW];
 #pragma unroll
@@ -408,6 +415,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
         double vown[D];  // node i's previous-step velocity (the a == i term of Me/dt vPrev)
 #pragma unroll
         for (int s = 0; s < D; ++s) vown[s] = V.q_prev[(long long)s * N + i];
+        const double town = V.T ? V.T[i] : 0.0;
         const int q0 = V.pei_ptr[m], q1 = V.pei_ptr[m + 1];
         for (int q = q0 + lane; q < q1; q += L) {
             int e, a, b;
@@ -422,13 +430,29 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
             double hg = 0, cv = 0;
             double ga[D];
             grad_col<D>(R, a, ga);
+            if (V.T) {
+                // Boussinesq (MomContEquation.inl:166-197): with T linear on the element,
+                // int N_a rho (1 - alpha (T - Tr)) = rho |e| / (d+1) ((1 + alpha Tr) -
+                // alpha (T_a + sum_c T_c) / (d+2)), and the H factor's mean is
+                // 1 - alpha (sum_c T_c / (d+1) - Tr)
+                const double ff = (1.0 + V.alpha * V.t_ref) - V.alpha * (town + sv4.w) / (D + 2);
+                const double fh = 1.0 - V.alpha * (sv4.w / (D + 1) - V.t_ref);
 #pragma unroll
-            for (int s = 0; s < D; ++s) {
-                bv[s] += R.rvn * V.g[s] + R.mab * (vown[s] + sv[s]);
-                hg += ga[s] * V.g[s];
-                cv += ga[s] * sv[s];
+                for (int s = 0; s < D; ++s) {
+                    bv[s] += R.rvn * V.g[s] * ff + R.mab * (vown[s] + sv[s]);
+                    hg += ga[s] * V.g[s];
+                    cv += ga[s] * sv[s];
+                }
+                bv[D] += R.tv * hg * fh + R.tvn * cv;
+            } else {
+#pragma unroll
+                for (int s = 0; s < D; ++s) {
+                    bv[s] += R.rvn * V.g[s] + R.mab * (vown[s] + sv[s]);
+                    hg += ga[s] * V.g[s];
+                    cv += ga[s] * sv[s];
+                }
+                bv[D] += R.tv * hg + R.tvn * cv;
             }
-            bv[D] += R.tv * hg + R.tvn * cv;
         }
 #pragma unroll
         for (int s = 0; s < NE; ++s) {

@@ -240,6 +240,8 @@ struct pfem_case {
     struct HeatBufs {
         pfem::DevBuf row_ptr, col, diag_pos, vals, b, erec, facets, fout, fkind, qgp, fterm, nf_ptr, nf_list;
         pfem::DevBuf x, r, z, p, q, dinv, part, scal;
+        pfem::DevBuf T, Tbc;  // node temperatures (the state) and Dirichlet temperatures
+        uint8_t dbit = 16;    // node flag of the Dirichlet temperatures (PFEM_NODE_DIRICHLET or _T)
         long long nnz = 0;
         int F = 0;
         bool has_q = false;
@@ -255,3 +257,14 @@ struct pfem_case {
     std::unique_ptr<pfem::Engine> engine;  // direct calls
     ~pfem_case();
 };
+
+namespace pfem {
+// cases stepped by the PSPG Picard engine (a Boussinesq case adds the heat equation)
+inline bool pspg_like(const pfem_case* c) {
+    return c->d.kind == PFEM_KIND_PSPG || c->d.kind == PFEM_KIND_BOUSSINESQ;
+}
+// cases that hold the heat equation
+inline bool heat_like(const pfem_case* c) {
+    return c->d.kind == PFEM_KIND_HEAT || c->d.kind == PFEM_KIND_BOUSSINESQ;
+}
+}  // namespace pfem

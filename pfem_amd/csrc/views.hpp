@@ -30,6 +30,7 @@ struct PspgView {
     float* vals32_ell;        // FP32 ELL copy for an FP32 multigrid hierarchy (null: none)
     double* erec;             // [E][16] element records of the assembly (pspg.hip)
     double* esv;              // [E][4] sums of the previous-step velocities over each element
+                              //        (and, with T, of the node temperatures in the 4th slot)
     double* binv;             // [(dim+1)^2][N] inverse diagonal node blocks (block Jacobi), SoA
     float* binv32;            // FP32 copy for the mixed-precision inner solve (may be null)
     long long nnz;
@@ -37,6 +38,10 @@ struct PspgView {
     double g[3];
     int problem, pad_problem;  // PFEM_PROBLEM_*
     double tau0, mreg;         // Bingham parameters
+    // Boussinesq: node temperatures (null: none) scaling the body force of F by
+    // 1 - alpha (T - Tr) at each Gauss point, and H by the same factor
+    const double* T;
+    double alpha, t_ref;
 };
 
 // The assembled PSPG matrix seen through its node structure.  Values are the CSR values

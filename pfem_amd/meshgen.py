@@ -319,6 +319,31 @@ def tank_nodes(dim: int, target_nodes: int, h=0.01, seed=0) -> PfemMesh:
     return PfemMesh(3, pos, conn, flags, h, name=f"tank3d_{nx}x{nx}x{nz}", surface=surface, meta=dict(nx=nx, nz=nz))
 
 
+def cavity(dim: int, n: int, h=None, seed=0) -> PfemMesh:
+    """Closed square / cubic cavity of side 1 on an n^dim-cell lattice, every wall node boundary,
+    fixed and Dirichlet (velocity): the lid-driven and heated-cavity cases of the heat,
+    fractional-step and Boussinesq solvers.  meta["top"], meta["bottom"], meta["lid"] mark the
+    top wall, the bottom wall, and the top nodes off the side walls."""
+    h = 1.0 / n if h is None else h
+    rng = np.random.default_rng(seed)
+    if dim == 2:
+        pos, conn, flags, _, _ = _lattice_2d(n, n, h, lambda i, j: (i == 0) | (j == 0) | (i == n) | (j == n), rng,
+                                             jitter=True)
+    else:
+        pos, conn, flags, _, _ = _lattice_3d(
+            n, n, n, h, lambda i, j, k: (i == 0) | (j == 0) | (k == 0) | (i == n) | (j == n) | (k == n), rng)
+    m = PfemMesh(dim, pos, conn, flags, h, name=f"cavity{dim}d_{n}")
+    top_z = n * h
+    z = pos[dim - 1]
+    side = np.zeros(pos.shape[1], dtype=bool)
+    for k in range(dim - 1):
+        side |= (np.abs(pos[k]) < 1e-12) | (np.abs(pos[k] - top_z) < 1e-12)
+    top = np.abs(z - top_z) < 1e-12
+    bottom = np.abs(z) < 1e-12
+    m.meta.update(n=n, seed=seed, top=top, bottom=bottom, lid=top & ~side)
+    return m
+
+
 # ----------------------------------------------------------------------------------------
 # Initial conditions (the examples' Problem.IC:initStates, evaluated in numpy)
 # ----------------------------------------------------------------------------------------

@@ -46,6 +46,7 @@ MUTATIONS = {
 
 HEAT_PINS = ["tests/test_oracle_heat_pins.py"]
 FRAC_PINS = ["tests/test_oracle_frac_pins.py"]
+BQ_PINS = ["tests/test_oracle_boussinesq_pins.py"]
 # (old, new, pin files) of the heat and fractional-step restatements
 MUTATIONS_PINNED = {
     # heat: L without the time step (HeatEquation.inl:245-250)
@@ -71,6 +72,11 @@ MUTATIONS_PINNED = {
     "frac_vstep_sign": ("{nodes[i] + d * N, P.dt * t}", "{nodes[i] + d * N, -P.dt * t}", FRAC_PINS),
     # FracStep: K coupling block (d, d2) read as (d, d) (FracStep.inl:108-112)
     "frac_k_block": ("Ke[i + d * ne][j + d2 * ne]};", "Ke[i + d * ne][j + d * ne]};", FRAC_PINS),
+    # Boussinesq: F factor with the sign of alpha flipped (MomContEquation.inl:170)
+    "bq_f_sign": ("bq ? P.rho * (1 - P.mat.alpha * (Tg(Ng) - P.mat.Tr)) : P.rho",
+                  "bq ? P.rho * (1 + P.mat.alpha * (Tg(Ng) - P.mat.Tr)) : P.rho", BQ_PINS),
+    # Boussinesq: H factor left at 1 (MomContEquation.inl:191-196)
+    "bq_h_factor": ("return bq ? 1 - P.mat.alpha * (Tg(Ng) - P.mat.Tr) : 1.0;", "return 1.0;", BQ_PINS),
     # FracStep: free nodes keep v_prev without dt g (FracStep.inl:244-252)
     "frac_free_rhs": ("S.b[nn + d * N] = vPrevQ[nn + d * N] + dt * P.g[d];", "S.b[nn + d * N] = vPrevQ[nn + d * N];",
                       FRAC_PINS),
