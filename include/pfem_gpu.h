@@ -83,7 +83,8 @@ typedef struct pfem_case_desc {
     int32_t kind; /* PFEM_KIND_* */
     double rho;   /* Material.rho (PSPG) */
     double mu;    /* Material.mu */
-    double gamma; /* Material.gamma (surface tension); must be 0 (FST is not implemented) */
+    double gamma; /* Material.gamma (surface tension); > 0 applies the FST facet term of the facets
+                     set by pfem_case_set_facets (PSPG, Boussinesq, WC; refused by FracStep) */
     double body_force[3];
     /* WC (ContEq / Material) */
     double K0, K0p, rho_star;
@@ -118,7 +119,7 @@ typedef struct pfem_case_desc {
        PSPG H factor 1 - alpha (T - Tr), T = N T_e at each Gauss point */
     double alpha_exp;       /* alpha: thermal expansion coefficient */
     double t_ref;           /* Tr: reference temperature */
-    double dgamma_dt;       /* DgammaDT: must be 0 (the surface-tension facet term is not implemented) */
+    double dgamma_dt;       /* DgammaDT: the Boussinesq surface-tension factor gamma + DgammaDT (T - Tr) */
 } pfem_case_desc;
 
 #define PFEM_PROBLEM_NEWTONIAN 0 /* "IncompNewtonNoT": K factor mu */
@@ -350,6 +351,10 @@ int pfem_case_footprint(pfem_case* c, int64_t* bytes);
  * including uploads, smallest element shape quality 6 sqrt(2) |e| / l_rms^3 (3D; 1 in 2D),
  * the multigrid smoother damping it implies (0.9, or 0.5 below quality 0.1)}. */
 int pfem_case_setup_times(pfem_case* c, double* out6);
+/* Diagnostics of the case's own engine (the direct calls; no counterpart in the reference):
+ * {ticks, replays of captured BiCGSTAB iteration chunks, captures, peak requests in flight}.
+ * Chunks are captured in HIP graphs when the environment sets PFEM_GRAPH=1. */
+int pfem_case_engine_stats(pfem_case* c, int64_t* out4);
 /* Bytes of the step workspace alone (what a serving request borrows from the arena). */
 int pfem_case_workspace_bytes(pfem_case* c, int64_t* bytes);
 /* Allocates the case's own step workspace and direct-call resources now (the direct calls
@@ -390,6 +395,18 @@ int pfem_heat_get_system(pfem_case* c, int32_t* row_ptr, int32_t* col_idx, doubl
  * HeatEquation.hpp:50) to krylov_rtol, the new temperatures.  PFEM_NOT_CONVERGED / PFEM_NAN
  * leave the temperatures unchanged. */
 int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats);
+
+/* ---- Surface tension ---------------------------------------------------------------------------
+ * The boundary facets of the case's mesh (Mesh::m_facets: nodes F x dim in Facet::m_nodesIndexes
+ * order, and each facet's element; pfem_mesher_get_facets gives both) for the FST term of
+ * getFST (MatricesBuilder.inl:389-402), which m_applyBCPSPG (MomContEquationPSPG.inl:155-186)
+ * adds to the momentum rows of the facet's element for every facet with a node on the free
+ * surface (PFEM_NODE_ONFS), and the WC m_applyBC (WCompNewton/MomEquation.inl:312-335) to the
+ * momentum force for every facet whose nodes all are; the factor is gamma, or for a Boussinesq
+ * case gamma + DgammaDT (T - Tr) at the facet's Gauss points.  pfem_case_set_mesh clears the
+ * facets; pfem_case_set_mesh_from / pfem_case_remesh set the mesher's facets when the case has
+ * surface tension.  F = 0 clears them. */
+int pfem_case_set_facets(pfem_case* c, int32_t F, const int32_t* nodes, const int32_t* elem);
 
 /* ---- Fractional step (PFEM_KIND_FRACSTEP) --------------------------------------------------
  * One MomContEqIncompNewton::solve with Solver.id = "FracStep" (MomContEquation.inl:296-303;

@@ -74,6 +74,7 @@ def lib():
         L.orc_picard_residual.argtypes = [C.c_int, C.c_int64, _u8p, C.c_int, _ip, _ip, _dp, _dp, _dp, _dp]
         L.orc_set_material.argtypes = [C.c_int, C.c_double, C.c_double]
         L.orc_set_boussinesq.argtypes = [C.c_double, C.c_double, _dp, C.c_int64]
+        L.orc_set_facets.argtypes = [C.c_int64, C.c_int, _ip, _ip, C.c_double, C.c_double]
         L.orc_wc_parts.argtypes = [C.c_int, C.c_int64, C.c_int64, _ip, _dp, _dp, _dp, _dp, C.c_double, _dp,
                                    C.c_double, C.c_double, C.c_int, C.c_double, _dp, _dp, _dp, _dp, C.c_int]
         L.orc_heat_assemble.restype = C.c_void_p
@@ -338,6 +339,27 @@ class material:
     def __exit__(self, *exc):
         lib().orc_set_material(0, 0.0, 0.0)
         lib().orc_set_boussinesq(0.0, 0.0, C.cast(None, _dp), 0)
+        return False
+
+
+class surface_tension:
+    """Context manager setting the boundary facets (nodes F x dim, element F) and gamma, DgammaDT
+    of the surface-tension term (getFST, MatricesBuilder.inl:389-402) for the PSPG and WC calls
+    inside it."""
+
+    def __init__(self, nodes, elem, gamma, DgammaDT=0.0):
+        self.nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        self.elem = np.ascontiguousarray(elem, dtype=np.int32)
+        self.gamma, self.dgdT = float(gamma), float(DgammaDT)
+
+    def __enter__(self):
+        F, d = self.nodes.shape
+        lib().orc_set_facets(F, d, _i(self.nodes), _i(self.elem), self.gamma, self.dgdT)
+        return self
+
+    def __exit__(self, *exc):
+        z = np.zeros(1, np.int32)
+        lib().orc_set_facets(0, 2, _i(z), _i(z), 0.0, 0.0)
         return False
 
 

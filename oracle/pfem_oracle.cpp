@@ -298,6 +298,77 @@ template <int D> static void ddevWC(double dd[MB<D>::nv][MB<D>::nv]) {
 }
 
 // --------------------------------------------------------------------------------------
+// Lower-dimensional (facet) quadrature and facet geometry, used by the heat equation's facet
+// terms and the surface-tension term.
+// --------------------------------------------------------------------------------------
+template <int D> struct QuadLD;
+template <> struct QuadLD<2> {  // segments on [-1, 1], refSize 2 (Mesh.cpp:479)
+    static constexpr double refSize = 2.0;
+    double N[3][2], w[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};  // Mesh.cpp:429
+    QuadLD() {
+        const double xi[3] = {-std::sqrt(3.0 / 5.0), 0, std::sqrt(3.0 / 5.0)};  // Mesh.cpp:356-358
+        for (int g = 0; g < 3; ++g) {
+            N[g][0] = (1 - xi[g]) / 2;  // Mesh.cpp:506-507
+            N[g][1] = (1 + xi[g]) / 2;
+        }
+    }
+};
+template <> struct QuadLD<3> {  // triangles, refSize 0.5 (Mesh.cpp:482)
+    static constexpr double refSize = 0.5;
+    double N[3][3], w[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};  // Mesh.cpp:451
+    QuadLD() {
+        const double gp[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}, {2.0 / 3.0, 1.0 / 6.0}};
+        for (int g = 0; g < 3; ++g) {
+            N[g][0] = 1 - gp[g][0] - gp[g][1];
+            N[g][1] = gp[g][0];
+            N[g][2] = gp[g][1];
+        }
+    }
+};
+
+// Facet::computeJ / computeDetJ (Facet.cpp:15-77) and computeNormal (Facet.cpp:130-210)
+template <int D>
+static void facetGeo(const double* pos, int64_t N, const int* fn, int out, double& detJ, double nrm[3]) {
+    auto P = [&](int node, int k) { return pos[node + k * N]; };
+    nrm[0] = nrm[1] = nrm[2] = 0;
+    if (D == 2) {
+        const double J00 = (P(fn[1], 0) - P(fn[0], 0)) / 2, J10 = (P(fn[1], 1) - P(fn[0], 1)) / 2;
+        detJ = std::sqrt(J00 * J00 + J10 * J10);
+        double nx = P(fn[1], 1) - P(fn[0], 1), ny = P(fn[0], 0) - P(fn[1], 0);
+        const double vx = P(out, 0) - P(fn[0], 0), vy = P(out, 1) - P(fn[0], 1);
+        if (nx * vx + ny * vy > 0) {
+            nx = -nx;
+            ny = -ny;
+        }
+        const double nn = std::sqrt(nx * nx + ny * ny);
+        nrm[0] = nx / nn;
+        nrm[1] = ny / nn;
+    } else {
+        double J[3][2];
+        for (int k = 0; k < 3; ++k) {
+            J[k][0] = P(fn[1], k) - P(fn[0], k);
+            J[k][1] = P(fn[2], k) - P(fn[0], k);
+        }
+        const double g0 = J[1][0] * J[2][1] - J[1][1] * J[2][0], g1 = J[2][0] * J[0][1] - J[2][1] * J[0][0],
+                     g2 = J[0][0] * J[1][1] - J[1][0] * J[0][1];
+        detJ = std::sqrt(g0 * g0 + g1 * g1 + g2 * g2);
+        double a[3], b[3], n[3], v[3];
+        for (int k = 0; k < 3; ++k) {
+            a[k] = P(fn[1], k) - P(fn[0], k);
+            b[k] = P(fn[2], k) - P(fn[0], k);
+            v[k] = P(out, k) - P(fn[0], k);
+        }
+        n[0] = a[1] * b[2] - a[2] * b[1];
+        n[1] = a[2] * b[0] - a[0] * b[2];
+        n[2] = a[0] * b[1] - a[1] * b[0];
+        if (v[0] * n[0] + v[1] * n[1] + v[2] * n[2] > 0)
+            for (double& c : n) c *= -1.0;
+        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        for (int k = 0; k < 3; ++k) nrm[k] = n[k] / nn;
+    }
+}
+
+// --------------------------------------------------------------------------------------
 // PSPG assembly: srcs/simulation/physics/IncompNewton/MomContEquationPSPG.inl:7-146.
 // --------------------------------------------------------------------------------------
 struct Triplet {
@@ -477,6 +548,125 @@ static std::vector<uint8_t> derivedFlags(const Params& P) {
     return f;
 }
 
+// Surface tension (MatricesBuilder.inl:167-214 getP / getT, :389-402 getFST): the boundary facets
+// of the mesh (nodes in Facet::m_nodesIndexes order, the element each belongs to), gamma and
+// DgammaDT, set by orc_set_facets.  Facet::getOutNode is the element's node that is not on the
+// facet.  A problem other than Boussinesq reads gamma alone (its DgammaDT is not read: 0).
+struct FstSet {
+    int64_t F = 0;
+    std::vector<int> nodes, elem;
+    double gamma = 0, dgdT = 0;
+};
+static FstSet g_fst;
+
+static bool fstActive() { return !(g_fst.gamma < 1e-15 && g_fst.dgdT < 1e-15); }
+
+// getFST of facet f: -sum_g fac_g B^T T(P) P w_g, times detJ refSize(dim - 1); fac_g = gamma, or
+// gamma + DgammaDT (T_g - Tr) with T_g = N_LD T_facet for a Boussinesq material
+template <int D>
+static void facetFST(const double* pos, int64_t N, const int* conn, int64_t f, const Material& mat,
+                     double FST[D * (D + 1)]) {
+    constexpr int ne = D + 1, nd = D * ne, nv = MB<D>::nv;
+    const int* fn = g_fst.nodes.data() + f * D;
+    const int* en = conn + (int64_t)g_fst.elem[f] * ne;
+    int out = en[0];
+    for (int a = 0; a < ne; ++a) {
+        bool onf = false;
+        for (int i = 0; i < D; ++i) onf = onf || en[a] == fn[i];
+        if (!onf) out = en[a];
+    }
+    double detJ, n[3];
+    facetGeo<D>(pos, N, fn, out, detJ, n);
+    double Pv[nv], Tm[nv][nv];
+    if (D == 2) {
+        Pv[0] = 1 - n[0] * n[0];
+        Pv[1] = 1 - n[1] * n[1];
+        Pv[2] = -n[0] * n[1];
+        Tm[0][0] = Pv[0] * Pv[0];
+        Tm[0][1] = Pv[2] * Pv[2];
+        Tm[0][2] = 2 * Pv[0] * Pv[2];
+        Tm[1][0] = Tm[0][1];
+        Tm[1][1] = Pv[1] * Pv[1];
+        Tm[1][2] = 2 * Pv[2] * Pv[1];
+        Tm[2][0] = Pv[0] * Pv[2];
+        Tm[2][1] = Pv[2] * Pv[1];
+        Tm[2][2] = Pv[0] * Pv[1] + Pv[2] * Pv[2];
+    } else {
+        // Voigt order xx, yy, zz, xy, xz, yz
+        Pv[0] = 1 - n[0] * n[0];
+        Pv[1] = 1 - n[1] * n[1];
+        Pv[2] = 1 - n[2] * n[2];
+        Pv[3] = -n[0] * n[1];
+        Pv[4] = -n[0] * n[2];
+        Pv[5] = -n[1] * n[2];
+        const double* P = Pv;
+        double (*T)[nv] = Tm;
+        T[0][0] = P[0] * P[0]; T[0][1] = P[3] * P[3]; T[0][2] = P[4] * P[4];
+        T[0][3] = 2 * P[0] * P[3]; T[0][5] = 2 * P[4] * P[3]; T[0][4] = 2 * P[0] * P[4];
+        T[1][0] = T[0][1]; T[1][1] = P[1] * P[1]; T[1][2] = P[5] * P[5];
+        T[1][3] = 2 * P[3] * P[1]; T[1][5] = 2 * P[5] * P[1]; T[1][4] = 2 * P[3] * P[5];
+        T[2][0] = T[0][2]; T[2][1] = T[1][2]; T[2][2] = P[2] * P[2];
+        T[2][3] = 2 * P[5] * P[4]; T[2][5] = 2 * P[5] * P[2]; T[2][4] = 2 * P[2] * P[4];
+        T[3][0] = P[0] * P[3]; T[3][1] = P[3] * P[1]; T[3][2] = P[4] * P[5];
+        T[3][3] = P[0] * P[1] + P[3] * P[3]; T[3][5] = P[4] * P[1] + P[5] * P[3]; T[3][4] = P[4] * P[3] + P[0] * P[5];
+        T[5][0] = P[3] * P[4]; T[5][1] = P[5] * P[1]; T[5][2] = P[5] * P[2];
+        T[5][3] = P[4] * P[1] + P[5] * P[3]; T[5][5] = P[1] * P[2] + P[5] * P[5]; T[5][4] = P[5] * P[4] + P[3] * P[2];
+        T[4][0] = P[0] * P[4]; T[4][1] = P[3] * P[5]; T[4][2] = P[2] * P[4];
+        T[4][3] = P[4] * P[3] + P[0] * P[5]; T[4][5] = P[5] * P[4] + P[3] * P[2]; T[4][4] = P[2] * P[0] + P[4] * P[4];
+    }
+    double TP[nv];
+    for (int r = 0; r < nv; ++r) {
+        double t = 0;
+        for (int c = 0; c < nv; ++c) t += Tm[r][c] * Pv[c];
+        TP[r] = t;
+    }
+    MB<D> mb;
+    Geo geo;
+    computeGeo<D>(pos, N, en, geo);
+    double G[D][ne], Bm[nv][nd];
+    mb.gradN(geo, G);
+    mb.B(G, Bm);
+    double BtTP[nd];
+    for (int c = 0; c < nd; ++c) {
+        double t = 0;
+        for (int r = 0; r < nv; ++r) t += Bm[r][c] * TP[r];
+        BtTP[c] = t;
+    }
+    QuadLD<D> q;
+    for (int c = 0; c < nd; ++c) FST[c] = 0;
+    for (int g = 0; g < 3; ++g) {
+        double fac = g_fst.gamma;
+        if (mat.problem == 2) {
+            double Tg = 0;
+            for (int i = 0; i < D; ++i) Tg += q.N[g][i] * mat.T[fn[i]];
+            fac = g_fst.gamma + g_fst.dgdT * (Tg - mat.Tr);
+        }
+        for (int c = 0; c < nd; ++c) FST[c] -= fac * BtTP[c] * q.w[g];
+    }
+    for (int c = 0; c < nd; ++c) FST[c] *= detJ * q.refSize;
+}
+
+// The facet loop of m_applyBCPSPG (MomContEquationPSPG.inl:155-186): every facet with a node on
+// the free surface (allNodes: every node, the WC test of MomEquation.inl:320-322) adds its FST to
+// the momentum rows of its element's nodes, in facet order.
+template <int D>
+static void addFST(const double* pos, int64_t N, const int* conn, const std::vector<uint8_t>& fl, const Material& mat,
+                   bool allNodes, double* b) {
+    if (!fstActive()) return;
+    constexpr int ne = D + 1;
+    for (int64_t f = 0; f < g_fst.F; ++f) {
+        const int* fn = g_fst.nodes.data() + f * D;
+        int nfs = 0;
+        for (int i = 0; i < D; ++i) nfs += (fl[fn[i]] & ONFS) ? 1 : 0;
+        if (allNodes ? nfs < D : nfs == 0) continue;
+        double FST[D * ne];
+        facetFST<D>(pos, N, conn, f, mat, FST);
+        const int* en = conn + (int64_t)g_fst.elem[f] * ne;
+        for (int a = 0; a < ne; ++a)
+            for (int d = 0; d < D; ++d) b[en[a] + d * N] += FST[a + d * ne];
+    }
+}
+
 // m_buildAbPSPG: PSPG.inl:7-146, including the default-triplet padding (PSPG.inl:17), the
 // serial unit-diagonal push-back (PSPG.inl:108-128) and Eigen 3.3's setFromTriplets
 // (duplicates summed per inner vector in triplet order), emitted here as CSR.
@@ -602,10 +792,11 @@ static void buildAbPSPG(const Params& P, const std::vector<uint8_t>& fl, const d
 // m_applyBCPSPG: PSPG.inl:149-235.  The FST loop is skipped because gamma = 0 (checked
 // by the caller); Dirichlet values are the host-evaluated "<Tag>V"(pos, t+dt) results.
 template <int D>
-static void applyBCPSPG(const Params& P, const std::vector<uint8_t>& fl, const double* qPrev, const double* dirVals,
-                        CsrSys& S) {
+static void applyBCPSPG(const Params& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
+                        const double* dirVals, CsrSys& S) {
     auto t0 = clk::now();
     const int64_t N = P.N;
+    addFST<D>(pos, N, P.conn, fl, P.mat, false, S.b.data());
     // column index (CSC view) of the CSR: for each column the (row, k) entries in row order
     std::vector<int64_t> ccnt(S.n + 1, 0);
     for (size_t k = 0; k < S.colIdx.size(); ++k) ccnt[S.colIdx[k] + 1]++;
@@ -854,7 +1045,7 @@ template <int D>
 static void assembleAndBC(const Params& P, const std::vector<uint8_t>& fl, const double* pos, const double* qPrev,
                           const double* qIter, const double* dirVals, CsrSys& S) {
     buildAbPSPG<D>(P, fl, pos, qPrev, qIter, S);
-    applyBCPSPG<D>(P, fl, qPrev, dirVals, S);
+    applyBCPSPG<D>(P, fl, pos, qPrev, dirVals, S);
 }
 
 // Residual callback of the PSPG Picard loop (MomContEquationPSPG.inl:314-372).  kind 2
@@ -1179,7 +1370,11 @@ static void wcStep(const WCParams& W, double dt, double* pos, double* v, double*
     std::vector<double> invMv(D * N), F(D * N);
     wcMomAssemble<D>(W, pos, v, p, rho, invMv.data(), F.data());
     for (int64_t i = 0; i < D * N; ++i) invMv[i] = 1 / invMv[i];
-    for (int64_t nn = 0; nn < N; ++nn) {  // m_applyBC: MomEquation.inl:305-374 (gamma = 0)
+    {  // the FST facet loop of m_applyBC (MomEquation.inl:312-335): facets with every node on the surface
+        Material wm;
+        addFST<D>(pos, N, W.conn, fl, wm, true, F.data());
+    }
+    for (int64_t nn = 0; nn < N; ++nn) {  // m_applyBC: MomEquation.inl:336-374
         const uint8_t f = fl[nn];
         if ((f & FREE) && !(f & BOUND)) {
             for (int d = 0; d < D; ++d) {
@@ -1256,73 +1451,6 @@ struct HeatParams {
 };
 
 // facet quadrature, nGPLD = 3 (HeatEquation.inl:12-23): Mesh.cpp:342-530 for dimension dim-1
-template <int D> struct QuadLD;
-template <> struct QuadLD<2> {  // segments on [-1, 1], refSize 2 (Mesh.cpp:479)
-    static constexpr double refSize = 2.0;
-    double N[3][2], w[3] = {5.0 / 18.0, 8.0 / 18.0, 5.0 / 18.0};  // Mesh.cpp:429
-    QuadLD() {
-        const double xi[3] = {-std::sqrt(3.0 / 5.0), 0, std::sqrt(3.0 / 5.0)};  // Mesh.cpp:356-358
-        for (int g = 0; g < 3; ++g) {
-            N[g][0] = (1 - xi[g]) / 2;  // Mesh.cpp:506-507
-            N[g][1] = (1 + xi[g]) / 2;
-        }
-    }
-};
-template <> struct QuadLD<3> {  // triangles, refSize 0.5 (Mesh.cpp:482)
-    static constexpr double refSize = 0.5;
-    double N[3][3], w[3] = {1.0 / 3.0, 1.0 / 3.0, 1.0 / 3.0};  // Mesh.cpp:451
-    QuadLD() {
-        const double gp[3][2] = {{1.0 / 6.0, 1.0 / 6.0}, {1.0 / 6.0, 2.0 / 3.0}, {2.0 / 3.0, 1.0 / 6.0}};
-        for (int g = 0; g < 3; ++g) {
-            N[g][0] = 1 - gp[g][0] - gp[g][1];
-            N[g][1] = gp[g][0];
-            N[g][2] = gp[g][1];
-        }
-    }
-};
-
-// Facet::computeJ / computeDetJ (Facet.cpp:15-77) and computeNormal (Facet.cpp:130-210)
-template <int D>
-static void facetGeo(const double* pos, int64_t N, const int* fn, int out, double& detJ, double nrm[3]) {
-    auto P = [&](int node, int k) { return pos[node + k * N]; };
-    nrm[0] = nrm[1] = nrm[2] = 0;
-    if (D == 2) {
-        const double J00 = (P(fn[1], 0) - P(fn[0], 0)) / 2, J10 = (P(fn[1], 1) - P(fn[0], 1)) / 2;
-        detJ = std::sqrt(J00 * J00 + J10 * J10);
-        double nx = P(fn[1], 1) - P(fn[0], 1), ny = P(fn[0], 0) - P(fn[1], 0);
-        const double vx = P(out, 0) - P(fn[0], 0), vy = P(out, 1) - P(fn[0], 1);
-        if (nx * vx + ny * vy > 0) {
-            nx = -nx;
-            ny = -ny;
-        }
-        const double nn = std::sqrt(nx * nx + ny * ny);
-        nrm[0] = nx / nn;
-        nrm[1] = ny / nn;
-    } else {
-        double J[3][2];
-        for (int k = 0; k < 3; ++k) {
-            J[k][0] = P(fn[1], k) - P(fn[0], k);
-            J[k][1] = P(fn[2], k) - P(fn[0], k);
-        }
-        const double g0 = J[1][0] * J[2][1] - J[1][1] * J[2][0], g1 = J[2][0] * J[0][1] - J[2][1] * J[0][0],
-                     g2 = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-        detJ = std::sqrt(g0 * g0 + g1 * g1 + g2 * g2);
-        double a[3], b[3], n[3], v[3];
-        for (int k = 0; k < 3; ++k) {
-            a[k] = P(fn[1], k) - P(fn[0], k);
-            b[k] = P(fn[2], k) - P(fn[0], k);
-            v[k] = P(out, k) - P(fn[0], k);
-        }
-        n[0] = a[1] * b[2] - a[2] * b[1];
-        n[1] = a[2] * b[0] - a[0] * b[2];
-        n[2] = a[0] * b[1] - a[1] * b[0];
-        if (v[0] * n[0] + v[1] * n[1] + v[2] * n[2] > 0)
-            for (double& c : n) c *= -1.0;
-        const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
-        for (int k = 0; k < 3; ++k) nrm[k] = n[k] / nn;
-    }
-}
-
 // m_buildAb: HeatEquation.inl:209-280 (triplets M then dt L per node pair of rows that are
 // neither Dirichlet nor free; unit diagonals pushed for Dirichlet or free nodes; b = sum of
 // M theta_prev per element; setFromTriplets as in buildAbPSPG, the default-triplet padding
@@ -1704,7 +1832,9 @@ static void buildFracVApp(const Params& P, const std::vector<uint8_t>& fl, const
     csrFromTriplets(N, iL, F.L);
     F.MK.b.assign(D * N, 0.0);
     for (const auto& db : ib) F.MK.b[db.first] += db.second;
-    // m_applyBCVAppStep (gamma = 0: no FST)
+    // m_applyBCVAppStep without its FST facet loop (surface tension is not restated for this
+    // solver: its pressure correction has no free-surface condition), then the free and
+    // Dirichlet rows
     CsrSys& S = F.MK;
     std::vector<int64_t> ccnt(S.n + 1, 0);
     for (int c : S.colIdx) ccnt[c + 1]++;
@@ -1934,10 +2064,10 @@ orc_sys* orc_pspg_assemble(int dim, int64_t N, int64_t E, const int* conn, const
     auto* h = new orc_sys();
     if (dim == 2) {
         buildAbPSPG<2>(P, fl, pos, q_prev, q_iter, h->S);
-        if (apply_bc) applyBCPSPG<2>(P, fl, q_prev, dir_vals, h->S);
+        if (apply_bc) applyBCPSPG<2>(P, fl, pos, q_prev, dir_vals, h->S);
     } else {
         buildAbPSPG<3>(P, fl, pos, q_prev, q_iter, h->S);
-        if (apply_bc) applyBCPSPG<3>(P, fl, q_prev, dir_vals, h->S);
+        if (apply_bc) applyBCPSPG<3>(P, fl, pos, q_prev, dir_vals, h->S);
     }
     return h;
 }
@@ -2227,6 +2357,15 @@ void orc_set_material(int problem, double tau0, double m_reg) {
     g_mat.problem = problem;
     g_mat.tau0 = tau0;
     g_mat.mReg = m_reg;
+}
+
+// Surface tension: F boundary facets (nodes F x dim, element F), gamma and DgammaDT (F = 0 clears)
+void orc_set_facets(int64_t F, int dim, const int* nodes, const int* elem, double gamma, double dgdT) {
+    g_fst.F = F;
+    g_fst.nodes.assign(nodes, nodes + F * dim);
+    g_fst.elem.assign(elem, elem + F);
+    g_fst.gamma = gamma;
+    g_fst.dgdT = dgdT;
 }
 
 // Boussinesq factors: thermal expansion alpha, reference temperature Tr, node temperatures

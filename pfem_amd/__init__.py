@@ -146,6 +146,12 @@ class Case:
         check(lib().pfem_case_set_mesh_from(self._h, mesher._h))
         self.N, self.E, _ = mesher.sizes()
 
+    def engine_stats(self) -> dict:
+        """Diagnostics of the case's own engine: ticks, graph replays and captures, peak in flight."""
+        o = (C.c_int64 * 4)()
+        check(lib().pfem_case_engine_stats(self._h, o))
+        return dict(ticks=o[0], graph_launches=o[1], graph_captures=o[2], peak_inflight=o[3])
+
     def setup_times(self) -> dict:
         """The last set_mesh: seconds in the pattern, ELL structure, multigrid hierarchy and in
         total, the smallest element shape quality (3D) and the smoother damping it implies."""
@@ -190,6 +196,14 @@ class Case:
     def set_dirichlet(self, vals):
         vals = np.ascontiguousarray(vals, dtype=np.float64).reshape(-1)
         check(lib().pfem_case_set_dirichlet(self._h, _dp(vals)))
+
+    def set_surface_facets(self, nodes, elem):
+        """The mesh's boundary facets (F x dim nodes, F elements) for the surface-tension term
+        (pfem_case_set_facets); set by set_mesh_from / remesh when the case has gamma > 0."""
+        nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+        elem = np.ascontiguousarray(elem, dtype=np.int32)
+        F = len(elem)
+        check(lib().pfem_case_set_facets(self._h, F, _ip(nodes) if F else None, _ip(elem) if F else None))
 
     def footprint(self) -> int:
         b = C.c_int64(0)

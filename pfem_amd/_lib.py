@@ -52,7 +52,7 @@ EXPORTS = [
     "pfem_mesher_get_nodes", "pfem_mesher_get_elements", "pfem_mesher_get_facets", "pfem_mesher_stats",
     "pfem_case_remesh", "pfem_case_set_mesh_from", "pfem_case_setup_times", "pfem_heat_set_facets",
     "pfem_heat_facet_points", "pfem_heat_set_flux", "pfem_heat_set_dirichlet", "pfem_heat_assemble",
-    "pfem_heat_system_nnz", "pfem_heat_get_system", "pfem_heat_step", "pfem_frac_step",
+    "pfem_heat_system_nnz", "pfem_heat_get_system", "pfem_heat_step", "pfem_frac_step", "pfem_case_set_facets", "pfem_case_engine_stats",
 ]
 
 
@@ -202,6 +202,8 @@ def lib():
         "pfem_heat_get_system": [_vp, _ip, _ip, _dp, _dp],
         "pfem_heat_step": [_vp, C.c_double, C.POINTER(StepStats)],
         "pfem_frac_step": [_vp, C.c_double, C.POINTER(StepStats)],
+        "pfem_case_set_facets": [_vp, C.c_int32, _ip, _ip],
+        "pfem_case_engine_stats": [_vp, C.POINTER(C.c_int64)],
     }
     for name, args in sig.items():
         fn = getattr(L, name)

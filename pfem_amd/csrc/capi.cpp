@@ -350,6 +350,7 @@ size_t persistent_bytes(pfem_case* c) {
                       &F.dv, &F.v_it, &F.v_itp, &F.p_itp, &F.pos_save, &F.res_part, &F.r, &F.z, &F.p, &F.q, &F.part,
                       &F.scal})
         b += p->bytes;
+    for (DevBuf* p : {&c->fst.nodes, &c->fst.elem, &c->fst.ptr, &c->fst.ent}) b += p->bytes;
     for (auto& L : c->amg.lev)
         for (DevBuf* p : {&L.nbr_ptr, &L.nbr_idx, &L.node_info, &L.self_rank, &L.agg_ptr, &L.agg_nodes, &L.fine_agg,
                           &L.gal_ptr, &L.gal_src, &L.ap_ptr, &L.ap_gptr, &L.ap_gsrc, &L.ap_soff,
@@ -525,16 +526,17 @@ int pfem_case_create(pfem_ctx* ctx, const pfem_case_desc* desc, pfem_case** out)
         PFEM_REQUIRE(d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_WC || d.kind == PFEM_KIND_HEAT ||
                          d.kind == PFEM_KIND_FRACSTEP || d.kind == PFEM_KIND_BOUSSINESQ,
                      PFEM_E_INVALID, "unknown case kind");
-        PFEM_REQUIRE(std::fabs(d.gamma) < 1e-15, PFEM_E_UNSUPPORTED,
-                     "surface tension (gamma > 0) is not supported: the FST facet term is not implemented");
+        PFEM_REQUIRE(d.gamma >= 0, PFEM_E_INVALID, "gamma must be >= 0");
+        // the fractional step's pressure correction has no free-surface condition, so the only
+        // flows the FST term acts on have no consistent pressure system there
+        PFEM_REQUIRE(d.kind != PFEM_KIND_FRACSTEP || d.gamma < 1e-15, PFEM_E_UNSUPPORTED,
+                     "surface tension is not supported by the fractional-step solver");
         PFEM_REQUIRE(d.mu >= 0, PFEM_E_INVALID, "mu must be >= 0");
         if (d.kind == PFEM_KIND_BOUSSINESQ) {
             PFEM_REQUIRE(d.cv > 0 && d.k_cond >= 0 && d.h_conv >= 0 && d.eps_rad >= 0, PFEM_E_INVALID,
                          "heat needs cv > 0, k >= 0, h >= 0, epsRad >= 0");
             PFEM_REQUIRE(d.problem == PFEM_PROBLEM_NEWTONIAN, PFEM_E_INVALID, "a Boussinesq fluid is Newtonian");
-            // MomContEquationPSPG.inl:160: the FST facet term also runs when DgammaDT > 0
-            PFEM_REQUIRE(std::fabs(d.dgamma_dt) < 1e-15, PFEM_E_UNSUPPORTED,
-                         "DgammaDT > 0 needs the surface-tension facet term, which is not implemented");
+            PFEM_REQUIRE(d.dgamma_dt >= 0, PFEM_E_INVALID, "DgammaDT must be >= 0");
         }
         if (d.kind == PFEM_KIND_PSPG || d.kind == PFEM_KIND_BOUSSINESQ) {
             PFEM_REQUIRE(d.rho > 0, PFEM_E_INVALID, "rho must be > 0");
@@ -620,6 +622,7 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
         c->has_mesh = false;
         c->ws.clear();
         c->cfl_valid = false;
+        c->fst.F = 0;  // the facets of the previous mesh no longer apply
         c->moving_dirichlet = false;
         for (int32_t i = 0; i < N; ++i)
             if ((flags[i] & PFEM_NODE_BOUND) && (flags[i] & PFEM_NODE_DIRICHLET) && !(flags[i] & PFEM_NODE_FIXED))
@@ -810,6 +813,21 @@ int pfem_case_footprint(pfem_case* c, int64_t* bytes) {
     return guarded([&] {
         PFEM_REQUIRE(c != nullptr && bytes != nullptr, PFEM_E_INVALID, "null argument");
         *bytes = (int64_t)(persistent_bytes(c) + pfem::workspace_bytes(c));
+        return PFEM_OK;
+    });
+}
+
+int pfem_case_engine_stats(pfem_case* c, int64_t* out4) {
+    return guarded([&] {
+        PFEM_REQUIRE(c != nullptr && out4 != nullptr, PFEM_E_INVALID, "null argument");
+        for (int k = 0; k < 4; ++k) out4[k] = 0;
+        if (!c->engine) return PFEM_OK;
+        long long g[2];
+        c->engine->graph_stats(g);
+        out4[0] = c->engine->ticks;
+        out4[1] = g[0];
+        out4[2] = g[1];
+        out4[3] = c->engine->peak_inflight;
         return PFEM_OK;
     });
 }
@@ -1670,6 +1688,56 @@ int pfem_heat_step(pfem_case* c, double dt, pfem_step_stats* stats) {
     });
 }
 
+// ---- surface-tension facets ---------------------------------------------------------------------
+int pfem_case_set_facets(pfem_case* c, int32_t F, const int32_t* nodes, const int32_t* elem) {
+    return guarded([&] {
+        require_mesh(c);
+        PFEM_REQUIRE(pfem::pspg_like(c) || c->d.kind == PFEM_KIND_WC, PFEM_E_STATE,
+                     "surface tension applies to PSPG, Boussinesq and WC cases");
+        PFEM_REQUIRE(F >= 0 && (F == 0 || (nodes && elem)), PFEM_E_INVALID, "invalid facets");
+        const int D = c->dim, ne = D + 1, N = c->N, E = c->E;
+        c->fst.F = 0;
+        if (F == 0) return PFEM_OK;
+        PFEM_HIP(hipSetDevice(c->ctx->device));
+        PFEM_REQUIRE((long long)F < (1LL << 30), PFEM_E_INVALID, "too many facets");
+        // the element connectivity (AoS) to check that each facet lies on its element
+        std::vector<int> soa((size_t)ne * E);
+        PFEM_HIP(hipMemcpyAsync(soa.data(), c->pat.conn.p, sizeof(int) * soa.size(), hipMemcpyDeviceToHost, c->stream));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        std::vector<int> cnt((size_t)N + 1, 0);
+        for (int32_t f = 0; f < F; ++f) {
+            const int e = elem[f];
+            PFEM_REQUIRE(e >= 0 && e < E, PFEM_E_INVALID, "facet element out of range");
+            for (int k = 0; k < D; ++k) {
+                const int n = nodes[(size_t)f * D + k];
+                bool in = false;
+                for (int a = 0; a < ne; ++a) in = in || soa[(size_t)a * E + e] == n;
+                PFEM_REQUIRE(n >= 0 && n < N && in, PFEM_E_INVALID, "a facet node is not a node of its element");
+            }
+            for (int a = 0; a < ne; ++a) cnt[(size_t)soa[(size_t)a * E + e] + 1]++;
+        }
+        for (int i = 0; i < N; ++i) cnt[i + 1] += cnt[i];
+        std::vector<int> cur(cnt.begin(), cnt.end() - 1);
+        std::vector<unsigned> ent((size_t)cnt[N]);
+        for (int32_t f = 0; f < F; ++f)  // facet order within each node's list
+            for (int a = 0; a < ne; ++a) {
+                const int n = soa[(size_t)a * E + elem[f]];
+                ent[(size_t)cur[n]++] = ((unsigned)f << 2) | (unsigned)a;
+            }
+        auto up = [&](pfem::DevBuf& d, const void* src, size_t bytes) {
+            d.reserve(bytes);
+            PFEM_HIP(hipMemcpyAsync(d.p, src, bytes, hipMemcpyHostToDevice, c->stream));
+        };
+        up(c->fst.nodes, nodes, sizeof(int) * (size_t)F * D);
+        up(c->fst.elem, elem, sizeof(int) * (size_t)F);
+        up(c->fst.ptr, cnt.data(), sizeof(int) * cnt.size());
+        up(c->fst.ent, ent.data(), sizeof(unsigned) * std::max<size_t>(ent.size(), 1));
+        PFEM_HIP(hipStreamSynchronize(c->stream));
+        c->fst.F = F;
+        return PFEM_OK;
+    });
+}
+
 // ---- fractional step ---------------------------------------------------------------------------
 int pfem_frac_step(pfem_case* c, double dt, pfem_step_stats* stats) {
     return guarded([&] {
@@ -1931,7 +1999,16 @@ int pfem_case_set_mesh_from(pfem_case* c, pfem_mesher* m) {
         M.get_nodes(pos.data(), st.data(), fl.data(), nullptr);
         int rc = pfem_case_set_mesh(c, N, E, M.elems().data(), fl.data(), pos.data());
         if (rc != PFEM_OK) return rc;
-        return pfem_case_set_state(c, st.data());
+        rc = pfem_case_set_state(c, st.data());
+        if (rc != PFEM_OK) return rc;
+        // the mesh's facets for the surface-tension term (Mesh::m_facets after the remesh)
+        const bool bq = c->d.kind == PFEM_KIND_BOUSSINESQ;
+        if ((pfem::pspg_like(c) || c->d.kind == PFEM_KIND_WC) &&
+            !(c->d.gamma < 1e-15 && (!bq || c->d.dgamma_dt < 1e-15))) {
+            const int F = M.n_facets();
+            if (F > 0) rc = pfem_case_set_facets(c, F, M.facets().data(), M.facet_elem().data());
+        }
+        return rc;
     });
 }
 

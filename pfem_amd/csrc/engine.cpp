@@ -11,6 +11,7 @@
 #include "runtime.hpp"
 
 #include <algorithm>
+#include <deque>
 #include <chrono>
 #include <cmath>
 #include <cstdlib>
@@ -226,6 +227,28 @@ WsLayout ws_layout(const pfem_case* c) {
 }
 }  // namespace
 
+// The case's surface-tension facets as the kernels see them (fst.F == 0: no term).  all_nodes:
+// the WC rule (a facet acts when every node is on the free surface), else the PSPG one (any node).
+FstView fst_view(const pfem_case* c, bool all_nodes) {
+    FstView v;
+    const bool bq = c->d.kind == PFEM_KIND_BOUSSINESQ;  // other problems read gamma alone
+    const bool on = !(c->d.gamma < 1e-15 && (!bq || c->d.dgamma_dt < 1e-15));
+    if (!on || c->fst.F == 0) return v;
+    v.F = c->fst.F;
+    v.all_nodes = all_nodes ? 1 : 0;
+    v.nodes = c->fst.nodes.as<int>();
+    v.elem = c->fst.elem.as<int>();
+    v.ptr = c->fst.ptr.as<int>();
+    v.ent = c->fst.ent.as<unsigned>();
+    v.gamma = c->d.gamma;
+    if (c->d.kind == PFEM_KIND_BOUSSINESQ) {
+        v.T = c->heat.T.as<double>();
+        v.dgdT = c->d.dgamma_dt;
+        v.t_ref = c->d.t_ref;
+    }
+    return v;
+}
+
 size_t workspace_bytes(const pfem_case* c) {
     if (!pspg_like(c) || !c->has_mesh) return 0;
     return ws_layout(c).total;
@@ -349,8 +372,26 @@ struct Engine::Impl {
     bool prof = false;
     hipEvent_t ev[8] = {};
     hipEvent_t pev[128] = {};
+    // the chunk of BiCGSTAB iterations (with the V-cycles between their phases) captured in a
+    // HIP graph and replayed whenever a tick repeats it exactly: the same work tables at the
+    // same device addresses, the same views, chunk length and iteration parity.  A chunk is
+    // captured the second time its key comes up among the last kGraphSeen eager chunks (a table
+    // set that changes every tick, as a busy server's does, is launched kernel by kernel and
+    // never pays for a capture).
+    bool graphs = false;
+    struct Captured {
+        std::vector<long long> key;
+        hipGraphExec_t exec = nullptr;
+    };
+    std::deque<Captured> gcache;  // the most recent captures, at most kGraphCache
+    static constexpr size_t kGraphCache = 8;
+    std::deque<std::vector<long long>> seen;  // keys of the recent eager chunks, at most kGraphSeen
+    static constexpr size_t kGraphSeen = 16;
+    long long graph_launches = 0, graph_captures = 0;
 
     ~Impl() {
+        for (auto& g : gcache)
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : pev)
@@ -416,6 +457,8 @@ Engine::Engine(int device, hipStream_t s, int slots, Arena* arena) : p_(new Impl
         if (w > 0.0 && w <= 1.5) p_->amg_omega = w;
     }
     if (const char* e = std::getenv("PFEM_AMG_REUSE")) p_->amg_reuse = std::atoi(e) != 0;
+    // PFEM_GRAPH=1 replays repeated BiCGSTAB iteration chunks from captured HIP graphs
+    if (const char* e = std::getenv("PFEM_GRAPH")) p_->graphs = std::atoi(e) != 0;
     // PFEM_AMG_ALPHA overrides the coarse-correction scaling (tuning)
     if (const char* e = std::getenv("PFEM_AMG_ALPHA")) {
         const double a = std::atof(e);
@@ -492,6 +535,7 @@ void Engine::Impl::load_views(Req* r) {
             pv.alpha = c->d.alpha_exp;
             pv.t_ref = c->d.t_ref;
         }
+        pv.fst = fst_view(c, false);
 
         PicardView& pc = h_pic[slot];
         pc = PicardView{};
@@ -623,6 +667,7 @@ void Engine::Impl::load_views(Req* r) {
             V.dt = r->dt;
             V.sc2 = r->sc * r->sc;
             V.meduri = c->d.meduri;
+            V.fst = fst_view(c, true);
         };
         if (c->d.wc_state == PFEM_STATE_FP32) {
             set(h_wc32[slot], c->wstate.as<float>());
@@ -890,6 +935,11 @@ void Engine::abort_all(int code, std::vector<Req*>& done) {
     waitq_.clear();
 }
 
+void Engine::graph_stats(long long out[2]) const {
+    out[0] = p_->graph_launches;
+    out[1] = p_->graph_captures;
+}
+
 void Engine::drain(std::vector<Req*>& done) {
     while (!idle()) tick(done);
 }
@@ -1143,21 +1193,71 @@ void Engine::tick(std::vector<Req*>& done) {
         vcycle_type(vamg64, T_AMG, which);
         vcycle_type(vamg32, T_AMG + A_N, which);
     };
-    for (int it = 0; it < chunk; ++it) {
-        const int p = (I.par + it) & 1;
-        for (int phase = 0; phase < 5; ++phase) {
-            for (int li = 0; li < 3; ++li) {
-                if (NT(T_K64 + li))
-                    launch_kry_phase<double>(phase, kLanes[li], vk64, T(T_K64 + li), NT(T_K64 + li), p, s,
-                                             (phase == 1 && prof_w == 0) ? I.pev + 2 * it : nullptr);
-                if (NT(T_K32 + li))
-                    launch_kry_phase<float>(phase, kLanes[li], vk32, T(T_K32 + li), NT(T_K32 + li), p, s,
-                                            (phase == 1 && prof_w == 1) ? I.pev + 2 * it : nullptr);
+    auto run_chunk = [&]() {
+        for (int it = 0; it < chunk; ++it) {
+            const int p = (I.par + it) & 1;
+            for (int phase = 0; phase < 5; ++phase) {
+                for (int li = 0; li < 3; ++li) {
+                    if (NT(T_K64 + li))
+                        launch_kry_phase<double>(phase, kLanes[li], vk64, T(T_K64 + li), NT(T_K64 + li), p, s,
+                                                 (phase == 1 && prof_w == 0) ? I.pev + 2 * it : nullptr);
+                    if (NT(T_K32 + li))
+                        launch_kry_phase<float>(phase, kLanes[li], vk32, T(T_K32 + li), NT(T_K32 + li), p, s,
+                                                (phase == 1 && prof_w == 1) ? I.pev + 2 * it : nullptr);
+                }
+                if (phase == 0) vcycle(0);
+                if (phase == 2) vcycle(1);
             }
-            if (phase == 0) vcycle(0);
-            if (phase == 2) vcycle(1);
+        }
+    };
+    bool replayed = false;
+    if (I.graphs && chunk > 0 && prof_w < 0) {
+        // everything the chunk's launches capture: its length and parity, the base addresses of
+        // the tables and views, and the tables' layout and contents
+        std::vector<long long> key;
+        key.reserve(16 + T_NTAB + 4 * total);
+        key.push_back(chunk);
+        key.push_back(I.par);
+        for (const void* p : {I.d_tab.p, I.d_k64.p, I.d_k32.p, I.d_amg64.p, I.d_amg32.p})
+            key.push_back((long long)(uintptr_t)p);
+        for (int t = 0; t <= T_NTAB; ++t) key.push_back((long long)off[t]);
+        for (size_t q = 0; q < total; ++q) {
+            const BlockTask& b = I.h_tab[q];
+            key.push_back(b.case_id);
+            key.push_back(b.block);
+            key.push_back(b.nblocks);
+            key.push_back(b.pad);
+        }
+        hipGraphExec_t exec = nullptr;
+        for (auto& g : I.gcache)
+            if (g.key == key) exec = g.exec;
+        bool again = false;
+        for (const auto& k : I.seen) again = again || k == key;
+        if (!exec && again) {
+            hipGraph_t g = nullptr;
+            PFEM_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+            run_chunk();
+            PFEM_HIP(hipStreamEndCapture(s, &g));
+            const hipError_t ie = hipGraphInstantiate(&exec, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            PFEM_HIP(ie);
+            if (I.gcache.size() == Impl::kGraphCache) {
+                PFEM_HIP(hipGraphExecDestroy(I.gcache.front().exec));
+                I.gcache.pop_front();
+            }
+            I.gcache.push_back(Impl::Captured{key, exec});
+            ++I.graph_captures;
+        }
+        if (exec) {
+            PFEM_HIP(hipGraphLaunch(exec, s));
+            replayed = true;
+            ++I.graph_launches;
+        } else {
+            if (I.seen.size() == Impl::kGraphSeen) I.seen.pop_front();
+            I.seen.push_back(std::move(key));
         }
     }
+    if (!replayed) run_chunk();
     if (nrun) I.par = (I.par + chunk) & 1;
     const WcView<double>* vw64 = I.d_wc64.as<WcView<double>>();
     const WcView<float>* vw32 = I.d_wc32.as<WcView<float>>();

@@ -30,6 +30,7 @@
 // sorted neighbours and their velocity components, which subtracts into b[(j, s)] in the
 // reference's order (Dirichlet node index, then component).
 #include "common.hpp"
+#include "fst.hpp"
 #include "views.hpp"
 
 namespace pfem {
@@ -487,6 +488,23 @@ __global__ __launch_bounds__(kBlock) void k_pspg_bc(const PspgView* __restrict__
         const uint8_t f = (uint8_t)inf.w;
         const bool isfree = f & NODE_FREE, bound = f & NODE_BOUND;
         const bool mom = !(bound || isfree), cont = !isfree;
+        // surface tension first (MomContEquationPSPG.inl:155-186): the FST entries of the facets
+        // whose element holds j, onto its momentum rows, before the rows are overwritten below
+        if (V.fst.F > 0 && V.fst.ptr[j] < V.fst.ptr[j + 1]) {
+            auto flag_of = [&](int n) { return (uint8_t)V.node_info[n].w; };
+            if (D == 2) {
+                double bb[2] = {V.b[j], V.b[(long long)N + j]};
+                fst_add<2>(V.fst, V.conn, V.E, V.pos, N, j, flag_of, bb);
+                V.b[j] = bb[0];
+                V.b[(long long)N + j] = bb[1];
+            } else {
+                double bb[3] = {V.b[j], V.b[(long long)N + j], V.b[2LL * N + j]};
+                fst_add<3>(V.fst, V.conn, V.E, V.pos, N, j, flag_of, bb);
+                V.b[j] = bb[0];
+                V.b[(long long)N + j] = bb[1];
+                V.b[2LL * N + j] = bb[2];
+            }
+        }
         // the same entries in the ELL copy: value (s, t) of node j at rank k
         const long long ebase = V.vals_ell ? (long long)V.ell_soff[j / kEllSlice] * nf * nf * kEllSlice + j % kEllSlice : 0;
         auto ell_set = [&](int k, int s, int t, double v) {

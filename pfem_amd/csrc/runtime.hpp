@@ -182,6 +182,8 @@ public:
     void abort_all(int code, std::vector<Req*>& done);
 
     void enable_profiling(bool on);
+    // captured iteration chunks: {graph launches, captures}
+    void graph_stats(long long out[2]) const;
     double prof_ms[2] = {0, 0};
     long long prof_count[2] = {0, 0};
     long long ticks = 0;
@@ -246,6 +248,11 @@ struct pfem_case {
         int F = 0;
         bool has_q = false;
     } heat;
+    // surface-tension facets (pfem_case_set_facets): nodes, elements, node -> (facet, local index)
+    struct FstBufs {
+        pfem::DevBuf nodes, elem, ptr, ent;
+        int F = 0;
+    } fst;
     // fractional-step cases: element records, the three graph matrices, right-hand sides, the
     // Picard iterate and the CG vectors
     struct FracBufs {

@@ -7,6 +7,18 @@
 
 namespace pfem {
 
+// Surface-tension facets of a case (fst.hpp gathers them per node).
+struct FstView {
+    int F = 0;                  // facets (0: no surface tension)
+    int all_nodes = 0;          // 1: a facet acts when every node is on the free surface (WC), 0: any node
+    const int* nodes = nullptr; // [F][dim] facet nodes (Facet::m_nodesIndexes order)
+    const int* elem = nullptr;  // [F] element of each facet
+    const int* ptr = nullptr;   // [N+1] node -> entries
+    const unsigned* ent = nullptr;  // f << 2 | a, a the node's local index in the facet's element
+    const double* T = nullptr;  // Boussinesq node temperatures (null: the factor is gamma)
+    double gamma = 0, dgdT = 0, t_ref = 0;
+};
+
 // PSPG assembly + boundary conditions (MomContEquationPSPG.inl:7-235).
 struct PspgView {
     int dim, N, E, mom_nnz;
@@ -42,6 +54,7 @@ struct PspgView {
     // 1 - alpha (T - Tr) at each Gauss point, and H by the same factor
     const double* T;
     double alpha, t_ref;
+    FstView fst;  // surface tension (fst.F == 0: none)
 };
 
 // The assembled PSPG matrix seen through its node structure.  Values are the CSR values
@@ -234,6 +247,7 @@ struct WcView {
     double dt;   // time step of the current launch
     double sc2;  // securityCoeff^2 of the CFL reduction
     int meduri, pad1;
+    FstView fst;  // surface tension (fst.F == 0: none)
 };
 
 // Picard loop helpers (MomContEquationPSPG.inl:293-372).

@@ -24,6 +24,7 @@
 //   (K V)_(i,s)             = mu |e| (B^T ddev B V)_(i,s),  ddev the deviatoric operator
 //   (D^T P)_(i,s)           = |e|/(d+1) d_sN_i sum P
 #include "common.hpp"
+#include "fst.hpp"
 #include "p1geom.hpp"
 #include "views.hpp"
 
@@ -336,13 +337,32 @@ __global__ __launch_bounds__(kBlock) void k_wc_mom_node(const WcView<S>* __restr
     for (int i = task.block * kBlock + threadIdx.x; i < N; i += task.nblocks * kBlock) {
         const uint8_t f = V.flags[i];
         const double invm = 1.0 / V.m_mom[i];
+        // the force with the surface-tension entries of the facets whose element holds i
+        // (WCompNewton/MomEquation.inl:312-335, added before the boundary conditions)
+        double fm[3] = {0.0, 0.0, 0.0};
+        for (int k = 0; k < V.dim; ++k) fm[k] = V.f_mom[(long long)k * N + i];
+        if (V.fst.F > 0 && V.fst.ptr[i] < V.fst.ptr[i + 1]) {
+            auto flag_of = [&](int n) { return V.flags[n]; };
+            if (V.dim == 2) {
+                double bb[2] = {fm[0], fm[1]};
+                fst_add<2>(V.fst, V.conn, V.E, V.pos, N, i, flag_of, bb);
+                fm[0] = bb[0];
+                fm[1] = bb[1];
+            } else {
+                double bb[3] = {fm[0], fm[1], fm[2]};
+                fst_add<3>(V.fst, V.conn, V.E, V.pos, N, i, flag_of, bb);
+                fm[0] = bb[0];
+                fm[1] = bb[1];
+                fm[2] = bb[2];
+            }
+        }
         double u2 = 0.0;
         for (int k = 0; k < V.dim; ++k) {
             const long long q = (long long)k * N + i;
             double acc;
             if ((f & NODE_FREE) && !(f & NODE_BOUND)) acc = V.g[k];
             else if ((f & NODE_BOUND) && (f & NODE_DIRICHLET)) acc = V.dir_vals[q];
-            else acc = invm * V.f_mom[q];
+            else acc = invm * fm[k];
             V.a[q] = (S)acc;
             const S vs = (S)((double)V.v[q] + 0.5 * dt * acc);
             V.v[q] = vs;

@@ -87,8 +87,8 @@ def test_boussinesq_steps_match_oracle(ctx, dim, heat_first):
 
 def test_boussinesq_refusals(ctx):
     with pytest.raises(pfem_amd.PfemError) as e:
-        pfem_amd.BoussinesqIncompNewton(ctx, 2, DgammaDT=1e-3)
-    assert e.value.code == _lib.PFEM_E_UNSUPPORTED
+        pfem_amd.BoussinesqIncompNewton(ctx, 2, DgammaDT=-1e-3)
+    assert e.value.code == _lib.PFEM_E_INVALID
     with pytest.raises(pfem_amd.PfemError) as e:
         pfem_amd.BoussinesqIncompNewton(ctx, 2, problem="Bingham", tau0=1.0, mReg=10.0)
     assert e.value.code == _lib.PFEM_E_INVALID

@@ -234,7 +234,7 @@ def test_invalid_arguments_raise(ctx):
     with pytest.raises(pfem_amd.PfemError):
         pfem_amd.Case(ctx, 4)
     with pytest.raises(pfem_amd.PfemError):
-        pfem_amd.Case(ctx, 2, gamma=0.07)  # surface tension is not implemented
+        pfem_amd.Case(ctx, 2, gamma=-0.07)  # a negative surface tension
     mesh = mg.dambreak_2d()
     bad = mg.PfemMesh(2, mesh.pos, mesh.conn.copy(), mesh.flags, mesh.h)
     bad.conn[0, 0] = mesh.n_nodes + 5

@@ -47,6 +47,7 @@ MUTATIONS = {
 HEAT_PINS = ["tests/test_oracle_heat_pins.py"]
 FRAC_PINS = ["tests/test_oracle_frac_pins.py"]
 BQ_PINS = ["tests/test_oracle_boussinesq_pins.py"]
+FST_PINS = ["tests/test_oracle_fst_pins.py"]
 # (old, new, pin files) of the heat and fractional-step restatements
 MUTATIONS_PINNED = {
     # heat: L without the time step (HeatEquation.inl:245-250)
@@ -77,6 +78,15 @@ MUTATIONS_PINNED = {
                   "bq ? P.rho * (1 + P.mat.alpha * (Tg(Ng) - P.mat.Tr)) : P.rho", BQ_PINS),
     # Boussinesq: H factor left at 1 (MomContEquation.inl:191-196)
     "bq_h_factor": ("return bq ? 1 - P.mat.alpha * (Tg(Ng) - P.mat.Tr) : 1.0;", "return 1.0;", BQ_PINS),
+    # FST: a wrong entry of the 3D T(P) (MatricesBuilder.inl:205)
+    "fst_t_entry": ("T[0][1] = P[3] * P[3];", "T[0][1] = P[3] * P[4];", FST_PINS),
+    # FST: sign of the 2D shear projector entry (MatricesBuilder.inl:176)
+    "fst_p_shear": ("Pv[2] = -n[0] * n[1];", "Pv[2] = n[0] * n[1];", FST_PINS),
+    # FST: without the reference element size (MatricesBuilder.inl:400)
+    "fst_refsize": ("for (int c = 0; c < nd; ++c) FST[c] *= detJ * q.refSize;",
+                    "for (int c = 0; c < nd; ++c) FST[c] *= detJ;", FST_PINS),
+    # FST: the WC every-node rule used for PSPG too (MomContEquationPSPG.inl:162-171)
+    "fst_any_node": ("if (allNodes ? nfs < D : nfs == 0) continue;", "if (nfs < D) continue;", FST_PINS),
     # FracStep: free nodes keep v_prev without dt g (FracStep.inl:244-252)
     "frac_free_rhs": ("S.b[nn + d * N] = vPrevQ[nn + d * N] + dt * P.g[d];", "S.b[nn + d * N] = vPrevQ[nn + d * N];",
                       FRAC_PINS),
