@@ -44,14 +44,15 @@ __device__ __forceinline__ T in_at(const AmgView<T>& V, int which, long long q) 
 
 // ---- setup --------------------------------------------------------------------------------
 
-// Galerkin coarse operator of level l >= 1: each entry block (I, J) sums the finer blocks of
-// its gather list.  L lanes per coarse node, lane k owns entry k of the node's list.
-template <typename T, int L>
-__device__ __forceinline__ void galerkin_body(const AmgView<T>& V, const BlockTask& task) {
+// Galerkin coarse operator of level l >= 1: entry block (I, J) = P^T A P = the sum over the
+// members i of aggregate I of the finer level's A P block (i, J), gathered in member order
+// (A P already holds the finer blocks (i, j) summed over j in J, level 0's masked to full rows
+// and columns).  L lanes per coarse node, lane k owns entry k of the node's list.
+template <typename T, typename ST, int L>
+__device__ __forceinline__ void galerkin_body(const AmgView<T>& V, const EllMatT<ST>& F, const BlockTask& task) {
     constexpr int NPB = kBlock / L;
     const int lane = threadIdx.x % L;
-    const int nf = V.A.nf, D = nf - 1;
-    const NodeMat<T>& F = V.F;
+    const int nf = V.A.nf, D = nf - 1, nf2 = nf * nf;
     for (long long I = (long long)task.block * NPB + threadIdx.x / L; I < V.A.N; I += (long long)task.nblocks * NPB) {
         const int4 inf = V.A.info[I];
         const int deg = inf.z;
@@ -65,22 +66,14 @@ __device__ __forceinline__ void galerkin_body(const AmgView<T>& V, const BlockTa
                 for (int t = 0; t < kMaxNf; ++t) blk[s][t] = 0.0;
             for (int q = V.gal_ptr[mc]; q < V.gal_ptr[mc + 1]; ++q) {
                 const int2 src = V.gal_src[q];
-                const int i = src.x, kf = src.y;
-                const int4 fi = F.info[i];
-                const int j = F.nbr_idx[F.nbr_ptr[i] + kf];
-                const uint8_t ffi = (uint8_t)fi.w;
-                const uint8_t ffj = V.fmask ? (uint8_t)F.info[j].w : (uint8_t)0;
+                const long long c = src.x / kEllSlice, r = src.x % kEllSlice;
+                const ST* b = F.vals + ((long long)F.soff[c] + src.y) * nf2 * kEllSlice + r;
 #pragma unroll
                 for (int s = 0; s < kMaxNf; ++s) {
                     if (s >= nf) continue;
-                    if (V.fmask && !row_full_flags(ffi, s, D)) continue;
-                    const long long rs = row_start_of(fi, s, D, F.mom_nnz) + kf;
 #pragma unroll
-                    for (int t = 0; t < kMaxNf; ++t) {
-                        if (t >= nf) continue;
-                        if (V.fmask && !row_full_flags(ffj, t, D)) continue;
-                        blk[s][t] += (double)F.vals[rs + (long long)t * fi.z];
-                    }
+                    for (int t = 0; t < kMaxNf; ++t)
+                        if (t < nf) blk[s][t] += (double)b[(long long)(s * nf + t) * kEllSlice];
                 }
             }
 #pragma unroll
@@ -99,15 +92,21 @@ __device__ __forceinline__ void galerkin_body(const AmgView<T>& V, const BlockTa
     }
 }
 
+template <typename T, typename ST>
+__device__ __forceinline__ void galerkin_lanes(const AmgView<T>& V, const EllMatT<ST>& F, const BlockTask& task) {
+    if (V.lanes == 4) galerkin_body<T, ST, 4>(V, F, task);
+    else if (V.lanes == 8) galerkin_body<T, ST, 8>(V, F, task);
+    else galerkin_body<T, ST, 16>(V, F, task);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_amg_galerkin(const AmgView<T>* __restrict__ views,
                                                          const BlockTask* __restrict__ tasks) {
     const BlockTask task = tasks[blockIdx.x];
     if (task.case_id < 0) return;
     const AmgView<T>& V = views[task.case_id];
-    if (V.lanes == 4) galerkin_body<T, 4>(V, task);
-    else if (V.lanes == 8) galerkin_body<T, 8>(V, task);
-    else galerkin_body<T, 16>(V, task);
+    if (V.FAPf.vals) galerkin_lanes<T, float>(V, V.FAPf, task);
+    else galerkin_lanes<T, T>(V, V.FAP, task);
 }
 
 // Block-Jacobi inverse of a coarse level (the diagonal nf x nf block of each node).

@@ -7,12 +7,14 @@
 // per field: coarse DOF (I, s) is the sum (restriction) / the copy (prolongation) of the
 // DOFs (i, s) of the nodes i of aggregate I.  At the level-0 -> 1 transfer only DOFs whose
 // row is a full assembled row take part (identity rows of boundary and free nodes are left
-// to the smoother).  Coarse operators are Galerkin products A_{l+1} = P^T A_l P; in the
-// node-block layout every coarse entry block (I, J) is the sum of the fine blocks (i, j)
-// with i in I and j in J, gathered through a precomputed list (no atomics, fixed order).
-// The post-smoothing uses A P (finer rows, coarse columns), built the same way, so that the
+// to the smoother).  A_l P (finer rows, coarse columns) holds in its block (i, J) the sum of
+// the fine blocks (i, j) over i's neighbours j in J, gathered through a precomputed list (no
+// atomics, fixed order).  The coarse operators are the Galerkin products A_{l+1} = P^T (A_l P):
+// block (I, J) is the sum of the A P blocks (i, J) over the members i of I, gathered the same
+// way (a node's neighbours fall in a few aggregates, so this reads about a third of the fine
+// blocks a direct sum over (i, j) would).  The post-smoothing uses A P too, so that the
 // prolongated correction's residual A (w + P z) = A w + (A P) z costs a product with A P
-// (a node's neighbours fall in a few aggregates) instead of a second product with A.
+// instead of a second product with A.
 #pragma once
 
 #include <vector>
@@ -40,7 +42,8 @@ struct AmgLevel {
     DevBuf agg_nodes;   // int  [members]
     DevBuf fine_agg;    // int  [N_finer] aggregate of each finer node (-1: not aggregated)
     DevBuf gal_ptr;     // int  [graph_nnz+1] Galerkin gather list of each coarse entry
-    DevBuf gal_src;     // int2 [finer graph entries] {finer row node i, rank k in i's list}
+    DevBuf gal_src;     // int2 [ap_nnz] {finer node i, rank m of the entry's column aggregate
+                        //      in i's A P row}
     // A_finer P: rows of the finer level, columns of this level's aggregates
     int ap_nnz = 0;     // entries (finer node, aggregate)
     int ap_lanes = 4;

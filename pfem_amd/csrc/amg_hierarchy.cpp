@@ -123,40 +123,6 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
             C.idx[m] = (int)(keys[m] % na);
         }
         for (int I = 0; I < na; ++I) C.ptr[I + 1] += C.ptr[I];
-        // Galerkin gather lists: finer entries (i, k) grouped by coarse entry, in finer order
-        std::vector<int> gcount(C.idx.size() + 1, 0);
-        std::vector<int> centry(G.idx.size(), -1);
-        for (int i = 0; i < G.N; ++i) {
-            if (!active[i]) continue;
-            const int I = agg[i];
-            for (int m = G.ptr[i]; m < G.ptr[i + 1]; ++m) {
-                const int j = G.idx[m];
-                if (!active[j]) continue;
-                const int J = agg[j];
-                const int* b = C.idx.data() + C.ptr[I];
-                const int* e = C.idx.data() + C.ptr[I + 1];
-                const int mc = (int)(std::lower_bound(b, e, J) - C.idx.data());
-                centry[m] = mc;
-                gcount[mc + 1]++;
-            }
-        }
-        for (size_t m = 0; m + 1 < gcount.size(); ++m) gcount[m + 1] += gcount[m];
-        std::vector<int2> gsrc((size_t)gcount.back());
-        std::vector<int> cursor(gcount.begin(), gcount.end() - 1);
-        for (int i = 0; i < G.N; ++i)
-            for (int m = G.ptr[i]; m < G.ptr[i + 1]; ++m)
-                if (centry[m] >= 0) gsrc[cursor[centry[m]]++] = int2{i, m - G.ptr[i]};
-        // aggregate members
-        std::vector<int> aptr((size_t)na + 1, 0), anodes;
-        for (int i = 0; i < G.N; ++i)
-            if (active[i]) aptr[agg[i] + 1]++;
-        for (int I = 0; I < na; ++I) aptr[I + 1] += aptr[I];
-        anodes.resize((size_t)aptr[na]);
-        {
-            std::vector<int> cur(aptr.begin(), aptr.end() - 1);
-            for (int i = 0; i < G.N; ++i)
-                if (active[i]) anodes[cur[agg[i]]++] = i;
-        }
         // A P structure: for each finer node i, the distinct aggregates of its active
         // neighbours (sorted) and, per (i, J), the ranks k of those neighbours in i's list
         std::vector<int> ap_ptr((size_t)G.N + 1, 0), ap_idx, ap_gptr(1, 0), ap_gsrc;
@@ -182,6 +148,40 @@ void build_amg(const Pattern& pat, int dim, int coarse_target, hipStream_t strea
                 apmax = std::max(apmax, ap_ptr[i + 1] - ap_ptr[i]);
             }
             PFEM_REQUIRE(apmax <= 255, E_UNSUPPORTED, "a node's neighbours span more than 255 aggregates");
+        }
+        // Galerkin gather lists: the A P entries (i, m) of each coarse entry (I, J), I = agg[i]
+        // and J the aggregate of i's rank-m entry, in finer node order
+        std::vector<int> gcount(C.idx.size() + 1, 0);
+        std::vector<int> centry(ap_idx.size(), -1);
+        for (int i = 0; i < G.N; ++i) {
+            if (!active[i]) continue;
+            const int I = agg[i];
+            const int* b = C.idx.data() + C.ptr[I];
+            const int* e = C.idx.data() + C.ptr[I + 1];
+            for (int m = ap_ptr[i]; m < ap_ptr[i + 1]; ++m) {
+                const int* f = std::lower_bound(b, e, ap_idx[m]);
+                PFEM_REQUIRE(f != e && *f == ap_idx[m], E_STATE, "A P entry outside the coarse graph");
+                const int mc = (int)(f - C.idx.data());
+                centry[m] = mc;
+                gcount[mc + 1]++;
+            }
+        }
+        for (size_t m = 0; m + 1 < gcount.size(); ++m) gcount[m + 1] += gcount[m];
+        std::vector<int2> gsrc((size_t)gcount.back());
+        std::vector<int> cursor(gcount.begin(), gcount.end() - 1);
+        for (int i = 0; i < G.N; ++i)
+            for (int m = ap_ptr[i]; m < ap_ptr[i + 1]; ++m)
+                if (centry[m] >= 0) gsrc[cursor[centry[m]]++] = int2{i, m - ap_ptr[i]};
+        // aggregate members
+        std::vector<int> aptr((size_t)na + 1, 0), anodes;
+        for (int i = 0; i < G.N; ++i)
+            if (active[i]) aptr[agg[i] + 1]++;
+        for (int I = 0; I < na; ++I) aptr[I + 1] += aptr[I];
+        anodes.resize((size_t)aptr[na]);
+        {
+            std::vector<int> cur(aptr.begin(), aptr.end() - 1);
+            for (int i = 0; i < G.N; ++i)
+                if (active[i]) anodes[cur[agg[i]]++] = i;
         }
         // node descriptors of the coarse level (every row full)
         std::vector<int4> cinfo((size_t)na);

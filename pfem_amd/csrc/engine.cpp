@@ -750,8 +750,8 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
             V.w = arr(w.amg_lv[l][3]);
             V.t = arr(w.amg_lv[l][4]);
             V.z = arr(w.amg_lv[l][5]);
-            V.F = av[l - 1].A;
-            V.fmask = l == 1 ? 1 : 0;
+            V.FAP = av[l - 1].AP;  // set by level l-1 (the loop runs from the finest level)
+            V.FAPf = av[l - 1].APf;
             V.gal_ptr = L.gal_ptr.as<int>();
             V.gal_src = L.gal_src.as<int2>();
             V.self_rank = L.self_rank.as<int>();
@@ -1035,7 +1035,7 @@ void Engine::tick(std::vector<Req*>& done) {
         if (op & OP_COPY) append(tab[T_COPY], sl, c->N, kElemCap, (int)r->copy);
         if (op & OP_UPD) append(tab[T_UPD], sl, c->N, kElemCap);
         if (op & OP_ASM) {
-            append(tab[c->dim == 2 ? T_ASM2 : T_ASM3], sl, (long long)c->N * 16, kElemCap);
+            append(tab[c->dim == 2 ? T_ASM2 : T_ASM3], sl, (long long)c->N * kAsmLanes, kElemCap);
             append(tab[T_BC], sl, c->N, kElemCap);
             append_rows(tab[T_PRE], sl, c->n, c->pat.ell_nsl);
         }
@@ -1132,14 +1132,15 @@ void Engine::tick(std::vector<Req*>& done) {
     const AmgView<float>* vamg32 = I.d_amg32.as<AmgView<float>>();
     // multigrid setup of both hierarchy types (TA: first table of the type)
     auto amg_setup = [&](auto* v, int TA) {
-        for (int l = 1; l < kAmgMaxLevels; ++l) {  // level l's Galerkin sums read level l-1
-            if (!NT(TA + A_G + l)) continue;
-            launch_amg_galerkin(v, T(TA + A_G + l), NT(TA + A_G + l), s);
-            launch_amg_bjac(v, T(TA + A_B + l), NT(TA + A_B + l), s);
+        for (int l = 0; l < kAmgMaxLevels; ++l) {
+            // A P of level l reads level l's final values; level l+1's Galerkin sums read it
+            if (NT(TA + A_AP + l)) launch_amg_ap(v, T(TA + A_AP + l), NT(TA + A_AP + l), s);
+            if (l + 1 < kAmgMaxLevels && NT(TA + A_G + l + 1)) {
+                launch_amg_galerkin(v, T(TA + A_G + l + 1), NT(TA + A_G + l + 1), s);
+                launch_amg_bjac(v, T(TA + A_B + l + 1), NT(TA + A_B + l + 1), s);
+            }
         }
         if (NT(TA + A_D)) launch_amg_dense(v, T(TA + A_D), NT(TA + A_D), s);
-        for (int l = 0; l < kAmgMaxLevels; ++l)  // A P reads level l's final values
-            if (NT(TA + A_AP + l)) launch_amg_ap(v, T(TA + A_AP + l), NT(TA + A_AP + l), s);
     };
     amg_setup(vamg64, T_AMG);
     amg_setup(vamg32, T_AMG + A_N);

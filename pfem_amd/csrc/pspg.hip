@@ -35,7 +35,6 @@
 
 namespace pfem {
 
-constexpr int kAsmLanes = 16;  // lanes per node in the assembly kernels
 
 template <int D>
 struct Geom {
@@ -298,9 +297,9 @@ __device__ __forceinline__ void add_block(const ERec<D>& R, int a, int b, bool m
 
 // Writes node i's block at rank k into the sliced ELL copy (EllMat layout: value (s, t) of
 // lane r = i % 64 of slice c = i / 64 at ((soff[c] + k) nf^2 + s nf + t) 64 + r); rows that
-// are not full hold zeros there (their unit diagonal comes from K2).  The 16 nodes of a
-// workgroup write the same 128-byte line of each (k, s, t), so the L2 merges the lanes'
-// 8-byte stores into whole lines.
+// are not full hold zeros there (their unit diagonal comes from K2).  The nodes of a
+// workgroup (kBlock / kAsmLanes) write the same 128-byte lines of each (k, s, t), so the L2
+// merges the lanes' 8-byte stores into whole lines.
 template <int NE>
 __device__ __forceinline__ void ell_put(const PspgView& V, long long i, int k, const double (&blk)[NE][NE],
                                         bool mom, bool cont) {

@@ -148,6 +148,10 @@ struct KrylovView {
 // preconditioner only shapes the search directions, the solve converges on the FP64 system).
 constexpr int kAmgMaxDense = 88;  // coarsest-level unknowns inverted densely in LDS
 constexpr int kRestrictLanes = 8;  // lanes per aggregate of the restriction
+// Lanes per node in the PSPG assembly gathers (pair blocks, diagonal block and b).  Measured on
+// the 262k-node 3D dam break / a 100k-node 2D column, off-diagonal + diagonal kernels: 16 lanes
+// 767 + 370 / 41 + 44 us, 8 lanes 747 + 298 / 34 + 29 us, 4 lanes 888 + 298 / 49 + 27 us.
+constexpr int kAsmLanes = 8;
 
 template <typename T>
 struct AmgView {
@@ -171,7 +175,7 @@ struct AmgView {
     T* z;                  // level >= 1 output
     // transfer to the next coarser level
     int next_N;
-    int fmask;             // this level's Galerkin sums read level 0 (full rows only)
+    int pad0;
     const int* agg_ptr;    // members (nodes of this level) of each next-level aggregate
     const int* agg_nodes;
     const int* fine_agg;   // next-level aggregate of each node of this level (-1: none)
@@ -189,10 +193,13 @@ struct AmgView {
     T* ap_out;             // AP values (written at set-up, padding ranks included)
     EllMatT<float> APf;    // level 0 of an FP64 hierarchy: AP held in FP32 instead (vals null: none)
     float* ap_out32;       // its values (written at set-up in place of ap_out)
-    // setup of this level (>= 1) from the finer level
-    NodeMat<T> F;
-    const int* gal_ptr;
-    const int2* gal_src;
+    // setup of this level (>= 1) from the finer level: entry block (I, J) is the sum of the
+    // finer level's A P blocks (i, J) over the members i of aggregate I
+    EllMatT<T> FAP;        // the finer level's A P
+    EllMatT<float> FAPf;   // ... or its FP32 copy (level 1 of an FP64 hierarchy whose level 0
+                           // holds A P in FP32; vals null: none)
+    const int* gal_ptr;    // [entries + 1] gather list of each entry of this level
+    const int2* gal_src;   // {finer node i, rank m of J in i's A P row}
     const int* self_rank;
     T* vals_out;
     T* binv_out;
