@@ -222,8 +222,10 @@ struct RefineView {
 };
 
 // Lanes per node of the weakly compressible node-gather passes (2D, 3D): a P1 node has
-// ~6 incident triangles in 2D and ~24 tetrahedra in 3D.
-constexpr int kWcLanes[2] = {8, 16};
+// ~6 incident triangles in 2D and ~24 tetrahedra in 3D.  On the 2.06M-node 3D tank the
+// momentum / continuity gathers take 991 / 284 us at 16 lanes, 830 / 255 at 8 and 919 / 312
+// at 4 (whole step 566, 626 and 572 steps/s).
+constexpr int kWcLanes[2] = {8, 8};
 
 // Weakly compressible explicit step (WCompNewton/Solver.cpp:236-276).
 template <typename S>
