@@ -50,7 +50,7 @@ enum : int { W_NONE = 0, W_K64 = 1, W_K32 = 2, W_IR = 3, W_POST = 4, W_RESID = 5
 int res_kind_of(const Req* r) { return r->res_kind >= 0 ? r->res_kind : r->c->d.residual; }
 
 enum Tab : int {
-    T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_PRE, T_RESU, T_IRU,
+    T_COPY, T_UPD, T_ASM2, T_ASM3, T_BC, T_RESU, T_IRU,
     T_RES, T_IRR = T_RES + 3, T_K64 = T_IRR + 3, T_K32 = T_K64 + 3, T_F64 = T_K32 + 3, T_F32 = T_F64 + 3,
     T_WCN = T_F32 + 3, T_WCE2 = T_WCN + 2, T_WCE3 = T_WCE2 + 2, T_CFE2 = T_WCE3 + 2, T_CFE3 = T_CFE2 + 2,
     T_WEL2 = T_CFE3 + 2, T_WEL3 = T_WEL2 + 2,  // WC continuity element pass (one lane per element)
@@ -1037,7 +1037,6 @@ void Engine::tick(std::vector<Req*>& done) {
         if (op & OP_ASM) {
             append(tab[c->dim == 2 ? T_ASM2 : T_ASM3], sl, (long long)c->N * kAsmLanes, kElemCap);
             append(tab[T_BC], sl, c->N, kElemCap);
-            append_rows(tab[T_PRE], sl, c->n, c->pat.ell_nsl);
         }
         if (op & OP_RES) {
             if (res_kind_of(r) == PFEM_RES_AX_F) append_rows(tab[T_RES + li], sl, c->n, c->pat.ell_nsl);
@@ -1126,8 +1125,8 @@ void Engine::tick(std::vector<Req*>& done) {
     if (NT(T_ASM2)) launch_pspg_assemble(2, vpv, T(T_ASM2), NT(T_ASM2), s);
     if (NT(T_ASM3)) launch_pspg_assemble(3, vpv, T(T_ASM3), NT(T_ASM3), s);
     if (NT(T_BC)) launch_pspg_bc(vpv, T(T_BC), NT(T_BC), s);
-    // (the assembly and the BC write the sliced ELL copy themselves)
-    if (NT(T_PRE)) launch_pspg_precond(vpv, T(T_PRE), NT(T_PRE), s);
+    // (the assembly and the BC write the sliced ELL copy and the FP32 copies themselves, and the
+    // diagonal gather the block-Jacobi inverse of each node's final diagonal block)
     const AmgView<double>* vamg64 = I.d_amg64.as<AmgView<double>>();
     const AmgView<float>* vamg32 = I.d_amg32.as<AmgView<float>>();
     // multigrid setup of both hierarchy types (TA: first table of the type)

@@ -9,7 +9,6 @@ namespace pfem {
 
 void launch_pspg_assemble(int dim, const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
-void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s);
 
 // ids: the cases (case_id of each entry) whose KScal::req_init is set, nids of them
 template <typename T>

@@ -23,7 +23,10 @@
 // of a node (every element around it) are spread over the node's lane group and reduced
 // with cross-lane shuffles.  Momentum rows of boundary nodes are not assembled
 // (MomContEquationPSPG.inl:68); K2 gives them, and every row of a free node, the unit
-// diagonal of MomContEquationPSPG.inl:108-128.
+// diagonal of MomContEquationPSPG.inl:108-128.  The diagonal gather also writes the
+// block-Jacobi inverse of the node's diagonal block as K2 leaves it (K2 changes a node's own
+// block only through its flags), and every write of a value also writes the FP32 copies the
+// mixed-precision modes keep, so no later pass re-reads the values.
 //
 // K2 is node-major and also atomic-free: the Dirichlet column elimination
 // (MomContEquationPSPG.inl:216-228) is done row by row — the owner of row (j, s) walks its
@@ -319,6 +322,28 @@ __device__ __forceinline__ void ell_put(const PspgView& V, long long i, int k, c
     }
 }
 
+// Writes value q of the node-block layout and, when the precision mode keeps one, its FP32 copy.
+__device__ __forceinline__ void put_val(const PspgView& V, long long q, double v) {
+    V.vals[q] = v;
+    if (V.vals32) V.vals32[q] = (float)v;
+}
+
+// Block-Jacobi inverse of node i (structure of arrays [(s nf + t)][N], FP64 and, when kept,
+// FP32) from its final diagonal block m[0..nf)[0..nf) (m[r][4..7] = I on entry; padding rows
+// of a 2D block hold the identity).
+template <int NE>
+__device__ __forceinline__ void put_binv(const PspgView& V, long long i, double (&m)[4][8]) {
+    gj_invert4<false>(m);
+#pragma unroll
+    for (int s = 0; s < NE; ++s)
+#pragma unroll
+        for (int t = 0; t < NE; ++t) {
+            const double v = m[s][4 + t];
+            V.binv[(long long)(s * NE + t) * V.N + i] = v;
+            if (V.binv32) V.binv32[(long long)(s * NE + t) * V.N + i] = (float)v;
+        }
+}
+
 __device__ __forceinline__ void decode(uint32_t w, int& e, int& a, int& b) {
     e = (int)(w >> 4);
     a = (int)((w >> 2) & 3u);
@@ -374,7 +399,7 @@ __global__ __launch_bounds__(kBlock) void k_pspg_offdiag(const PspgView* __restr
                 if (s < D && !mom) continue;
                 const long long rs = (s < D) ? (long long)s * V.mom_nnz + inf.x : (long long)D * V.mom_nnz + inf.y;
 #pragma unroll
-                for (int t = 0; t < NE; ++t) V.vals[rs + (long long)t * deg + k] = blk[s][t];
+                for (int t = 0; t < NE; ++t) put_val(V, rs + (long long)t * deg + k, blk[s][t]);
             }
             ell_put<NE>(V, i, k, blk, mom, cont);
         }
@@ -398,7 +423,17 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
         const int4 inf = V.node_info[i];
         const int deg = inf.z;
         const uint8_t f = (uint8_t)inf.w;
-        if (f & NODE_FREE) continue;  // no elements: rows and b set by K2
+        if (f & NODE_FREE) {  // no elements: rows and b set by K2, block-Jacobi inverse I
+            if (lane == 0) {
+                double m[4][8];
+#pragma unroll
+                for (int r = 0; r < 4; ++r)
+#pragma unroll
+                    for (int c = 0; c < 8; ++c) m[r][c] = (c == r || c - 4 == r) ? 1.0 : 0.0;
+                put_binv<NE>(V, i, m);
+            }
+            continue;
+        }
         const bool mom = (f & (NODE_BOUND | NODE_FREE)) == 0;
         const int nb = V.nbr_ptr[i];
         int self = 0;  // rank of i in its own neighbourhood (sorted)
@@ -467,9 +502,26 @@ __global__ __launch_bounds__(kBlock) void k_pspg_diag(const PspgView* __restrict
                 if (s < D && !mom) continue;
                 const long long rs = (s < D) ? (long long)s * V.mom_nnz + inf.x : (long long)D * V.mom_nnz + inf.y;
 #pragma unroll
-                for (int t = 0; t < NE; ++t) V.vals[rs + (long long)t * deg + self] = blk[s][t];
+                for (int t = 0; t < NE; ++t) put_val(V, rs + (long long)t * deg + self, blk[s][t]);
             }
             ell_put<NE>(V, i, self, blk, mom, true);
+            // block-Jacobi inverse of the block as K2 leaves it: the velocity rows of a node
+            // without momentum rows are unit rows, and a Dirichlet wall node's own velocity
+            // columns are eliminated from its continuity row (k_pspg_bc)
+            const bool elim = (f & NODE_BOUND) && (f & NODE_DIRICHLET);
+            double m[4][8];
+#pragma unroll
+            for (int r = 0; r < 4; ++r)
+#pragma unroll
+                for (int c = 0; c < 8; ++c) {
+                    double v = (c == r || c - 4 == r) ? 1.0 : 0.0;
+                    if (r < NE && c < NE) {
+                        if (r < D) v = mom ? blk[r][c] : (r == c ? 1.0 : 0.0);
+                        else v = (elim && c < D) ? 0.0 : blk[r][c];
+                    }
+                    m[r][c] = v;
+                }
+            put_binv<NE>(V, i, m);
         }
     }
 }
@@ -518,11 +570,11 @@ __global__ __launch_bounds__(kBlock) void k_pspg_bc(const PspgView* __restrict__
                                         (long long)D * inf.z);
         if (bound || isfree)
             for (int s = 0; s < D; ++s) {
-                V.vals[(long long)s * V.mom_nnz + inf.x] = 1.0;
+                put_val(V, (long long)s * V.mom_nnz + inf.x, 1.0);
                 ell_set(self, s, s, 1.0);
             }
         if (isfree) {
-            V.vals[(long long)D * V.mom_nnz + inf.y] = 1.0;
+            put_val(V, (long long)D * V.mom_nnz + inf.y, 1.0);
             ell_set(self, D, D, 1.0);
             V.b[(long long)D * N + j] = 0.0;
             if (!bound)
@@ -545,65 +597,12 @@ __global__ __launch_bounds__(kBlock) void k_pspg_bc(const PspgView* __restrict__
                     const long long rs = (s < D) ? (long long)s * V.mom_nnz + inf.x : (long long)D * V.mom_nnz + inf.y;
                     const long long pos = rs + (long long)d * deg + k;
                     V.b[(long long)s * N + j] -= V.vals[pos] * vbc;
-                    V.vals[pos] = 0.0;
+                    put_val(V, pos, 0.0);
                     ell_set(k, s, d, 0.0);
                 }
             }
         }
     }
-}
-
-// Block-Jacobi preconditioner: the inverse of each node's (dim+1) x (dim+1) diagonal
-// block A[(i,s),(i,t)] (FP64 Gauss-Jordan with partial pivoting), stored structure of
-// arrays; plus the FP32 copies of A and of the inverses for the mixed-precision solve.
-__global__ __launch_bounds__(kBlock) void k_pspg_precond(const PspgView* __restrict__ views,
-                                                         const BlockTask* __restrict__ tasks) {
-    const BlockTask task = tasks[blockIdx.x];
-    if (task.case_id < 0) return;
-    const PspgView& V = views[task.case_id];
-    const int N = V.N, D = V.dim, nf = D + 1;
-    const long long stride = (long long)task.nblocks * kBlock;
-    for (long long i = (long long)task.block * kBlock + threadIdx.x; i < N; i += stride) {
-        const int4 inf = V.node_info[i];
-        const uint8_t f = (uint8_t)inf.w;
-        const bool mom_full = (f & (NODE_BOUND | NODE_FREE)) == 0, cont_full = (f & NODE_FREE) == 0;
-        const long long cont_start = (long long)D * V.mom_nnz + inf.y;
-        // rank of i in its own neighbourhood, from the diagonal of the continuity row
-        const int self = cont_full ? (int)(V.diag_pos[(long long)D * N + i] - cont_start - (long long)D * inf.z) : 0;
-        double m[4][8];
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int t = 0; t < 8; ++t) m[s][t] = (t - 4 == s) ? 1.0 : 0.0;
-#pragma unroll
-        for (int s = 0; s < 4; ++s) {
-            if (s >= nf) {
-                m[s][s] = 1.0;  // padding rows (2D): identity
-                continue;
-            }
-            const bool full = (s < D) ? mom_full : cont_full;
-            const long long rs = (s < D) ? (long long)s * V.mom_nnz + inf.x : cont_start;
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (t >= nf) continue;
-                if (full) m[s][t] = V.vals[rs + (long long)t * inf.z + self];
-                else m[s][t] = (s == t) ? V.vals[rs] : 0.0;
-            }
-        }
-        gj_invert4<false>(m);
-#pragma unroll
-        for (int s = 0; s < 4; ++s)
-#pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                if (s >= nf || t >= nf) continue;
-                const double v = m[s][4 + t];
-                V.binv[(long long)(s * nf + t) * N + i] = v;
-                if (V.binv32) V.binv32[(long long)(s * nf + t) * N + i] = (float)v;
-            }
-    }
-    if (V.vals32)
-        for (long long k = (long long)task.block * kBlock + threadIdx.x; k < V.nnz; k += stride)
-            V.vals32[k] = (float)V.vals[k];
 }
 
 // Launchers ---------------------------------------------------------------------------
@@ -623,8 +622,5 @@ void launch_pspg_bc(const PspgView* views, const BlockTask* tasks, int ntasks, h
     k_pspg_bc<<<ntasks, kBlock, 0, s>>>(views, tasks);
 }
 
-void launch_pspg_precond(const PspgView* views, const BlockTask* tasks, int ntasks, hipStream_t s) {
-    k_pspg_precond<<<ntasks, kBlock, 0, s>>>(views, tasks);
-}
 
 }  // namespace pfem
