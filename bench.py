@@ -1042,8 +1042,8 @@ def main():
             "picard": "Ax_f, minRes 1e-6, maxIter 10",
             "linear_solve": "BiCGSTAB to true relres <= 1e-8 (and ||r|| <= 0.1 minRes), right-preconditioned by "
                             "aggregation multigrid (2D >= 2k nodes; 3D >= 50k FP64 / >= 5k mixed) or block Jacobi",
-            "batched": f"continuous batching ({args.slots} slots), PFEM_PREC_MIXED: FP64 BiCGSTAB, FP32 3D "
-                       f"multigrid, FP32 WC state",
+            "batched": f"continuous batching ({args.slots} slots), PFEM_PREC_MIXED: FP64 BiCGSTAB, multigrid "
+                       f"with FP32 level-0 values, FP32 WC state",
             "plain": "one request at a time through the direct calls, FP64 throughout",
             "slots": args.slots, "arena_GiB": args.budget_gb, "stream_cases": n_req,
         },

@@ -62,9 +62,12 @@ extern "C" {
 
 #define PFEM_PREC_FP64 0  /* FP64 assembly, SpMV and Krylov */
 #define PFEM_PREC_MIXED 1    /* FP64 BiCGSTAB (converges on the FP64 system) preconditioned by
-                                the multigrid hierarchy in FP32 (values, inverses, vectors and
-                                SpMVs; the coarsest dense inverse stays FP64) for 3D cases; 2D
-                                hierarchies and block-Jacobi cases run as FP64 */
+                                the multigrid with FP32 values at level 0 (the residual's
+                                product over an FP32 copy of A and the post-smoothing product
+                                A P held in FP32; FP64 vectors, coarse levels and coarsest
+                                dense inverse), 3D cases from 5,000 nodes; block-Jacobi cases
+                                run as FP64.  PFEM_AMG_FP32=1 in the environment when the mesh
+                                is set runs a 3D hierarchy wholly in FP32 instead */
 #define PFEM_PREC_MIXED_IR 2 /* FP32 inner BiCGSTAB (FP32 vectors) on the FP32 copy of A inside
                                 FP64 iterative refinement; falls back to FP64 BiCGSTAB when the
                                 FP32 corrections stop reducing the FP64 residual */

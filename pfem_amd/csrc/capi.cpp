@@ -683,6 +683,8 @@ int pfem_case_set_mesh(pfem_case* c, int32_t N, int32_t E, const int32_t* conn, 
             PFEM_HIP(hipMemsetAsync(c->q_prev.p, 0, sizeof(double) * n, c->stream));
             c->lanes = pfem::pick_lanes((double)c->pat.graph_nnz / (double)N);
             c->amg = pfem::AmgHierarchy{};
+            const char* f32 = std::getenv("PFEM_AMG_FP32");
+            c->amg_all_fp32 = f32 && std::atoi(f32) != 0;
             if (wants_amg(c)) pfem::build_amg(c->pat, dim, pfem::kAmgMaxDense / (dim + 1), c->stream, c->amg);
             lap(2, t0);
             if (c->d.kind == PFEM_KIND_BOUSSINESQ) setup_heat(c);

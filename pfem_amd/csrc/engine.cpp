@@ -165,17 +165,21 @@ struct WsLayout {
 // 8 % faster than FP32 operators with FP64 vectors.  2D hierarchies keep FP64 vectors: an
 // FP32 V-cycle raised the 2D BiCGSTAB iteration count by 20-40 % (their level 0 reads FP32
 // values instead, amg_l0_fp32).
+// A hierarchy wholly in FP32 (values, inverses and vectors of every level): only for 3D
+// PFEM_PREC_MIXED cases set up with PFEM_AMG_FP32=1.  On the 512-request serving stream it
+// takes more BiCGSTAB iterations than an FP64 hierarchy with FP32 level-0 values (below):
+// 24,313 against 23,519, and 210.6 against 220.4 steps/s; in 2D 31,053 iterations.
 bool amg_fp32(const pfem_case* c) {
-    return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 3;
+    return c->amg_all_fp32 && c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 3;
 }
 
-// PFEM_PREC_MIXED 2D cases keep the FP64 hierarchy but read FP32 values at level 0: the
-// residual t = r - A w (an FP32 ELL copy of the system) and the post-smoothing product
+// PFEM_PREC_MIXED cases otherwise keep the FP64 hierarchy but read FP32 values at level 0:
+// the residual t = r - A w (an FP32 ELL copy of the system) and the post-smoothing product
 // (A P) z (A P held in FP32), with FP64 vectors and sums.  The iteration counts do not
-// change, and with non-temporal loads FP32 values nearly halve those kernels' time
+// rise, and with non-temporal loads FP32 values nearly halve those kernels' time
 // (measured on gfx950, ELL SpMV of a 900k-node 2D system: 91 -> 52 us).
 bool amg_l0_fp32(const pfem_case* c) {
-    return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && c->dim == 2;
+    return c->d.precision == PFEM_PREC_MIXED && c->amg.levels > 1 && !amg_fp32(c);
 }
 
 WsLayout ws_layout(const pfem_case* c) {

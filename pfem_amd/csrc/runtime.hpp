@@ -215,6 +215,7 @@ struct pfem_case {
     // smallest element shape quality 6 sqrt(2) |e| / l_rms^3 of the last mesh (3D; 1 for a regular
     // tet) and the multigrid smoother damping cap it implies (see pfem_case_set_mesh)
     double min_quality = 1.0, amg_omega_cap = 1e30;
+    bool amg_all_fp32 = false;  // PFEM_AMG_FP32=1 at set_mesh: 3D mixed hierarchies wholly in FP32
     double setup_s[4] = {0, 0, 0, 0};  // last pfem_case_set_mesh: pattern, ELL, multigrid, total (seconds)
     pfem_ctx* ctx = nullptr;
     pfem_case_desc d{};

@@ -144,13 +144,16 @@ def test_solve_relres_on_oracle_system(ctx, name, precision, precond):
     assert np.linalg.norm(x - xs) / np.linalg.norm(xs) <= 1e-6
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed_all_fp32"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_amg_cuts_iterations(ctx, dim, precision):
+def test_amg_cuts_iterations(ctx, dim, precision, monkeypatch):
     """On a mesh large enough for a multi-level hierarchy, the multigrid-preconditioned solve
     reaches the same true relative residual on the oracle's system in a fraction of the
-    block-Jacobi iterations (mixed: FP32 level-0 values under the FP64 2D hierarchy, FP32 3D
-    hierarchy)."""
+    block-Jacobi iterations (mixed: FP32 level-0 values under the FP64 hierarchy;
+    mixed_all_fp32: PFEM_AMG_FP32=1, a 3D hierarchy wholly in FP32, 2D as mixed)."""
+    if precision == "mixed_all_fp32":
+        monkeypatch.setenv("PFEM_AMG_FP32", "1")
+        precision = "mixed"
     mesh = mg.column_2d_nodes(20000, h=0.01, seed=2) if dim == 2 else mg.column_3d_nodes(30000, h=0.01, seed=2)
     say(f"[amg] dim={dim} {precision} N={mesh.n_nodes}")
     q = mg.pspg_hydrostatic(mesh)
