@@ -336,12 +336,14 @@ int pfem_server_stats(pfem_server* s, int64_t* out8);
 int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4);
 
 /* Instrumentation (no counterpart in the reference): back-to-back timings, in
- * microseconds per launch, of kernels on this case's assembled system.  out8: {the FP64
+ * microseconds per launch, of kernels on this case's assembled system.  out10: {the FP64
  * BiCGSTAB SpMV kernel (v = A ph with the fused (rh, v)) as a solve launches it, node-block
  * SpMV FP32 (0 unless PFEM_PREC_MIXED_IR), streaming read of the nnz FP64 values, copy
- * kernel, bytes moved by the copy, nnz, node-graph nnz, lanes per node}.  The case needs an
- * assembled system and its step workspace. */
-int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8);
+ * kernel, bytes moved by the copy, nnz, node-graph nnz, lanes per node, the multigrid's
+ * level-0 residual t = r - A w over the FP64 ELL copy as a V-cycle launches it (0 without a
+ * multigrid), its workgroup count}.  The case needs an assembled system and its step
+ * workspace. */
+int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out10);
 /* Instrumentation: XCD (XCC) id on which each of nblocks single-wavefront workgroups ran. */
 int pfem_probe_xcd(pfem_ctx* ctx, int32_t nblocks, int32_t* out);
 

@@ -328,10 +328,12 @@ class Case:
         """Back-to-back timings (microseconds per launch) on the assembled system: the FP64
         BiCGSTAB SpMV kernel as a solve launches it, the FP32 node-block SpMV, a streaming read
         of the nnz FP64 values and a copy kernel, with the copy's bytes, nnz, node-graph nnz
-        and lanes per node."""
-        out = np.zeros(8)
+        and lanes per node; then the multigrid's level-0 residual kernel (0 without a
+        multigrid) and its workgroup count."""
+        out = np.zeros(10)
         check(lib().pfem_bench_kernels(self._h, int(reps), _dp(out)))
-        keys = ("kry_v_us", "spmv32_us", "stream_read_us", "copy_us", "copy_bytes", "nnz", "graph_nnz", "lanes")
+        keys = ("kry_v_us", "spmv32_us", "stream_read_us", "copy_us", "copy_bytes", "nnz", "graph_nnz", "lanes",
+                "amg_res0_us", "amg_res0_blocks")
         return dict(zip(keys, out.tolist()))
 
     def wc_next_dt(self, security_coeff, max_dt):

@@ -1219,10 +1219,11 @@ int pfem_profile_spmv(pfem_case* c, int32_t enable, double* out4) {
     });
 }
 
-int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
+int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out10) {
     return guarded([&] {
         require_system(c);
-        PFEM_REQUIRE(out8 && reps > 0, PFEM_E_INVALID, "invalid arguments");
+        PFEM_REQUIRE(out10 && reps > 0, PFEM_E_INVALID, "invalid arguments");
+        double* out8 = out10;
         PFEM_HIP(hipSetDevice(c->ctx->device));
         hipEvent_t e0, e1;
         PFEM_HIP(hipEventCreateWithFlags(&e0, hipEventDisableSystemFence));
@@ -1240,7 +1241,7 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
         };
         const int n = c->n;
         double* w = c->ws.kv64;
-        for (int k = 0; k < 8; ++k) out8[k] = 0;
+        for (int k = 0; k < 10; ++k) out10[k] = 0;
         {  // the BiCGSTAB SpMV kernel (v = A ph with the fused (rh, v)) as a solve launches it
             const int nb = pfem::row_blocks(n, c->pat.ell_nsl);
             std::vector<pfem::BlockTask> tasks((size_t)nb);
@@ -1291,6 +1292,40 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out8) {
         out8[5] = (double)nnz;
         out8[6] = (double)c->pat.graph_nnz;
         out8[7] = (double)c->lanes;
+        if (c->amg.levels > 1) {  // the multigrid's level-0 residual t = r - A w over the FP64 ELL copy
+            const int nb = (int)std::min<long long>((c->N + pfem::kBlock - 1) / pfem::kBlock, 8192);
+            std::vector<pfem::BlockTask> tasks((size_t)nb);
+            for (int b = 0; b < nb; ++b) tasks[(size_t)b] = pfem::BlockTask{0, b, nb, 0};
+            DevBuf d_tasks, d_view, d_done;
+            d_done.reserve(sizeof(int));
+            PFEM_HIP(hipMemsetAsync(d_done.p, 0, sizeof(int), c->stream));
+            PFEM_HIP(hipMemsetAsync(w + 2LL * n, 0, sizeof(double) * (size_t)n, c->stream));
+            pfem::AmgView<double> av{};
+            av.A = node_mat<double>(c->pat, c->dim, c->ws.vals);
+            av.E0.N = c->N;
+            av.E0.nf = c->dim + 1;
+            av.E0.nsl = c->pat.ell_nsl;
+            av.E0.soff = c->pat.ell_soff.as<int>();
+            av.E0.idx = c->pat.ell_idx.as<int>();
+            av.E0.vals = c->ws.vals_ell;
+            av.level = 0;
+            av.lanes = c->lanes;
+            av.mask = 1;
+            av.done_flag = d_done.as<int>();
+            av.r0[0] = c->q_prev.as<double>();
+            av.w = w + 2LL * n;
+            av.t = w + 3LL * n;
+            d_tasks.reserve(sizeof(pfem::BlockTask) * tasks.size());
+            d_view.reserve(sizeof(av));
+            PFEM_HIP(hipMemcpyAsync(d_tasks.p, tasks.data(), sizeof(pfem::BlockTask) * tasks.size(),
+                                    hipMemcpyHostToDevice, c->stream));
+            PFEM_HIP(hipMemcpyAsync(d_view.p, &av, sizeof(av), hipMemcpyHostToDevice, c->stream));
+            out10[8] = timed([&] {
+                pfem::launch_amg_res<double>(d_view.as<pfem::AmgView<double>>(), d_tasks.as<pfem::BlockTask>(), nb, 0,
+                                             c->stream);
+            });
+            out10[9] = (double)nb;
+        }
         (void)hipEventDestroy(e0);
         (void)hipEventDestroy(e1);
         return PFEM_OK;
