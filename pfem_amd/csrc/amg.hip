@@ -174,7 +174,7 @@ __device__ __forceinline__ void ap_body(const AmgView<T>& V, const BlockTask& ta
             const int qe = m < pdeg ? V.ap_gptr[e0 + m + 1] : 0;
             for (int q = m < pdeg ? V.ap_gptr[e0 + m] : 0; q < qe; ++q) {
                 const int k = V.ap_gsrc[q];
-                const uint8_t fj = V.mask ? (uint8_t)A.info[A.nbr_idx[anb + k]].w : (uint8_t)0;
+                const uint8_t fj = V.mask ? V.flags[A.nbr_idx[anb + k]] : (uint8_t)0;
 #pragma unroll
                 for (int s = 0; s < kMaxNf; ++s) {
                     if (s >= nf) continue;
@@ -400,7 +400,7 @@ __global__ __launch_bounds__(kBlock) void k_amg_restrict(const AmgView<T>* __res
         double acc[kMaxNf] = {0.0, 0.0, 0.0, 0.0};
         for (int m = V.agg_ptr[I] + lane; m < V.agg_ptr[I + 1]; m += L) {
             const int i = V.agg_nodes[m];
-            const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
+            const uint8_t f = V.mask ? V.flags[i] : (uint8_t)0;
 #pragma unroll
             for (int s = 0; s < kMaxNf; ++s)
                 if (s < nf && (!V.mask || row_full_flags(f, s, D))) acc[s] += (double)V.t[i + (long long)s * N];
@@ -456,7 +456,7 @@ __device__ __forceinline__ void post_ell(const AmgView<T>& V, const BlockTask& t
 #pragma unroll
         for (int t = 0; t < NF; ++t) d[t] = V.t[i + (long long)t * N] - al * y[t];
         const int I = V.fine_agg[i];
-        const uint8_t f = V.mask ? (uint8_t)V.A.info[i].w : (uint8_t)0;
+        const uint8_t f = V.mask ? V.flags[i] : (uint8_t)0;
 #pragma unroll
         for (int s = 0; s < NF; ++s) {
             T acc = 0;

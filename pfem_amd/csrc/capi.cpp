@@ -1311,6 +1311,7 @@ int pfem_bench_kernels(pfem_case* c, int32_t reps, double* out10) {
             av.level = 0;
             av.lanes = c->lanes;
             av.mask = 1;
+            av.flags = c->pat.flags.as<uint8_t>();
             av.done_flag = d_done.as<int>();
             av.r0[0] = c->q_prev.as<double>();
             av.w = w + 2LL * n;

@@ -731,6 +731,7 @@ void Engine::Impl::load_amg(pfem_case* c, int slot, const KrylovView<double>& k,
             }
             V.lanes = c->lanes;
             V.mask = 1;
+            V.flags = c->pat.flags.as<uint8_t>();
             V.r0[0] = k.p;
             V.r0[1] = k.s;
             V.out0[0] = k.ph;

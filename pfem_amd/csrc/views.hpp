@@ -162,6 +162,7 @@ struct AmgView {
     int level;
     int lanes;             // SpMV lanes per node on this level
     int mask;              // level 0: transfers and Galerkin sums use full rows only
+    const uint8_t* flags;  // level 0: the nodes' flags (the masks read them; 1 byte per node)
     int nc;                // coarsest level: unknowns of the dense inverse
     double omega;          // block-Jacobi damping of the smoother
     double alpha;          // scaling of the coarse-grid correction (over-correction of the
