@@ -588,8 +588,12 @@ void Engine::Impl::load_views(Req* r) {
         if (k.amg) {
             if (amg_fp32(c))
                 load_amg<float>(c, slot, k, h_amg32, d_amg32, node_mat<float>(P, dim, w.vals32), w.binv32);
-            else
+            else {
                 load_amg<double>(c, slot, k, h_amg64, d_amg64, node_mat<double>(P, dim, w.vals), w.binv);
+                // K_p / K_s write the level-0 pre-smoothing (the FP64 V-cycle launches none)
+                k.mg_w0 = reinterpret_cast<double*>(w.amg_w0);
+                k.mg_om = std::min(amg_omega, c->amg_omega_cap);
+            }
         }
 
         KrylovView<float>& f = h_k32[slot];
@@ -1185,8 +1189,9 @@ void Engine::tick(std::vector<Req*>& done) {
         for (int l = 0; l < kAmgMaxLevels; ++l) {
             if (!NT(TA + A_VN + l)) break;
             const int tr = TA + (l == 0 ? A_VN : A_VS) + l;
-            // the pre-smoothing of levels >= 1 is part of the restriction from the finer level
-            if (l == 0) launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
+            // the pre-smoothing of levels >= 1 is part of the restriction from the finer level,
+            // that of level 0 in FP64 hierarchies part of K_p / K_s (KrylovView::mg_w0)
+            if (l == 0 && TA != T_AMG) launch_amg_pre(v, T(TA + A_VN + l), NT(TA + A_VN + l), which, s);
             launch_amg_res(v, T(tr), NT(tr), which, s);
             launch_amg_restrict(v, T(TA + A_VR + l), NT(TA + A_VR + l), s);
         }

@@ -53,6 +53,18 @@ __device__ __forceinline__ void block_apply(const T* __restrict__ binv, int node
     }
 }
 
+// The multigrid's level-0 pre-smoothing w = om B^-1 in of node i (FP64 hierarchies), written
+// by the kernel that produced `in` (the same products and order as the V-cycle's own pass).
+template <typename T>
+__device__ __forceinline__ void smooth_w0(const KrylovView<T>& V, long long i, const T (&in)[kMaxNf]) {
+    T out[kMaxNf];
+    block_apply<T>(V.binv, V.nodes, V.nf, i, in, out);
+    const T om = (T)V.mg_om;
+#pragma unroll
+    for (int s = 0; s < kMaxNf; ++s)
+        if (s < V.nf) V.mg_w0[(long long)s * V.nodes + i] = (double)(om * out[s]);
+}
+
 // Sum of partial slots over the producer's workgroups, in block order; valid in every thread.
 template <int NV>
 __device__ __forceinline__ void reduce_partials(const double* __restrict__ P, int nblk, const int (&slot)[NV],
@@ -199,7 +211,10 @@ __global__ __launch_bounds__(kBlock) void k_kry_p(const KrylovView<T>* __restric
                 V.p[q] = pv[f];
             }
         }
-        if (V.amg) continue;  // ph = M^-1 p comes from the V-cycle
+        if (V.amg) {  // ph = M^-1 p comes from the V-cycle; its pre-smoothing starts here
+            if (V.mg_w0) smooth_w0<T>(V, i, pv);
+            continue;
+        }
         block_apply<T>(V.binv, V.nodes, nf, i, pv, ph);
 #pragma unroll
         for (int f = 0; f < kMaxNf; ++f)
@@ -311,7 +326,10 @@ __global__ __launch_bounds__(kBlock) void k_kry_s(const KrylovView<T>* __restric
                 part[0] += (double)sv[f] * (double)sv[f];
             }
         }
-        if (V.amg) continue;  // sh = M^-1 s comes from the V-cycle
+        if (V.amg) {  // sh = M^-1 s comes from the V-cycle; its pre-smoothing starts here
+            if (V.mg_w0) smooth_w0<T>(V, i, sv);
+            continue;
+        }
         block_apply<T>(V.binv, V.nodes, nf, i, sv, sh);
 #pragma unroll
         for (int f = 0; f < kMaxNf; ++f)

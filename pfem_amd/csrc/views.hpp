@@ -141,6 +141,10 @@ struct KrylovView {
     double* partials;    // [kMaxCaseBlocks * kPartStride]
     unsigned* counters;  // [8]
     KScal* sc;
+    // FP64 multigrid: K_p / K_s also write the V-cycle's level-0 pre-smoothing w = om B^-1 p
+    // (or s) from the vector they hold (null: the V-cycle's own pre-smoothing launch does it)
+    double* mg_w0;
+    double mg_om;
 };
 
 // One level of a case's aggregation multigrid (amg.hpp) as seen by the setup and V-cycle
